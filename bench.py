@@ -1,0 +1,168 @@
+"""Benchmark of the FSILBM3D fluid hot path on one MI355X.
+
+Workload: the stencil benchmark configuration -- D3Q19 512x256x256 fully
+periodic block, SRT, tau 0.55, initial state the equilibrium of a seeded
+density 1+U(-1e-3,1e-3) and velocity U(-0.05,0.05) per cell (splitmix64, seed
+20261202).  One step = one fused collide+push-stream pass over every cell
+(the body of main.f90:93-147 for a block with no bodies and periodic faces).
+
+Prints ONE JSON line.  value = MLUPS of the FP64 plain form (whole job, one
+GPU), measured host-side around exactly --steps steps between device
+synchronisations; the FP32-deviation form is reported beside it, with its
+velocity L2 error against FP64.  roofline: the fused kernel's algorithmic
+bytes per launch (FP64: 19 reads + 19 writes x 8 B = 304 B per cell) over
+its average duration from HIP events on the block's stream.  cpu_baseline:
+the reference program itself (oracle/_ref, compiled from the reference
+sources) timed on this host's cores for a few steps of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+DIMS = (512, 256, 256)
+NU = 0.05 * 1.0 / 3.0        # Uref 0.05, Lref 1, Re 3 -> tau 0.55 (FluidDomain.f90:452)
+HBM_PEAK_GBS = 8000.0
+B_CELL = {0: 304, 1: 152}
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_form(prec, f0, steps, warmup, nlaunch):
+    from fsilbm3d_amd.block import LBMBlock, synchronize
+    N = DIMS[0] * DIMS[1] * DIMS[2]
+    blk = LBMBlock(DIMS, bc=(301,) * 6, nu=NU, precision=prec)
+    blk.upload(f0)
+    blk.steps(warmup, 0.0, 1.0)
+    synchronize()
+    t0 = time.perf_counter()
+    ev_ms = blk.time_steps(steps, float(warmup), 1.0)
+    synchronize()
+    t1 = time.perf_counter()
+    kern_ms = blk.time_collide(nlaunch)
+    synchronize()
+    blk.close()
+    secs = t1 - t0
+    mlups = N * steps / secs / 1e6
+    achieved = B_CELL[prec] * N / (kern_ms * 1e-3) / 1e9
+    return dict(mlups=mlups, ms_per_step=secs / steps * 1e3, event_ms_per_step=ev_ms / steps,
+                kernel_ms=kern_ms, achieved_gbs=achieved)
+
+
+def l2_fp32_vs_fp64(f0, nsteps):
+    import numpy as np
+    from fsilbm3d_amd.block import LBMBlock
+    u = {}
+    for prec in (0, 1):
+        blk = LBMBlock(DIMS, bc=(301,) * 6, nu=NU, precision=prec)
+        blk.upload(f0)
+        blk.steps(nsteps, 0.0, 1.0)
+        u[prec] = blk.calculate_macro_quantities()[1]
+        blk.close()
+    return float(np.sqrt(((u[1] - u[0]) ** 2).sum() / (u[0] ** 2).sum()))
+
+
+def cpu_baseline(steps):
+    from oracle.refcase import Case, time_reference_steps
+    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
+    case = Case(dims=DIMS, bc=(301,) * 6, Re=3.0, Uref=0.05, UrefType=9, Lref=1.0, LrefType=1, TrefType=0)
+    secs = time_reference_steps(case, steps, threads)
+    N = DIMS[0] * DIMS[1] * DIMS[2]
+    tot = sum(secs)
+    return {"value": N * len(secs) / tot / 1e6 if tot > 0 else None, "unit": "MLUPS", "cores": threads,
+            "kind": "reference",
+            "sample": f"{len(secs)} steps of the 512x256x256 periodic SRT block (reference program built with "
+                      f"amdflang -O3 -fopenmp, {threads} OpenMP threads, uniform initial state: per-step work is "
+                      f"value-independent); seconds per step {[round(s, 3) for s in secs]}"}
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the FP64 fused kernel from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
+    calibration), or None when it has not been measured."""
+    p = os.path.join(ROOT, "profiles", "collide_push_fp64_512x256x256_pmc.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        return json.load(open(p)).get("hbm_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--nlaunch", type=int, default=50)
+    ap.add_argument("--l2-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
+              "decomposition is a design note, not a measured figure.", file=sys.stderr)
+        sys.exit(2)
+    from tests.golden.cases import seeded_state
+    log("bench: seeding 512x256x256 state")
+    f0 = seeded_state(DIMS, 20261202)
+    log("bench: FP64 plain form")
+    r64 = run_form(0, f0, args.steps, args.warmup, args.nlaunch)
+    log(f"bench: FP64 {r64}")
+    r32 = None
+    l2 = None
+    if not args.no_fp32:
+        log("bench: FP32-deviation form")
+        r32 = run_form(1, f0, args.steps, args.warmup, args.nlaunch)
+        log(f"bench: FP32 {r32}")
+        if args.l2_steps > 0:
+            log(f"bench: FP32 vs FP64 velocity L2 after {args.l2_steps} steps")
+            l2 = l2_fp32_vs_fp64(f0, args.l2_steps)
+    cpu = None
+    if not args.no_cpu:
+        log(f"bench: reference CPU baseline, {args.cpu_steps} steps")
+        try:
+            cpu = cpu_baseline(args.cpu_steps)
+        except Exception as e:  # reported, not fatal: the baseline is not the target
+            cpu = {"value": None, "unit": "MLUPS", "cores": None, "kind": "reference", "sample": f"failed: {e}"}
+    N = DIMS[0] * DIMS[1] * DIMS[2]
+    out = {
+        "metric": "FSI-step MLUPS, 1 MI355X: FP32+learned vs FP64 plain @drag/lift err<1e-3; %HBM",
+        "value": r64["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded equilibrium state)",
+        "config": {"workload": "stencil benchmark: D3Q19 512x256x256 periodic, SRT tau 0.55, FP64 plain form",
+                   "cells": N, "collision": "SRT", "form": "fp64-plain"},
+        "roofline": {"bound": "hbm", "achieved": r64["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": r64["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "k_collide_push<1,false,false,false>", "kernel_ms": r64["kernel_ms"],
+                     "bytes_per_cell": B_CELL[0]},
+        "cpu_baseline": cpu,
+    }
+    if r32 is not None:
+        out["fp32_deviation"] = {"value": r32["mlups"], "unit": "MLUPS", "ms_per_step": r32["ms_per_step"],
+                                 "speedup_vs_fp64": r32["mlups"] / r64["mlups"],
+                                 "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                                              "unit": "GB/s", "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
+                                              "kernel_ms": r32["kernel_ms"], "bytes_per_cell": B_CELL[1]},
+                                 "velocity_l2_rel_vs_fp64": l2, "l2_steps": args.l2_steps}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,95 @@
+"""ctypes binding of the C-ABI in include/fsilbm3d_amd.h (libfsil.so).
+
+The library is the only compute path: there is no CPU fallback, and loading
+fails loudly when the shared object has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfsil.so")
+
+# every exported symbol with (restype, argtypes); the CPU test checks this
+# list against include/fsilbm3d_amd.h
+_i, _d, _pi, _pd = C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double)
+SIGNATURES = {
+    "fsil_version": (_i, []),
+    "fsil_last_error": (C.c_char_p, []),
+    "fsil_init": (_i, [_i]),
+    "fsil_finalize": (_i, []),
+    "fsil_synchronize": (_i, []),
+    "fsil_block_create": (_i, [_i, _pi, _pd, _pi, _i, _pd, _pd, _pd, _i, _pi]),
+    "fsil_block_destroy": (_i, [_i]),
+    "fsil_set_flow": (_i, [_i, _d, _pd, _pd, _i, _pd, _d, _d, _d]),
+    "fsil_set_time": (_i, [_i, _d]),
+    "fsil_get_volume_force": (_i, [_i, _pd]),
+    "fsil_f_upload": (_i, [_i, _pd]),
+    "fsil_f_download": (_i, [_i, _pd]),
+    "fsil_set_boundary_conditions": (_i, [_i]),
+    "fsil_macro_download": (_i, [_i, _pd, _pd]),
+    "fsil_collide_stream_bc": (_i, [_i]),
+    "fsil_steps": (_i, [_i, _i, _d, _d]),
+    "fsil_time_collide": (_i, [_i, _i, _pd]),
+    "fsil_time_steps": (_i, [_i, _i, _d, _d, _pd]),
+    "fsil_ibm_define_bodies": (_i, [_i, _i, _pi, _pi, _pi, _pi, _pi]),
+    "fsil_ibm_set_params": (_i, [_i, _i, _d, _d, _d]),
+    "fsil_ibm_interaction_force": (_i, [_i, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
+    "fsil_ibm_band_info": (_i, [_i, _pi, _pi]),
+}
+
+_lib = None
+_lock = threading.Lock()
+
+
+class FsilError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libfsil.so (built by fsilbm3d_amd/build.py); raise if absent."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise FsilError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
+                            " (there is no CPU fallback)")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise FsilError(load().fsil_last_error().decode())
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_pd)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_pi)
+
+
+_initialised = False
+
+
+def init(device: int = 0):
+    global _initialised
+    lib = load()
+    if not _initialised:
+        check(lib.fsil_init(device))
+        _initialised = True
+    return lib

@@ -1,0 +1,900 @@
+// C-ABI of the FSILBM3D MI355X hot-path library: block lifetime, state
+// transfer, the fused fluid step and the immersed-boundary coupling.
+// Declarations and the reference routine each entry point replaces are in
+// include/fsilbm3d_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fsilbm3d_amd.h"
+#include "fsil_common.h"
+#include "fsil_ibm_kernels.h"
+#include "fsil_lbm_kernels.h"
+#include "fsil_lbm32_kernels.h"
+
+using namespace fsil;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return 1;
+}
+
+#define HIPCHK(call)                                                                     \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess) return fail("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                      __FILE__, __LINE__);                               \
+  } while (0)
+
+template <typename T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // grows (never shrinks) the allocation; contents are not preserved
+  hipError_t ensure(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+    if (e == hipSuccess) n = std::max<size_t>(count, 1);
+    return e;
+  }
+};
+
+struct Ibm {
+  int nbodies = 0;
+  std::vector<int> nmark, nnode, nelem;      // per body
+  std::vector<int> moff, noff, eoff;         // prefix offsets (markers, nodes, elements)
+  int M = 0, nnodes = 0, nelems = 0;
+  std::vector<int> marker_elem;              // global element id of each marker
+  int ntol = 0;
+  double dtol = 0.0, Uref = 1.0, denIn = 1.0;
+  // host scratch
+  std::vector<int> Ei;                       // 12*M (1-based grid indices)
+  std::vector<float> Ew;                     // 12*M
+  std::vector<int> bidx;                     // 64*M band index per entry
+  std::vector<std::pair<long long, int>> keyed;
+  std::vector<long long> band_cell;
+  std::vector<int> band_ptr, entry;
+  std::vector<int> bmap;
+  int bx0 = 0, by0 = 0, bz0 = 0, bnx = 0, bny = 0, bnz = 0;
+  int nband = 0;
+  // device
+  DevBuf<double> Exyz, Evel, Ea, Eforce, fT, tolpart, band_u, band_F, elem_xc, lod;
+  DevBuf<float> dEw;
+  DevBuf<int> dbidx, dptr, dentry, dbmap, dmarker_elem, dof_ptr, dof_list, dof_kind;
+  DevBuf<long long> dcell;
+  std::vector<double> h_tolpart;
+  int ndof = 0;
+  bool active = false;  // band valid for the next collision
+};
+
+struct Block {
+  int id = 0;
+  int xd = 0, yd = 0, zd = 0;
+  long long N = 0;
+  double dh = 1, xmin = 0, ymin = 0, zmin = 0, xmax = 0, ymax = 0, zmax = 0;
+  int bc[6] = {0, 0, 0, 0, 0, 0};
+  int model = 1;
+  double tau = 1, Omega = 1, Omega2 = 1;
+  int precision = 0;  // 0 FP64 plain, 1 FP32 deviation storage
+  DevBuf<double> fA, fB;
+  DevBuf<float> gA, gB;
+  DevBuf<double> Mc, Mf;
+  // flow conditions (FlowCondition.f90:11-26)
+  double denIn = 1, uvwIn[3] = {0, 0, 0}, shear[3] = {0, 0, 0};
+  int vkind = 0;
+  double VFin[3] = {0, 0, 0}, VFamp = 0, VFfreq = 0, VFphi = 0;
+  double VF[3] = {0, 0, 0};
+  double blktime = 0;
+  // auxiliary planes
+  DevBuf<double> nb_den[6], nb_u[6], hw[6];
+  bool hw_valid[6] = {false, false, false, false, false, false};
+  Ibm ibm;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+std::map<int, std::unique_ptr<Block>> g_blocks;
+int g_next_handle = 1;
+int g_device = -1;
+
+Block *get_block(int h) {
+  auto it = g_blocks.find(h);
+  if (it == g_blocks.end()) {
+    fail("invalid block handle %d", h);
+    return nullptr;
+  }
+  return it->second.get();
+}
+
+int plane_size(const Block &b, int face) {
+  const int axis = face >> 1;
+  return axis == 0 ? b.yd * b.zd : (axis == 1 ? b.xd * b.zd : b.xd * b.yd);
+}
+
+constexpr double kPi = 3.141592653589793;  // ConstParams.f90:36
+
+// update_volume_force_ (FluidDomain.f90:1174-1180), host libm sin
+void update_vf(Block &b) {
+  b.VF[0] = b.VFin[0] + b.VFamp * std::sin(2.0 * kPi * b.VFfreq * b.blktime + b.VFphi / 180.0 * kPi);
+  b.VF[1] = b.VFin[1];
+  b.VF[2] = b.VFin[2];
+}
+
+// evaluate_oscillatory_velocity (FluidDomain.f90:1814-1824), host libm cos
+void osc_velocity(const Block &b, double t, double v[3]) {
+  v[0] = b.uvwIn[0] + b.shear[0] * std::cos(2 * kPi * b.shear[1] * t + b.shear[2] / 180.0 * kPi);
+  v[1] = b.uvwIn[1];
+  v[2] = b.uvwIn[2];
+}
+
+void eval_velocity_host(const Block &b, double t, double xc, double yc, double zc, double v[3]) {
+  if (b.vkind == 0) {
+    v[0] = ((b.uvwIn[0] + 0.0 * b.shear[0]) + yc * b.shear[1]) + zc * b.shear[2];
+    v[1] = ((b.uvwIn[1] + xc * b.shear[0]) + 0.0 * b.shear[1]) + zc * b.shear[2];
+    v[2] = ((b.uvwIn[2] + xc * b.shear[0]) + yc * b.shear[1]) + 0.0 * b.shear[2];
+  } else if (b.vkind == 2) {
+    osc_velocity(b, t, v);
+  } else {
+    v[0] = v[1] = v[2] = 0.0;
+  }
+}
+
+bool is_active_face(int code) {
+  return !(code == BC_PERIODIC || code == BC_FLUID || code == BC_FLUID_FATHER);
+}
+
+int launch_faces(Block &b) {
+  for (int face = 0; face < 6; ++face) {
+    const int code = b.bc[face];
+    if (!is_active_face(code)) continue;
+    FaceArgs a{};
+    a.f = b.fA.p;
+    a.xd = b.xd; a.yd = b.yd; a.zd = b.zd; a.N = b.N;
+    a.face = face; a.code = code;
+    a.dh = b.dh; a.xmin = b.xmin; a.ymin = b.ymin; a.zmin = b.zmin;
+    const double lo[3] = {b.xmin, b.ymin, b.zmin}, hi[3] = {b.xmax, b.ymax, b.zmax};
+    const int axis = face >> 1;
+    a.wallc = (face & 1) ? hi[axis] : lo[axis];
+    a.hwc = (face & 1) ? hi[axis] + b.dh * 0.5 : lo[axis] - b.dh * 0.5;
+    a.denIn = b.denIn;
+    for (int k = 0; k < 3; ++k) { a.uvw[k] = b.uvwIn[k]; a.shear[k] = b.shear[k]; }
+    a.vkind = b.vkind;
+    if (b.vkind == 2) osc_velocity(b, b.blktime, a.uvw_osc);
+    if (b.vkind != 0 && b.vkind != 2) a.vkind = 3;  // velocity left at zero
+    a.nb_den = b.nb_den[face].p;
+    a.nb_u = b.nb_u[face].p;
+    a.hw = b.hw_valid[face] ? b.hw[face].p : nullptr;
+    const int np = plane_size(b, face);
+    const dim3 grid((np + 255) / 256), block(256);
+    switch (face) {
+      case 0: hipLaunchKernelGGL(k_face<0>, grid, block, 0, b.stream, a); break;
+      case 1: hipLaunchKernelGGL(k_face<1>, grid, block, 0, b.stream, a); break;
+      case 2: hipLaunchKernelGGL(k_face<2>, grid, block, 0, b.stream, a); break;
+      case 3: hipLaunchKernelGGL(k_face<3>, grid, block, 0, b.stream, a); break;
+      case 4: hipLaunchKernelGGL(k_face<4>, grid, block, 0, b.stream, a); break;
+      case 5: hipLaunchKernelGGL(k_face<5>, grid, block, 0, b.stream, a); break;
+    }
+    HIPCHK(hipGetLastError());
+  }
+  return 0;
+}
+
+dim3 cell_block(const Block &b) {
+  // one block spans whole z-rows where possible so the z-shifted stores of a
+  // row stay inside one workgroup (and one XCD's L2)
+  int tz = b.zd <= 256 ? b.zd : 256;
+  int ty = std::max(1, 256 / tz);
+  ty = std::min(ty, b.yd);
+  return dim3(tz, ty, 1);
+}
+
+template <int MODEL>
+void launch_collide_model(const CollideArgs &a, bool force, bool band, bool aux, dim3 grid,
+                          dim3 blk, hipStream_t s) {
+#define L(F, B, X) \
+  if (force == F && band == B && aux == X) { hipLaunchKernelGGL((k_collide_push<MODEL, F, B, X>), grid, blk, 0, s, a); return; }
+  L(false, false, false) L(true, false, false) L(false, true, false) L(true, true, false)
+  L(false, false, true) L(true, false, true) L(false, true, true) L(true, true, true)
+#undef L
+}
+
+int collide_stream(Block &b) {
+  CollideArgs a{};
+  a.fin = b.fA.p;
+  a.fout = b.fB.p;
+  a.xd = b.xd; a.yd = b.yd; a.zd = b.zd; a.N = b.N;
+  a.Omega = b.Omega; a.Omega2 = b.Omega2; a.dh = b.dh;
+  a.c1 = 1.0 - 0.5 * b.Omega;
+  a.hO = 0.5 * b.Omega; a.cT1 = 0.5 - 0.25 * b.Omega;
+  a.hO2 = 0.5 * b.Omega2; a.cT2 = 0.5 - 0.25 * b.Omega2;
+  for (int k = 0; k < 3; ++k) {
+    a.hv[k] = 0.5 * b.VF[k] * b.dh;
+    a.VF[k] = b.VF[k];
+  }
+  a.Mc = b.Mc.p;
+  a.Mf = b.Mf.p;
+  const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
+  const bool band = b.ibm.active && b.ibm.nband > 0;
+  if (band) {
+    a.bx0 = b.ibm.bx0; a.by0 = b.ibm.by0; a.bz0 = b.ibm.bz0;
+    a.bnx = b.ibm.bnx; a.bny = b.ibm.bny; a.bnz = b.ibm.bnz;
+    a.bmap = b.ibm.dbmap.p;
+    a.band_u = b.ibm.band_u.p;
+    a.band_F = b.ibm.band_F.p;
+    a.nband = b.ibm.nband;
+  }
+  int m102 = 0, mhw = 0;
+  for (int face = 0; face < 6; ++face) {
+    if (b.bc[face] == BC_NEQ_DIRICHLET) m102 |= 1 << face;
+    if (b.bc[face] == BC_WALL_HALFWAY || b.bc[face] == BC_MOVING_WALL_HALFWAY) mhw |= 1 << face;
+    a.nb_den[face] = b.nb_den[face].p;
+    a.nb_u[face] = b.nb_u[face].p;
+    a.hw[face] = b.hw[face].p;
+  }
+  a.aux_face_mask102 = m102;
+  a.aux_face_mask_hw = mhw;
+  const bool aux = (m102 | mhw) != 0;
+  const dim3 blk = cell_block(b);
+  const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
+  if (b.model == 1) launch_collide_model<1>(a, force, band, aux, grid, blk, b.stream);
+  else if (b.model == 2) launch_collide_model<2>(a, force, band, aux, grid, blk, b.stream);
+  else launch_collide_model<3>(a, force, band, aux, grid, blk, b.stream);
+  HIPCHK(hipGetLastError());
+  for (int face = 0; face < 6; ++face)
+    if ((mhw >> face) & 1) b.hw_valid[face] = true;
+  std::swap(b.fA.p, b.fB.p);
+  std::swap(b.fA.n, b.fB.n);
+  return 0;
+}
+
+// FP32-deviation form: all-periodic blocks (the stencil benchmark case)
+int collide_stream32(Block &b) {
+  Collide32Args a{};
+  a.gin = b.gA.p;
+  a.gout = b.gB.p;
+  a.xd = b.xd; a.yd = b.yd; a.zd = b.zd; a.N = b.N;
+  a.Omega = (float)b.Omega;
+  a.c1 = (float)(1.0 - 0.5 * b.Omega);
+  a.dt3 = (float)(3.0 * b.dh);
+  for (int k = 0; k < 3; ++k) {
+    a.hv[k] = (float)(0.5 * b.VF[k] * b.dh);
+    a.VF[k] = (float)b.VF[k];
+  }
+  const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
+  const dim3 blk = cell_block(b);
+  const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
+  if (force) hipLaunchKernelGGL(k_collide_push32<true>, grid, blk, 0, b.stream, a);
+  else hipLaunchKernelGGL(k_collide_push32<false>, grid, blk, 0, b.stream, a);
+  HIPCHK(hipGetLastError());
+  std::swap(b.gA.p, b.gB.p);
+  std::swap(b.gA.n, b.gB.n);
+  return 0;
+}
+
+int macro32(Block &b, double *den, double *uuu) {
+  DevBuf<double> dd, du;
+  HIPCHK(dd.ensure(b.N));
+  HIPCHK(du.ensure(3 * (size_t)b.N));
+  hipLaunchKernelGGL(k_macro32, dim3((unsigned)((b.N + 255) / 256)), dim3(256), 0, b.stream, b.gA.p,
+                     b.N, (float)(0.5 * b.VF[0] * b.dh), (float)(0.5 * b.VF[1] * b.dh),
+                     (float)(0.5 * b.VF[2] * b.dh), dd.p, du.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(den, dd.p, b.N * sizeof(double), hipMemcpyDeviceToHost, b.stream));
+  HIPCHK(hipMemcpyAsync(uuu, du.p, 3 * b.N * sizeof(double), hipMemcpyDeviceToHost, b.stream));
+  HIPCHK(hipStreamSynchronize(b.stream));
+  return 0;
+}
+
+// Peskin 4-point kernel (Solidbody.f90:822-833)
+double peskin(double x) {
+  const double r = std::fabs(x);
+  if (r < 1.0) return (3.0 - 2.0 * r + std::sqrt(1.0 + 4.0 * r * (1.0 - r))) * 0.125;
+  if (r < 2.0) return (5.0 - 2.0 * r - std::sqrt(-7.0 + 4.0 * r * (3.0 - r))) * 0.125;
+  return 0.0;
+}
+
+// trimedindex (Solidbody.f90:834-866): 1-based indices, wrap/mirror by face code
+int trim_index(int i, int nd, int *out, int bclo, int bchi, std::string &why) {
+  for (int k = -1; k <= 2; ++k) {
+    int v = i + k;
+    if (v < 1) {
+      if (bclo == BC_PERIODIC) v += nd;
+      else if ((bclo == BC_SYMMETRIC || bclo == BC_WALL) && v == 0) v = 2;
+      else if (bclo == BC_WALL_HALFWAY && v == 0) v = 1;
+      else { why = "index out of min bound " + std::to_string(v); return 1; }
+    } else if (v > nd) {
+      if (bchi == BC_PERIODIC) v -= nd;
+      else if ((bchi == BC_SYMMETRIC || bchi == BC_WALL) && v == nd + 1) v = nd - 1;
+      else if (bchi == BC_WALL_HALFWAY && v == nd + 1) v = nd;
+      else { why = "index out of max bound " + std::to_string(v); return 1; }
+    }
+    out[k + 1] = v;
+  }
+  return 0;
+}
+
+// UpdateElmtInterp_ (Solidbody.f90:760-821) for every body + band assembly
+int build_stencil_and_band(Block &b, const double *Exyz) {
+  Ibm &I = b.ibm;
+  const int M = I.M;
+  I.Ei.resize(12 * (size_t)M);
+  I.Ew.resize(12 * (size_t)M);
+  const double invdh = 1.0 / b.dh;
+  const double org[3] = {b.xmin, b.ymin, b.zmin};
+  const int nd[3] = {b.xd, b.yd, b.zd};
+  for (int body = 0; body < I.nbodies; ++body) {
+    const int m0 = I.moff[body], m1 = I.moff[body + 1];
+    if (m1 == m0) continue;
+    int i0[3];
+    double x0[3];
+    for (int k = 0; k < 3; ++k) {
+      const int ii = (int)std::floor((Exyz[3 * (size_t)m0 + k] - org[k]) * invdh);
+      x0[k] = org[k] + (double)ii * b.dh;
+      i0[k] = ii + 1;
+    }
+    for (int m = m0; m < m1; ++m) {
+      for (int k = 0; k < 3; ++k) {
+        double off = (Exyz[3 * (size_t)m + k] - x0[k]) * invdh;
+        int idx = (int)std::floor(off);
+        off = off - (double)idx;
+        idx = idx + i0[k];
+        std::string why;
+        if (trim_index(idx, nd[k], &I.Ei[12 * (size_t)m + 4 * k], b.bc[2 * k], b.bc[2 * k + 1], why))
+          return fail("immersed-boundary stencil of marker %d (body %d): %s", m - m0 + 1, body + 1,
+                      why.c_str());
+        for (int j = -1; j <= 2; ++j)
+          I.Ew[12 * (size_t)m + 4 * k + j + 1] = (float)peskin((double)j - off);
+      }
+    }
+  }
+  // entries sorted by (cell, entry id): the reference's serial update order
+  const size_t E = 64 * (size_t)M;
+  I.keyed.resize(E);
+  for (int m = 0; m < M; ++m) {
+    const int *ix = &I.Ei[12 * (size_t)m];
+    for (int x = 0; x < 4; ++x)
+      for (int y = 0; y < 4; ++y)
+        for (int z = 0; z < 4; ++z) {
+          const long long cell =
+              ((long long)(ix[x] - 1) * b.yd + (ix[4 + y] - 1)) * b.zd + (ix[8 + z] - 1);
+          const int e = 64 * m + 16 * x + 4 * y + z;
+          I.keyed[e] = {cell, e};
+        }
+  }
+  std::sort(I.keyed.begin(), I.keyed.end());
+  I.band_cell.clear();
+  I.band_ptr.clear();
+  I.entry.resize(E);
+  I.bidx.resize(E);
+  for (size_t k = 0; k < E; ++k) {
+    if (k == 0 || I.keyed[k].first != I.keyed[k - 1].first) {
+      I.band_ptr.push_back((int)k);
+      I.band_cell.push_back(I.keyed[k].first);
+    }
+    I.entry[k] = I.keyed[k].second;
+    I.bidx[I.keyed[k].second] = (int)I.band_cell.size() - 1;
+  }
+  I.band_ptr.push_back((int)E);
+  I.nband = (int)I.band_cell.size();
+  // bounding box of the band and its local map
+  int lo[3] = {b.xd, b.yd, b.zd}, hi[3] = {-1, -1, -1};
+  for (long long c : I.band_cell) {
+    const int x = (int)(c / ((long long)b.yd * b.zd));
+    const int y = (int)((c / b.zd) % b.yd);
+    const int z = (int)(c % b.zd);
+    lo[0] = std::min(lo[0], x); hi[0] = std::max(hi[0], x);
+    lo[1] = std::min(lo[1], y); hi[1] = std::max(hi[1], y);
+    lo[2] = std::min(lo[2], z); hi[2] = std::max(hi[2], z);
+  }
+  I.bx0 = lo[0]; I.by0 = lo[1]; I.bz0 = lo[2];
+  I.bnx = hi[0] - lo[0] + 1; I.bny = hi[1] - lo[1] + 1; I.bnz = hi[2] - lo[2] + 1;
+  I.bmap.assign((size_t)I.bnx * I.bny * I.bnz, -1);
+  for (int k = 0; k < I.nband; ++k) {
+    const long long c = I.band_cell[k];
+    const int x = (int)(c / ((long long)b.yd * b.zd));
+    const int y = (int)((c / b.zd) % b.yd);
+    const int z = (int)(c % b.zd);
+    I.bmap[((size_t)(x - I.bx0) * I.bny + (y - I.by0)) * I.bnz + (z - I.bz0)] = k;
+  }
+  return 0;
+}
+
+template <typename T>
+int upload(DevBuf<T> &d, const T *h, size_t n, hipStream_t s) {
+  HIPCHK(d.ensure(n));
+  if (n) HIPCHK(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return 0;
+}
+
+}  // namespace
+
+// ===========================================================================
+extern "C" {
+
+const char *fsil_last_error(void) { return g_err.c_str(); }
+
+int fsil_version(void) { return 1; }
+
+int fsil_init(int device) {
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (n <= 0) return fail("no HIP device visible");
+  if (device < 0 || device >= n) return fail("device %d out of range (%d devices)", device, n);
+  HIPCHK(hipSetDevice(device));
+  g_device = device;
+  return 0;
+}
+
+int fsil_finalize(void) {
+  for (auto &kv : g_blocks) {
+    Block &b = *kv.second;
+    if (b.stream) (void)hipStreamSynchronize(b.stream);
+    if (b.ev0) (void)hipEventDestroy(b.ev0);
+    if (b.ev1) (void)hipEventDestroy(b.ev1);
+    if (b.stream) (void)hipStreamDestroy(b.stream);
+  }
+  g_blocks.clear();
+  return 0;
+}
+
+int fsil_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return 0;
+}
+
+int fsil_block_create(int id, const int *dims, const double *geom, const int *bc, int iCollidModel,
+                      const double *relax, const double *m_collid, const double *m_force,
+                      int precision, int *handle) {
+  if (g_device < 0) return fail("fsil_init has not been called");
+  if (!dims || !geom || !bc || !relax || !handle) return fail("null argument to fsil_block_create");
+  if (dims[0] < 4 || dims[1] < 4 || dims[2] < 4) return fail("block dimensions must be >= 4");
+  if (dims[0] > 32767 || dims[1] > 32767 || dims[2] > 32767)
+    return fail("Grid number exceeds 32767");  // FluidDomain.f90:88-91
+  if (!(iCollidModel == 1 || iCollidModel == 2 || iCollidModel == 3))
+    return fail("collision model %d not provided (1 SRT, 2 TRT, 3 MRT)", iCollidModel);
+  if (iCollidModel == 3 && (!m_collid || !m_force)) return fail("MRT needs M_COLLID and M_FORCE");
+  if (!(precision == 0 || precision == 1)) return fail("precision must be 0 (FP64) or 1 (FP32 deviation)");
+  if (precision == 1 && iCollidModel != 1) return fail("FP32-deviation form supports SRT only");
+  if (precision == 1)
+    for (int k = 0; k < 6; ++k)
+      if (bc[k] != BC_PERIODIC) return fail("FP32-deviation form supports periodic blocks only");
+  for (int k = 0; k < 6; ++k) {
+    const int c = bc[k];
+    const bool ok = c == 0 || c == 1 || c == 101 || c == 102 || c == 103 || c == 104 || c == 201 ||
+                    c == 202 || c == 203 || c == 204 || c == 301 || c == 302;
+    if (!ok) return fail("face %d has no such boundary condition %d", k + 1, c);
+  }
+  for (int a = 0; a < 3; ++a) {
+    const bool p0 = bc[2 * a] == BC_PERIODIC, p1 = bc[2 * a + 1] == BC_PERIODIC;
+    if (p0 != p1) return fail("Periodic boundaries must appear in pairs: %d %d", bc[2 * a], bc[2 * a + 1]);
+  }
+  auto bp = std::make_unique<Block>();
+  Block &b = *bp;
+  b.id = id;
+  b.xd = dims[0]; b.yd = dims[1]; b.zd = dims[2];
+  b.N = (long long)b.xd * b.yd * b.zd;
+  b.dh = geom[0]; b.xmin = geom[1]; b.ymin = geom[2]; b.zmin = geom[3];
+  b.xmax = geom[4]; b.ymax = geom[5]; b.zmax = geom[6];
+  for (int k = 0; k < 6; ++k) b.bc[k] = bc[k];
+  b.model = iCollidModel;
+  b.tau = relax[0]; b.Omega = relax[1]; b.Omega2 = relax[2];
+  b.precision = precision;
+  HIPCHK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&b.ev0));
+  HIPCHK(hipEventCreate(&b.ev1));
+  const size_t nf = (size_t)Q * b.N;
+  if (precision == 0) {
+    HIPCHK(b.fA.ensure(nf));
+    HIPCHK(b.fB.ensure(nf));
+  } else {
+    HIPCHK(b.gA.ensure(nf));
+    HIPCHK(b.gB.ensure(nf));
+  }
+  if (iCollidModel == 3) {
+    HIPCHK(b.Mc.ensure(Q * Q));
+    HIPCHK(b.Mf.ensure(Q * Q));
+    HIPCHK(hipMemcpy(b.Mc.p, m_collid, Q * Q * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b.Mf.p, m_force, Q * Q * sizeof(double), hipMemcpyHostToDevice));
+  }
+  for (int face = 0; face < 6; ++face) {
+    const int np = plane_size(b, face);
+    if (bc[face] == BC_NEQ_DIRICHLET) {
+      HIPCHK(b.nb_den[face].ensure(np));
+      HIPCHK(b.nb_u[face].ensure(3 * (size_t)np));
+    }
+    if (bc[face] == BC_WALL_HALFWAY || bc[face] == BC_MOVING_WALL_HALFWAY)
+      HIPCHK(b.hw[face].ensure((size_t)Q * np));
+  }
+  const int h = g_next_handle++;
+  g_blocks[h] = std::move(bp);
+  *handle = h;
+  return 0;
+}
+
+int fsil_block_destroy(int h) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  (void)hipStreamSynchronize(b->stream);
+  (void)hipEventDestroy(b->ev0);
+  (void)hipEventDestroy(b->ev1);
+  (void)hipStreamDestroy(b->stream);
+  g_blocks.erase(h);
+  return 0;
+}
+
+int fsil_set_flow(int h, double denIn, const double *uvwIn, const double *shearRateIn,
+                  int velocityKind, const double *volumeForceIn, double volumeForceAmp,
+                  double volumeForceFreq, double volumeForcePhi) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!uvwIn || !shearRateIn || !volumeForceIn) return fail("null argument to fsil_set_flow");
+  b->denIn = denIn;
+  for (int k = 0; k < 3; ++k) {
+    b->uvwIn[k] = uvwIn[k];
+    b->shear[k] = shearRateIn[k];
+    b->VFin[k] = volumeForceIn[k];
+  }
+  b->vkind = velocityKind;
+  b->VFamp = volumeForceAmp;
+  b->VFfreq = volumeForceFreq;
+  b->VFphi = volumeForcePhi;
+  update_vf(*b);
+  return 0;
+}
+
+int fsil_set_time(int h, double blktime) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  b->blktime = blktime;
+  update_vf(*b);
+  return 0;
+}
+
+int fsil_get_volume_force(int h, double *vf) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  for (int k = 0; k < 3; ++k) vf[k] = b->VF[k];
+  return 0;
+}
+
+int fsil_f_upload(int h, const double *fIn) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!fIn) return fail("null fIn");
+  const size_t nf = (size_t)Q * b->N;
+  if (b->precision == 0) {
+    HIPCHK(hipMemcpyAsync(b->fA.p, fIn, nf * sizeof(double), hipMemcpyHostToDevice, b->stream));
+  } else {
+    // FP32 deviation storage: g = f - w, rounded once
+    std::vector<float> g(nf);
+    for (int q = 0; q < Q; ++q)
+      for (long long n = 0; n < b->N; ++n) g[q * b->N + n] = (float)(fIn[q * b->N + n] - wt(q));
+    HIPCHK(hipMemcpyAsync(b->gA.p, g.data(), nf * sizeof(float), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int fsil_f_download(int h, double *fIn) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!fIn) return fail("null fIn");
+  const size_t nf = (size_t)Q * b->N;
+  if (b->precision == 0) {
+    HIPCHK(hipMemcpyAsync(fIn, b->fA.p, nf * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  } else {
+    std::vector<float> g(nf);
+    HIPCHK(hipMemcpyAsync(g.data(), b->gA.p, nf * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (int q = 0; q < Q; ++q)
+      for (long long n = 0; n < b->N; ++n) fIn[q * b->N + n] = (double)g[q * b->N + n] + wt(q);
+  }
+  return 0;
+}
+
+int fsil_set_boundary_conditions(int h) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (b->precision != 0) return 0;  // FP32 form: periodic blocks only (checked at creation)
+  // Code 102 reads den/uuu of the first interior plane.  At start-up the
+  // reference calls this right after initialise_, where den = denIn and uuu is
+  // the inflow profile at every node (FluidDomain.f90:533-545), so those fields
+  // are what the planes hold here.
+  for (int face = 0; face < 6; ++face) {
+    if (b->bc[face] != BC_NEQ_DIRICHLET) continue;
+    const int np = plane_size(*b, face);
+    std::vector<double> den(np), uu(3 * (size_t)np);
+    const int axis = face >> 1, hi = face & 1;
+    const int nda = axis == 0 ? b->xd : (axis == 1 ? b->yd : b->zd);
+    const int p2 = hi ? nda - 2 : 1;
+    const int n0 = axis == 0 ? b->yd : b->xd, n1 = axis == 2 ? b->yd : b->zd;
+    for (int A = 0; A < n0; ++A)
+      for (int B = 0; B < n1; ++B) {
+        int c[3];
+        c[axis] = p2;
+        c[axis == 0 ? 1 : 0] = A;
+        c[axis == 2 ? 1 : 2] = B;
+        double v[3];
+        eval_velocity_host(*b, 0.0, b->xmin + b->dh * c[0], b->ymin + b->dh * c[1],
+                           b->zmin + b->dh * c[2], v);
+        const int t = A * n1 + B;
+        den[t] = b->denIn;
+        uu[t] = v[0]; uu[np + t] = v[1]; uu[2 * (size_t)np + t] = v[2];
+      }
+    HIPCHK(hipMemcpyAsync(b->nb_den[face].p, den.data(), np * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync(b->nb_u[face].p, uu.data(), 3 * np * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  if (launch_faces(*b)) return 1;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int fsil_macro_download(int h, double *den, double *uuu) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!den || !uuu) return fail("null den/uuu");
+  if (b->precision != 0) return macro32(*b, den, uuu);
+  DevBuf<double> dd, du;
+  HIPCHK(dd.ensure(b->N));
+  HIPCHK(du.ensure(3 * (size_t)b->N));
+  const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
+  hipLaunchKernelGGL(k_macro, dim3((unsigned)((b->N + 255) / 256)), dim3(256), 0, b->stream, b->fA.p,
+                     b->N, hv0, hv1, hv2, dd.p, du.p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(den, dd.p, b->N * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipMemcpyAsync(uuu, du.p, 3 * b->N * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int fsil_collide_stream_bc(int h) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (b->precision == 0) {
+    if (collide_stream(*b)) return 1;
+    if (launch_faces(*b)) return 1;
+  } else {
+    if (collide_stream32(*b)) return 1;
+  }
+  b->ibm.active = false;
+  return 0;
+}
+
+int fsil_steps(int h, int nsteps, double t0, double dt) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  for (int s = 0; s < nsteps; ++s) {
+    b->blktime = t0 + dt * (double)(s + 1);
+    update_vf(*b);
+    if (fsil_collide_stream_bc(h)) return 1;
+  }
+  return 0;
+}
+
+int fsil_time_collide(int h, int nlaunch, double *avg_ms) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (nlaunch <= 0 || !avg_ms) return fail("bad arguments to fsil_time_collide");
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  for (int s = 0; s < nlaunch; ++s) {
+    if (b->precision == 0) {
+      if (collide_stream(*b)) return 1;
+    } else {
+      if (collide_stream32(*b)) return 1;
+    }
+  }
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  *avg_ms = (double)ms / nlaunch;
+  return 0;
+}
+
+int fsil_time_steps(int h, int nsteps, double t0, double dt, double *total_ms) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (nsteps <= 0 || !total_ms) return fail("bad arguments to fsil_time_steps");
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  if (fsil_steps(h, nsteps, t0, dt)) return 1;
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  *total_ms = ms;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// immersed boundary
+int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *nnodes,
+                           const int *nelems, const int *marker_elem, const int *elem_nodes) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (b->precision != 0) return fail("immersed boundary coupling runs on the FP64 form");
+  if (nbodies < 0) return fail("negative body count");
+  Ibm &I = b->ibm;
+  I = Ibm();
+  I.nbodies = nbodies;
+  I.nmark.assign(nmarkers, nmarkers + nbodies);
+  I.nnode.assign(nnodes, nnodes + nbodies);
+  I.nelem.assign(nelems, nelems + nbodies);
+  I.moff.assign(nbodies + 1, 0);
+  I.noff.assign(nbodies + 1, 0);
+  I.eoff.assign(nbodies + 1, 0);
+  for (int k = 0; k < nbodies; ++k) {
+    I.moff[k + 1] = I.moff[k] + I.nmark[k];
+    I.noff[k + 1] = I.noff[k] + I.nnode[k];
+    I.eoff[k + 1] = I.eoff[k] + I.nelem[k];
+  }
+  I.M = I.moff[nbodies];
+  I.nnodes = I.noff[nbodies];
+  I.nelems = I.eoff[nbodies];
+  if ((long long)I.M * 64 > (1LL << 30)) return fail("too many markers");
+  // global element id per marker; DOF lists (lodFlow) in marker order
+  I.marker_elem.resize(I.M);
+  I.ndof = 6 * I.nnodes;
+  std::vector<std::vector<int>> lists(I.ndof);
+  for (int body = 0; body < nbodies; ++body)
+    for (int m = I.moff[body]; m < I.moff[body + 1]; ++m) {
+      const int el = marker_elem[m];
+      if (el < 0 || el >= I.nelem[body]) return fail("marker %d has element %d out of range", m, el);
+      const int ge = I.eoff[body] + el;
+      I.marker_elem[m] = ge;
+      const int n0 = elem_nodes[2 * ge], n1 = elem_nodes[2 * ge + 1];
+      if (n0 < 0 || n0 >= I.nnode[body] || n1 < 0 || n1 >= I.nnode[body])
+        return fail("element %d has a node out of range", el);
+      const int g0 = 6 * (I.noff[body] + n0), g1 = 6 * (I.noff[body] + n1);
+      for (int k = 0; k < 3; ++k) lists[g0 + k].push_back(m);
+      for (int k = 0; k < 3; ++k) lists[g1 + k].push_back(m);
+      for (int k = 0; k < 3; ++k) lists[g0 + 3 + k].push_back(m);
+      for (int k = 0; k < 3; ++k) lists[g1 + 3 + k].push_back(m);
+    }
+  std::vector<int> ptr(I.ndof + 1, 0), lst, kind(I.ndof);
+  for (int d = 0; d < I.ndof; ++d) {
+    ptr[d + 1] = ptr[d] + (int)lists[d].size();
+    lst.insert(lst.end(), lists[d].begin(), lists[d].end());
+    kind[d] = d % 6;
+  }
+  if (upload(I.dof_ptr, ptr.data(), ptr.size(), b->stream)) return 1;
+  if (upload(I.dof_list, lst.data(), lst.size(), b->stream)) return 1;
+  if (upload(I.dof_kind, kind.data(), kind.size(), b->stream)) return 1;
+  if (upload(I.dmarker_elem, I.marker_elem.data(), I.marker_elem.size(), b->stream)) return 1;
+  HIPCHK(I.Exyz.ensure(3 * (size_t)I.M));
+  HIPCHK(I.Evel.ensure(3 * (size_t)I.M));
+  HIPCHK(I.Ea.ensure(I.M));
+  HIPCHK(I.Eforce.ensure(3 * (size_t)I.M));
+  HIPCHK(I.fT.ensure(3 * (size_t)I.M));
+  HIPCHK(I.lod.ensure(I.ndof));
+  HIPCHK(I.elem_xc.ensure(3 * (size_t)std::max(1, I.nelems)));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int fsil_ibm_set_params(int h, int ntolLBM, double dtolLBM, double Uref, double denIn) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  b->ibm.ntol = ntolLBM;
+  b->ibm.dtol = dtolLBM;
+  b->ibm.Uref = Uref;
+  b->ibm.denIn = denIn;
+  return 0;
+}
+
+int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const double *Evel,
+                               const double *Ea, const double *elem_xc, double *Eforce,
+                               double *lodFlow, int *iters, double *dmax) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Ibm &I = b->ibm;
+  I.active = false;
+  if (I.nbodies == 0 || I.M == 0) {
+    if (iters) *iters = 0;
+    if (dmax) *dmax = 0.0;
+    return 0;
+  }
+  if (!Exyz || !Evel || !Ea || !elem_xc) return fail("null marker arrays");
+  if (build_stencil_and_band(*b, Exyz)) return 1;
+  hipStream_t s = b->stream;
+  const int M = I.M;
+  if (upload(I.Exyz, Exyz, 3 * (size_t)M, s)) return 1;
+  if (upload(I.Evel, Evel, 3 * (size_t)M, s)) return 1;
+  if (upload(I.Ea, Ea, (size_t)M, s)) return 1;
+  if (upload(I.elem_xc, elem_xc, 3 * (size_t)I.nelems, s)) return 1;
+  if (upload(I.dEw, I.Ew.data(), I.Ew.size(), s)) return 1;
+  if (upload(I.dbidx, I.bidx.data(), I.bidx.size(), s)) return 1;
+  if (upload(I.dcell, I.band_cell.data(), I.band_cell.size(), s)) return 1;
+  if (upload(I.dptr, I.band_ptr.data(), I.band_ptr.size(), s)) return 1;
+  if (upload(I.dentry, I.entry.data(), I.entry.size(), s)) return 1;
+  if (upload(I.dbmap, I.bmap.data(), I.bmap.size(), s)) return 1;
+  HIPCHK(I.band_u.ensure(3 * (size_t)I.nband));
+  HIPCHK(I.band_F.ensure(3 * (size_t)I.nband));
+  const int nblkM = (M + 255) / 256;
+  HIPCHK(I.tolpart.ensure(nblkM));
+  I.h_tolpart.resize(nblkM);
+  const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
+  hipLaunchKernelGGL(k_band_macro, dim3((I.nband + 255) / 256), dim3(256), 0, s, b->fA.p, b->N,
+                     I.dcell.p, I.nband, hv0, hv1, hv2, I.band_u.p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_zero, dim3((3 * M + 255) / 256), dim3(256), 0, s, I.Eforce.p, 3LL * M);
+  HIPCHK(hipGetLastError());
+  // MDF loop (Solidbody.f90:893-906)
+  const double invh3p = 0.5 * dt * ((1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh)) / I.denIn;
+  int it = 0;
+  double dm = 1e10;
+  double dsum = 0.0;
+  for (int body = 0; body < I.nbodies; ++body) dsum = dsum + (double)I.nmark[body];
+  while (it < I.ntol && dm > I.dtol) {
+    hipLaunchKernelGGL(k_interp, dim3(nblkM), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p,
+                       I.nband, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(k_correct, dim3((I.nband + 255) / 256), dim3(256), 0, s, I.nband, I.dptr.p,
+                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(I.h_tolpart.data(), I.tolpart.p, nblkM * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    double tol = 0.0;
+    for (int k = 0; k < nblkM; ++k) tol = tol + I.h_tolpart[k];
+    if (!std::isfinite(tol)) return fail("Nan found in PenaltyForce");
+    dm = tol / (dsum * I.Uref);
+    ++it;
+  }
+  const double invh3s = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
+  hipLaunchKernelGGL(k_spread, dim3((I.nband + 255) / 256), dim3(256), 0, s, I.nband, I.dptr.p,
+                     I.dentry.p, I.dEw.p, I.Eforce.p, invh3s, b->VF[0], b->VF[1], b->VF[2], I.band_F.p);
+  HIPCHK(hipGetLastError());
+  if (I.ndof > 0) {
+    hipLaunchKernelGGL(k_loads, dim3((I.ndof + 255) / 256), dim3(256), 0, s, I.ndof, I.dof_ptr.p,
+                       I.dof_list.p, I.dof_kind.p, I.dmarker_elem.p, I.elem_xc.p, I.Exyz.p,
+                       I.Eforce.p, I.lod.p);
+    HIPCHK(hipGetLastError());
+  }
+  if (Eforce) HIPCHK(hipMemcpyAsync(Eforce, I.Eforce.p, 3 * (size_t)M * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (lodFlow && I.ndof > 0) HIPCHK(hipMemcpyAsync(lodFlow, I.lod.p, (size_t)I.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  I.active = true;
+  if (iters) *iters = it;
+  if (dmax) *dmax = dm;
+  return 0;
+}
+
+int fsil_ibm_band_info(int h, int *nband, int *bbox) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (nband) *nband = b->ibm.nband;
+  if (bbox) {
+    bbox[0] = b->ibm.bx0; bbox[1] = b->ibm.by0; bbox[2] = b->ibm.bz0;
+    bbox[3] = b->ibm.bnx; bbox[4] = b->ibm.bny; bbox[5] = b->ibm.bnz;
+  }
+  return 0;
+}
+
+}  // extern "C"

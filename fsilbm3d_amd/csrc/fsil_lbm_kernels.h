@@ -1,0 +1,472 @@
+// D3Q19 fluid kernels for gfx950 (FP64 plain form).
+//
+// Exactness: this file is compiled with -ffp-contract=off, so no multiply-add
+// is fused; every expression below is written in the association order of the
+// reference's Fortran (gaoak/FSILBM3D src/FluidDomain.f90).  Terms whose lattice
+// component e is 0 are skipped: in IEEE arithmetic x + (+-0) == x for every
+// nonzero x, and a running sum that starts at +0 never becomes -0, so skipping
+// them changes at most the sign of an intermediate zero, never a stored value.
+#pragma once
+#include "fsil_common.h"
+
+namespace fsil {
+
+// ---------------------------------------------------------------------------
+// per-direction helpers, fully unrolled at compile time
+template <int E>
+__device__ __forceinline__ double sgn(double v) {
+  return E > 0 ? v : -v;
+}
+
+// uxyz(q) = u1*e1 + u2*e2 + u3*e3   (FluidDomain.f90:1219)
+template <int q>
+__device__ __forceinline__ double dotu(double u1, double u2, double u3) {
+  constexpr int a = ex(q), b = ey(q), c = ez(q);
+  if constexpr (a == 0 && b == 0 && c == 0) return 0.0;
+  else if constexpr (a != 0 && b != 0) return sgn<a>(u1) + sgn<b>(u2);
+  else if constexpr (a != 0 && c != 0) return sgn<a>(u1) + sgn<c>(u3);
+  else if constexpr (b != 0 && c != 0) return sgn<b>(u2) + sgn<c>(u3);
+  else if constexpr (a != 0) return sgn<a>(u1);
+  else if constexpr (b != 0) return sgn<b>(u2);
+  else return sgn<c>(u3);
+}
+
+// (e_i - u_i + 3*uxyz*e_i)   (FluidDomain.f90:1222-1224)
+template <int E>
+__device__ __forceinline__ double guo_a(double ui, double t3) {
+  if constexpr (E == 0) return -ui;
+  else if constexpr (E > 0) return (1.0 - ui) + t3;
+  else return (-1.0 - ui) - t3;
+}
+
+// Flb(q) = dt3*wt*(A1*F1 + A2*F2 + A3*F3)
+template <int q>
+__device__ __forceinline__ double guo_force(double dt3, double ux, double u1, double u2,
+                                            double u3, double F1, double F2, double F3) {
+  const double t3 = 3.0 * ux;
+  const double s = (guo_a<ex(q)>(u1, t3) * F1 + guo_a<ey(q)>(u2, t3) * F2) +
+                   guo_a<ez(q)>(u3, t3) * F3;
+  return (dt3 * wt(q)) * s;
+}
+
+// Sequential moments of one cell (FluidDomain.f90:1136-1139).
+__device__ __forceinline__ void moments(const double (&f)[Q], double &rho, double &m1,
+                                        double &m2, double &m3) {
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) s = s + f[q];
+  rho = s;
+  // e_x != 0: 1,2,7,8,9,10,11,12,13,14 ; e_y: 3,4,7,8,9,10,15,16,17,18 ;
+  // e_z: 5,6,11,12,13,14,15,16,17,18 (q ascending)
+  m1 = 0.0;
+  m1 = m1 + f[1]; m1 = m1 - f[2]; m1 = m1 + f[7]; m1 = m1 - f[8]; m1 = m1 + f[9];
+  m1 = m1 - f[10]; m1 = m1 + f[11]; m1 = m1 - f[12]; m1 = m1 + f[13]; m1 = m1 - f[14];
+  m2 = 0.0;
+  m2 = m2 + f[3]; m2 = m2 - f[4]; m2 = m2 + f[7]; m2 = m2 + f[8]; m2 = m2 - f[9];
+  m2 = m2 - f[10]; m2 = m2 + f[15]; m2 = m2 - f[16]; m2 = m2 + f[17]; m2 = m2 - f[18];
+  m3 = 0.0;
+  m3 = m3 + f[5]; m3 = m3 - f[6]; m3 = m3 + f[11]; m3 = m3 + f[12]; m3 = m3 - f[13];
+  m3 = m3 - f[14]; m3 = m3 + f[15]; m3 = m3 + f[16]; m3 = m3 - f[17]; m3 = m3 - f[18];
+}
+
+// ---------------------------------------------------------------------------
+// Arguments of the fused collide + push-stream kernel.
+struct CollideArgs {
+  const double *__restrict__ fin;  // pre-collision state, SoA [q][x][y][z]
+  double *__restrict__ fout;       // post-stream state (before boundary faces)
+  int xd, yd, zd;
+  long long N;
+  double Omega, Omega2, dh;
+  double c1;        // 1 - 0.5*Omega
+  double hO, cT1;   // TRT: 0.5*Omega, 0.5-0.25*Omega
+  double hO2, cT2;  // TRT: 0.5*Omega2, 0.5-0.25*Omega2
+  double hv[3];     // 0.5*VF*dh   (numerator shift of the velocity moment)
+  double VF[3];     // uniform volume force
+  const double *__restrict__ Mc;  // MRT M_COLLID, Fortran column-major 19x19
+  const double *__restrict__ Mf;  // MRT M_FORCE
+  // immersed-boundary band overlay (cells whose velocity/force come from IBM)
+  int bx0, by0, bz0, bnx, bny, bnz;
+  const int *__restrict__ bmap;       // bbox-local map -> band index, -1 outside band
+  const double *__restrict__ band_u;  // [3][nband] corrected velocity
+  const double *__restrict__ band_F;  // [3][nband] spread force (+VF)
+  int nband;
+  // auxiliary plane outputs for boundary codes 102 (den/u of the first
+  // interior plane) and 203/204 (post-collision boundary plane)
+  int aux_face_mask102, aux_face_mask_hw;
+  double *nb_den[6];
+  double *nb_u[6];   // [3][plane]
+  double *hw[6];     // [q][plane]
+};
+
+// plane index of a cell on face `face` in the reference's (outer, inner) loop order
+__device__ __forceinline__ int face_plane_index(int face, int x, int y, int z, int yd, int zd) {
+  const int axis = face >> 1;
+  if (axis == 0) return y * zd + z;
+  if (axis == 1) return x * zd + z;
+  return x * yd + y;
+}
+
+template <int MODEL, bool FORCE, bool BAND, bool AUX>
+__global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int x = blockIdx.z;
+  if (z >= a.zd || y >= a.yd) return;
+  const long long N = a.N;
+  const long long n = ((long long)x * a.yd + y) * a.zd + z;
+
+  double f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = __builtin_nontemporal_load(a.fin + q * N + n);
+
+  double rho, m1, m2, m3;
+  moments(f, rho, m1, m2, m3);
+  double u1 = (m1 + a.hv[0]) / rho;
+  double u2 = (m2 + a.hv[1]) / rho;
+  double u3 = (m3 + a.hv[2]) / rho;
+  bool forced = FORCE;
+  double F1 = a.VF[0], F2 = a.VF[1], F3 = a.VF[2];
+  if constexpr (BAND) {
+    const int lx = x - a.bx0, ly = y - a.by0, lz = z - a.bz0;
+    if ((unsigned)lx < (unsigned)a.bnx && (unsigned)ly < (unsigned)a.bny &&
+        (unsigned)lz < (unsigned)a.bnz) {
+      const int b = a.bmap[((long long)lx * a.bny + ly) * a.bnz + lz];
+      if (b >= 0) {
+        u1 = a.band_u[b];
+        u2 = a.band_u[a.nband + b];
+        u3 = a.band_u[2 * a.nband + b];
+        F1 = a.band_F[b];
+        F2 = a.band_F[a.nband + b];
+        F3 = a.band_F[2 * a.nband + b];
+        forced = true;
+      }
+    }
+  }
+  if constexpr (AUX) {
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      if (!((a.aux_face_mask102 >> face) & 1)) continue;
+      const int axis = face >> 1, hi = face & 1;
+      const int c = axis == 0 ? x : (axis == 1 ? y : z);
+      const int nd = axis == 0 ? a.xd : (axis == 1 ? a.yd : a.zd);
+      if (c == (hi ? nd - 2 : 1)) {
+        const int p = face_plane_index(face, x, y, z, a.yd, a.zd);
+        const int np = axis == 0 ? a.yd * a.zd : (axis == 1 ? a.xd * a.zd : a.xd * a.yd);
+        a.nb_den[face][p] = rho;
+        a.nb_u[face][p] = u1;
+        a.nb_u[face][np + p] = u2;
+        a.nb_u[face][2 * np + p] = u3;
+      }
+    }
+  }
+
+  // equilibrium difference fEq - f  (FluidDomain.f90:1217-1220)
+  double uSqr = u1 * u1;
+  uSqr = uSqr + u2 * u2;
+  uSqr = uSqr + u3 * u3;
+  const double base = 1.0 - 1.5 * uSqr;
+  const double dt3 = 3.0 * a.dh;
+  double post[Q];
+#define FSIL_FEQ(q)                                                       \
+  const double ux##q = dotu<q>(u1, u2, u3);                               \
+  const double fe##q = (wt(q) * rho) * (base + ux##q * (3.0 + 4.5 * ux##q)) - f[q]; \
+  const double fl##q = forced ? guo_force<q>(dt3, ux##q, u1, u2, u3, F1, F2, F3) : 0.0;
+  FSIL_FEQ(0) FSIL_FEQ(1) FSIL_FEQ(2) FSIL_FEQ(3) FSIL_FEQ(4) FSIL_FEQ(5) FSIL_FEQ(6)
+  FSIL_FEQ(7) FSIL_FEQ(8) FSIL_FEQ(9) FSIL_FEQ(10) FSIL_FEQ(11) FSIL_FEQ(12) FSIL_FEQ(13)
+  FSIL_FEQ(14) FSIL_FEQ(15) FSIL_FEQ(16) FSIL_FEQ(17) FSIL_FEQ(18)
+#undef FSIL_FEQ
+  const double fe[Q] = {fe0, fe1, fe2, fe3, fe4, fe5, fe6, fe7, fe8, fe9,
+                        fe10, fe11, fe12, fe13, fe14, fe15, fe16, fe17, fe18};
+  const double fl[Q] = {fl0, fl1, fl2, fl3, fl4, fl5, fl6, fl7, fl8, fl9,
+                        fl10, fl11, fl12, fl13, fl14, fl15, fl16, fl17, fl18};
+  if constexpr (MODEL == 1) {
+    // SRT  (FluidDomain.f90:1227)
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double t = f[q] + a.Omega * fe[q];
+      post[q] = forced ? t + a.c1 * fl[q] : t;
+    }
+  } else if constexpr (MODEL == 2) {
+    // TRT  (FluidDomain.f90:1230-1235)
+    constexpr int P[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+    constexpr int M[9] = {2, 4, 6, 10, 9, 14, 13, 18, 17};
+    {
+      const double t = a.Omega * fe[0];
+      post[0] = f[0] + (forced ? t + a.c1 * fl[0] : t);
+    }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int p = P[k], m = M[k];
+      double tp = a.hO * (fe[p] + fe[m]);
+      double tm = a.hO2 * (fe[p] - fe[m]);
+      if (forced) {
+        tp = tp + a.cT1 * (fl[p] + fl[m]);
+        tm = tm + a.cT2 * (fl[p] - fl[m]);
+      }
+      post[p] = f[p] + (tp + tm);
+      post[m] = f[m] + (tp - tm);
+    }
+  } else {
+    // MRT  (FluidDomain.f90:1238): f + MATMUL(M_COLLID,fEq) + MATMUL(M_FORCE,Flb)
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+      double s = 0.0;
+      for (int j = 0; j < Q; ++j) s = s + a.Mc[j * Q + i] * fe[j];
+      double t = f[i] + s;
+      if (forced) {
+        double r = 0.0;
+        for (int j = 0; j < Q; ++j) r = r + a.Mf[j * Q + i] * fl[j];
+        t = t + r;
+      }
+      post[i] = t;
+    }
+  }
+
+  if constexpr (AUX) {
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      if (!((a.aux_face_mask_hw >> face) & 1)) continue;
+      const int axis = face >> 1, hi = face & 1;
+      const int c = axis == 0 ? x : (axis == 1 ? y : z);
+      const int nd = axis == 0 ? a.xd : (axis == 1 ? a.yd : a.zd);
+      if (c == (hi ? nd - 1 : 0)) {
+        const int p = face_plane_index(face, x, y, z, a.yd, a.zd);
+        const int np = axis == 0 ? a.yd * a.zd : (axis == 1 ? a.xd * a.zd : a.xd * a.yd);
+        for (int q = 0; q < Q; ++q) a.hw[face][(long long)q * np + p] = post[q];
+      }
+    }
+  }
+
+  // push-stream with periodic wrap in every direction (FluidDomain.f90:1514-1625)
+  const int xp = x + 1 == a.xd ? 0 : x + 1, xm = x == 0 ? a.xd - 1 : x - 1;
+  const int yp = y + 1 == a.yd ? 0 : y + 1, ym = y == 0 ? a.yd - 1 : y - 1;
+  const int zp = z + 1 == a.zd ? 0 : z + 1, zm = z == 0 ? a.zd - 1 : z - 1;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int X = ex(q) > 0 ? xp : (ex(q) < 0 ? xm : x);
+    const int Y = ey(q) > 0 ? yp : (ey(q) < 0 ? ym : y);
+    const int Z = ez(q) > 0 ? zp : (ez(q) < 0 ? zm : z);
+    const long long d = ((long long)X * a.yd + Y) * a.zd + Z;
+    __builtin_nontemporal_store(post[q], a.fout + q * N + d);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// macroscopic fields of the stored state (FluidDomain.f90:1128-1145)
+__global__ __launch_bounds__(256) void k_macro(const double *__restrict__ f, long long N,
+                                               double hv0, double hv1, double hv2,
+                                               double *__restrict__ den, double *__restrict__ u) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  double g[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) g[q] = f[q * N + n];
+  double rho, m1, m2, m3;
+  moments(g, rho, m1, m2, m3);
+  den[n] = rho;
+  u[n] = (m1 + hv0) / rho;
+  u[N + n] = (m2 + hv1) / rho;
+  u[2 * N + n] = (m3 + hv2) / rho;
+}
+
+// ---------------------------------------------------------------------------
+// Boundary faces (FluidDomain.f90:616-1126), one launch per face in the order
+// x-, x+, y-, y+, z-, z+ so that edges and corners see earlier faces' values.
+struct FaceArgs {
+  double *__restrict__ f;
+  int xd, yd, zd;
+  long long N;
+  int face, code;
+  double dh, xmin, ymin, zmin, wallc, hwc;
+  double denIn, uvw[3], shear[3];
+  int vkind;        // 0 shear profile, 2 oscillatory (velocity given in uvw_osc)
+  double uvw_osc[3];
+  const double *__restrict__ nb_den;  // 102: den of first interior plane
+  const double *__restrict__ nb_u;    // 102: u of first interior plane
+  const double *__restrict__ hw;      // 203/204: saved post-collision plane (nullptr: skip)
+};
+
+// evaluate_velocity (FluidDomain.f90:1791-1824) at node (xc,yc,zc)
+__device__ __forceinline__ void eval_velocity(const FaceArgs &a, double xc, double yc,
+                                              double zc, double v[3]) {
+  if (a.vkind == 0) {
+    v[0] = ((a.uvw[0] + 0.0 * a.shear[0]) + yc * a.shear[1]) + zc * a.shear[2];
+    v[1] = ((a.uvw[1] + xc * a.shear[0]) + 0.0 * a.shear[1]) + zc * a.shear[2];
+    v[2] = ((a.uvw[2] + xc * a.shear[0]) + yc * a.shear[1]) + 0.0 * a.shear[2];
+  } else {
+    v[0] = a.uvw_osc[0];
+    v[1] = a.uvw_osc[1];
+    v[2] = a.uvw_osc[2];
+  }
+}
+
+// calculate_distribution_funcion (FluidDomain.f90:1827-1834), one direction
+template <int q>
+__device__ __forceinline__ double feq_bc(double rho, double v1, double v2, double v3,
+                                         double uSqr) {
+  const double ux = dotu<q>(v1, v2, v3);
+  return (wt(q) * rho) * (((1.0 + 3.0 * ux) + (4.5 * ux) * ux) - 1.5 * uSqr);
+}
+
+template <int q>
+__device__ __forceinline__ double mw_term(double rho, double v1, double v2, double v3) {
+  // 2.0*wt*density*uxyz*3.0  (FluidDomain.f90:1842)
+  const double ux = dotu<q>(v1, v2, v3);
+  return (((2.0 * wt(q)) * rho) * ux) * 3.0;
+}
+
+template <int FACE>
+struct FaceSets;
+template <> struct FaceSets<0> { static constexpr int in[5] = {1, 7, 9, 11, 13}; static constexpr int sym[5] = {2, 8, 10, 12, 14}; };
+template <> struct FaceSets<1> { static constexpr int in[5] = {2, 8, 10, 12, 14}; static constexpr int sym[5] = {1, 7, 9, 11, 13}; };
+template <> struct FaceSets<2> { static constexpr int in[5] = {3, 7, 8, 15, 17}; static constexpr int sym[5] = {4, 9, 10, 16, 18}; };
+template <> struct FaceSets<3> { static constexpr int in[5] = {4, 9, 10, 16, 18}; static constexpr int sym[5] = {3, 7, 8, 15, 17}; };
+template <> struct FaceSets<4> { static constexpr int in[5] = {5, 11, 12, 15, 16}; static constexpr int sym[5] = {6, 13, 14, 17, 18}; };
+template <> struct FaceSets<5> { static constexpr int in[5] = {6, 13, 14, 17, 18}; static constexpr int sym[5] = {5, 11, 12, 15, 16}; };
+
+template <int q>
+__device__ __forceinline__ double feq_dir(double rho, double v1, double v2, double v3, double uSqr) {
+  return feq_bc<q>(rho, v1, v2, v3, uSqr);
+}
+
+// runtime-q dispatch over the compile-time feq helpers
+__device__ __forceinline__ double feq_q(int q, double rho, double v1, double v2, double v3,
+                                        double uSqr) {
+  switch (q) {
+#define C(k) case k: return feq_bc<k>(rho, v1, v2, v3, uSqr);
+    C(0) C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16) C(17) C(18)
+#undef C
+  }
+  return 0.0;
+}
+__device__ __forceinline__ double mw_q(int q, double rho, double v1, double v2, double v3) {
+  switch (q) {
+#define C(k) case k: return mw_term<k>(rho, v1, v2, v3);
+    C(0) C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16) C(17) C(18)
+#undef C
+  }
+  return 0.0;
+}
+
+template <int FACE>
+__global__ __launch_bounds__(256) void k_face(FaceArgs a) {
+  constexpr int axis = FACE >> 1, hi = FACE & 1;
+  const int nd0 = axis == 0 ? a.yd : a.xd;             // outer in-plane extent
+  const int nd1 = axis == 2 ? a.yd : a.zd;             // inner in-plane extent
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nd0 * nd1) return;
+  const int A = t / nd1, B = t - A * nd1;
+  const int nda = axis == 0 ? a.xd : (axis == 1 ? a.yd : a.zd);
+  const int pb = hi ? nda - 1 : 0, p2 = hi ? nda - 2 : 1, p3 = hi ? nda - 3 : 2;
+  int c[3], c2[3], c3[3];
+  c[axis] = pb; c2[axis] = p2; c3[axis] = p3;
+  const int oa = axis == 0 ? 1 : 0, ia = axis == 2 ? 1 : 2;
+  c[oa] = c2[oa] = c3[oa] = A;
+  c[ia] = c2[ia] = c3[ia] = B;
+  const long long N = a.N;
+  const long long n = ((long long)c[0] * a.yd + c[1]) * a.zd + c[2];
+  const long long n2 = ((long long)c2[0] * a.yd + c2[1]) * a.zd + c2[2];
+  const long long n3 = ((long long)c3[0] * a.yd + c3[1]) * a.zd + c3[2];
+  double co[3] = {a.xmin + a.dh * (double)c[0], a.ymin + a.dh * (double)c[1],
+                  a.zmin + a.dh * (double)c[2]};
+  co[axis] = a.wallc;
+  constexpr const int *S = FaceSets<FACE>::in;
+  switch (a.code) {
+    case BC_EQ_DIRICHLET: {
+      double v[3];
+      eval_velocity(a, co[0], co[1], co[2], v);
+      double uSqr = v[0] * v[0];
+      uSqr = uSqr + v[1] * v[1];
+      uSqr = uSqr + v[2] * v[2];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) a.f[q * N + n] = feq_q(q, a.denIn, v[0], v[1], v[2], uSqr);
+      break;
+    }
+    case BC_NEQ_DIRICHLET: {
+      double v[3];
+      eval_velocity(a, co[0], co[1], co[2], v);
+      double uSqr = v[0] * v[0];
+      uSqr = uSqr + v[1] * v[1];
+      uSqr = uSqr + v[2] * v[2];
+      const int np = nd0 * nd1;
+      const double r2 = a.nb_den[t];
+      const double w1 = a.nb_u[t], w2 = a.nb_u[np + t], w3 = a.nb_u[2 * np + t];
+      double wSqr = w1 * w1;
+      wSqr = wSqr + w2 * w2;
+      wSqr = wSqr + w3 * w3;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const int q = S[k];
+        const double fe = feq_q(q, a.denIn, v[0], v[1], v[2], uSqr);
+        const double fi = feq_q(q, r2, w1, w2, w3, wSqr);
+        a.f[q * N + n] = fe + (a.f[q * N + n2] - fi);
+      }
+      break;
+    }
+    case BC_EXTRAP1:
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = a.f[S[k] * N + n2];
+      break;
+    case BC_EXTRAP2:
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        a.f[S[k] * N + n] = 2.0 * a.f[S[k] * N + n2] - a.f[S[k] * N + n3];
+      break;
+    case BC_WALL: {
+      double tmp[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) tmp[k] = a.f[oppo(S[k]) * N + n];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      break;
+    }
+    case BC_WALL_HALFWAY: {
+      if (!a.hw) break;
+      const int np = nd0 * nd1;
+      double tmp[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) tmp[k] = a.hw[(long long)oppo(S[k]) * np + t];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      break;
+    }
+    case BC_MOVING_WALL: {
+      double v[3];
+      eval_velocity(a, co[0], co[1], co[2], v);
+      double tmp[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        tmp[k] = a.f[oppo(S[k]) * N + n] + mw_q(S[k], a.denIn, v[0], v[1], v[2]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      break;
+    }
+    case BC_MOVING_WALL_HALFWAY: {
+      if (!a.hw) break;
+      co[axis] = a.hwc;
+      double v[3];
+      eval_velocity(a, co[0], co[1], co[2], v);
+      const int np = nd0 * nd1;
+      double tmp[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k)
+        tmp[k] = a.hw[(long long)oppo(S[k]) * np + t] + mw_q(S[k], a.denIn, v[0], v[1], v[2]);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      break;
+    }
+    case BC_SYMMETRIC: {
+      constexpr const int *M = FaceSets<FACE>::sym;
+      double tmp[5];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) tmp[k] = a.f[M[k] * N + n];
+#pragma unroll
+      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+}  // namespace fsil

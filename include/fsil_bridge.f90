@@ -1,0 +1,137 @@
+! fsil_bridge.f90 -- ISO_C_BINDING interface to libfsil.so (include/fsilbm3d_amd.h)
+! for a Fortran host such as the reference's FSILBM3D program.
+!
+! Arrays are passed as the host's own Fortran arrays (contiguous, column-major):
+! fIn(zDim,yDim,xDim,0:18), den(zDim,yDim,xDim), uuu(zDim,yDim,xDim,1:3),
+! v_Exyz(3,M), v_Evel(3,M), v_Ea(M), v_Eforce(3,M), lodFlow(6*nND).
+! fsil_check() follows the reference's error convention: print the message
+! and stop (e.g. FluidDomain.f90:703-704).
+module fsil_bridge
+    use, intrinsic :: iso_c_binding
+    implicit none
+    private
+    public :: fsil_init, fsil_finalize, fsil_synchronize, fsil_block_create, fsil_block_destroy, &
+              fsil_set_flow, fsil_set_time, fsil_get_volume_force, fsil_f_upload, fsil_f_download, &
+              fsil_set_boundary_conditions, fsil_macro_download, fsil_collide_stream_bc, fsil_steps, &
+              fsil_ibm_define_bodies, fsil_ibm_set_params, fsil_ibm_interaction_force, fsil_check, &
+              fsil_version
+
+    interface
+        integer(c_int) function fsil_version() bind(C, name='fsil_version')
+            import :: c_int
+        end function
+        type(c_ptr) function fsil_last_error_c() bind(C, name='fsil_last_error')
+            import :: c_ptr
+        end function
+        integer(c_int) function fsil_init(device) bind(C, name='fsil_init')
+            import :: c_int
+            integer(c_int), value :: device
+        end function
+        integer(c_int) function fsil_finalize() bind(C, name='fsil_finalize')
+            import :: c_int
+        end function
+        integer(c_int) function fsil_synchronize() bind(C, name='fsil_synchronize')
+            import :: c_int
+        end function
+        integer(c_int) function fsil_block_create(id, dims, geom, bc, iCollidModel, relax, m_collid, &
+                                                  m_force, precision, handle) bind(C, name='fsil_block_create')
+            import :: c_int, c_double
+            integer(c_int), value :: id, iCollidModel, precision
+            integer(c_int), intent(in) :: dims(3), bc(6)
+            real(c_double), intent(in) :: geom(7), relax(3), m_collid(19,19), m_force(19,19)
+            integer(c_int), intent(out) :: handle
+        end function
+        integer(c_int) function fsil_block_destroy(handle) bind(C, name='fsil_block_destroy')
+            import :: c_int
+            integer(c_int), value :: handle
+        end function
+        integer(c_int) function fsil_set_flow(handle, denIn, uvwIn, shearRateIn, velocityKind, volumeForceIn, &
+                                              volumeForceAmp, volumeForceFreq, volumeForcePhi) &
+                                              bind(C, name='fsil_set_flow')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, velocityKind
+            real(c_double), value :: denIn, volumeForceAmp, volumeForceFreq, volumeForcePhi
+            real(c_double), intent(in) :: uvwIn(3), shearRateIn(3), volumeForceIn(3)
+        end function
+        integer(c_int) function fsil_set_time(handle, blktime) bind(C, name='fsil_set_time')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), value :: blktime
+        end function
+        integer(c_int) function fsil_get_volume_force(handle, vf) bind(C, name='fsil_get_volume_force')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), intent(out) :: vf(3)
+        end function
+        integer(c_int) function fsil_f_upload(handle, fIn) bind(C, name='fsil_f_upload')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), intent(in) :: fIn(*)
+        end function
+        integer(c_int) function fsil_f_download(handle, fIn) bind(C, name='fsil_f_download')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), intent(out) :: fIn(*)
+        end function
+        integer(c_int) function fsil_set_boundary_conditions(handle) bind(C, name='fsil_set_boundary_conditions')
+            import :: c_int
+            integer(c_int), value :: handle
+        end function
+        integer(c_int) function fsil_macro_download(handle, den, uuu) bind(C, name='fsil_macro_download')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), intent(out) :: den(*), uuu(*)
+        end function
+        integer(c_int) function fsil_collide_stream_bc(handle) bind(C, name='fsil_collide_stream_bc')
+            import :: c_int
+            integer(c_int), value :: handle
+        end function
+        integer(c_int) function fsil_steps(handle, nsteps, t0, dt) bind(C, name='fsil_steps')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, nsteps
+            real(c_double), value :: t0, dt
+        end function
+        integer(c_int) function fsil_ibm_define_bodies(handle, nbodies, nmarkers, nnodes, nelems, marker_elem, &
+                                                       elem_nodes) bind(C, name='fsil_ibm_define_bodies')
+            import :: c_int
+            integer(c_int), value :: handle, nbodies
+            integer(c_int), intent(in) :: nmarkers(*), nnodes(*), nelems(*), marker_elem(*), elem_nodes(*)
+        end function
+        integer(c_int) function fsil_ibm_set_params(handle, ntolLBM, dtolLBM, Uref, denIn) &
+                                                    bind(C, name='fsil_ibm_set_params')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, ntolLBM
+            real(c_double), value :: dtolLBM, Uref, denIn
+        end function
+        integer(c_int) function fsil_ibm_interaction_force(handle, dt, Exyz, Evel, Ea, elem_xc, Eforce, lodFlow, &
+                                                           iters, dmax) bind(C, name='fsil_ibm_interaction_force')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), value :: dt
+            real(c_double), intent(in) :: Exyz(*), Evel(*), Ea(*), elem_xc(*)
+            real(c_double), intent(out) :: Eforce(*), lodFlow(*)
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: dmax
+        end function
+    end interface
+
+contains
+
+    subroutine fsil_check(rc, where)
+        integer(c_int), intent(in) :: rc
+        character(len=*), intent(in) :: where
+        character(kind=c_char), pointer :: msg(:)
+        character(len=1024) :: text
+        integer :: i
+        if (rc == 0) return
+        call c_f_pointer(fsil_last_error_c(), msg, [1024])
+        text = ' '
+        do i = 1, 1024
+            if (msg(i) == c_null_char) exit
+            text(i:i) = msg(i)
+        enddo
+        write(*,*) 'fsil error in ', where, ': ', trim(text)
+        stop
+    end subroutine fsil_check
+
+end module fsil_bridge

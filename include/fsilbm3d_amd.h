@@ -1,0 +1,129 @@
+/*
+ * fsilbm3d_amd.h -- C-ABI of the MI355X-native FSILBM3D fluid/coupling hot
+ * path (libfsil.so, gfx950).
+ *
+ * Every entry point returns 0 on success and nonzero on error; the message is
+ * then available from fsil_last_error() (the reference's convention is to
+ * `stop` with a message, which the Fortran binding in include/fsil_bridge.f90
+ * does on a nonzero return).  Arguments are plain pointers and sizes; arrays
+ * are the reference's host arrays in Fortran memory order.  The device owns
+ * the distribution functions of each block (A-B buffers, SoA by direction,
+ * z fastest) between calls; the host sees them only through
+ * fsil_f_download / fsil_macro_download.
+ *
+ * Reference citations are to gaoak/FSILBM3D src/.
+ */
+#ifndef FSILBM3D_AMD_H
+#define FSILBM3D_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library version (1). */
+int fsil_version(void);
+
+/* Message of the last failed call on this thread. */
+const char *fsil_last_error(void);
+
+/* Select the HIP device; must precede fsil_block_create.  No reference
+ * counterpart (the reference is a CPU program; cf. omp_set_num_threads at
+ * main.f90:36). */
+int fsil_init(int device);
+
+/* Destroy every block and release device memory. */
+int fsil_finalize(void);
+
+/* Wait for all device work. */
+int fsil_synchronize(void);
+
+/* Create one fluid block: allocate_fluid_ + the relaxation part of
+ * initialise_ (FluidDomain.f90:378-431, 433-522).
+ *   dims[3]   xDim, yDim, zDim                     (FluidDomain.f90:73)
+ *   geom[7]   dh, xmin, ymin, zmin, xmax, ymax, zmax (FluidDomain.f90:75,93-104)
+ *   bc[6]     BndConds x-,x+,y-,y+,z-,z+: 0,1,101-104,201-204,301,302
+ *   iCollidModel 1 SRT, 2 TRT, 3 MRT              (FluidDomain.f90:1225-1238)
+ *   relax[3]  tau, Omega, Omega2                    (FluidDomain.f90:449-464)
+ *   m_collid, m_force  19x19 Fortran-order M_COLLID, M_FORCE for MRT, else NULL
+ *   precision 0: FP64 plain; 1: FP32 storage of f - w (periodic SRT blocks)
+ *   handle    out: block handle */
+int fsil_block_create(int id, const int *dims, const double *geom, const int *bc, int iCollidModel,
+                      const double *relax, const double *m_collid, const double *m_force,
+                      int precision, int *handle);
+
+/* Free a block. */
+int fsil_block_destroy(int handle);
+
+/* Flow conditions the block's boundaries and volume force read from the
+ * reference's `flow` global (FlowCondition.f90:11-26): denIn, uvwIn[3],
+ * shearRateIn[3], velocityKind, volumeForceIn[3], volumeForceAmp/Freq/Phi. */
+int fsil_set_flow(int handle, double denIn, const double *uvwIn, const double *shearRateIn,
+                  int velocityKind, const double *volumeForceIn, double volumeForceAmp,
+                  double volumeForceFreq, double volumeForcePhi);
+
+/* Set LBMblks%blktime and evaluate update_volume_force_ at it
+ * (main.f90:97, FluidDomain.f90:1174-1180). */
+int fsil_set_time(int handle, double blktime);
+
+/* The volume force of the current block time (vf[3]). */
+int fsil_get_volume_force(int handle, double *vf);
+
+/* Copy fIn(z,y,x,0:18) (pre-collision, post-boundary state) to / from the
+ * device.  Replaces nothing: the reference's fIn lives in host memory. */
+int fsil_f_upload(int handle, const double *fIn);
+int fsil_f_download(int handle, double *fIn);
+
+/* set_boundary_conditions_ on the stored state as called once at start-up
+ * (main.f90:63; FluidDomain.f90:616-1126). */
+int fsil_set_boundary_conditions(int handle);
+
+/* calculate_macro_quantities_ of the stored state (FluidDomain.f90:1128-1145,
+ * as main.f90:107 calls it): den(z,y,x), uuu(z,y,x,1:3). */
+int fsil_macro_download(int handle, double *den, double *uuu);
+
+/* One fluid update of a block: collision_ (with the immersed-boundary
+ * velocity/force of the last fsil_ibm_interaction_force on its band cells,
+ * the uniform volume force elsewhere), halfwayBCset_, streaming_ and
+ * set_boundary_conditions_ (LBMBlockComm.f90:293-304; FluidDomain.f90:567-614,
+ * 616-1126, 1208-1263, 1514-1625), fused into one pass plus boundary faces. */
+int fsil_collide_stream_bc(int handle);
+
+/* nsteps fluid-only steps at block times t0+dt*(s+1) (volume force updated
+ * each step); the body of main.f90:93-147 for a block carrying no bodies. */
+int fsil_steps(int handle, int nsteps, double t0, double dt);
+
+/* Timing helpers (HIP events on the block's stream): average duration of the
+ * fused collide+stream kernel over nlaunch launches; total of nsteps steps. */
+int fsil_time_collide(int handle, int nlaunch, double *avg_ms);
+int fsil_time_steps(int handle, int nsteps, double t0, double dt, double *total_ms);
+
+/* Immersed bodies carried by the block (FindCarrierFluidBlock,
+ * FluidDomain.f90:1998-2017; PlateBuild_, Solidbody.f90:1051-1071):
+ * nmarkers/nnodes/nelems per body; marker_elem[M] element (0-based, within
+ * its body) of each marker (vtor); elem_nodes[2*sum(nelems)] node0/node1
+ * (0-based, within the body) of each element. */
+int fsil_ibm_define_bodies(int handle, int nbodies, const int *nmarkers, const int *nnodes,
+                           const int *nelems, const int *marker_elem, const int *elem_nodes);
+
+/* ntolLBM, dtolLBM, Uref, denIn of the direct-forcing loop
+ * (set_solidbody_parameters, Solidbody.f90:328-348). */
+int fsil_ibm_set_params(int handle, int ntolLBM, double dtolLBM, double Uref, double denIn);
+
+/* FSInteraction_force after UpdatePosVelArea_ (Solidbody.f90:589-602,
+ * 760-1049): discrete-delta stencils, the multi-direct-forcing loop on the
+ * band macros of the stored state, force spreading and the beam loads.
+ *   Exyz[3M], Evel[3M], Ea[M]  markers of all bodies, carried order
+ *   elem_xc[3*nelems]          beam axis midpoints (moment arm origin)
+ *   Eforce[3M] out             v_Eforce;  lodFlow[6*sum(nnodes)] out
+ *   iters, dmax out            iterations and final residual */
+int fsil_ibm_interaction_force(int handle, double dt, const double *Exyz, const double *Evel,
+                               const double *Ea, const double *elem_xc, double *Eforce,
+                               double *lodFlow, int *iters, double *dmax);
+
+/* Band size and bounding box (x0,y0,z0,nx,ny,nz) of the last coupling call. */
+int fsil_ibm_band_info(int handle, int *nband, int *bbox);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FSILBM3D_AMD_H */

@@ -1,0 +1,117 @@
+"""Shared drivers: run a parity case on the CPU oracle or on the device."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+from fsilbm3d_amd import params as P  # noqa: E402
+from fsilbm3d_amd.solid import RigidPlate  # noqa: E402
+
+
+def derived(case):
+    """nu, tau, Omega, Omega2, MRT matrices, Uref, Tref of a refcase.Case."""
+    if case.UrefType == 0:
+        uref = abs(case.uvwIn[0])
+    else:
+        uref = case.Uref
+    lref = case.Lref
+    tref = lref / uref if case.TrefType == 0 else case.Tref
+    nu = P.viscosity(uref, lref, case.Re)
+    tau, om = P.srt(nu, case.dh)
+    om2 = P.trt_omega2(om, case.params[0]) if case.collide == 2 else om
+    Mc = Mf = None
+    if case.collide == 3:
+        a, b = P.mrt_matrices(om)
+        Mc, Mf = P.fortran_order(a), P.fortran_order(b)
+    return dict(nu=nu, tau=tau, Omega=om, Omega2=om2, Mc=Mc, Mf=Mf, Uref=uref, Tref=tref)
+
+
+def make_bodies(pc, denIn):
+    out = []
+    for p in pc.plates:
+        out.append(RigidPlate(p.nodes(), p.elements(), XYZo=p.XYZo, XYZAmpl=p.XYZAmpl, XYZPhi=p.XYZPhi,
+                              AoAo=p.AoAo, AoAAmpl=p.AoAAmpl, AoAPhi=p.AoAPhi, Freq=p.Freq,
+                              initXYZVel=p.initXYZVel, denIn=denIn, IBPenaltyAlpha=pc.case.IBPenaltyAlpha))
+    return out
+
+
+def flow_kwargs(case):
+    return dict(denIn=case.denIn, uvwIn=case.uvwIn, shearRateIn=case.shearRateIn, velocityKind=case.velocityKind,
+                volumeForceIn=case.volumeForceIn, volumeForceAmp=case.volumeForceAmp,
+                volumeForceFreq=case.volumeForceFreq, volumeForcePhi=case.volumeForcePhi)
+
+
+def run_oracle(pc, f0=None, nsteps=None, record=None):
+    """Run a ParityCase on the C oracle; returns the final state (and the
+    per-step IBM outputs when record is a list)."""
+    from oracle.oracle import OracleBlock
+    case = pc.case
+    d = derived(case)
+    ob = OracleBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["Omega"], d["Omega2"], d["Mc"], d["Mf"],
+                     **flow_kwargs(case))
+    f = np.array(pc.initial_state() if f0 is None else f0, dtype=np.float64, copy=True)
+    ob.init_boundary(f, 0.0)
+    bodies = make_bodies(pc, case.denIn)
+    ibm = dict(ntol=case.ntolLBM, dtol=case.dtolLBM, Uref=d["Uref"], denIn=case.denIn)
+    n = pc.nsteps if nsteps is None else nsteps
+    time = 0.0
+    for _ in range(n):
+        time = time + case.dh
+        bl = None
+        if bodies:
+            for b in bodies:
+                if b.v_move == 1:
+                    b.update_markers()
+            bl = [dict(Exyz=b.Exyz.copy(), Evel=b.Evel.copy(), Ea=b.Ea.copy()) for b in bodies]
+            xcs = [b.element_centres() for b in bodies]
+        out = ob.step(f, time, bl, ibm)
+        if bodies:
+            if record is not None:
+                from oracle.oracle import lod_flow
+                lods = [lod_flow(E, bb["Exyz"], b.marker_elem, b.elem_nodes, xc, b.nND)
+                        for E, bb, b, xc in zip(out["Eforce"], bl, bodies, xcs)]
+                record.append(dict(iters=out["iters"], Eforce=out["Eforce"], lod=lods))
+            dts = case.dh / float(case.numsubstep)
+            for isub in range(1, case.numsubstep + 1):
+                for b in bodies:
+                    b.structure(time, isub, case.dh, dts)
+    return f
+
+
+def run_device(pc, f0=None, nsteps=None, record=None):
+    """Run a ParityCase through the C-ABI on the GPU; returns the final state."""
+    from fsilbm3d_amd.block import LBMBlock
+    from fsilbm3d_amd.fsi import FSIBlock
+    case = pc.case
+    d = derived(case)
+    blk = LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params)
+    blk.set_flow(**flow_kwargs(case))
+    f = np.array(pc.initial_state() if f0 is None else f0, dtype=np.float64, copy=True)
+    blk.upload(f)
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    bodies = make_bodies(pc, case.denIn)
+    fsi = FSIBlock(blk, bodies, ntolLBM=case.ntolLBM, dtolLBM=case.dtolLBM, Uref=d["Uref"], denIn=case.denIn,
+                   numsubstep=case.numsubstep)
+    n = pc.nsteps if nsteps is None else nsteps
+    time = 0.0
+    for _ in range(n):
+        time = time + case.dh
+        fsi.step(time)
+        if record is not None and bodies:
+            record.append(dict(iters=fsi.iters, Eforce=fsi.Eforce[:fsi.M].copy(),
+                               lod=[fsi.body_loads(i).copy() for i in range(len(bodies))]))
+    out = blk.download()
+    blk.close()
+    return out
+
+
+def mismatch(a, b):
+    d = np.abs(a - b)
+    return int((a != b).sum()), float(d.max()) if d.size else 0.0

@@ -1,0 +1,59 @@
+"""The C-ABI library: it loads, exports exactly the entry points that
+include/fsilbm3d_amd.h declares, and fails loudly when it is missing.  No
+compute calls (no GPU here)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from fsilbm3d_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fsilbm3d_amd.h")
+
+
+def header_symbols():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fsil_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_matches_binding_table():
+    assert header_symbols() == sorted(_lib.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "build the library first (__graft_entry__.build())"
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in header_symbols():
+        assert hasattr(lib, name), name
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r"\bT (fsil_[a-z0-9_]+)", out))
+    assert exported == set(header_symbols())
+
+
+def test_binding_loads_and_reports_version():
+    lib = _lib.load()
+    assert lib.fsil_version() == 1
+    assert isinstance(lib.fsil_last_error(), bytes)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.FsilError):
+        _lib.load()
+
+
+def test_fortran_binding_compiles():
+    """include/fsil_bridge.f90 (the ISO_C_BINDING module a Fortran host uses)
+    compiles with the image's Fortran compiler."""
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not present")
+    src = os.path.join(ROOT, "include", "fsil_bridge.f90")
+    out = subprocess.run([fc, "-c", "-ffree-form", src, "-module-dir", "/tmp", "-o", "/tmp/fsil_bridge.o"],
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr

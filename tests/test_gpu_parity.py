@@ -1,0 +1,121 @@
+"""Device parity: the HIP path through the C-ABI against the reference
+program's fixtures and the CPU oracle, bit for bit (FP64 plain form)."""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from tests.golden.cases import BY_NAME, CASES, ParityCase, Plate, load_fixture, seeded_state
+from tests.helpers import mismatch, run_device, run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("pc", CASES, ids=[c.name for c in CASES])
+def test_device_matches_reference_fixture(pc):
+    _say(f"[gpu] fixture case {pc.name} dims={pc.case.dims} steps={pc.nsteps}")
+    fx = load_fixture(pc.name)
+    f = run_device(pc)
+    n, m = mismatch(f, fx["f_final"])
+    assert n == 0, f"{pc.name}: {n} values differ from the reference, max |diff| {m:.3e}"
+
+
+@pytest.mark.parametrize("name", ["ibm_static_plate", "ibm_heaving_plate", "ibm_pitching_two_plates"])
+def test_device_ibm_forces_match_oracle(name):
+    """Marker forces v_Eforce, beam loads lodFlow and the iteration count of
+    every step equal the oracle's exactly."""
+    pc = BY_NAME[name]
+    _say(f"[gpu] ibm forces {name}")
+    ro, rd = [], []
+    fo = run_oracle(pc, record=ro)
+    fd = run_device(pc, record=rd)
+    assert mismatch(fo, fd)[0] == 0
+    for so, sd in zip(ro, rd):
+        assert so["iters"] == sd["iters"]
+        Eo = np.concatenate(so["Eforce"])
+        assert np.array_equal(Eo, sd["Eforce"])
+        for lo, ld in zip(so["lod"], sd["lod"]):
+            assert np.array_equal(lo, ld)
+
+
+def _c1_like(nsteps, dims=(128, 64, 64)):
+    """The heaving-plate configuration at its full size: 128x64x64, inflow
+    0.05, Re 100 on a 16-cell chord, 101/103/302 faces, 17 chord nodes x 16
+    span samples = 256 markers, heave 8 cells at 0.25 U/L, 3 MDF iterations."""
+    from oracle.refcase import Case
+    case = Case(dims=dims, bc=(101, 103, 302, 302, 302, 302), Re=100.0, uvwIn=(0.05, 0.0, 0.0),
+                UrefType=0, Uref=0.05, LrefType=1, Lref=16.0, TrefType=0, ntolLBM=3, dtolLBM=1e-14)
+    plate = Plate(17, -8.0, 8.0, 8.0, 16, XYZo=(40.0, 32.0, 32.0), XYZAmpl=(0.0, 8.0, 0.0), Freq=7.8125e-4)
+    return ParityCase("c1_heaving_plate", case, nsteps=nsteps, seed=41, ubase=(0.05, 0, 0), neq=0.0,
+                      drho=0.0, du=0.0, plates=[plate])
+
+
+def test_full_size_heaving_plate_matches_oracle():
+    pc = _c1_like(4)
+    _say("[gpu] 128x64x64 heaving plate, 4 steps vs oracle")
+    ro, rd = [], []
+    fo = run_oracle(pc, record=ro)
+    fd = run_device(pc, record=rd)
+    n, m = mismatch(fo, fd)
+    assert n == 0, f"{n} differ, max {m:.3e}"
+    for so, sd in zip(ro, rd):
+        assert so["iters"] == sd["iters"] == 3
+        assert np.array_equal(np.concatenate(so["Eforce"]), sd["Eforce"])
+
+
+@pytest.mark.parametrize("model", [1, 2, 3])
+def test_random_periodic_models_match_oracle(model):
+    from oracle.refcase import Case
+    case = Case(dims=(40, 24, 20), bc=(301,) * 6, collide=model, params=(0.15,) + (0.0,) * 9, Re=3.0,
+                Uref=0.05, UrefType=9, Lref=1.0, LrefType=1, TrefType=0, volumeForceIn=(1e-5, 0.0, -2e-6))
+    pc = ParityCase(f"rand_periodic_{model}", case, nsteps=5, seed=100 + model, neq=0.03)
+    _say(f"[gpu] random periodic model {model}")
+    n, m = mismatch(run_oracle(pc), run_device(pc))
+    assert n == 0, f"{n} differ, max {m:.3e}"
+
+
+def test_odd_sizes_and_all_open_faces():
+    """Ragged sizes (not multiples of the 64-lane wave) and every face a
+    non-periodic code."""
+    from oracle.refcase import Case
+    case = Case(dims=(13, 11, 70), bc=(102, 104, 203, 202, 201, 302), Re=5.0, uvwIn=(0.04, 0.0, 0.0),
+                shearRateIn=(1e-3, -2e-3, 5e-4), UrefType=9, Uref=0.05, LrefType=1, Lref=1.0, TrefType=0)
+    pc = ParityCase("ragged_open", case, nsteps=4, seed=7, ubase=(0.04, 0, 0), neq=0.01)
+    _say("[gpu] ragged sizes 13x11x70 all open faces")
+    n, m = mismatch(run_oracle(pc), run_device(pc))
+    assert n == 0, f"{n} differ, max {m:.3e}"
+
+
+def test_macro_quantities_match_oracle():
+    from fsilbm3d_amd.block import LBMBlock
+    from oracle import oracle as O
+    dims = (20, 16, 12)
+    f = seeded_state(dims, 9, neq=0.02)
+    blk = LBMBlock(dims, bc=(301,) * 6, nu=0.05 / 3)
+    blk.set_flow(volumeForceIn=(1e-4, 2e-5, -3e-5))
+    blk.upload(f)
+    blk.update_volume_force(0.0)
+    den, uuu = blk.calculate_macro_quantities()
+    od, ou = O.macro(f, (1e-4, 2e-5, -3e-5), 1.0)
+    assert np.array_equal(den, od) and np.array_equal(uuu, ou)
+    assert np.array_equal(blk.download(), f)
+    blk.close()
+
+
+def test_stencil_out_of_bounds_is_an_error():
+    """A marker whose stencil leaves a non-periodic, non-mirrored face is
+    rejected on the host before any kernel runs (Solidbody.f90:850-862)."""
+    from fsilbm3d_amd._lib import FsilError
+    from oracle.refcase import Case
+    case = Case(dims=(12, 12, 12), bc=(101, 103, 302, 302, 302, 302), Re=10.0, uvwIn=(0.05, 0, 0), UrefType=9,
+                Uref=0.05, LrefType=1, Lref=1.0, TrefType=0)
+    pc = ParityCase("oob", case, nsteps=1, seed=3, ubase=(0.05, 0, 0),
+                    plates=[Plate(3, -1.0, 1.0, 1.0, 2, XYZo=(0.6, 6.0, 6.0))])
+    with pytest.raises(FsilError, match="out of min bound"):
+        run_device(pc)

@@ -1,0 +1,88 @@
+"""Host-side logic: derived parameters and rigid-body kinematics."""
+import math
+
+import numpy as np
+
+from fsilbm3d_amd import params as P
+from fsilbm3d_amd.solid import RigidPlate, aoa_to_ttt
+
+
+def test_relaxation_times():
+    tau, om = P.srt(0.05 * 1.0 / 3.0, 1.0)
+    assert abs(tau - 0.55) < 1e-15 and abs(om - 1 / 0.55) < 1e-15
+    # TRT with magic parameter lambda: (1/om+ - 1/2)(1/om- - 1/2) = lambda
+    lam = 0.1875
+    om2 = P.trt_omega2(om, lam)
+    assert abs((1 / om - 0.5) * (1 / om2 - 0.5) - lam) < 1e-14
+
+
+def test_mrt_reduces_to_srt_when_all_rates_equal(monkeypatch):
+    for name in ("S0", "S1", "S2", "S4", "S10", "S16"):
+        monkeypatch.setattr(P, name, 1.3)
+    Mc, Mf = P.mrt_matrices(1.3)
+    assert np.allclose(Mc, 1.3 * np.eye(19), atol=1e-13)
+    assert np.allclose(Mf, np.eye(19) - 0.65 * np.eye(19), atol=1e-13)
+
+
+def test_mrt_conserves_mass_and_momentum_rows():
+    Mc, _ = P.mrt_matrices(1 / 0.55)
+    ee = P.EE
+    # the relaxation leaves conserved moments untouched: 1^T Mc = 0, e^T Mc = 0
+    assert np.allclose(np.ones(19) @ Mc, 0.0, atol=1e-13)
+    for k in range(3):
+        assert np.allclose(ee[:, k] @ Mc, 0.0, atol=1e-13)
+
+
+def test_block_extent_periodic_adds_one_cell():
+    ext, per = P.block_extent((10, 8, 6), 0.5, (1.0, 2.0, 3.0), (301, 301, 302, 302, 101, 103))
+    assert ext == (1.0 + 0.5 * 10, 2.0 + 0.5 * 7, 3.0 + 0.5 * 5)
+    assert per == (1, 0, 0)
+
+
+def _plate(**kw):
+    nodes = np.array([[-2.0 + i, 0, 0, 2.0, 2.0, 0, 0, 1.0] for i in range(5)])
+    els = [(i + 1, i + 2, 4) for i in range(4)]
+    return RigidPlate(nodes, els, **kw)
+
+
+def test_static_plate_markers_closed_form():
+    p = _plate(XYZo=(8.0, 6.0, 6.0))
+    assert p.M == 16 and p.v_move == 0
+    # element-centre x, span samples z = 6 + (s + 0.5) - 2
+    xs = sorted(set(p.Exyz[:, 0]))
+    assert xs == [6.5, 7.5, 8.5, 9.5]
+    assert sorted(set(p.Exyz[:, 2])) == [4.5, 5.5, 6.5, 7.5]
+    assert (p.Exyz[:, 1] == 6.0).all()
+    assert (p.Evel == 0.0).all()
+    assert (p.Ea == -2.0).all()          # dl * len * (-2 alpha rho)
+    assert abs(p.Asfac - 4.0) < 1e-15
+
+
+def test_heaving_plate_kinematics():
+    A, f, phi = 1.5, 0.02, 30.0
+    p = _plate(XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0, A, 0), XYZPhi=(0, phi, 0), Freq=f)
+    assert p.v_move == 1
+    t = 3.0
+    p.structure(t, 1, 1.0, 1.0)
+    p.update_markers()
+    w = 2 * math.pi * f
+    y = 6.0 + A * math.cos(w * t + math.radians(phi))
+    v = -w * A * math.sin(w * t + math.radians(phi))
+    assert np.allclose(p.Exyz[:, 1], y, atol=1e-14)
+    assert np.allclose(p.Evel[:, 1], v, atol=1e-15)
+    assert np.allclose(p.Evel[:, [0, 2]], 0.0)
+
+
+def test_pitch_rotation_matrix_is_orthonormal_and_rotates_marker_velocity():
+    T = np.array(aoa_to_ttt([0.1, -0.2, 0.3]))
+    assert np.allclose(T @ T.T, np.eye(3), atol=1e-14)
+    # prescribed pitching: every marker moves with the rigid velocity W x r
+    p = _plate(XYZo=(8.0, 6.0, 6.0), AoAAmpl=(0, 0, 10.0), Freq=0.05, AoAPhi=(0, 0, -90.0))
+    p.structure(0.0, 1, 0.0, 1.0)
+    p.update_markers()
+    W = p.vel[0, 3:6]
+    r = p.Exyz - np.array(p.XYZ)
+    assert np.linalg.norm(W) > 0
+    assert np.allclose(p.Evel, np.cross(W, r), atol=1e-12)
+    # and every node shares the same angular velocity
+    assert np.allclose(p.vel[:, 3:6], W, atol=0)
