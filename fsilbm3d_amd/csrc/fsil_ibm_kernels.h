@@ -9,6 +9,7 @@
 // exactly as the reference does, with no atomics.
 #pragma once
 #include "fsil_common.h"
+#include "fsil_lbm_kernels.h"
 
 namespace fsil {
 
@@ -24,16 +25,8 @@ __global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f
   double g[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) g[q] = f[q * N + n];
-  double s = 0.0;
-#pragma unroll
-  for (int q = 0; q < Q; ++q) s = s + g[q];
-  double m1 = 0.0, m2 = 0.0, m3 = 0.0;
-  m1 = m1 + g[1]; m1 = m1 - g[2]; m1 = m1 + g[7]; m1 = m1 - g[8]; m1 = m1 + g[9];
-  m1 = m1 - g[10]; m1 = m1 + g[11]; m1 = m1 - g[12]; m1 = m1 + g[13]; m1 = m1 - g[14];
-  m2 = m2 + g[3]; m2 = m2 - g[4]; m2 = m2 + g[7]; m2 = m2 + g[8]; m2 = m2 - g[9];
-  m2 = m2 - g[10]; m2 = m2 + g[15]; m2 = m2 - g[16]; m2 = m2 + g[17]; m2 = m2 - g[18];
-  m3 = m3 + g[5]; m3 = m3 - g[6]; m3 = m3 + g[11]; m3 = m3 + g[12]; m3 = m3 - g[13];
-  m3 = m3 - g[14]; m3 = m3 + g[15]; m3 = m3 + g[16]; m3 = m3 - g[17]; m3 = m3 - g[18];
+  double s, m1, m2, m3;
+  moments(g, s, m1, m2, m3);
   band_u[b] = (m1 + hv0) / s;
   band_u[nband + b] = (m2 + hv1) / s;
   band_u[2 * nband + b] = (m3 + hv2) / s;

@@ -42,11 +42,14 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int x = blockIdx.z;
   if (z >= a.zd || y >= a.yd) return;
-  const long long N = a.N;
-  const long long n = ((long long)x * a.yd + y) * a.zd + z;
+  const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
+  const size_t plane = (size_t)a.N * sizeof(float);
+  const char *gin = reinterpret_cast<const char *>(a.gin);
+  const unsigned nb = n << 2;
   float g[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) g[q] = __builtin_nontemporal_load(a.gin + q * N + n);
+  for (int q = 0; q < Q; ++q)
+    g[q] = __builtin_nontemporal_load(reinterpret_cast<const float *>(gin + q * plane + nb));
   float dr = 0.f, cr = 0.f, j1 = 0.f, c1 = 0.f, j2 = 0.f, c2 = 0.f, j3 = 0.f, c3 = 0.f;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -80,16 +83,21 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
     }
     post[q] = t;
   }
-  const int xp = x + 1 == a.xd ? 0 : x + 1, xm = x == 0 ? a.xd - 1 : x - 1;
-  const int yp = y + 1 == a.yd ? 0 : y + 1, ym = y == 0 ? a.yd - 1 : y - 1;
-  const int zp = z + 1 == a.zd ? 0 : z + 1, zm = z == 0 ? a.zd - 1 : z - 1;
+  const unsigned yz = (unsigned)a.yd * (unsigned)a.zd, zd = (unsigned)a.zd;
+  const unsigned sxp = x + 1 == a.xd ? 0u - (unsigned)(a.xd - 1) * yz : yz;
+  const unsigned sxm = x == 0 ? (unsigned)(a.xd - 1) * yz : 0u - yz;
+  const unsigned syp = y + 1 == a.yd ? 0u - (unsigned)(a.yd - 1) * zd : zd;
+  const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
+  const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
+  const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
+  char *gout = reinterpret_cast<char *>(a.gout);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int X = ex(q) > 0 ? xp : (ex(q) < 0 ? xm : x);
-    const int Y = ey(q) > 0 ? yp : (ey(q) < 0 ? ym : y);
-    const int Z = ez(q) > 0 ? zp : (ez(q) < 0 ? zm : z);
-    const long long d = ((long long)X * a.yd + Y) * a.zd + Z;
-    __builtin_nontemporal_store(post[q], a.gout + q * N + d);
+    const unsigned sx = ex(q) > 0 ? sxp : (ex(q) < 0 ? sxm : 0u);
+    const unsigned sy = ey(q) > 0 ? syp : (ey(q) < 0 ? sym : 0u);
+    const unsigned sz = ez(q) > 0 ? szp : (ez(q) < 0 ? szm : 0u);
+    const unsigned d = n + sx + sy + sz;
+    __builtin_nontemporal_store(post[q], reinterpret_cast<float *>(gout + q * plane + (d << 2)));
   }
 }
 

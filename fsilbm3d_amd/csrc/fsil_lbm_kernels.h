@@ -52,20 +52,19 @@ __device__ __forceinline__ double guo_force(double dt3, double ux, double u1, do
 // Sequential moments of one cell (FluidDomain.f90:1136-1139).
 __device__ __forceinline__ void moments(const double (&f)[Q], double &rho, double &m1,
                                         double &m2, double &m3) {
-  double s = 0.0;
+  // the reference's sums start from +0; 0 + a == a for every a but -0, and a
+  // population is never -0, so each sum starts from its first term
+  double s = f[0];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) s = s + f[q];
+  for (int q = 1; q < Q; ++q) s = s + f[q];
   rho = s;
   // e_x != 0: 1,2,7,8,9,10,11,12,13,14 ; e_y: 3,4,7,8,9,10,15,16,17,18 ;
   // e_z: 5,6,11,12,13,14,15,16,17,18 (q ascending)
-  m1 = 0.0;
-  m1 = m1 + f[1]; m1 = m1 - f[2]; m1 = m1 + f[7]; m1 = m1 - f[8]; m1 = m1 + f[9];
+  m1 = f[1]; m1 = m1 - f[2]; m1 = m1 + f[7]; m1 = m1 - f[8]; m1 = m1 + f[9];
   m1 = m1 - f[10]; m1 = m1 + f[11]; m1 = m1 - f[12]; m1 = m1 + f[13]; m1 = m1 - f[14];
-  m2 = 0.0;
-  m2 = m2 + f[3]; m2 = m2 - f[4]; m2 = m2 + f[7]; m2 = m2 + f[8]; m2 = m2 - f[9];
+  m2 = f[3]; m2 = m2 - f[4]; m2 = m2 + f[7]; m2 = m2 + f[8]; m2 = m2 - f[9];
   m2 = m2 - f[10]; m2 = m2 + f[15]; m2 = m2 - f[16]; m2 = m2 + f[17]; m2 = m2 - f[18];
-  m3 = 0.0;
-  m3 = m3 + f[5]; m3 = m3 - f[6]; m3 = m3 + f[11]; m3 = m3 + f[12]; m3 = m3 - f[13];
+  m3 = f[5]; m3 = m3 - f[6]; m3 = m3 + f[11]; m3 = m3 + f[12]; m3 = m3 - f[13];
   m3 = m3 - f[14]; m3 = m3 + f[15]; m3 = m3 + f[16]; m3 = m3 - f[17]; m3 = m3 - f[18];
 }
 
@@ -106,18 +105,21 @@ __device__ __forceinline__ int face_plane_index(int face, int x, int y, int z, i
   return x * yd + y;
 }
 
+// One cell: load its 19 pre-collision populations, collide, push the 19
+// post-collision values to the neighbours (x+e_q, periodic wrap).
 template <int MODEL, bool FORCE, bool BAND, bool AUX>
-__global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int x = blockIdx.z;
-  if (z >= a.zd || y >= a.yd) return;
-  const long long N = a.N;
-  const long long n = ((long long)x * a.yd + y) * a.zd + z;
+__device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, int y, int z) {
+  // 32-bit cell index and byte offsets (N < 2^29 for every block this library
+  // accepts), so each access is a scalar plane base plus one 32-bit offset
+  const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
+  const size_t plane = (size_t)a.N * sizeof(double);
+  const char *fin = reinterpret_cast<const char *>(a.fin);
+  const unsigned nb = n << 3;
 
   double f[Q];
 #pragma unroll
-  for (int q = 0; q < Q; ++q) f[q] = __builtin_nontemporal_load(a.fin + q * N + n);
+  for (int q = 0; q < Q; ++q)
+    f[q] = __builtin_nontemporal_load(reinterpret_cast<const double *>(fin + q * plane + nb));
 
   double rho, m1, m2, m3;
   moments(f, rho, m1, m2, m3);
@@ -237,18 +239,33 @@ __global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
     }
   }
 
-  // push-stream with periodic wrap in every direction (FluidDomain.f90:1514-1625)
-  const int xp = x + 1 == a.xd ? 0 : x + 1, xm = x == 0 ? a.xd - 1 : x - 1;
-  const int yp = y + 1 == a.yd ? 0 : y + 1, ym = y == 0 ? a.yd - 1 : y - 1;
-  const int zp = z + 1 == a.zd ? 0 : z + 1, zm = z == 0 ? a.zd - 1 : z - 1;
+  // push-stream with periodic wrap in every direction (FluidDomain.f90:1514-1625):
+  // destination offset = n + per-axis step, the step wrapping at the block edge
+  const unsigned yz = (unsigned)a.yd * (unsigned)a.zd, zd = (unsigned)a.zd;
+  const unsigned sxp = x + 1 == a.xd ? 0u - (unsigned)(a.xd - 1) * yz : yz;
+  const unsigned sxm = x == 0 ? (unsigned)(a.xd - 1) * yz : 0u - yz;
+  const unsigned syp = y + 1 == a.yd ? 0u - (unsigned)(a.yd - 1) * zd : zd;
+  const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
+  const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
+  const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
+  char *fout = reinterpret_cast<char *>(a.fout);
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
-    const int X = ex(q) > 0 ? xp : (ex(q) < 0 ? xm : x);
-    const int Y = ey(q) > 0 ? yp : (ey(q) < 0 ? ym : y);
-    const int Z = ez(q) > 0 ? zp : (ez(q) < 0 ? zm : z);
-    const long long d = ((long long)X * a.yd + Y) * a.zd + Z;
-    __builtin_nontemporal_store(post[q], a.fout + q * N + d);
+    const unsigned sx = ex(q) > 0 ? sxp : (ex(q) < 0 ? sxm : 0u);
+    const unsigned sy = ey(q) > 0 ? syp : (ey(q) < 0 ? sym : 0u);
+    const unsigned sz = ez(q) > 0 ? szp : (ez(q) < 0 ? szm : 0u);
+    const unsigned d = n + sx + sy + sz;
+    __builtin_nontemporal_store(post[q], reinterpret_cast<double *>(fout + q * plane + (d << 3)));
   }
+}
+
+template <int MODEL, bool FORCE, bool BAND, bool AUX>
+__global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int x = blockIdx.z;
+  if (z >= a.zd || y >= a.yd) return;
+  collide_push_cell<MODEL, FORCE, BAND, AUX>(a, x, y, z);
 }
 
 // ---------------------------------------------------------------------------
