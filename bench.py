@@ -95,6 +95,32 @@ def traffic_from_profiles():
         return None
 
 
+def copy_gbs(nbytes=2 << 30, reps=10):
+    import numpy as np
+    from fsilbm3d_amd import _lib
+    lib = _lib.init(0)
+    out = np.zeros(1)
+    _lib.check(lib.fsil_copy_bandwidth(int(nbytes), int(reps), _lib.dptr(out)))
+    return float(out[0])
+
+
+def pmc_driver():
+    """Kernels for the HBM-counter runs: k_copy over 2 GiB (calibration: 4 GiB
+    of traffic per launch) and 3 launches of the FP64 fused kernel."""
+    import numpy as np
+    from fsilbm3d_amd.block import LBMBlock, synchronize
+    copy_gbs(2 << 30, 1)
+    blk = LBMBlock(DIMS, bc=(301,) * 6, nu=NU, precision=0)
+    f = np.empty((19,) + DIMS)
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    f[:] = w[:, None, None, None]
+    blk.upload(f)
+    blk.steps(3, 0.0, 1.0)
+    synchronize()
+    blk.close()
+    log("pmc driver done")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -105,12 +131,17 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--pmc-driver", action="store_true",
+                    help="profiling driver: one 2 GiB copy kernel, then 3 launches of the FP64 fused kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
               "decomposition is a design note, not a measured figure.", file=sys.stderr)
         sys.exit(2)
     from tests.golden.cases import seeded_state
+    if args.pmc_driver:
+        pmc_driver()
+        return
     log("bench: seeding 512x256x256 state")
     f0 = seeded_state(DIMS, 20261202)
     log("bench: FP64 plain form")
@@ -133,6 +164,7 @@ def main():
         except Exception as e:  # reported, not fatal: the baseline is not the target
             cpu = {"value": None, "unit": "MLUPS", "cores": None, "kind": "reference", "sample": f"failed: {e}"}
     N = DIMS[0] * DIMS[1] * DIMS[2]
+    cp = copy_gbs()
     out = {
         "metric": "FSI-step MLUPS, 1 MI355X: FP32+learned vs FP64 plain @drag/lift err<1e-3; %HBM",
         "value": r64["mlups"],
@@ -148,6 +180,7 @@ def main():
         "data": "synthetic (seeded equilibrium state)",
         "config": {"workload": "stencil benchmark: D3Q19 512x256x256 periodic, SRT tau 0.55, FP64 plain form",
                    "cells": N, "collision": "SRT", "form": "fp64-plain"},
+        "copy_kernel_gbs": cp,
         "roofline": {"bound": "hbm", "achieved": r64["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": r64["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "k_collide_push<1,false,false,false>", "kernel_ms": r64["kernel_ms"],

@@ -886,6 +886,32 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   return 0;
 }
 
+int fsil_copy_bandwidth(long long nbytes, int reps, double *gbs) {
+  if (g_device < 0) return fail("fsil_init has not been called");
+  if (nbytes < 1024 || reps <= 0 || !gbs) return fail("bad arguments to fsil_copy_bandwidth");
+  const long long n = nbytes / (long long)sizeof(double);
+  DevBuf<double> a, b;
+  HIPCHK(a.ensure(n));
+  HIPCHK(b.ensure(n));
+  const dim3 grid((unsigned)((n + 255) / 256)), blk(256);
+  hipLaunchKernelGGL(k_zero, grid, blk, 0, 0, a.p, n);
+  hipLaunchKernelGGL(k_copy, grid, blk, 0, 0, a.p, b.p, n);
+  HIPCHK(hipGetLastError());
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, 0));
+  for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, grid, blk, 0, 0, a.p, b.p, n);
+  HIPCHK(hipEventRecord(e1, 0));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *gbs = 2.0 * (double)n * sizeof(double) * reps / ((double)ms * 1e-3) / 1e9;
+  return 0;
+}
+
 int fsil_ibm_band_info(int h, int *nband, int *bbox) {
   Block *b = get_block(h);
   if (!b) return 1;

@@ -37,6 +37,13 @@ __global__ void k_zero(double *__restrict__ p, long long n) {
   if (i < n) p[i] = 0.0;
 }
 
+// plain copy (bandwidth calibration of the HBM counters and the roofline)
+__global__ __launch_bounds__(256) void k_copy(const double *__restrict__ a, double *__restrict__ b,
+                                              long long n) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
 // wave64 sum through DPP/permute lanes (deterministic order)
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

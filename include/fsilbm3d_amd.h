@@ -120,6 +120,11 @@ int fsil_ibm_interaction_force(int handle, double dt, const double *Exyz, const 
                                const double *Ea, const double *elem_xc, double *Eforce,
                                double *lodFlow, int *iters, double *dmax);
 
+/* Bandwidth of a plain device copy of nbytes (read + write), reps launches:
+ * the calibration for HBM counters and the achievable-bandwidth context of the
+ * roofline.  No reference counterpart (measurement helper). */
+int fsil_copy_bandwidth(long long nbytes, int reps, double *gbs);
+
 /* Band size and bounding box (x0,y0,z0,nx,ny,nz) of the last coupling call. */
 int fsil_ibm_band_info(int handle, int *nband, int *bbox);
 
