@@ -257,3 +257,99 @@ def lod_flow(Eforce: np.ndarray, Exyz: np.ndarray, marker_elem: np.ndarray, elem
         for k in range(3):
             lod[6 * n1 + 3 + k] = lod[6 * n1 + 3 + k] + 0.5 * Mm[k]
     return np.array(lod)
+
+
+class OracleBeam:
+    """A plate body on the C restatement of the beam solver (fsil_oracle_beam.c)."""
+
+    _sig_done = False
+
+    @classmethod
+    def _sigs(cls):
+        if cls._sig_done:
+            return
+        L = lib()
+        L.orc_beam_create.restype = C.c_void_p
+        L.orc_beam_create.argtypes = [C.c_int, C.c_int, _pd, _pi, _pi, C.c_int, _pi, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, _pd, _pd, _pd, _pd, _pd, _pd, _pd, C.c_double, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                      C.c_double, C.c_int]
+        L.orc_beam_free.argtypes = [C.c_void_p]
+        L.orc_beam_structure.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double]
+        L.orc_beam_structure.restype = C.c_int
+        L.orc_beam_markers.argtypes = [C.c_void_p, C.c_double, C.c_double, _pd, _pd, _pd]
+        L.orc_beam_element_centres.argtypes = [C.c_void_p, _pd]
+        L.orc_beam_nmarkers.argtypes = [C.c_void_p]
+        L.orc_beam_nmarkers.restype = C.c_int
+        L.orc_beam_set_lodflow.argtypes = [C.c_void_p, _pd]
+        L.orc_beam_state.argtypes = [C.c_void_p, _pd, _pd, _pd, _pd]
+        L.orc_beam_info.argtypes = [C.c_void_p, _pi, _pi, _pd]
+        L.orc_beam_asfac.argtypes = [C.c_void_p]
+        L.orc_beam_asfac.restype = C.c_double
+        L.orc_beam_marker_elem.argtypes = [C.c_void_p, _pi, _pi]
+        cls._sig_done = True
+
+    def __init__(self, nodes8, elements, constraints, *, iBodyModel=2, isMotionGiven=(1, 1, 1, 1, 1, 1), denR=1.0,
+                 KB=0.1, KS=1000.0, psR=0.3, Freq=0.0, XYZo=(0, 0, 0), XYZAmpl=(0, 0, 0), XYZPhi=(0, 0, 0),
+                 AoAo=(0, 0, 0), AoAAmpl=(0, 0, 0), AoAPhi=(0, 0, 0), initXYZVel=(0, 0, 0), Lref=1.0, Uref=1.0,
+                 denIn=1.0, dampK=0.0, dampM=0.0, GeoGamma=1.0, NewmarkGamma=0.5, NewmarkBeta=0.25, dtolFEM=1e-10,
+                 ntolFEM=20, IBPenaltyAlpha=1.0):
+        self._sigs()
+        nodes8 = np.ascontiguousarray(nodes8, dtype=np.float64)
+        el = np.ascontiguousarray(elements, dtype=np.int32)
+        cons = np.ascontiguousarray(constraints, dtype=np.int32)
+        v = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self.nND, self.nEL = len(nodes8), len(el)
+        self.h = lib().orc_beam_create(self.nND, self.nEL, _d(nodes8), _i(el), _i(cons), int(iBodyModel),
+                                       _i(np.array(isMotionGiven, dtype=np.int32)), denR, KB, KS, psR, Freq,
+                                       _d(v(XYZo)), _d(v(XYZAmpl)), _d(v(XYZPhi)), _d(v(AoAo)), _d(v(AoAAmpl)),
+                                       _d(v(AoAPhi)), _d(v(initXYZVel)), Lref, Uref, denIn, dampK, dampM, GeoGamma,
+                                       NewmarkGamma, NewmarkBeta, dtolFEM, int(ntolFEM))
+        self.iBodyModel = int(iBodyModel)
+        self.alpha, self.denIn = float(IBPenaltyAlpha), float(denIn)
+        self.M = lib().orc_beam_nmarkers(self.h)
+        self.marker_elem = np.zeros(self.M, np.int32)
+        self.elem_nodes = np.zeros((self.nEL, 2), np.int32)
+        lib().orc_beam_marker_elem(self.h, _i(self.marker_elem), _i(self.elem_nodes))
+        self.Asfac = lib().orc_beam_asfac(self.h)
+        self.v_move = int(sum(abs(x) for x in initXYZVel) > 1e-5 or sum(abs(x) for x in XYZAmpl) > 1e-5
+                          or sum(abs(x) for x in AoAAmpl) > 1e-5 or sum(isMotionGiven) < 6)
+        self.Exyz = np.zeros((self.M, 3))
+        self.Evel = np.zeros((self.M, 3))
+        self.Ea = np.zeros(self.M)
+        self.update_markers()
+
+    def __del__(self):
+        try:
+            lib().orc_beam_free(self.h)
+        except Exception:
+            pass
+
+    def update_markers(self):
+        lib().orc_beam_markers(self.h, self.alpha, self.denIn, _d(self.Exyz), _d(self.Evel), _d(self.Ea))
+
+    def element_centres(self):
+        xc = np.zeros((self.nEL, 3))
+        lib().orc_beam_element_centres(self.h, _d(xc))
+        return xc
+
+    def set_lodflow(self, lod):
+        lib().orc_beam_set_lodflow(self.h, _d(np.ascontiguousarray(lod, dtype=np.float64)))
+
+    def structure(self, time, isubstep, deltat, subdeltat):
+        rc = lib().orc_beam_structure(self.h, float(time), int(isubstep), float(deltat), float(subdeltat))
+        if rc:
+            raise RuntimeError("CG solver breakdown")
+
+    def state(self):
+        g = 6 * self.nND
+        out = [np.zeros(g) for _ in range(4)]
+        lib().orc_beam_state(self.h, *[_d(a) for a in out])
+        return [a.reshape(self.nND, 6) for a in out]
+
+    def info(self):
+        a = np.zeros(1, np.int32)
+        b = np.zeros(1, np.int32)
+        c = np.zeros(1)
+        lib().orc_beam_info(self.h, _i(a), _i(b), _d(c))
+        return int(a[0]), int(b[0]), float(c[0])

@@ -88,6 +88,12 @@ class Plate:
     AoAPhi: tuple = (0.0, 0.0, 0.0)
     Freq: float = 0.0
     initXYZVel: tuple = (0.0, 0.0, 0.0)
+    iBodyModel: int = 1
+    denR: float = 1.0
+    KB: float = 0.1
+    KS: float = 1000.0
+    psR: float = 0.3
+    isMotionGiven: tuple = (1, 1, 1, 1, 1, 1)
 
     def nodes(self):
         return plate_nodes(self.n_nodes, self.x_lo, self.lspan, self.rspan)
@@ -97,7 +103,8 @@ class Plate:
 
     def group(self) -> Group:
         txt = plate_text([row[:3] for row in self.nodes()], self.lspan, self.rspan, (0.0, 0.0, 1.0), self.nspan)
-        return Group(plate=txt, iBodyModel=1, firstXYZ=self.XYZo, XYZAmpl=self.XYZAmpl, XYZPhi=self.XYZPhi,
+        return Group(plate=txt, iBodyModel=self.iBodyModel, denR=self.denR, KB=self.KB, KS=self.KS, psR=self.psR,
+                     isMotionGiven=self.isMotionGiven, firstXYZ=self.XYZo, XYZAmpl=self.XYZAmpl, XYZPhi=self.XYZPhi,
                      AoAo=self.AoAo, AoAAmpl=self.AoAAmpl, AoAPhi=self.AoAPhi, freq=self.Freq,
                      initXYZVel=self.initXYZVel)
 
@@ -172,6 +179,17 @@ CASES: List[ParityCase] = [
                plates=[Plate(4, -1.5, 2.0, 2.0, 3, XYZo=(7.0, 6.0, 7.0), AoAo=(0.0, 0.0, 10.0), AoAAmpl=(0.0, 0.0, 15.0),
                              Freq=0.03, XYZAmpl=(0.0, 0.5, 0.0)),
                        Plate(4, -1.5, 1.5, 1.5, 3, XYZo=(15.0, 6.5, 7.0), initXYZVel=(0.01, 0.0, 0.0))]),
+    ParityCase("fem_clamped_heaving_plate", _open(dims=(20, 12, 12), bc=(101, 103, 302, 302, 302, 302), ntolLBM=3,
+                                                  dtolLBM=1e-14, numsubstep=2, uvwIn=(0.05, 0.0, 0.0)),
+               nsteps=3, seed=34, ubase=(0.05, 0, 0), neq=0.005,
+               plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.0, 0.0), Freq=0.03,
+                             iBodyModel=2, denR=1.0, KB=0.1, KS=1000.0)]),
+    ParityCase("fem_flexible_plate", _open(dims=(20, 12, 12), bc=(101, 103, 302, 302, 302, 302), ntolLBM=10,
+                                           dtolLBM=1e-6, numsubstep=4, uvwIn=(0.05, 0.0, 0.0), Lref=8.0, Re=20.0,
+                                           UrefType=0),
+               nsteps=5, seed=35, ubase=(0.05, 0, 0), neq=0.002,
+               plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.0, 0.0), Freq=0.01,
+                             iBodyModel=2, denR=1.0, KB=0.1, KS=1000.0)]),
 ]
 
 BY_NAME: Dict[str, ParityCase] = {c.name: c for c in CASES}

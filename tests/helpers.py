@@ -47,17 +47,36 @@ def flow_kwargs(case):
                 volumeForceFreq=case.volumeForceFreq, volumeForcePhi=case.volumeForcePhi)
 
 
+def oracle_bodies(pc, d):
+    """Plate bodies on the oracle's beam restatement (rigid or elastic)."""
+    from oracle.oracle import OracleBeam
+    case = pc.case
+    out = []
+    for p in pc.plates:
+        cons = np.zeros((p.n_nodes, 6), np.int32)
+        cons[0] = 1
+        els = np.array(p.elements(), np.int32)
+        out.append(OracleBeam(p.nodes(), els, cons, iBodyModel=p.iBodyModel, isMotionGiven=p.isMotionGiven,
+                              denR=p.denR, KB=p.KB, KS=p.KS, psR=p.psR, Freq=p.Freq, XYZo=p.XYZo, XYZAmpl=p.XYZAmpl,
+                              XYZPhi=p.XYZPhi, AoAo=p.AoAo, AoAAmpl=p.AoAAmpl, AoAPhi=p.AoAPhi,
+                              initXYZVel=p.initXYZVel, Lref=case.Lref, Uref=d["Uref"], denIn=case.denIn,
+                              dampK=case.dampK, dampM=case.dampM, GeoGamma=case.GeoGamma,
+                              NewmarkGamma=case.NewmarkGamma, NewmarkBeta=case.NewmarkBeta, dtolFEM=case.dtolFEM,
+                              ntolFEM=case.ntolFEM, IBPenaltyAlpha=case.IBPenaltyAlpha))
+    return out
+
+
 def run_oracle(pc, f0=None, nsteps=None, record=None):
     """Run a ParityCase on the C oracle; returns the final state (and the
     per-step IBM outputs when record is a list)."""
-    from oracle.oracle import OracleBlock
+    from oracle.oracle import OracleBlock, lod_flow
     case = pc.case
     d = derived(case)
     ob = OracleBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["Omega"], d["Omega2"], d["Mc"], d["Mf"],
                      **flow_kwargs(case))
     f = np.array(pc.initial_state() if f0 is None else f0, dtype=np.float64, copy=True)
     ob.init_boundary(f, 0.0)
-    bodies = make_bodies(pc, case.denIn)
+    bodies = oracle_bodies(pc, d)
     ibm = dict(ntol=case.ntolLBM, dtol=case.dtolLBM, Uref=d["Uref"], denIn=case.denIn)
     n = pc.nsteps if nsteps is None else nsteps
     time = 0.0
@@ -66,21 +85,26 @@ def run_oracle(pc, f0=None, nsteps=None, record=None):
         bl = None
         if bodies:
             for b in bodies:
-                if b.v_move == 1:
+                if b.v_move == 1 or b.iBodyModel == 2:
                     b.update_markers()
             bl = [dict(Exyz=b.Exyz.copy(), Evel=b.Evel.copy(), Ea=b.Ea.copy()) for b in bodies]
             xcs = [b.element_centres() for b in bodies]
         out = ob.step(f, time, bl, ibm)
         if bodies:
+            lods = [lod_flow(E, bb["Exyz"], b.marker_elem, b.elem_nodes, xc, b.nND)
+                    for E, bb, b, xc in zip(out["Eforce"], bl, bodies, xcs)]
+            for b, lod in zip(bodies, lods):
+                b.set_lodflow(lod)
             if record is not None:
-                from oracle.oracle import lod_flow
-                lods = [lod_flow(E, bb["Exyz"], b.marker_elem, b.elem_nodes, xc, b.nND)
-                        for E, bb, b, xc in zip(out["Eforce"], bl, bodies, xcs)]
-                record.append(dict(iters=out["iters"], Eforce=out["Eforce"], lod=lods))
+                record.append(dict(iters=out["iters"], Eforce=out["Eforce"], lod=lods,
+                                   Exyz=[bb["Exyz"] for bb in bl]))
             dts = case.dh / float(case.numsubstep)
             for isub in range(1, case.numsubstep + 1):
                 for b in bodies:
                     b.structure(time, isub, case.dh, dts)
+            if record is not None:
+                record[-1]["beam"] = [b.state() for b in bodies]
+                record[-1]["fem_info"] = [b.info() for b in bodies]
     return f
 
 
