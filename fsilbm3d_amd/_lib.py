@@ -41,6 +41,12 @@ SIGNATURES = {
     "fsil_ibm_interaction_force": (_i, [_i, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
     "fsil_ibm_band_info": (_i, [_i, _pi, _pi]),
     "fsil_copy_bandwidth": (_i, [C.c_longlong, _i, _pd]),
+    "fsil_fem_define_bodies": (_i, [_i, _i, _pi, _pi, _pd, _pi, _pi, _pi, _pi, _pd, _pd, _d, _d, _d, _d]),
+    "fsil_fem_set_solver": (_i, [_i, _d, _d, _d, _d, _d, _d, _i]),
+    "fsil_fem_update_markers": (_i, [_i]),
+    "fsil_fem_substeps": (_i, [_i, _d, _i, _d]),
+    "fsil_fem_state": (_i, [_i, _i, _pd, _pd, _pd, _pd]),
+    "fsil_fem_info": (_i, [_i, _i, _pi, _pi, _pd]),
 }
 
 _lib = None

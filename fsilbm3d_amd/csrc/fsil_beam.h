@@ -1,0 +1,458 @@
+// Co-rotational 3-D Timoshenko beam (plate bodies) for the device FEM, FP64.
+//
+// Element routines are __host__ __device__: the host runs the body set-up
+// (material, triads, rotated mass matrix) with the C library's libm, the
+// device runs the Newmark / Newton / block-Jacobi PCG sub-steps.  Every
+// expression keeps the association order of gaoak/FSILBM3D
+// src/SolidSolver.f90 (routine citations at each function).  Integer powers
+// x**n are evaluated by binary exponentiation, as the reference binary does.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace fsil {
+
+constexpr int NED = 12;  // element degrees of freedom
+
+struct BeamElem {
+  int node0, node1, Nspan, pad;
+  int bc[NED];
+  double x00[NED], x0[NED], x1[NED], xnxt[NED];
+  double dx0, dy0, dz0, len0, dx1, dy1, dz1, len1;
+  double xll0, xmm0, xnn0, xll1, xmm1, xnn1;
+  double Lspan, spanlen, area00, geoFRM;
+  double dirc00[3], dirc0[3];
+  double tee[9], tn1[9], tn2[9];  // [row*3 + col]
+  double prop[8];
+  double mas[NED * NED], stf[NED * NED], geo[NED * NED], coef[NED * NED];
+  double rot[9];
+  double fint[NED], fload[NED], fdamp[NED];  // per-element contributions (internal force, loads)
+};
+
+__host__ __device__ inline double bpow(double x, int n) {
+  double res = 0.0, cur = x;
+  bool have = false;
+  while (n) {
+    if (n & 1) {
+      if (have) res = res * cur;
+      else { res = cur; have = true; }
+    }
+    n >>= 1;
+    if (n) cur = cur * cur;
+  }
+  return have ? res : 1.0;
+}
+
+__host__ __device__ inline double bdot3(const double *a, const double *b) {
+  double s = 0.0;
+  for (int k = 0; k < 3; ++k) s = s + a[k] * b[k];
+  return s;
+}
+
+// C = A*B (3x3, k ascending); C may alias A or B
+__host__ __device__ inline void bmm3(const double *A, const double *B, double *C) {
+  double T[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s = s + A[i * 3 + k] * B[k * 3 + j];
+      T[i * 3 + j] = s;
+    }
+  for (int k = 0; k < 9; ++k) C[k] = T[k];
+}
+
+__host__ __device__ inline void bmv3(const double *A, const double *v, double *out) {
+  double t[3];
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) s = s + A[i * 3 + k] * v[k];
+    t[i] = s;
+  }
+  out[0] = t[0]; out[1] = t[1]; out[2] = t[2];
+}
+
+// get_angle_triad (SolidSolver.f90:909-952)
+__host__ __device__ inline void angle_triad(const double *t11, const double *t22, double *t) {
+  double rr[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s = s + t22[i * 3 + k] * t11[j * 3 + k];
+      rr[i * 3 + j] = s;
+    }
+  const double dtx = (rr[7] - rr[5]) / 2.0;
+  const double dty = (rr[2] - rr[6]) / 2.0;
+  const double dtz = (rr[3] - rr[1]) / 2.0;
+  double tr = rr[0] + rr[4] + rr[8];
+  tr = (tr - 1.0) / 2.0;
+  if (tr > 1.0) tr = 1.0;
+  if (tr < -1.0) tr = -1.0;
+  const double sint = sqrt(dtx * dtx + dty * dty + dtz * dtz);
+  const double theta = acos(tr);
+  if (sint < 1.0e-10 || theta < 1.0e-10) {
+    t[0] = dtx; t[1] = dty; t[2] = dtz;
+  } else {
+    const double f = theta / sint;
+    t[0] = f * dtx; t[1] = f * dty; t[2] = f * dtz;
+  }
+}
+
+// AoAtoTTT with its pi constant (SolidSolver.f90:2404-2472)
+__host__ __device__ inline void aoa_to_ttt(const double *aoa, double *T) {
+  const double pi_ = 3.141562653589793;
+  double R[3][9];
+  for (int a = 0; a < 3; ++a) {
+    double c = cos(aoa[a]), s = sin(aoa[a]);
+    if (fabs(aoa[a]) < 1.0e-5) { c = 1.0; s = 0.0; }
+    if (fabs(aoa[a] - 0.5 * pi_) < 1.0e-5) { c = 0.0; s = 1.0; }
+    if (fabs(aoa[a] + 0.5 * pi_) < 1.0e-5) { c = 0.0; s = -1.0; }
+    for (int k = 0; k < 9; ++k) R[a][k] = 0.0;
+    if (a == 0) { R[0][0] = 1.0; R[0][4] = c; R[0][7] = s; R[0][5] = -s; R[0][8] = c; }
+    else if (a == 1) { R[1][0] = c; R[1][6] = -s; R[1][4] = 1.0; R[1][2] = s; R[1][8] = c; }
+    else { R[2][0] = c; R[2][3] = s; R[2][1] = -s; R[2][4] = c; R[2][8] = 1.0; }
+  }
+  const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  bmm3(R[2], I, T);
+  bmm3(R[1], T, T);
+  bmm3(R[0], T, T);
+}
+
+// Segment_FiniteRot (SolidSolver.f90:1066-1124)
+__host__ __device__ inline void finite_rot(double t1, double t2, double t3, double *rr) {
+  const double tt = sqrt(bpow(t1, 2) + bpow(t2, 2) + bpow(t3, 2));
+  const double ss = sin(tt), cc = cos(tt);
+  const double r2[9] = {0.0, -t3, t2, t3, 0.0, -t1, -t2, t1, 0.0};
+  double r3[9];
+  r3[0] = -t3 * t3 - t2 * t2; r3[1] = t2 * t1; r3[2] = t3 * t1;
+  r3[3] = t1 * t2; r3[4] = -t3 * t3 - t1 * t1; r3[5] = t3 * t2;
+  r3[6] = t1 * t3; r3[7] = t2 * t3; r3[8] = -t2 * t2 - t1 * t1;
+  double c1, c2;
+  if (tt < 1.0e-10) { c1 = 1.0; c2 = 0.5; }
+  else { c1 = ss / tt; c2 = (1.0 - cc) / bpow(tt, 2); }
+  for (int k = 0; k < 9; ++k) {
+    const double r1 = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+    rr[k] = r1 + r2[k] * c1 + r3[k] * c2;
+  }
+}
+
+// Segment_BuildAxisDirTriad (SolidSolver.f90:624-697)
+__host__ __device__ inline void axis_dir_triad(double l, double m, double n, const double *d, double *triad) {
+  double ex[3] = {l, m, n}, ey[3], ez[3], dir[3] = {d[0], d[1], d[2]};
+  double dd = sqrt(bdot3(ex, ex));
+  if (dd > 1.0e-14) for (int k = 0; k < 3; ++k) ex[k] = ex[k] / dd;
+  dd = sqrt(bdot3(dir, dir));
+  if (dd > 1.0e-14) for (int k = 0; k < 3; ++k) dir[k] = dir[k] / dd;
+  const double proj = bdot3(dir, ex);
+  for (int k = 0; k < 3; ++k) ey[k] = dir[k] - proj * ex[k];
+  dd = sqrt(bdot3(ey, ey));
+  if (dd <= 1.0e-10) {
+    if (fabs(ex[2]) > 0.995) {
+      ey[0] = 0.0; ey[1] = 1.0; ey[2] = 0.0;
+      ez[0] = -ex[2]; ez[1] = 0.0; ez[2] = 0.0;
+    } else {
+      dd = sqrt(ex[0] * ex[0] + ex[1] * ex[1]);
+      ey[0] = -ex[1] / dd; ey[1] = ex[0] / dd; ey[2] = 0.0;
+      ez[0] = -ex[0] * ex[2] / dd; ez[1] = -ex[1] * ex[2] / dd; ez[2] = dd;
+    }
+  } else {
+    for (int k = 0; k < 3; ++k) ey[k] = ey[k] / dd;
+    ez[0] = ex[1] * ey[2] - ex[2] * ey[1];
+    ez[1] = ex[2] * ey[0] - ex[0] * ey[2];
+    ez[2] = ex[0] * ey[1] - ex[1] * ey[0];
+    dd = sqrt(bdot3(ez, ez));
+    if (dd > 1.0e-14) for (int k = 0; k < 3; ++k) ez[k] = ez[k] / dd;
+  }
+  for (int k = 0; k < 3; ++k) {
+    triad[k * 3 + 0] = ex[k];
+    triad[k * 3 + 1] = ey[k];
+    triad[k * 3 + 2] = ez[k];
+  }
+}
+
+// Segment_cptdxyz1 (SolidSolver.f90:119-129)
+__host__ __device__ inline void el_cptdxyz1(BeamElem &e) {
+  e.dx1 = e.x1[6] - e.x1[0];
+  e.dy1 = e.x1[7] - e.x1[1];
+  e.dz1 = e.x1[8] - e.x1[2];
+  e.len1 = sqrt(e.dx1 * e.dx1 + e.dy1 * e.dy1 + e.dz1 * e.dz1);
+  e.xll1 = e.dx1 / e.len1;
+  e.xmm1 = e.dy1 / e.len1;
+  e.xnn1 = e.dz1 / e.len1;
+}
+
+// Segment_MapReferencePosToCurrent (SolidSolver.f90:1126-1137)
+__host__ __device__ inline void el_map_pos(const BeamElem &e, double *out, const double *T, const double *XYZ,
+                                           const double *AoA) {
+  double a[3], b[3];
+  bmv3(T, e.x00, a);
+  bmv3(T, e.x00 + 6, b);
+  for (int k = 0; k < 3; ++k) {
+    out[k] = a[k] + XYZ[k];
+    out[3 + k] = AoA[k];
+    out[6 + k] = b[k] + XYZ[k];
+    out[9 + k] = AoA[k];
+  }
+}
+
+// Segment_FormMassMatrix (SolidSolver.f90:318-363)
+__host__ __device__ inline void el_form_mass(BeamElem &e) {
+  const double area = e.prop[2], rho = e.prop[3], ziy = e.prop[6], ziz = e.prop[7];
+  for (int k = 0; k < NED * NED; ++k) e.mas[k] = 0.0;
+  const double roal = rho * area * e.len0 / 2.0;
+  e.mas[0 * NED + 0] = roal; e.mas[1 * NED + 1] = roal; e.mas[2 * NED + 2] = roal;
+  e.mas[3 * NED + 3] = roal * (ziy + ziz) / area;
+  e.mas[4 * NED + 4] = roal * ziy / area;
+  e.mas[5 * NED + 5] = roal * ziz / area;
+  for (int k = 0; k < 6; ++k) e.mas[(6 + k) * NED + 6 + k] = e.mas[k * NED + k];
+}
+
+// Segment_FormStiffMatrix (SolidSolver.f90:365-478)
+__host__ __device__ inline void el_form_stiff(BeamElem &e) {
+  const double emod = e.prop[0], gmod = e.prop[1], area = e.prop[2], zix = e.prop[5], ziy = e.prop[6],
+               ziz = e.prop[7];
+  const double L = e.len0;
+  double *K = e.stf;
+  for (int k = 0; k < NED * NED; ++k) K[k] = 0.0;
+  const double Inv = 1.0 / L;
+  const double ksy = 5.0 / 6.0, ksz = 5.0 / 6.0;
+  const double phiy = 12.0 * emod * ziz / (ksy * gmod * area * L * L);
+  const double phiz = 12.0 * emod * ziy / (ksz * gmod * area * L * L);
+  const double ky1 = 12.0 * emod * ziz / (bpow(L, 3) * (1.0 + phiy));
+  const double ky2 = 6.0 * emod * ziz / (bpow(L, 2) * (1.0 + phiy));
+  const double ky3 = (4.0 + phiy) * emod * ziz / (L * (1.0 + phiy));
+  const double ky4 = (2.0 - phiy) * emod * ziz / (L * (1.0 + phiy));
+  const double kz1 = 12.0 * emod * ziy / (bpow(L, 3) * (1.0 + phiz));
+  const double kz2 = 6.0 * emod * ziy / (bpow(L, 2) * (1.0 + phiz));
+  const double kz3 = (4.0 + phiz) * emod * ziy / (L * (1.0 + phiz));
+  const double kz4 = (2.0 - phiz) * emod * ziy / (L * (1.0 + phiz));
+#define KK(i, j) K[(i) * NED + (j)]
+  KK(0, 0) = area * emod * Inv; KK(1, 1) = ky1; KK(2, 2) = kz1; KK(3, 3) = gmod * zix * Inv; KK(4, 4) = kz3;
+  KK(5, 5) = ky3;
+  for (int k = 0; k < 6; ++k) KK(6 + k, 6 + k) = KK(k, k);
+  KK(0, 6) = -KK(0, 0);
+  KK(1, 5) = ky2; KK(1, 7) = -ky1; KK(1, 11) = ky2; KK(5, 7) = -ky2; KK(5, 11) = ky4; KK(7, 11) = -ky2;
+  KK(2, 4) = -kz2; KK(2, 8) = -kz1; KK(2, 10) = -kz2; KK(4, 8) = kz2; KK(4, 10) = kz4; KK(8, 10) = kz2;
+  KK(3, 9) = -KK(3, 3);
+  KK(6, 0) = KK(0, 6);
+  KK(5, 1) = KK(1, 5); KK(7, 1) = KK(1, 7); KK(11, 1) = KK(1, 11); KK(7, 5) = KK(5, 7); KK(11, 5) = KK(5, 11);
+  KK(11, 7) = KK(7, 11);
+  KK(4, 2) = KK(2, 4); KK(8, 2) = KK(2, 8); KK(10, 2) = KK(2, 10); KK(8, 4) = KK(4, 8); KK(10, 4) = KK(4, 10);
+  KK(10, 8) = KK(8, 10);
+  KK(9, 3) = KK(3, 9);
+#undef KK
+}
+
+// Segment_FormGeomMatrix (SolidSolver.f90:480-600)
+__host__ __device__ inline void el_form_geom(BeamElem &e) {
+  const double s = e.geoFRM;
+  const double emod = e.prop[0], gmod = e.prop[1], area = e.prop[2], zix = e.prop[5], ziy = e.prop[6],
+               ziz = e.prop[7];
+  const double L = e.len0;
+  double *G = e.geo;
+  for (int k = 0; k < NED * NED; ++k) G[k] = 0.0;
+  const double ksy = 5.0 / 6.0, ksz = 5.0 / 6.0;
+  const double phiy = 12.0 * emod * ziz / (ksy * gmod * area * L * L);
+  const double phiz = 12.0 * emod * ziy / (ksz * gmod * area * L * L);
+  const double py2 = bpow(1.0 + phiy, 2), pz2 = bpow(1.0 + phiz, 2);
+  const double gy1 = s / L * (6.0 / 5.0 + 2.0 * phiy + phiy * phiy) / py2;
+  const double gy2 = s / L * (L / 10.0) / py2;
+  const double gy3 = s / L * (2.0 * L * L / 15.0 + phiy * L * L / 6.0 + phiy * phiy * L * L / 12.0) / py2;
+  const double gy4 = s / L * (-(L * L / 30.0) - phiy * L * L / 6.0 - phiy * phiy * L * L / 12.0) / py2;
+#define GG(i, j) G[(i) * NED + (j)]
+  GG(1, 1) = gy1; GG(5, 5) = gy3; GG(7, 7) = gy1; GG(11, 11) = gy3;
+  GG(1, 5) = gy2; GG(1, 7) = -gy1; GG(1, 11) = gy2; GG(5, 7) = -gy2; GG(5, 11) = gy4; GG(7, 11) = -gy2;
+  const double gz1 = s / L * (6.0 / 5.0 + 2.0 * phiz + phiz * phiz) / pz2;
+  const double gz2 = s / L * (L / 10.0) / pz2;
+  const double gz3 = s / L * (2.0 * L * L / 15.0 + phiz * L * L / 6.0 + phiz * phiz * L * L / 12.0) / pz2;
+  const double gz4 = s / L * (-(L * L / 30.0) - phiz * L * L / 6.0 - phiz * phiz * L * L / 12.0) / pz2;
+  GG(2, 2) = gz1; GG(4, 4) = gz3; GG(8, 8) = gz1; GG(10, 10) = gz3;
+  GG(2, 4) = -gz2; GG(2, 8) = -gz1; GG(2, 10) = -gz2; GG(4, 8) = gz2; GG(4, 10) = gz4; GG(8, 10) = gz2;
+  const double gt = s * zix / (area * L);
+  GG(3, 3) = gt; GG(3, 9) = -gt; GG(9, 9) = gt;
+  GG(5, 1) = GG(1, 5); GG(7, 1) = GG(1, 7); GG(11, 1) = GG(1, 11); GG(7, 5) = GG(5, 7); GG(11, 5) = GG(5, 11);
+  GG(11, 7) = GG(7, 11);
+  GG(4, 2) = GG(2, 4); GG(8, 2) = GG(2, 8); GG(10, 2) = GG(2, 10); GG(8, 4) = GG(4, 8); GG(10, 4) = GG(4, 10);
+  GG(10, 8) = GG(8, 10);
+  GG(9, 3) = GG(3, 9);
+#undef GG
+}
+
+// RotateMatrix + RKR (SolidSolver.f90:699-748): ek <- R^T ek R by 3x3 blocks
+__host__ __device__ inline void el_rotate(BeamElem &e) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) e.rot[i * 3 + j] = e.tee[j * 3 + i];
+}
+__host__ __device__ inline void el_rkr(const BeamElem &e, double *ek) {
+  const double *r = e.rot;
+  double rt[9];
+  for (int in = 0; in < 3; ++in)
+    for (int jn = 0; jn < 3; ++jn) rt[jn * 3 + in] = r[in * 3 + jn];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      const int j1 = i * 3, j2 = j * 3;
+      double kt[9];
+      for (int k = 0; k < 3; ++k)
+        for (int ii = 0; ii < 3; ++ii) {
+          double s = 0.0;
+          for (int jj = 0; jj < 3; ++jj) s = s + ek[(j1 + k) * NED + j2 + jj] * r[jj * 3 + ii];
+          kt[k * 3 + ii] = s;
+        }
+      for (int k = 0; k < 3; ++k)
+        for (int ii = 0; ii < 3; ++ii) {
+          double s = 0.0;
+          for (int jj = 0; jj < 3; ++jj) s = s + rt[k * 3 + jj] * kt[jj * 3 + ii];
+          ek[(j1 + k) * NED + j2 + ii] = s;
+        }
+    }
+}
+
+// Segment_BodyStress_D (SolidSolver.f90:750-839): local internal force -> e.fint
+__host__ __device__ inline void el_body_stress(BeamElem &e) {
+  const double du = e.dx1 - e.dx0, dv = e.dy1 - e.dy0, dw = e.dz1 - e.dz0;
+  const double dl = ((e.dx0 + e.dx1) * du + (e.dy0 + e.dy1) * dv + (e.dz0 + e.dz1) * dw) / (e.len0 + e.len1);
+  double t[3], a[3], b[3];
+  const double *T = e.tee;
+  angle_triad(e.tee, e.tn1, t);
+  a[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
+  a[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
+  a[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
+  angle_triad(e.tee, e.tn2, t);
+  b[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
+  b[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
+  b[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
+  const double ub[NED] = {0.0, 0.0, 0.0, a[0], a[1], a[2], dl, 0.0, 0.0, b[0], b[1], b[2]};
+  e.geoFRM = dl * e.prop[0] * e.prop[2] / e.len0;
+  double fb[NED];
+  for (int i = 0; i < NED; ++i) {
+    double s = 0.0;
+    for (int k = 0; k < NED; ++k) s = s + e.stf[i * NED + k] * ub[k];
+    fb[i] = s;
+  }
+  for (int i = 0; i < NED; ++i) e.fint[i] = 0.0;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      e.fint[0 + i] = e.fint[0 + i] + T[i * 3 + j] * fb[0 + j];
+      e.fint[3 + i] = e.fint[3 + i] + T[i * 3 + j] * fb[3 + j];
+      e.fint[6 + i] = e.fint[6 + i] + T[i * 3 + j] * fb[6 + j];
+      e.fint[9 + i] = e.fint[9 + i] + T[i * 3 + j] * fb[9 + j];
+    }
+}
+
+// Segment_UpdateMatrix (SolidSolver.f90:131-157)
+__host__ __device__ inline void el_update_matrix(BeamElem &e, const double *c, double gamma, double dampM,
+                                                 double dampK) {
+  el_form_geom(e);
+  el_rotate(e);
+  el_rkr(e, e.stf);
+  el_rkr(e, e.geo);
+  for (int k = 0; k < NED * NED; ++k) {
+    const double tanv = e.stf[k] + gamma * e.geo[k];
+    e.coef[k] = tanv + c[0] * e.mas[k] + c[1] * dampM * e.mas[k];
+  }
+  if (dampK > 0.0)
+    for (int k = 0; k < NED * NED; ++k) e.coef[k] = e.coef[k] + c[1] * dampK * e.stf[k];
+}
+
+// Segment_UpdateLoad (SolidSolver.f90:159-193) -> e.fload (mass load, then the
+// stiffness-damping load added in the same order as two LocToGlobal calls)
+__host__ __device__ inline void el_update_load(BeamElem &e, const double *c, double dampM, double dampK,
+                                               const double *dspO, const double *dsp, const double *vel,
+                                               const double *acc) {
+  double qM[NED], qC[NED], qMC[NED];
+  const int n0 = e.node0, n1 = e.node1;
+  for (int i = 0; i < 6; ++i) {
+    qM[i] = c[0] * (dspO[6 * n0 + i] - dsp[6 * n0 + i]) + c[2] * vel[6 * n0 + i] + c[3] * acc[6 * n0 + i];
+    qM[i + 6] = c[0] * (dspO[6 * n1 + i] - dsp[6 * n1 + i]) + c[2] * vel[6 * n1 + i] + c[3] * acc[6 * n1 + i];
+    qC[i] = c[1] * (dspO[6 * n0 + i] - dsp[6 * n0 + i]) + c[4] * vel[6 * n0 + i] + c[5] * acc[6 * n0 + i];
+    qC[i + 6] = c[1] * (dspO[6 * n1 + i] - dsp[6 * n1 + i]) + c[4] * vel[6 * n1 + i] + c[5] * acc[6 * n1 + i];
+  }
+  for (int i = 0; i < NED; ++i) qMC[i] = qM[i] + dampM * qC[i];
+  for (int i = 0; i < NED; ++i) e.fload[i] = 0.0;
+  for (int i = 0; i < 3; ++i) {
+    e.fload[i] = e.mas[i * NED + i] * qMC[i];
+    e.fload[i + 6] = e.mas[(i + 6) * NED + i + 6] * qMC[i + 6];
+  }
+  for (int i = 0; i < 3; ++i) {
+    double s = 0.0, t = 0.0;
+    for (int k = 0; k < 3; ++k) s = s + e.mas[(3 + i) * NED + 3 + k] * qMC[3 + k];
+    for (int k = 0; k < 3; ++k) t = t + e.mas[(9 + i) * NED + 9 + k] * qMC[9 + k];
+    e.fload[3 + i] = s;
+    e.fload[9 + i] = t;
+  }
+  if (dampK > 0.0)
+    for (int i = 0; i < NED; ++i) {
+      double s = 0.0;
+      for (int k = 0; k < NED; ++k) s = s + e.stf[i * NED + k] * qC[k];
+      e.fdamp[i] = dampK * s;
+    }
+}
+
+// Segment_UpdateTriad_D + Segment_MakeTriad_ee (SolidSolver.f90:966-1064)
+__host__ __device__ inline void el_update_triads(BeamElem &e, const double *dspnn) {
+  double rr[9];
+  finite_rot(dspnn[6 * e.node0 + 3], dspnn[6 * e.node0 + 4], dspnn[6 * e.node0 + 5], rr);
+  bmm3(rr, e.tn1, e.tn1);
+  finite_rot(dspnn[6 * e.node1 + 3], dspnn[6 * e.node1 + 4], dspnn[6 * e.node1 + 5], rr);
+  bmm3(rr, e.tn2, e.tn2);
+  // MakeTriad_ee
+  double *E = e.tee;
+  for (int k = 0; k < 9; ++k) E[k] = 0.0;
+  E[0] = e.xll1; E[3] = e.xmm1; E[6] = e.xnn1;
+  double t[3], aa[9];
+  angle_triad(e.tn1, e.tn2, t);
+  finite_rot(t[0] / 2.0, t[1] / 2.0, t[2] / 2.0, rr);
+  bmm3(rr, e.tn1, aa);
+  double r2e1 = 0.0, r3e1 = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    r2e1 = r2e1 + aa[k * 3 + 1] * E[k * 3 + 0];
+    r3e1 = r3e1 + aa[k * 3 + 2] * E[k * 3 + 0];
+  }
+  for (int j = 0; j < 3; ++j) {
+    E[j * 3 + 1] = aa[j * 3 + 1] - r2e1 * (aa[j * 3 + 0] + E[j * 3 + 0]) / 2.0;
+    E[j * 3 + 2] = aa[j * 3 + 2] - r3e1 * (aa[j * 3 + 0] + E[j * 3 + 0]) / 2.0;
+  }
+  const double c1[3] = {E[0], E[3], E[6]};
+  double c2[3] = {E[1], E[4], E[7]};
+  const double p = bdot3(c2, c1);
+  for (int k = 0; k < 3; ++k) E[k * 3 + 1] = E[k * 3 + 1] - p * E[k * 3 + 0];
+  for (int k = 0; k < 3; ++k) c2[k] = E[k * 3 + 1];
+  const double dd = sqrt(bdot3(c2, c2));
+  if (dd > 1.0e-14)
+    for (int k = 0; k < 3; ++k) E[k * 3 + 1] = E[k * 3 + 1] / dd;
+  const double a0 = E[3] * E[7] - E[6] * E[4];
+  const double a1 = E[6] * E[1] - E[0] * E[7];
+  const double a2 = E[0] * E[4] - E[3] * E[1];
+  E[2] = a0; E[5] = a1; E[8] = a2;
+}
+
+// Invert6x6 (SolidSolver.f90:2234-2279), Gauss-Jordan with partial pivoting
+__host__ __device__ inline int invert6(const double *A, double *Ai) {
+  double aug[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) { aug[i][j] = A[i * 6 + j]; aug[i][6 + j] = 0.0; }
+  for (int i = 0; i < 6; ++i) aug[i][6 + i] = 1.0;
+  for (int i = 0; i < 6; ++i) {
+    int p = i;
+    double pivot = fabs(aug[i][i]);
+    for (int k = i + 1; k < 6; ++k)
+      if (fabs(aug[k][i]) > pivot) { pivot = fabs(aug[k][i]); p = k; }
+    if (pivot <= 1.0e-30) {
+      for (int k = 0; k < 36; ++k) Ai[k] = 0.0;
+      return i + 1;
+    }
+    if (p != i)
+      for (int j = 0; j < 12; ++j) { const double t = aug[i][j]; aug[i][j] = aug[p][j]; aug[p][j] = t; }
+    pivot = aug[i][i];
+    for (int j = 0; j < 12; ++j) aug[i][j] = aug[i][j] / pivot;
+    for (int k = 0; k < 6; ++k)
+      if (k != i) {
+        const double factor = aug[k][i];
+        for (int j = 0; j < 12; ++j) aug[k][j] = aug[k][j] - factor * aug[i][j];
+      }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) Ai[i * 6 + j] = aug[i][6 + j];
+  return 0;
+}
+
+}  // namespace fsil

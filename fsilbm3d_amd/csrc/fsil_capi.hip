@@ -19,6 +19,7 @@
 #include "fsil_ibm_kernels.h"
 #include "fsil_lbm_kernels.h"
 #include "fsil_lbm32_kernels.h"
+#include "fsil_fem_kernels.h"
 
 using namespace fsil;
 
@@ -87,8 +88,31 @@ struct Ibm {
   DevBuf<int> dbidx, dptr, dentry, dbmap, dmarker_elem, dof_ptr, dof_list, dof_kind;
   DevBuf<long long> dcell;
   std::vector<double> h_tolpart;
+  std::vector<double> hExyz;                 // host copy of device-computed markers
   int ndof = 0;
   bool active = false;  // band valid for the next collision
+};
+
+// Device FEM state of the bodies carried by a block.
+struct FemBodyHost {
+  int iBodyModel = 1;
+  double Freq = 0, XYZo[3] = {0, 0, 0}, XYZAmpl[3] = {0, 0, 0}, XYZPhi[3] = {0, 0, 0}, AoAo[3] = {0, 0, 0},
+         AoAAmpl[3] = {0, 0, 0}, AoAPhi[3] = {0, 0, 0}, initXYZVel[3] = {0, 0, 0};
+  double TTT0[9];
+};
+
+struct Fem {
+  int nbodies = 0, nEL = 0, nND = 0, gEQ = 0;
+  std::vector<FemBodyHost> bh;
+  std::vector<FemBodyDesc> desc;
+  double dampK = 0, dampM = 0, GeoGamma = 1, NewmarkGamma = 0.5, NewmarkBeta = 0.25, dtolFEM = 1e-10;
+  int ntolFEM = 20;
+  DevBuf<FemBodyDesc> ddesc;
+  DevBuf<BeamElem> el;
+  DevBuf<double> pos, dsp, vel, acc, dspO, velO, accO, lodInte, lodEffe, lodExte, vBC, x, r, p, Ap, z, xF, lbuf, blk,
+      beta;
+  DevBuf<int> fixed, dof_ptr, dof_e, dof_i, node_ptr, node_e, node_h, body_of_elem, elem_first_marker, body_goff;
+  bool markers_on_device = false;
 };
 
 struct Block {
@@ -113,6 +137,7 @@ struct Block {
   DevBuf<double> nb_den[6], nb_u[6], hw[6];
   bool hw_valid[6] = {false, false, false, false, false, false};
   Ibm ibm;
+  Fem fem;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -428,6 +453,149 @@ int upload(DevBuf<T> &d, const T *h, size_t n, hipStream_t s) {
   HIPCHK(d.ensure(n));
   if (n) HIPCHK(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
   return 0;
+}
+
+constexpr double kMPi = 3.141592653589793;  // m_pi (SolidSolver.f90:1261)
+
+// Beam_Initialise + calculate_angle_material (isKB = 1) + InitTriadANDFormMass
+// for one body, on the host (SolidSolver.f90:60-117, 1401-1510, 1533-1616)
+void fem_init_body(const double *xyz8, int nND, int nEL, const int *elems, const int *cons, int iBodyModel,
+                   const int *isMotionGiven, const double *mat, const double *kin, double Lref, double Uref,
+                   double denIn, FemBodyHost &bh, std::vector<BeamElem> &els, std::vector<double> &pos,
+                   std::vector<double> &vel) {
+  const double denR = mat[0], KB = mat[1], KS = mat[2], psR = mat[3];
+  bh.iBodyModel = iBodyModel;
+  bh.Freq = kin[0];
+  for (int k = 0; k < 3; ++k) {
+    bh.XYZo[k] = kin[1 + k];
+    bh.XYZAmpl[k] = kin[4 + k];
+    bh.XYZPhi[k] = kin[7 + k] / 180.0 * kMPi;
+    bh.AoAo[k] = kin[10 + k] / 180.0 * kMPi;
+    bh.AoAAmpl[k] = kin[13 + k] / 180.0 * kMPi;
+    bh.AoAPhi[k] = kin[16 + k] / 180.0 * kMPi;
+    bh.initXYZVel[k] = kin[19 + k];
+  }
+  els.assign(nEL, BeamElem{});
+  for (int n = 0; n < nEL; ++n) {
+    BeamElem &e = els[n];
+    const int p0 = elems[3 * n] - 1, p1 = elems[3 * n + 1] - 1;
+    e.node0 = p0; e.node1 = p1; e.Nspan = elems[3 * n + 2];
+    for (int k = 0; k < 3; ++k) {
+      e.x00[k] = xyz8[8 * p0 + k];
+      e.x00[6 + k] = xyz8[8 * p1 + k];
+      e.x00[3 + k] = 0.0;
+      e.x00[9 + k] = 0.0;
+    }
+    e.area00 = std::sqrt(bpow(e.x00[6] - e.x00[0], 2) + bpow(e.x00[7] - e.x00[1], 2) + bpow(e.x00[8] - e.x00[2], 2));
+    for (int k = 0; k < 6; ++k) { e.bc[k] = cons[6 * p0 + k]; e.bc[6 + k] = cons[6 * p1 + k]; }
+    e.Lspan = 0.5 * (xyz8[8 * p0 + 3] + xyz8[8 * p1 + 3]);
+    e.spanlen = 0.5 * (xyz8[8 * p0 + 4] + xyz8[8 * p1 + 4]) + e.Lspan;
+    for (int k = 0; k < 3; ++k) e.dirc00[k] = 0.5 * (xyz8[8 * p0 + 5 + k] + xyz8[8 * p1 + 5 + k]);
+    double s = 0.0;
+    for (int k = 0; k < 3; ++k) s = s + bpow(e.dirc00[k], 2);
+    const double nrm = std::sqrt(s);
+    for (int k = 0; k < 3; ++k) e.dirc00[k] = e.dirc00[k] / nrm;
+    if (e.bc[0] == 1) for (int k = 0; k < 6; ++k) e.bc[k] = isMotionGiven[k];
+    if (e.bc[6] == 1) for (int k = 0; k < 6; ++k) e.bc[6 + k] = isMotionGiven[k];
+    const double len = e.spanlen;
+    const double th = std::sqrt(KB / KS * 12.0) * Lref;
+    e.prop[2] = len * th;
+    e.prop[0] = KS * denIn * bpow(Uref, 2) * Lref * len / e.prop[2];
+    e.prop[1] = e.prop[0] / (2.0 * (1.0 + psR));
+    e.prop[3] = denR * len * Lref * denIn / e.prop[2];
+    const double ratio = th / len;
+    e.prop[5] = len * bpow(th, 3) / 3.0 * (1.0 - 0.63 * ratio + 0.052 * bpow(ratio, 5));
+    e.prop[6] = len * bpow(th, 3) / 12.0;
+    e.prop[7] = th * bpow(len, 3) / 12.0;
+  }
+  const double t = 0.0;
+  double XYZ[3], AoA[3], AoAd[3], Tn[9];
+  for (int k = 0; k < 3; ++k) {
+    XYZ[k] = bh.XYZo[k] + bh.XYZAmpl[k] * std::cos(2.0 * kMPi * bh.Freq * t + bh.XYZPhi[k]) + bh.initXYZVel[k] * t;
+    AoA[k] = bh.AoAo[k] + bh.AoAAmpl[k] * std::cos(2.0 * kMPi * bh.Freq * t + bh.AoAPhi[k]);
+  }
+  aoa_to_ttt(AoA, bh.TTT0);
+  aoa_to_ttt(AoA, Tn);
+  angle_triad(bh.TTT0, Tn, AoAd);
+  for (auto &e : els) {
+    el_map_pos(e, e.x0, bh.TTT0, XYZ, AoAd);
+    bmv3(bh.TTT0, e.dirc00, e.dirc0);
+    for (int k = 0; k < NED; ++k) { e.x1[k] = e.x0[k]; e.xnxt[k] = e.x1[k]; }
+    e.dx0 = e.x0[6] - e.x0[0];
+    e.dy0 = e.x0[7] - e.x0[1];
+    e.dz0 = e.x0[8] - e.x0[2];
+    e.len0 = std::sqrt(e.dx0 * e.dx0 + e.dy0 * e.dy0 + e.dz0 * e.dz0);
+    e.xll0 = e.dx0 / e.len0;
+    e.xmm0 = e.dy0 / e.len0;
+    e.xnn0 = e.dz0 / e.len0;
+    el_cptdxyz1(e);
+  }
+  pos.assign(6 * (size_t)nND, 0.0);
+  vel.assign(6 * (size_t)nND, 0.0);
+  for (auto &e : els)
+    for (int k = 0; k < 6; ++k) { pos[6 * e.node0 + k] = e.x1[k]; pos[6 * e.node1 + k] = e.x1[6 + k]; }
+  if (iBodyModel == 1) {
+    double UVW[3], W1[3], W2[3], W3[3];
+    for (int k = 0; k < 3; ++k) {
+      UVW[k] = -(2.0 * kMPi * bh.Freq * bh.XYZAmpl[k] * std::sin(2.0 * kMPi * bh.Freq * t + bh.XYZPhi[k])) + bh.initXYZVel[k];
+      W1[k] = -(2.0 * kMPi * bh.Freq * bh.AoAAmpl[k] * std::sin(2.0 * kMPi * bh.Freq * t + bh.AoAPhi[k]));
+    }
+    W2[0] = W1[0] * std::cos(AoA[1]) + W1[2];
+    W2[1] = W1[0] * std::sin(AoA[1]) * std::sin(AoA[2]) + W1[1] * std::cos(AoA[2]);
+    W2[2] = W1[0] * std::sin(AoA[1]) * std::cos(AoA[2]) - W1[1] * std::sin(AoA[2]);
+    bmv3(bh.TTT0, W2, W3);
+    for (int n = 0; n < nND; ++n) {
+      double rel[3];
+      for (int k = 0; k < 3; ++k) rel[k] = pos[6 * n + k] - XYZ[k];
+      vel[6 * n + 0] = (W3[1] * rel[2] - W3[2] * rel[1]) + UVW[0];
+      vel[6 * n + 1] = (W3[2] * rel[0] - W3[0] * rel[2]) + UVW[1];
+      vel[6 * n + 2] = (W3[0] * rel[1] - W3[1] * rel[0]) + UVW[2];
+      for (int k = 0; k < 3; ++k) vel[6 * n + 3 + k] = W3[k];
+    }
+  }
+  for (auto &e : els) {
+    axis_dir_triad(e.xll0, e.xmm0, e.xnn0, e.dirc0, e.tn1);
+    for (int k = 0; k < 9; ++k) { e.tn2[k] = e.tn1[k]; e.tee[k] = e.tn1[k]; }
+    el_form_mass(e);
+    el_rotate(e);
+    el_rkr(e, e.mas);
+  }
+}
+
+// prescribed placement of a sub-step, host libm (SolidSolver.f90:1826-1869)
+void fem_kinematics(const FemBodyHost &bh, double time, int isub, double deltat, double subdt, FemBodyDesc &d) {
+  const double tt = time - deltat + (double)isub * subdt;
+  double AoA[3];
+  for (int k = 0; k < 3; ++k) {
+    d.XYZ[k] = bh.XYZo[k] + bh.XYZAmpl[k] * std::cos(2.0 * kMPi * bh.Freq * tt + bh.XYZPhi[k]) + bh.initXYZVel[k] * tt;
+    AoA[k] = bh.AoAo[k] + bh.AoAAmpl[k] * std::cos(2.0 * kMPi * bh.Freq * tt + bh.AoAPhi[k]);
+  }
+  aoa_to_ttt(AoA, d.T);
+  angle_triad(bh.TTT0, d.T, d.AoAd);
+  if (bh.iBodyModel == 1) {
+    double W1[3], W2[3];
+    for (int k = 0; k < 3; ++k) {
+      d.UVW[k] = -(2.0 * kMPi * bh.Freq * bh.XYZAmpl[k] * std::sin(2.0 * kMPi * bh.Freq * tt + bh.XYZPhi[k])) + bh.initXYZVel[k];
+      W1[k] = -(2.0 * kMPi * bh.Freq * bh.AoAAmpl[k] * std::sin(2.0 * kMPi * bh.Freq * tt + bh.AoAPhi[k]));
+    }
+    W2[0] = W1[0] * std::cos(AoA[1]) + W1[2];
+    W2[1] = W1[0] * std::sin(AoA[1]) * std::sin(AoA[2]) + W1[1] * std::cos(AoA[2]);
+    W2[2] = W1[0] * std::sin(AoA[1]) * std::cos(AoA[2]) - W1[1] * std::sin(AoA[2]);
+    bmv3(d.T, W2, d.W3);
+  }
+}
+
+FemArrays fem_arrays(Block &b) {
+  Fem &F = b.fem;
+  FemArrays A{};
+  A.el = F.el.p;
+  A.pos = F.pos.p; A.dsp = F.dsp.p; A.vel = F.vel.p; A.acc = F.acc.p; A.lod = b.ibm.lod.p;
+  A.dspO = F.dspO.p; A.velO = F.velO.p; A.accO = F.accO.p; A.lodInte = F.lodInte.p; A.lodEffe = F.lodEffe.p;
+  A.lodExte = F.lodExte.p; A.vBC = F.vBC.p; A.x = F.x.p; A.r = F.r.p; A.p = F.p.p; A.Ap = F.Ap.p; A.z = F.z.p;
+  A.xF = F.xF.p; A.lbuf = F.lbuf.p; A.blk = F.blk.p; A.fixed = F.fixed.p;
+  A.dof_ptr = F.dof_ptr.p; A.dof_e = F.dof_e.p; A.dof_i = F.dof_i.p;
+  A.node_ptr = F.node_ptr.p; A.node_e = F.node_e.p; A.node_h = F.node_h.p;
+  return A;
 }
 
 }  // namespace
@@ -820,14 +988,20 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     if (dmax) *dmax = 0.0;
     return 0;
   }
-  if (!Exyz || !Evel || !Ea || !elem_xc) return fail("null marker arrays");
-  if (build_stencil_and_band(*b, Exyz)) return 1;
   hipStream_t s = b->stream;
   const int M = I.M;
-  if (upload(I.Exyz, Exyz, 3 * (size_t)M, s)) return 1;
-  if (upload(I.Evel, Evel, 3 * (size_t)M, s)) return 1;
-  if (upload(I.Ea, Ea, (size_t)M, s)) return 1;
-  if (upload(I.elem_xc, elem_xc, 3 * (size_t)I.nelems, s)) return 1;
+  if (!Exyz) {
+    // markers computed on the device by fsil_fem_update_markers
+    if (!b->fem.markers_on_device) return fail("no marker arrays given and no device markers computed");
+    if (build_stencil_and_band(*b, I.hExyz.data())) return 1;
+  } else {
+    if (!Evel || !Ea || !elem_xc) return fail("null marker arrays");
+    if (build_stencil_and_band(*b, Exyz)) return 1;
+    if (upload(I.Exyz, Exyz, 3 * (size_t)M, s)) return 1;
+    if (upload(I.Evel, Evel, 3 * (size_t)M, s)) return 1;
+    if (upload(I.Ea, Ea, (size_t)M, s)) return 1;
+    if (upload(I.elem_xc, elem_xc, 3 * (size_t)I.nelems, s)) return 1;
+  }
   if (upload(I.dEw, I.Ew.data(), I.Ew.size(), s)) return 1;
   if (upload(I.dbidx, I.bidx.data(), I.bidx.size(), s)) return 1;
   if (upload(I.dcell, I.band_cell.data(), I.band_cell.size(), s)) return 1;
@@ -883,6 +1057,214 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   I.active = true;
   if (iters) *iters = it;
   if (dmax) *dmax = dm;
+  return 0;
+}
+
+int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, const double *xyz8,
+                           const int *elems, const int *cons, const int *iBodyModel, const int *isMotionGiven,
+                           const double *mat, const double *kin, double Lref, double Uref, double denIn,
+                           double alpha) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (b->precision != 0) return fail("immersed boundary coupling runs on the FP64 form");
+  if (nbodies <= 0) return fail("fsil_fem_define_bodies needs at least one body");
+  Fem &F = b->fem;
+  F = Fem();
+  F.nbodies = nbodies;
+  std::vector<BeamElem> all_el;
+  std::vector<double> all_pos, all_vel;
+  std::vector<int> nmark(nbodies), me, en, fixed;
+  std::vector<std::vector<std::pair<int, int>>> dofl;
+  std::vector<std::vector<std::pair<int, int>>> nodel;
+  std::vector<int> body_of_elem, first_marker, goff(nbodies);
+  std::vector<double> beta(nbodies);
+  size_t xo = 0, eo = 0, co = 0;
+  int mcount = 0;
+  F.desc.assign(nbodies, FemBodyDesc{});
+  F.bh.assign(nbodies, FemBodyHost{});
+  for (int bi = 0; bi < nbodies; ++bi) {
+    if (nND[bi] < 2 || nEL[bi] < 1) return fail("body %d: needs nodes and elements", bi + 1);
+    for (int e = 0; e < nEL[bi]; ++e)
+      for (int k = 0; k < 2; ++k) {
+        const int nd = elems[3 * (eo + e) + k];
+        if (nd < 1 || nd > nND[bi]) return fail("body %d element %d: node %d out of range", bi + 1, e + 1, nd);
+      }
+    std::vector<BeamElem> els;
+    std::vector<double> pos, vel;
+    fem_init_body(xyz8 + 8 * xo, nND[bi], nEL[bi], elems + 3 * eo, cons + 6 * xo, iBodyModel[bi],
+                  isMotionGiven + 6 * bi, mat + 4 * bi, kin + 22 * bi, Lref, Uref, denIn, F.bh[bi], els, pos, vel);
+    FemBodyDesc &d = F.desc[bi];
+    d.nND = nND[bi]; d.nEL = nEL[bi]; d.eoff = F.nEL; d.noff = F.nND; d.goff = 6 * F.nND;
+    d.iBodyModel = iBodyModel[bi];
+    goff[bi] = d.goff;
+    beta[bi] = -(alpha * 2.0 * denIn);
+    const int g0 = (int)fixed.size();
+    fixed.resize(g0 + 6 * (size_t)nND[bi], 0);
+    dofl.resize(g0 + 6 * (size_t)nND[bi]);
+    nodel.resize(F.nND + (size_t)nND[bi]);
+    int mb = 0;
+    for (int e = 0; e < nEL[bi]; ++e) {
+      const BeamElem &E = els[e];
+      const int ge = F.nEL + e;
+      for (int i = 0; i < NED; ++i) {
+        const int gl = i < 6 ? 6 * E.node0 + i : 6 * E.node1 + (i - 6);
+        dofl[g0 + gl].push_back({ge, i});
+        if (E.bc[i] > 0) fixed[g0 + gl] = 1;
+      }
+      nodel[F.nND + E.node0].push_back({ge, 0});
+      nodel[F.nND + E.node1].push_back({ge, 1});
+      body_of_elem.push_back(bi);
+      first_marker.push_back(mcount + mb);
+      for (int s2 = 0; s2 < E.Nspan; ++s2) me.push_back(e);
+      en.push_back(E.node0);
+      en.push_back(E.node1);
+      mb += E.Nspan;
+    }
+    nmark[bi] = mb;
+    mcount += mb;
+    all_el.insert(all_el.end(), els.begin(), els.end());
+    all_pos.insert(all_pos.end(), pos.begin(), pos.end());
+    all_vel.insert(all_vel.end(), vel.begin(), vel.end());
+    F.nEL += nEL[bi];
+    F.nND += nND[bi];
+    xo += nND[bi];
+    eo += nEL[bi];
+    co += 6 * (size_t)nND[bi];
+  }
+  F.gEQ = 6 * F.nND;
+  // the immersed-boundary side of the same bodies
+  std::vector<int> nn(nND, nND + nbodies), ne(nEL, nEL + nbodies);
+  if (fsil_ibm_define_bodies(h, nbodies, nmark.data(), nn.data(), ne.data(), me.data(), en.data())) return 1;
+  std::vector<int> dptr(F.gEQ + 1, 0), de, di, nptr(F.nND + 1, 0), ne2, nh;
+  for (int g = 0; g < F.gEQ; ++g) {
+    if (dofl[g].empty()) return fail("degree of freedom %d belongs to no element", g);
+    dptr[g + 1] = dptr[g] + (int)dofl[g].size();
+    for (auto &pr : dofl[g]) { de.push_back(pr.first); di.push_back(pr.second); }
+  }
+  for (int n = 0; n < F.nND; ++n) {
+    nptr[n + 1] = nptr[n] + (int)nodel[n].size();
+    for (auto &pr : nodel[n]) { ne2.push_back(pr.first); nh.push_back(pr.second); }
+  }
+  hipStream_t s = b->stream;
+  std::vector<double> zeros(F.gEQ, 0.0);
+  if (upload(F.el, all_el.data(), all_el.size(), s)) return 1;
+  if (upload(F.pos, all_pos.data(), all_pos.size(), s)) return 1;
+  if (upload(F.vel, all_vel.data(), all_vel.size(), s)) return 1;
+  if (upload(F.dsp, zeros.data(), zeros.size(), s)) return 1;
+  if (upload(F.acc, zeros.data(), zeros.size(), s)) return 1;
+  if (upload(F.fixed, fixed.data(), fixed.size(), s)) return 1;
+  if (upload(F.dof_ptr, dptr.data(), dptr.size(), s)) return 1;
+  if (upload(F.dof_e, de.data(), de.size(), s)) return 1;
+  if (upload(F.dof_i, di.data(), di.size(), s)) return 1;
+  if (upload(F.node_ptr, nptr.data(), nptr.size(), s)) return 1;
+  if (upload(F.node_e, ne2.data(), ne2.size(), s)) return 1;
+  if (upload(F.node_h, nh.data(), nh.size(), s)) return 1;
+  if (upload(F.body_of_elem, body_of_elem.data(), body_of_elem.size(), s)) return 1;
+  if (upload(F.elem_first_marker, first_marker.data(), first_marker.size(), s)) return 1;
+  if (upload(F.body_goff, goff.data(), goff.size(), s)) return 1;
+  if (upload(F.beta, beta.data(), beta.size(), s)) return 1;
+  for (DevBuf<double> *v : {&F.dspO, &F.velO, &F.accO, &F.lodInte, &F.lodEffe, &F.lodExte, &F.vBC, &F.x, &F.r, &F.p,
+                            &F.Ap, &F.z, &F.xF})
+    HIPCHK(v->ensure(F.gEQ));
+  HIPCHK(F.lbuf.ensure((size_t)F.nEL * NED));
+  HIPCHK(F.blk.ensure((size_t)F.nND * 36));
+  HIPCHK(F.ddesc.ensure(nbodies));
+  // lodFlow starts at zero (Beam_InitLoad)
+  if (upload(b->ibm.lod, zeros.data(), zeros.size(), s)) return 1;
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsil_fem_set_solver(int h, double dampK, double dampM, double GeoGamma, double NewmarkGamma,
+                        double NewmarkBeta, double dtolFEM, int ntolFEM) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  F.dampK = dampK; F.dampM = dampM; F.GeoGamma = GeoGamma; F.NewmarkGamma = NewmarkGamma;
+  F.NewmarkBeta = NewmarkBeta; F.dtolFEM = dtolFEM; F.ntolFEM = ntolFEM;
+  return 0;
+}
+
+int fsil_fem_update_markers(int h) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  Ibm &I = b->ibm;
+  if (F.nbodies == 0) return fail("no FEM bodies defined");
+  FemMarkerArgs a{};
+  a.el = F.el.p; a.pos = F.pos.p; a.vel = F.vel.p;
+  a.body_of_elem = F.body_of_elem.p; a.elem_first_marker = F.elem_first_marker.p; a.body_goff = F.body_goff.p;
+  a.beta = F.beta.p; a.nEL_total = F.nEL;
+  a.Exyz = I.Exyz.p; a.Evel = I.Evel.p; a.Ea = I.Ea.p; a.elem_xc = I.elem_xc.p;
+  hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 255) / 256), dim3(256), 0, b->stream, a);
+  HIPCHK(hipGetLastError());
+  I.hExyz.resize(3 * (size_t)I.M);
+  HIPCHK(hipMemcpyAsync(I.hExyz.data(), I.Exyz.p, 3 * (size_t)I.M * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  F.markers_on_device = true;
+  return 0;
+}
+
+int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (F.nbodies == 0) return fail("no FEM bodies defined");
+  if (numsubstep < 1) return fail("numsubstep must be >= 1");
+  const double subdt = deltat / (double)numsubstep;
+  FemArrays A = fem_arrays(*b);
+  std::vector<std::vector<FemBodyDesc>> staged(numsubstep, F.desc);
+  for (int isub = 1; isub <= numsubstep; ++isub) {
+    std::vector<FemBodyDesc> &D = staged[isub - 1];
+    for (int bi = 0; bi < F.nbodies; ++bi) {
+      FemBodyDesc &d = D[bi];
+      fem_kinematics(F.bh[bi], time, isub, deltat, subdt, d);
+      double *c = d.coeffs;
+      c[2] = 1.0 / (F.NewmarkBeta * subdt);
+      c[1] = F.NewmarkGamma * c[2];
+      c[0] = c[2] / subdt;
+      c[3] = 0.5 / F.NewmarkBeta - 1.0;
+      c[4] = F.NewmarkGamma / F.NewmarkBeta - 1.0;
+      c[5] = subdt * (0.5 * F.NewmarkGamma / F.NewmarkBeta - 1.0);
+      c[6] = subdt * (1.0 - F.NewmarkGamma);
+      c[7] = subdt * F.NewmarkGamma;
+      d.ntolFEM = F.ntolFEM; d.dtolFEM = F.dtolFEM; d.dampK = F.dampK; d.dampM = F.dampM; d.GeoGamma = F.GeoGamma;
+    }
+    HIPCHK(hipMemcpyAsync(F.ddesc.p, D.data(), D.size() * sizeof(FemBodyDesc), hipMemcpyHostToDevice, b->stream));
+    hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(256), 0, b->stream, F.ddesc.p, A);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipMemcpyAsync(F.desc.data(), F.ddesc.p, F.desc.size() * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int bi = 0; bi < F.nbodies; ++bi)
+    if (F.desc[bi].status) return fail("ERROR: CG solver breakdown (body %d)", bi + 1);
+  return 0;
+}
+
+int fsil_fem_state(int h, int body, double *pos, double *vel, double *acc, double *dsp) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (body < 0 || body >= F.nbodies) return fail("body %d out of range", body);
+  const FemBodyDesc &d = F.desc[body];
+  const size_t n = 6 * (size_t)d.nND;
+  hipStream_t s = b->stream;
+  if (pos) HIPCHK(hipMemcpyAsync(pos, F.pos.p + d.goff, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (vel) HIPCHK(hipMemcpyAsync(vel, F.vel.p + d.goff, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (acc) HIPCHK(hipMemcpyAsync(acc, F.acc.p + d.goff, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (dsp) HIPCHK(hipMemcpyAsync(dsp, F.dsp.p + d.goff, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsil_fem_info(int h, int body, int *newton_iters, int *cg_iters, double *dnorm) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (body < 0 || body >= F.nbodies) return fail("body %d out of range", body);
+  if (newton_iters) *newton_iters = F.desc[body].newton_iters;
+  if (cg_iters) *cg_iters = F.desc[body].cg_iters;
+  if (dnorm) *dnorm = F.desc[body].dnorm;
   return 0;
 }
 

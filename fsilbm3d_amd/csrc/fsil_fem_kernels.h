@@ -1,0 +1,339 @@
+// Device FEM sub-step: one workgroup per body, FP64.
+//
+// Beam_structure for iBodyModel 2 (SolidSolver.f90:1859-1914): prescribed
+// positions, Newmark coefficients, full Newton with a block-Jacobi
+// preconditioned conjugate-gradient solve (SolidSolver.f90:1949-2020) and the
+// displacement/triad update, then the velocity/acceleration update.  For
+// iBodyModel 1 the same launch applies the prescribed rigid kinematics.
+//
+// Parallel structure inside a workgroup: element work (matrices, internal
+// force, loads, triads) one thread per element; element-by-element
+// mat-vec products one thread per (element, row); the 6x6 block inverses one
+// thread per node; every assembly (LocToGlobal) is a per-DOF sequential sum
+// over that DOF's elements in element order, and every dot product is a
+// sequential sum in DOF order by one thread, so the arithmetic is the
+// reference's (only the device's libm sin/cos/acos can differ in the last
+// place from the host C library's).
+#pragma once
+#include "fsil_beam.h"
+
+namespace fsil {
+
+struct FemBodyDesc {
+  int nND, nEL, eoff, goff, noff;   // element / dof / node offsets into the global arrays
+  int iBodyModel, ntolFEM, pad;
+  double dtolFEM, dampK, dampM, GeoGamma;
+  double coeffs[8];
+  double XYZ[3], AoAd[3], T[9];     // prescribed placement of this sub-step (host libm)
+  double UVW[3], W3[3];             // rigid velocities (iBodyModel 1)
+  // outputs
+  int newton_iters, cg_iters, status, pad2;
+  double dnorm;
+};
+
+struct FemArrays {
+  BeamElem *el;
+  double *pos, *dsp, *vel, *acc, *lod;          // lod: lodFlow from the immersed-boundary coupling
+  double *dspO, *velO, *accO, *lodInte, *lodEffe, *lodExte, *vBC;
+  double *x, *r, *p, *Ap, *z, *xF, *lbuf, *blk; // lbuf: 12 per element; blk: 36 per node
+  int *fixed;
+  const int *dof_ptr, *dof_e, *dof_i;           // dof -> (element, local index) in element order
+  const int *node_ptr, *node_e, *node_h;        // node -> (element, half 0/1) in element order
+};
+
+__device__ inline void fem_sync() { __syncthreads(); }
+
+// element-by-element product out = A*v for one body (Beam_MatrixMultipy,
+// SolidSolver.f90:2035-2049)
+__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const double *v, double *out) {
+  const int nE = D.nEL, g0 = D.goff;
+  for (int t = threadIdx.x; t < nE * NED; t += blockDim.x) {
+    const int e = t / NED, i = t - e * NED;
+    const BeamElem &E = A.el[D.eoff + e];
+    double s = 0.0;
+    for (int k = 0; k < NED; ++k) {
+      const int gk = k < 6 ? 6 * E.node0 + k : 6 * E.node1 + (k - 6);
+      s = s + E.coef[i * NED + k] * v[g0 + gk];
+    }
+    A.lbuf[(size_t)(D.eoff + e) * NED + i] = s;
+  }
+  fem_sync();
+  for (int g = threadIdx.x; g < 6 * D.nND; g += blockDim.x) {
+    double s = 0.0;
+    for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
+      s = s + A.lbuf[(size_t)A.dof_e[k] * NED + A.dof_i[k]];
+    out[g0 + g] = s;
+  }
+  fem_sync();
+}
+
+__device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D) {
+  const int g0 = D.goff;
+  for (int g = threadIdx.x; g < 6 * D.nND; g += blockDim.x) {
+    const int node = g / 6, i = g - node * 6;
+    const double *B = A.blk + (size_t)(D.noff + node) * 36;
+    double s = 0.0;
+    for (int k = 0; k < 6; ++k) s = s + B[i * 6 + k] * A.r[g0 + 6 * node + k];
+    A.z[g0 + g] = A.fixed[g0 + g] ? 0.0 : s;
+  }
+  fem_sync();
+}
+
+// sequential dot product in DOF order, broadcast through LDS
+__device__ inline double fem_dot(const double *a, const double *b, int g0, int n, double *shared) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s = s + a[g0 + i] * b[g0 + i];
+    *shared = s;
+  }
+  fem_sync();
+  const double v = *shared;
+  fem_sync();
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArrays A) {
+  __shared__ double sh_val;
+  __shared__ int sh_flag;
+  FemBodyDesc &D = descs[blockIdx.x];
+  const int nE = D.nEL, nN = D.nND, nG = 6 * nN, g0 = D.goff;
+  if (D.iBodyModel == 1) {
+    // rigid prescribed motion (SolidSolver.f90:1826-1858)
+    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+      BeamElem &E = A.el[D.eoff + e];
+      el_map_pos(E, E.xnxt, D.T, D.XYZ, D.AoAd);
+      double dn[3];
+      bmv3(D.T, E.dirc00, dn);
+      for (int k = 0; k < NED; ++k) E.x1[k] = E.xnxt[k];
+      el_cptdxyz1(E);
+      axis_dir_triad(E.xll1, E.xmm1, E.xnn1, dn, E.tn1);
+      for (int k = 0; k < 9; ++k) { E.tn2[k] = E.tn1[k]; E.tee[k] = E.tn1[k]; }
+    }
+    fem_sync();
+    for (int n = threadIdx.x; n < nN; n += blockDim.x) {
+      const int last = A.node_ptr[D.noff + n + 1] - 1;
+      const BeamElem &E = A.el[A.node_e[last]];
+      const double *src = A.node_h[last] ? E.x1 + 6 : E.x1;
+      for (int k = 0; k < 6; ++k) A.pos[g0 + 6 * n + k] = src[k];
+      double rel[3];
+      for (int k = 0; k < 3; ++k) rel[k] = A.pos[g0 + 6 * n + k] - D.XYZ[k];
+      const double *W = D.W3;
+      A.vel[g0 + 6 * n + 0] = (W[1] * rel[2] - W[2] * rel[1]) + D.UVW[0];
+      A.vel[g0 + 6 * n + 1] = (W[2] * rel[0] - W[0] * rel[2]) + D.UVW[1];
+      A.vel[g0 + 6 * n + 2] = (W[0] * rel[1] - W[1] * rel[0]) + D.UVW[2];
+      for (int k = 0; k < 3; ++k) A.vel[g0 + 6 * n + 3 + k] = W[k];
+    }
+    if (threadIdx.x == 0) { D.newton_iters = 0; D.cg_iters = 0; D.status = 0; D.dnorm = 0.0; }
+    return;
+  }
+  const double *c = D.coeffs;
+  // prescribed positions of this sub-step and InitDspVelAccATTimeT (SolidSolver.f90:1916-1930)
+  for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+    BeamElem &E = A.el[D.eoff + e];
+    el_map_pos(E, E.xnxt, D.T, D.XYZ, D.AoAd);
+  }
+  fem_sync();
+  for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+    const int last = A.dof_ptr[g0 + g + 1] - 1;
+    const BeamElem &E = A.el[A.dof_e[last]];
+    const int li = A.dof_i[last];
+    A.vBC[g0 + g] = E.xnxt[li] - E.x1[li];
+    A.lodExte[g0 + g] = (A.lod[g0 + g] + 0.0) + 0.0;   // lodFlow + lodGrav + lodRepl (gravity, repulsion zero)
+    A.dspO[g0 + g] = A.dsp[g0 + g];
+    A.velO[g0 + g] = A.vel[g0 + g];
+    A.accO[g0 + g] = A.acc[g0 + g];
+  }
+  fem_sync();
+  int iter;
+  int cg_total = 0;
+  int status = 0;
+  double dnorm = 1.0;
+  for (iter = 1; iter <= D.ntolFEM; ++iter) {
+    // UpdateMatrixANDLoad (SolidSolver.f90:1932-1947)
+    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+      BeamElem &E = A.el[D.eoff + e];
+      el_form_stiff(E);
+      el_body_stress(E);
+      el_update_matrix(E, c, D.GeoGamma, D.dampM, D.dampK);
+      el_update_load(E, c, D.dampM, D.dampK, A.dspO + g0, A.dsp + g0, A.vel + g0, A.acc + g0);
+    }
+    fem_sync();
+    for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+      double li = 0.0;
+      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k) li = li + A.el[A.dof_e[k]].fint[A.dof_i[k]];
+      A.lodInte[g0 + g] = li;
+      double le = A.lodExte[g0 + g] - li;
+      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k) {
+        le = le + A.el[A.dof_e[k]].fload[A.dof_i[k]];
+        if (D.dampK > 0.0) le = le + A.el[A.dof_e[k]].fdamp[A.dof_i[k]];
+      }
+      A.lodEffe[g0 + g] = le;
+    }
+    fem_sync();
+    // CG_Solve (SolidSolver.f90:1949-2020), block-Jacobi preconditioner
+    for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+      A.xF[g0 + g] = A.fixed[g0 + g] ? (iter == 1 ? A.vBC[g0 + g] : 0.0) : 0.0;
+      A.x[g0 + g] = A.xF[g0 + g];
+    }
+    for (int n = threadIdx.x; n < nN; n += blockDim.x) {
+      double B[36], Bi[36];
+      for (int k = 0; k < 36; ++k) B[k] = 0.0;
+      for (int k = A.node_ptr[D.noff + n]; k < A.node_ptr[D.noff + n + 1]; ++k) {
+        const BeamElem &E = A.el[A.node_e[k]];
+        const int o = A.node_h[k] ? 6 : 0;
+        for (int i = 0; i < 6; ++i)
+          for (int j = 0; j < 6; ++j) B[i * 6 + j] = B[i * 6 + j] + E.coef[(o + i) * NED + o + j];
+      }
+      for (int i = 0; i < 6; ++i)
+        if (A.fixed[g0 + 6 * n + i]) {
+          for (int j = 0; j < 6; ++j) { B[i * 6 + j] = 0.0; B[j * 6 + i] = 0.0; }
+          B[i * 6 + i] = 1.0;
+        }
+      if (invert6(B, Bi)) {
+        for (int k = 0; k < 36; ++k) Bi[k] = 0.0;
+        for (int i = 0; i < 6; ++i) Bi[i * 6 + i] = 1.0;
+      }
+      for (int k = 0; k < 36; ++k) A.blk[(size_t)(D.noff + n) * 36 + k] = Bi[k];
+    }
+    fem_sync();
+    fem_matvec(A, D, A.x, A.Ap);
+    for (int g = threadIdx.x; g < nG; g += blockDim.x)
+      A.r[g0 + g] = A.fixed[g0 + g] ? 0.0 : A.lodEffe[g0 + g] - A.Ap[g0 + g];
+    fem_sync();
+    fem_precond(A, D);
+    const double err = 1.0e-6;
+    double rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val));
+    if (!(rn <= err)) {
+      for (int g = threadIdx.x; g < nG; g += blockDim.x) A.p[g0 + g] = A.z[g0 + g];
+      fem_sync();
+      double rsold = fem_dot(A.r, A.z, g0, nG, &sh_val);
+      if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
+      for (int it = 1; it <= 10000; ++it) {
+        ++cg_total;
+        fem_matvec(A, D, A.p, A.Ap);
+        for (int g = threadIdx.x; g < nG; g += blockDim.x)
+          if (A.fixed[g0 + g]) A.Ap[g0 + g] = 0.0;
+        fem_sync();
+        const double pAp = fem_dot(A.p, A.Ap, g0, nG, &sh_val);
+        if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
+        const double alpha = rsold / pAp;
+        for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+          double xv = A.x[g0 + g] + alpha * A.p[g0 + g];
+          if (A.fixed[g0 + g]) xv = A.xF[g0 + g];
+          A.x[g0 + g] = xv;
+          double rv = A.r[g0 + g] - alpha * A.Ap[g0 + g];
+          if (A.fixed[g0 + g]) rv = 0.0;
+          A.r[g0 + g] = rv;
+        }
+        fem_sync();
+        rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val));
+        if (rn <= err) break;
+        fem_precond(A, D);
+        const double rsnew = fem_dot(A.r, A.z, g0, nG, &sh_val);
+        if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
+        const double beta = rsnew / rsold;
+        for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+          double pv = A.z[g0 + g] + beta * A.p[g0 + g];
+          if (A.fixed[g0 + g]) pv = 0.0;
+          A.p[g0 + g] = pv;
+        }
+        fem_sync();
+        rsold = rsnew;
+      }
+      if (status) break;
+    }
+    // UpdateDspANDTride (SolidSolver.f90:2304-2338); the ramp factor is 1
+    for (int g = threadIdx.x; g < nG; g += blockDim.x) A.dsp[g0 + g] = A.dsp[g0 + g] + A.x[g0 + g];
+    if (threadIdx.x == 0) sh_val = 0.0;
+    fem_sync();
+    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+      BeamElem &E = A.el[D.eoff + e];
+      for (int k = 0; k < 6; ++k) {
+        E.x1[k] = E.x0[k] + A.dsp[g0 + 6 * E.node0 + k];
+        E.x1[6 + k] = E.x0[6 + k] + A.dsp[g0 + 6 * E.node1 + k];
+      }
+      el_cptdxyz1(E);
+      el_update_triads(E, A.x + g0);
+    }
+    if (threadIdx.x == 0) {
+      double mx = -INFINITY;
+      for (int g = 0; g < nG; ++g) {
+        const double v = A.x[g0 + g] * A.x[g0 + g];
+        if (v > mx) mx = v;
+      }
+      sh_val = fabs(mx);
+    }
+    fem_sync();
+    for (int n = threadIdx.x; n < nN; n += blockDim.x) {
+      const int last = A.node_ptr[D.noff + n + 1] - 1;
+      const BeamElem &E = A.el[A.node_e[last]];
+      const double *src = A.node_h[last] ? E.x1 + 6 : E.x1;
+      for (int k = 0; k < 6; ++k) A.pos[g0 + 6 * n + k] = src[k];
+    }
+    dnorm = sh_val;
+    fem_sync();
+    if (dnorm <= D.dtolFEM) break;
+  }
+  // UpdateVelAcc (SolidSolver.f90:2350-2359)
+  for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+    const double a = c[0] * (A.dsp[g0 + g] - A.dspO[g0 + g]) - c[2] * A.velO[g0 + g] - c[3] * A.accO[g0 + g];
+    A.acc[g0 + g] = a;
+    A.vel[g0 + g] = A.velO[g0 + g] + c[6] * A.accO[g0 + g] + c[7] * a;
+  }
+  if (threadIdx.x == 0) {
+    D.newton_iters = iter > D.ntolFEM ? D.ntolFEM : iter;
+    D.cg_iters = cg_total;
+    D.status = status;
+    D.dnorm = dnorm;
+  }
+  (void)sh_flag;
+}
+
+// PlateUpdatePosVelArea_ (Solidbody.f90:604-646): one thread per element,
+// markers written in the body's marker order; elem_xc = beam axis midpoints.
+struct FemMarkerArgs {
+  const BeamElem *el;
+  const double *pos, *vel;
+  const int *body_of_elem, *elem_first_marker, *body_goff;
+  const double *beta;   // per body: -(alpha*2*denIn)
+  int nEL_total;
+  double *Exyz, *Evel, *Ea, *elem_xc;
+};
+
+__global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nEL_total) return;
+  const BeamElem &E = a.el[e];
+  const int body = a.body_of_elem[e];
+  const int g0 = a.body_goff[body];
+  const int i1 = E.node0, i2 = E.node1;
+  double txyz[3], tvel[3], om[3], dirc[3];
+  for (int k = 0; k < 3; ++k) {
+    txyz[k] = 0.5 * (a.pos[g0 + 6 * i1 + k] + a.pos[g0 + 6 * i2 + k]);
+    tvel[k] = 0.5 * (a.vel[g0 + 6 * i1 + k] + a.vel[g0 + 6 * i2 + k]);
+    om[k] = 0.5 * (a.vel[g0 + 6 * i1 + 3 + k] + a.vel[g0 + 6 * i2 + 3 + k]);
+  }
+  const double dl = E.spanlen / (double)E.Nspan;
+  const double area = dl * E.len1 * a.beta[body];
+  for (int k = 0; k < 3; ++k) dirc[k] = 0.5 * (E.tn1[k * 3 + 1] + E.tn2[k * 3 + 1]);
+  const double nrm = sqrt(bdot3(dirc, dirc));
+  if (nrm > 1.0e-12) for (int k = 0; k < 3; ++k) dirc[k] = dirc[k] / nrm;
+  else for (int k = 0; k < 3; ++k) dirc[k] = E.tee[k * 3 + 1];
+  const int m0 = a.elem_first_marker[e];
+  for (int s = 0; s < E.Nspan; ++s) {
+    const double ls = dl * (0.5 + (double)s) - E.Lspan;
+    double r[3], w[3];
+    for (int k = 0; k < 3; ++k) r[k] = dirc[k] * ls;
+    w[0] = om[1] * r[2] - om[2] * r[1];
+    w[1] = om[2] * r[0] - om[0] * r[2];
+    w[2] = om[0] * r[1] - om[1] * r[0];
+    for (int k = 0; k < 3; ++k) {
+      a.Exyz[3LL * (m0 + s) + k] = txyz[k] + r[k];
+      a.Evel[3LL * (m0 + s) + k] = tvel[k] + w[k];
+    }
+    a.Ea[m0 + s] = area;
+  }
+  for (int k = 0; k < 3; ++k) a.elem_xc[3LL * e + k] = 0.5 * (E.x1[k] + E.x1[6 + k]);
+}
+
+}  // namespace fsil

@@ -9,9 +9,108 @@ from typing import List, Optional
 
 import numpy as np
 
+import dataclasses
+
 from . import _lib
 from .block import LBMBlock
 from .solid import RigidPlate
+
+
+@dataclasses.dataclass
+class PlateSpec:
+    """A plate body as the reference's inputs describe it: plate.dat rows
+    (POINT, ELEMENT, CONSTRAINT) plus the SolidBody group parameters of
+    inFlow.dat (Solidbody.f90:121-155)."""
+    nodes8: np.ndarray                 # nND x 8: x y z Lspan Rspan dirX dirY dirZ
+    elements: np.ndarray               # nEL x 3: i j Nspan (1-based node ids)
+    constraints: np.ndarray            # nND x 6
+    iBodyModel: int = 1
+    isMotionGiven: tuple = (1, 1, 1, 1, 1, 1)
+    denR: float = 1.0
+    KB: float = 0.1
+    KS: float = 1000.0
+    psR: float = 0.3
+    Freq: float = 0.0
+    XYZo: tuple = (0.0, 0.0, 0.0)
+    XYZAmpl: tuple = (0.0, 0.0, 0.0)
+    XYZPhi: tuple = (0.0, 0.0, 0.0)
+    AoAo: tuple = (0.0, 0.0, 0.0)
+    AoAAmpl: tuple = (0.0, 0.0, 0.0)
+    AoAPhi: tuple = (0.0, 0.0, 0.0)
+    initXYZVel: tuple = (0.0, 0.0, 0.0)
+
+    def kin(self):
+        return [self.Freq, *self.XYZo, *self.XYZAmpl, *self.XYZPhi, *self.AoAo, *self.AoAAmpl, *self.AoAPhi,
+                *self.initXYZVel]
+
+
+class DeviceBodyBlock:
+    """A fluid block whose plate bodies live on the device beam solver: the
+    whole IBM_FEM (LBMBlockComm.f90:320-338) runs through the C-ABI."""
+
+    def __init__(self, block: LBMBlock, plates: List[PlateSpec], *, ntolLBM: int, dtolLBM: float, Uref: float,
+                 denIn: float, Lref: float, numsubstep: int = 1, IBPenaltyAlpha: float = 1.0, dampK: float = 0.0,
+                 dampM: float = 0.0, GeoGamma: float = 1.0, NewmarkGamma: float = 0.5, NewmarkBeta: float = 0.25,
+                 dtolFEM: float = 1e-10, ntolFEM: int = 20):
+        self.block = block
+        self.lib = lib = block.lib
+        self.numsubstep = int(numsubstep)
+        nb = len(plates)
+        nND = np.array([len(p.nodes8) for p in plates], np.int32)
+        nEL = np.array([len(p.elements) for p in plates], np.int32)
+        xyz8 = np.ascontiguousarray(np.concatenate([np.asarray(p.nodes8, np.float64) for p in plates]))
+        els = np.ascontiguousarray(np.concatenate([np.asarray(p.elements, np.int32) for p in plates]))
+        cons = np.ascontiguousarray(np.concatenate([np.asarray(p.constraints, np.int32) for p in plates]))
+        ibm = np.array([p.iBodyModel for p in plates], np.int32)
+        img = np.ascontiguousarray(np.array([p.isMotionGiven for p in plates], np.int32).reshape(-1))
+        mat = np.ascontiguousarray(np.array([[p.denR, p.KB, p.KS, p.psR] for p in plates], np.float64).reshape(-1))
+        kin = np.ascontiguousarray(np.array([p.kin() for p in plates], np.float64).reshape(-1))
+        _lib.check(lib.fsil_fem_define_bodies(block.handle, nb, _lib.iptr(nND), _lib.iptr(nEL), _lib.dptr(xyz8),
+                                              _lib.iptr(els), _lib.iptr(cons), _lib.iptr(ibm), _lib.iptr(img),
+                                              _lib.dptr(mat), _lib.dptr(kin), float(Lref), float(Uref),
+                                              float(denIn), float(IBPenaltyAlpha)))
+        _lib.check(lib.fsil_fem_set_solver(block.handle, dampK, dampM, GeoGamma, NewmarkGamma, NewmarkBeta,
+                                           dtolFEM, int(ntolFEM)))
+        _lib.check(lib.fsil_ibm_set_params(block.handle, int(ntolLBM), float(dtolLBM), float(Uref), float(denIn)))
+        self.nND = nND
+        self.M = int(sum(int(np.asarray(p.elements)[:, 2].sum()) for p in plates))
+        self.Eforce = np.zeros((max(self.M, 1), 3))
+        self.lodFlow = np.zeros(6 * int(nND.sum()))
+        self.iters = 0
+        self.dmax = 0.0
+
+    def interaction_force(self, dt: float):
+        _lib.check(self.lib.fsil_fem_update_markers(self.block.handle))
+        it = np.zeros(1, dtype=np.int32)
+        dm = np.zeros(1)
+        _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
+                                                       _lib.dptr(self.Eforce), _lib.dptr(self.lodFlow),
+                                                       _lib.iptr(it), _lib.dptr(dm)))
+        self.iters, self.dmax = int(it[0]), float(dm[0])
+
+    def body_loads(self, i: int) -> np.ndarray:
+        off = 6 * int(self.nND[:i].sum())
+        return self.lodFlow[off:off + 6 * int(self.nND[i])]
+
+    def state(self, i: int):
+        n = 6 * int(self.nND[i])
+        out = [np.zeros(n) for _ in range(4)]
+        _lib.check(self.lib.fsil_fem_state(self.block.handle, i, *[_lib.dptr(a) for a in out]))
+        return [a.reshape(-1, 6) for a in out]
+
+    def info(self, i: int):
+        a = np.zeros(1, np.int32)
+        b = np.zeros(1, np.int32)
+        c = np.zeros(1)
+        _lib.check(self.lib.fsil_fem_info(self.block.handle, i, _lib.iptr(a), _lib.iptr(b), _lib.dptr(c)))
+        return int(a[0]), int(b[0]), float(c[0])
+
+    def step(self, time: float):
+        blk = self.block
+        blk.update_volume_force(time)
+        self.interaction_force(blk.dh)
+        _lib.check(self.lib.fsil_fem_substeps(blk.handle, float(time), self.numsubstep, blk.dh))
+        blk.collision_streaming_bc()
 
 
 class FSIBlock:

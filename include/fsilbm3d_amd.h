@@ -120,6 +120,38 @@ int fsil_ibm_interaction_force(int handle, double dt, const double *Exyz, const 
                                const double *Ea, const double *elem_xc, double *Eforce,
                                double *lodFlow, int *iters, double *dmax);
 
+/* Plate bodies on the device beam solver, carried by the block (read_solid_files,
+ * Beam_ReadBuild, Beam_Initialise, calculate_angle_material with isKB = 1:
+ * Solidbody.f90:73-216, SolidSolver.f90:1270-1446, 1533-1616).  Per body b:
+ *   nND[b], nEL[b]; xyz8: nND rows "x y z Lspan Rspan dirX dirY dirZ" (plate.dat POINT);
+ *   elems: nEL rows "i j Nspan" (1-based nodes); cons: nND rows of 6 (CONSTRAINT);
+ *   iBodyModel[b] 1 rigid / 2 elastic; isMotionGiven[6b..6b+5];
+ *   mat[4b..]: denR, KB, KS, psR;  kin[22b..]: Freq, XYZo[3], XYZAmpl[3], XYZPhi[3] (deg),
+ *   AoAo[3], AoAAmpl[3], AoAPhi[3] (deg), initXYZVel[3];  Lref, Uref, denIn, IBPenaltyAlpha.
+ * Also defines the bodies' immersed-boundary markers (as fsil_ibm_define_bodies). */
+int fsil_fem_define_bodies(int handle, int nbodies, const int *nND, const int *nEL, const double *xyz8,
+                           const int *elems, const int *cons, const int *iBodyModel, const int *isMotionGiven,
+                           const double *mat, const double *kin, double Lref, double Uref, double denIn,
+                           double alpha);
+
+/* Set_SolidSolver_Params (SolidSolver.f90:1249-1264). */
+int fsil_fem_set_solver(int handle, double dampK, double dampM, double GeoGamma, double NewmarkGamma,
+                        double NewmarkBeta, double dtolFEM, int ntolFEM);
+
+/* PlateUpdatePosVelArea_ for every FEM body on the device (Solidbody.f90:604-646);
+ * fsil_ibm_interaction_force with Exyz == NULL then uses these markers. */
+int fsil_fem_update_markers(int handle);
+
+/* numsubstep calls of Beam_structure for every body (LBMBlockComm.f90:333-335,
+ * SolidSolver.f90:1820-2359), sub-step deltat/numsubstep, loads = the lodFlow of
+ * the last fsil_ibm_interaction_force. */
+int fsil_fem_substeps(int handle, double time, int numsubstep, double deltat);
+
+/* Node state of body b (6*nND each, node-major), and the last sub-step's
+ * Newton iterations, conjugate-gradient iterations and displacement norm. */
+int fsil_fem_state(int handle, int body, double *pos, double *vel, double *acc, double *dsp);
+int fsil_fem_info(int handle, int body, int *newton_iters, int *cg_iters, double *dnorm);
+
 /* Bandwidth of a plain device copy of nbytes (read + write), reps launches:
  * the calibration for HBM counters and the achievable-bandwidth context of the
  * roofline.  No reference counterpart (measurement helper). */

@@ -108,10 +108,25 @@ def run_oracle(pc, f0=None, nsteps=None, record=None):
     return f
 
 
-def run_device(pc, f0=None, nsteps=None, record=None):
-    """Run a ParityCase through the C-ABI on the GPU; returns the final state."""
+def plate_specs(pc):
+    from fsilbm3d_amd.fsi import PlateSpec
+    out = []
+    for p in pc.plates:
+        cons = np.zeros((p.n_nodes, 6), np.int32)
+        cons[0] = 1
+        out.append(PlateSpec(nodes8=p.nodes(), elements=np.array(p.elements(), np.int32), constraints=cons,
+                             iBodyModel=p.iBodyModel, isMotionGiven=p.isMotionGiven, denR=p.denR, KB=p.KB, KS=p.KS,
+                             psR=p.psR, Freq=p.Freq, XYZo=p.XYZo, XYZAmpl=p.XYZAmpl, XYZPhi=p.XYZPhi, AoAo=p.AoAo,
+                             AoAAmpl=p.AoAAmpl, AoAPhi=p.AoAPhi, initXYZVel=p.initXYZVel))
+    return out
+
+
+def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None):
+    """Run a ParityCase through the C-ABI on the GPU; returns the final state.
+    Bodies run on the device beam solver when device_fem (default: when any
+    plate is elastic), else with the host rigid kinematics."""
     from fsilbm3d_amd.block import LBMBlock
-    from fsilbm3d_amd.fsi import FSIBlock
+    from fsilbm3d_amd.fsi import DeviceBodyBlock, FSIBlock
     case = pc.case
     d = derived(case)
     blk = LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params)
@@ -120,17 +135,31 @@ def run_device(pc, f0=None, nsteps=None, record=None):
     blk.upload(f)
     blk.update_volume_force(0.0)
     blk.set_boundary_conditions()
-    bodies = make_bodies(pc, case.denIn)
-    fsi = FSIBlock(blk, bodies, ntolLBM=case.ntolLBM, dtolLBM=case.dtolLBM, Uref=d["Uref"], denIn=case.denIn,
-                   numsubstep=case.numsubstep)
+    if device_fem is None:
+        device_fem = any(p.iBodyModel == 2 for p in pc.plates)
+    if device_fem and pc.plates:
+        bodies = pc.plates
+        fsi = DeviceBodyBlock(blk, plate_specs(pc), ntolLBM=case.ntolLBM, dtolLBM=case.dtolLBM, Uref=d["Uref"],
+                              denIn=case.denIn, Lref=case.Lref, numsubstep=case.numsubstep,
+                              IBPenaltyAlpha=case.IBPenaltyAlpha, dampK=case.dampK, dampM=case.dampM,
+                              GeoGamma=case.GeoGamma, NewmarkGamma=case.NewmarkGamma, NewmarkBeta=case.NewmarkBeta,
+                              dtolFEM=case.dtolFEM, ntolFEM=case.ntolFEM)
+    else:
+        bodies = make_bodies(pc, case.denIn)
+        fsi = FSIBlock(blk, bodies, ntolLBM=case.ntolLBM, dtolLBM=case.dtolLBM, Uref=d["Uref"], denIn=case.denIn,
+                       numsubstep=case.numsubstep)
     n = pc.nsteps if nsteps is None else nsteps
     time = 0.0
     for _ in range(n):
         time = time + case.dh
         fsi.step(time)
         if record is not None and bodies:
-            record.append(dict(iters=fsi.iters, Eforce=fsi.Eforce[:fsi.M].copy(),
-                               lod=[fsi.body_loads(i).copy() for i in range(len(bodies))]))
+            rec = dict(iters=fsi.iters, Eforce=fsi.Eforce[:fsi.M].copy(),
+                       lod=[fsi.body_loads(i).copy() for i in range(len(bodies))])
+            if isinstance(fsi, DeviceBodyBlock):
+                rec["beam"] = [fsi.state(i) for i in range(len(bodies))]
+                rec["fem_info"] = [fsi.info(i) for i in range(len(bodies))]
+            record.append(rec)
     out = blk.download()
     blk.close()
     return out

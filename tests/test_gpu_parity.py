@@ -17,8 +17,12 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("pc", CASES, ids=[c.name for c in CASES])
+RIGID = [c for c in CASES if not any(p.iBodyModel == 2 for p in c.plates)]
+
+
+@pytest.mark.parametrize("pc", RIGID, ids=[c.name for c in RIGID])
 def test_device_matches_reference_fixture(pc):
+    """Every fluid / rigid-body fixture bit for bit (elastic bodies: test_gpu_fem.py)."""
     _say(f"[gpu] fixture case {pc.name} dims={pc.case.dims} steps={pc.nsteps}")
     fx = load_fixture(pc.name)
     f = run_device(pc)
