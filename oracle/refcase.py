@@ -62,6 +62,18 @@ class Group:
 
 
 @dataclasses.dataclass
+class SubBlock:
+    """A further fluid block (FluidBlocks section, FluidDomain.f90:69-85):
+    a refined son block (dh = father dh / 2) with BCfluid (0) faces."""
+    dims: Sequence[int]
+    dh: float
+    origin: Sequence[float]
+    bc: Sequence[int] = (0, 0, 0, 0, 0, 0)
+    collide: int = 1
+    params: Sequence[float] = (0.0,) * 10
+
+
+@dataclasses.dataclass
 class Case:
     dims: Sequence[int] = (16, 8, 8)            # xDim, yDim, zDim
     dh: float = 1.0
@@ -99,6 +111,12 @@ class Case:
     groups: List[Group] = dataclasses.field(default_factory=list)
     npsize: int = 1
     info_every: Optional[int] = None    # write DatInfo every this many steps
+    sons: List[SubBlock] = dataclasses.field(default_factory=list)
+    interpolateScheme: int = 1
+
+    def blocks(self):
+        """(dims, dh, origin) of every block, the root first."""
+        return [(tuple(self.dims), self.dh, tuple(self.origin))] + [(tuple(b.dims), b.dh, tuple(b.origin)) for b in self.sons]
 
 
 def _f(x: float) -> str:
@@ -178,7 +196,7 @@ def inflow_text(case: Case, nsteps: int, restart: bool = True) -> str:
     a("# ntolLBM dtolLBM")
     a(f"{case.ntolLBM} {_f(case.dtolLBM)}")
     a("# interpolateScheme")
-    a("1")
+    a(str(case.interpolateScheme))
     a("SolidBody")
     a(f"{_f(case.IBPenaltyAlpha)} {_f(case.GeoGamma)}")
     a(f"{_f(case.NewmarkGamma)} {_f(case.NewmarkBeta)}")
@@ -206,17 +224,23 @@ def inflow_text(case: Case, nsteps: int, restart: bool = True) -> str:
         if gi < len(case.groups) - 1:
             a("=")
     a("FluidBlocks")
-    a("1")
-    a("# ID iCollideModel offsetOutput outputtype")
-    a(f"1 {case.collide} 0 0")
-    a("# xDim yDim zDim")
-    a(" ".join(str(int(d)) for d in case.dims))
-    a("# dh xmin ymin zmin")
-    a(f"{_f(case.dh)} {_v(case.origin)}")
-    a("# BndConds")
-    a(" ".join(str(int(b)) for b in case.bc))
-    a("# params")
-    a(_v(case.params))
+    a(str(1 + len(case.sons)))
+    blocks = [(1, case.collide, case.dims, case.dh, case.origin, case.bc, case.params)]
+    for i, b in enumerate(case.sons):
+        blocks.append((2 + i, b.collide, b.dims, b.dh, b.origin, b.bc, b.params))
+    for k, (bid, col, dims, dh, org, bc, prm) in enumerate(blocks):
+        a("# ID iCollideModel offsetOutput outputtype")
+        a(f"{bid} {col} 0 0")
+        a("# xDim yDim zDim")
+        a(" ".join(str(int(d)) for d in dims))
+        a("# dh xmin ymin zmin")
+        a(f"{_f(dh)} {_v(org)}")
+        a("# BndConds")
+        a(" ".join(str(int(v)) for v in bc))
+        a("# params")
+        a(_v(prm))
+        if k < len(blocks) - 1:
+            a("=")
     a("ProbingFluid")
     a("0 1")
     a("ProbingSolid")
@@ -244,27 +268,41 @@ def plate_text(nodes_xyz, lspan, rspan, direction, elements_nspan, constraint_fi
     return "\n".join(L) + "\n"
 
 
-def write_continue(path: str, f: np.ndarray, case: Case, step: int = 0, time_over_tref: float = 0.0):
-    """f has shape (19, xDim, yDim, zDim), C order: the Fortran fIn(z,y,x,q)
-    memory order."""
-    xd, yd, zd = case.dims
-    assert f.shape == (19, xd, yd, zd) and f.dtype == np.float64
+def write_continue(path: str, fs, case: Case, step: int = 0, time_over_tref: float = 0.0):
+    """fs: one state per block (root first), each of shape (19, xDim, yDim,
+    zDim) in C order = the Fortran fIn(z,y,x,q) memory order."""
+    if isinstance(fs, np.ndarray):
+        fs = [fs]
+    blocks = case.blocks()
+    assert len(fs) == len(blocks)
     with open(path, "wb") as fh:
-        fh.write(np.array([1, step], dtype="<i4").tobytes())
+        fh.write(np.array([len(blocks), step], dtype="<i4").tobytes())
         fh.write(np.array([time_over_tref], dtype="<f8").tobytes())
-        fh.write(np.array([case.origin[0], case.origin[1], case.origin[2], case.dh], dtype="<f8").tobytes())
-        fh.write(np.array([xd, yd, zd], dtype="<i4").tobytes())
-        fh.write(np.ascontiguousarray(f).astype("<f8").tobytes())
+        for f, (dims, dh, org) in zip(fs, blocks):
+            xd, yd, zd = dims
+            assert f.shape == (19, xd, yd, zd) and f.dtype == np.float64
+            fh.write(np.array([org[0], org[1], org[2], dh], dtype="<f8").tobytes())
+            fh.write(np.array([xd, yd, zd], dtype="<i4").tobytes())
+            fh.write(np.ascontiguousarray(f).astype("<f8").tobytes())
 
 
 def read_continue(path: str, dims):
-    xd, yd, zd = dims
+    """Returns (step, time, f) for a single-block file; for several blocks f
+    is the list of states (dims: list of block dims)."""
     raw = open(path, "rb").read()
     nb, step = np.frombuffer(raw[:8], dtype="<i4")
     time = np.frombuffer(raw[8:16], dtype="<f8")[0]
-    off = 16 + 32 + 12
-    f = np.frombuffer(raw[off:off + 19 * xd * yd * zd * 8], dtype="<f8").reshape(19, xd, yd, zd).copy()
-    return int(step), float(time), f
+    multi = isinstance(dims[0], (tuple, list))
+    dl = list(dims) if multi else [dims]
+    off = 16
+    out = []
+    for d in dl:
+        xd, yd, zd = d
+        off += 32 + 12
+        n = 19 * xd * yd * zd
+        out.append(np.frombuffer(raw[off:off + n * 8], dtype="<f8").reshape(19, xd, yd, zd).copy())
+        off += n * 8
+    return int(step), float(time), (out if multi else out[0])
 
 
 @dataclasses.dataclass
@@ -306,7 +344,8 @@ def run_reference(case: Case, f0: np.ndarray, nsteps: int, keep: bool = False,
         outs = sorted(x for x in os.listdir(os.path.join(wd, "DatContinue")) if x != "continue")
         if not outs:
             raise RuntimeError("reference wrote no continue file\n" + p.stdout[-3000:])
-        step, time, f = read_continue(os.path.join(wd, "DatContinue", outs[-1]), case.dims)
+        dims = [b[0] for b in case.blocks()] if case.sons else case.dims
+        step, time, f = read_continue(os.path.join(wd, "DatContinue", outs[-1]), dims)
         secs = [float(m) for m in re.findall(r"Time for\s+one step:\s*([-+0-9.Ee]+)", p.stdout)]
         return RefResult(f=f, step=step, time=time, stdout=p.stdout, workdir=wd, step_seconds=secs)
     finally:

@@ -20,7 +20,7 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
-from oracle.refcase import Case, Group, plate_text  # noqa: E402
+from oracle.refcase import Case, Group, SubBlock, plate_text  # noqa: E402
 
 WT = np.array([1.0 / 3.0] + [1.0 / 18.0] * 6 + [1.0 / 36.0] * 12)
 EE = np.array([[0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0],
@@ -121,8 +121,21 @@ class ParityCase:
     ubase: tuple = (0.0, 0.0, 0.0)
     plates: List[Plate] = dataclasses.field(default_factory=list)
 
-    def initial_state(self) -> np.ndarray:
-        return seeded_state(self.case.dims, self.seed, self.drho, self.du, self.neq, self.ubase)
+    def initial_state(self):
+        f = seeded_state(self.case.dims, self.seed, self.drho, self.du, self.neq, self.ubase)
+        if not self.case.sons:
+            return f
+        out = [f]
+        for k, sb in enumerate(self.case.sons):
+            fs = seeded_state(sb.dims, self.seed * 7919 + k + 1, self.drho, self.du, self.neq, self.ubase)
+            # father nodes that coincide with son nodes carry the son's values, as
+            # the reference's restart reads them from the finer block
+            r = int(round(self.case.dh / sb.dh))
+            o = [int(round((sb.origin[a] - self.case.origin[a]) / self.case.dh)) for a in range(3)]
+            n = [(sb.dims[a] - 1) // r + 1 for a in range(3)]
+            f[:, o[0]:o[0] + n[0], o[1]:o[1] + n[1], o[2]:o[2] + n[2]] = fs[:, ::r, ::r, ::r]
+            out.append(fs)
+        return out
 
     def full_case(self) -> Case:
         c = dataclasses.replace(self.case)
@@ -190,9 +203,18 @@ CASES: List[ParityCase] = [
                nsteps=5, seed=35, ubase=(0.05, 0, 0), neq=0.002,
                plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.0, 0.0), Freq=0.01,
                              iBodyModel=2, denR=1.0, KB=0.1, KS=1000.0)]),
+    ParityCase("twoblock_linear", _open(dims=(16, 12, 12), bc=(101, 103, 302, 302, 302, 302),
+                                        sons=[SubBlock(dims=(13, 11, 11), dh=0.5, origin=(4.0, 3.0, 3.0))]),
+               nsteps=3, seed=41, ubase=(0.05, 0, 0), neq=0.01),
+    ParityCase("twoblock_quartic_periodic", _periodic(dims=(16, 12, 12), interpolateScheme=2,
+                                                      volumeForceIn=(1e-5, 0.0, 0.0),
+                                                      sons=[SubBlock(dims=(13, 13, 11), dh=0.5, origin=(5.0, 2.0, 3.0))]),
+               nsteps=3, seed=42, neq=0.01),
 ]
 
 BY_NAME: Dict[str, ParityCase] = {c.name: c for c in CASES}
+SINGLE = [c for c in CASES if not c.case.sons]      # one fluid block
+TREE = [c for c in CASES if c.case.sons]            # a root block with refined sons
 
 
 def fixture_path(name: str) -> str:

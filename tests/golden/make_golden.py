@@ -27,7 +27,12 @@ def make(pc):
     if pc.plates:
         case.info_every = 1
     r = run_reference(case, pc.initial_state(), pc.nsteps, keep=True)
-    out = {"f_final": r.f, "step": np.array(r.step)}
+    if isinstance(r.f, list):
+        out = {"f_final": r.f[0], "step": np.array(r.step)}
+        for k, fs in enumerate(r.f[1:]):
+            out[f"f_final_son{k + 1}"] = fs
+    else:
+        out = {"f_final": r.f, "step": np.array(r.step)}
     for g in range(len(pc.plates)):
         p = os.path.join(r.workdir, "DatInfo", f"Group{g + 1:03d}_forces.dat")
         rows = [l.split() for l in open(p) if l.strip() and not l.lstrip().startswith(("ZONE", "VARIABLES", "TITLE", "variables"))]

@@ -108,6 +108,47 @@ def run_oracle(pc, f0=None, nsteps=None, record=None):
     return f
 
 
+def son_derived(case, sb):
+    """Omega / Omega2 / MRT matrices of a son block (viscosity of the root)."""
+    nu = derived(case)["nu"]
+    tau, om = P.srt(nu, sb.dh)
+    om2 = P.trt_omega2(om, sb.params[0]) if sb.collide == 2 else om
+    Mc = Mf = None
+    if sb.collide == 3:
+        a, b = P.mrt_matrices(om)
+        Mc, Mf = P.fortran_order(a), P.fortran_order(b)
+    return dict(nu=nu, tau=tau, Omega=om, Omega2=om2, Mc=Mc, Mf=Mf)
+
+
+def run_oracle_tree(pc, fs0=None, nsteps=None):
+    """Run a father/son ParityCase on the oracle tree; returns [father, sons...]."""
+    from oracle.oracle import OracleBlock
+    from oracle.oracle_tree import OracleTree, comm_pair
+    case = pc.case
+    d = derived(case)
+    root = OracleBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["Omega"], d["Omega2"], d["Mc"],
+                       d["Mf"], **flow_kwargs(case))
+    sons, pairs, taus = [], [], [d["tau"]]
+    for sb in case.sons:
+        sd = son_derived(case, sb)
+        sons.append(OracleBlock(sb.dims, sb.dh, sb.origin, sb.bc, sb.collide, sd["Omega"], sd["Omega2"], sd["Mc"],
+                                sd["Mf"], **flow_kwargs(case)))
+        per = [int(sb.bc[2 * a] == 301) for a in range(3)]
+        pairs.append(comm_pair((case.dims, case.dh, case.origin), (sb.dims, sb.dh, sb.origin), sb.bc, per))
+        taus.append(sd["tau"])
+    fs = [np.array(a, dtype=np.float64, copy=True) for a in (pc.initial_state() if fs0 is None else fs0)]
+    root.init_boundary(fs[0], 0.0)
+    for k, sb in enumerate(sons):
+        sb.init_boundary(fs[1 + k], 0.0)
+    tree = OracleTree(root, sons, pairs, taus, case.interpolateScheme)
+    n = pc.nsteps if nsteps is None else nsteps
+    time = 0.0
+    for _ in range(n):
+        time = time + case.dh
+        tree.step(fs, time)
+    return fs
+
+
 def plate_specs(pc):
     from fsilbm3d_amd.fsi import PlateSpec
     out = []

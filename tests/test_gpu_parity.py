@@ -17,7 +17,7 @@ def _say(msg):
     sys.stdout.flush()
 
 
-RIGID = [c for c in CASES if not any(p.iBodyModel == 2 for p in c.plates)]
+RIGID = [c for c in CASES if not c.case.sons and not any(p.iBodyModel == 2 for p in c.plates)]
 
 
 @pytest.mark.parametrize("pc", RIGID, ids=[c.name for c in RIGID])
