@@ -41,6 +41,13 @@ SIGNATURES = {
     "fsil_ibm_interaction_force": (_i, [_i, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
     "fsil_ibm_band_info": (_i, [_i, _pi, _pi]),
     "fsil_copy_bandwidth": (_i, [C.c_longlong, _i, _pd]),
+    "fsil_tree_add_son": (_i, [_i, _i, _i, _pi]),
+    "fsil_tree_pair_info": (_i, [_i, _i, _pi]),
+    "fsil_extract_interpolate_layer": (_i, [_i, _i]),
+    "fsil_interpolation_father_to_son": (_i, [_i, _i, _i]),
+    "fsil_deliver_son_to_father": (_i, [_i, _i]),
+    "fsil_tree_steps": (_i, [_i, _i, _d, _d]),
+    "fsil_time_tree_steps": (_i, [_i, _i, _d, _d, _pd]),
     "fsil_fem_define_bodies": (_i, [_i, _i, _pi, _pi, _pd, _pi, _pi, _pi, _pi, _pd, _pd, _d, _d, _d, _d]),
     "fsil_fem_set_solver": (_i, [_i, _d, _d, _d, _d, _d, _d, _i]),
     "fsil_fem_update_markers": (_i, [_i]),

@@ -120,9 +120,12 @@ class LBMBlock:
         _lib.check(self.lib.fsil_collide_stream_bc(self.handle))
 
     def steps(self, nsteps: int, t0: float, dt: float):
-        """nsteps pure-fluid steps (no immersed bodies) at times t0+dt, t0+2dt, ..."""
+        """nsteps pure-fluid steps (no immersed bodies), the time accumulated from t0 by dt."""
         _lib.check(self.lib.fsil_steps(self.handle, int(nsteps), float(t0), float(dt)))
-        self.blktime = t0 + dt * nsteps
+        t = float(t0)
+        for _ in range(int(nsteps)):
+            t = t + dt
+        self.blktime = t
 
     def time_collide(self, nlaunch: int) -> float:
         ms = np.zeros(1)

@@ -20,6 +20,7 @@
 #include "fsil_lbm_kernels.h"
 #include "fsil_lbm32_kernels.h"
 #include "fsil_fem_kernels.h"
+#include "fsil_tree_kernels.h"
 
 using namespace fsil;
 
@@ -115,6 +116,29 @@ struct Fem {
   bool markers_on_device = false;
 };
 
+// A stream shared by the blocks of one tree (father and sons run in order).
+struct StreamHolder {
+  hipStream_t s = nullptr;
+  ~StreamHolder() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+// Father/son pair of the block tree (CommPair, LBMBlockComm.f90:11-18); the
+// index arrays are kept 1-based as in the reference.
+struct Pair {
+  int son = 0;  // son block handle
+  int sds[6], s[6], f[6], si[6], fi[6];
+  int dimS[3], dimF[3];
+  int scheme = 1;
+  DevBuf<double> planes;  // per active face: level-1 and level-2 coarse planes
+  long long off[6] = {0, 0, 0, 0, 0, 0};
+  int extracted = 0;      // 2 once both levels hold planes of the same father step
+};
+
 struct Block {
   int id = 0;
   int xd = 0, yd = 0, zd = 0;
@@ -138,6 +162,10 @@ struct Block {
   bool hw_valid[6] = {false, false, false, false, false, false};
   Ibm ibm;
   Fem fem;
+  // block tree: the father's handle (0 for a root) and the son pairs
+  int father = 0;
+  std::vector<std::unique_ptr<Pair>> sons;
+  std::shared_ptr<StreamHolder> sh;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
@@ -623,7 +651,6 @@ int fsil_finalize(void) {
     if (b.stream) (void)hipStreamSynchronize(b.stream);
     if (b.ev0) (void)hipEventDestroy(b.ev0);
     if (b.ev1) (void)hipEventDestroy(b.ev1);
-    if (b.stream) (void)hipStreamDestroy(b.stream);
   }
   g_blocks.clear();
   return 0;
@@ -671,7 +698,9 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   b.model = iCollidModel;
   b.tau = relax[0]; b.Omega = relax[1]; b.Omega2 = relax[2];
   b.precision = precision;
-  HIPCHK(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+  b.sh = std::make_shared<StreamHolder>();
+  HIPCHK(hipStreamCreateWithFlags(&b.sh->s, hipStreamNonBlocking));
+  b.stream = b.sh->s;
   HIPCHK(hipEventCreate(&b.ev0));
   HIPCHK(hipEventCreate(&b.ev1));
   const size_t nf = (size_t)Q * b.N;
@@ -709,7 +738,19 @@ int fsil_block_destroy(int h) {
   (void)hipStreamSynchronize(b->stream);
   (void)hipEventDestroy(b->ev0);
   (void)hipEventDestroy(b->ev1);
-  (void)hipStreamDestroy(b->stream);
+  // drop the pair that names this block as a son, and detach its own sons
+  if (b->father) {
+    auto it = g_blocks.find(b->father);
+    if (it != g_blocks.end()) {
+      auto &v = it->second->sons;
+      v.erase(std::remove_if(v.begin(), v.end(), [h](const std::unique_ptr<Pair> &p) { return p->son == h; }),
+              v.end());
+    }
+  }
+  for (auto &p : b->sons) {
+    auto it = g_blocks.find(p->son);
+    if (it != g_blocks.end()) it->second->father = 0;
+  }
   g_blocks.erase(h);
   return 0;
 }
@@ -858,8 +899,10 @@ int fsil_collide_stream_bc(int h) {
 int fsil_steps(int h, int nsteps, double t0, double dt) {
   Block *b = get_block(h);
   if (!b) return 1;
+  double t = t0;
   for (int s = 0; s < nsteps; ++s) {
-    b->blktime = t0 + dt * (double)(s + 1);
+    t = t + dt;  // main.f90:95 accumulates the time
+    b->blktime = t;
     update_vf(*b);
     if (fsil_collide_stream_bc(h)) return 1;
   }
@@ -1302,6 +1345,309 @@ int fsil_ibm_band_info(int h, int *nband, int *bbox) {
     bbox[0] = b->ibm.bx0; bbox[1] = b->ibm.by0; bbox[2] = b->ibm.bz0;
     bbox[3] = b->ibm.bnx; bbox[4] = b->ibm.bny; bbox[5] = b->ibm.bnz;
   }
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// block tree (LBMBlockComm.f90)
+namespace {
+
+void share_stream(Block &b, const std::shared_ptr<StreamHolder> &sh) {
+  if (b.stream != sh->s) (void)hipStreamSynchronize(b.stream);
+  b.sh = sh;
+  b.stream = sh->s;
+  for (auto &p : b.sons) {
+    auto it = g_blocks.find(p->son);
+    if (it != g_blocks.end()) share_stream(*it->second, sh);
+  }
+}
+
+void face_extents(const Pair &p, int face, int &aF, int &bF, int &aS, int &bS) {
+  const int axis = face >> 1;
+  const int ia = axis == 0 ? 1 : 0, ib = axis == 2 ? 1 : 2;
+  aF = p.dimF[ia]; bF = p.dimF[ib];
+  aS = p.dimS[ia]; bS = p.dimS[ib];
+}
+
+TreeArgs tree_args(Block &F, Block &S, const Pair &p) {
+  TreeArgs t{};
+  t.ff = F.fA.p; t.fs = S.fA.p; t.planes = p.planes.p;
+  t.fxd = F.xd; t.fyd = F.yd; t.fzd = F.zd; t.fN = F.N;
+  t.sxd = S.xd; t.syd = S.yd; t.szd = S.zd; t.sN = S.N;
+  for (int a = 0; a < 3; ++a) {
+    t.f0[a] = p.f[2 * a] - 1;
+    t.fi0[a] = p.fi[2 * a] - 1;
+    t.si0[a] = p.si[2 * a] - 1;
+    t.sicnt[a] = p.si[2 * a + 1] >= p.si[2 * a] ? (p.si[2 * a + 1] - p.si[2 * a]) / 2 + 1 : 0;
+  }
+  for (int face = 0; face < 6; ++face) {
+    TreeFace &tf = t.face[face];
+    tf.active = p.sds[face] != 0;
+    tf.fixedF = p.f[face] - 1;
+    tf.fixedS = p.s[face] - 1;
+    tf.fixedFi = p.fi[face] - 1;
+    tf.fixedSi = p.si[face] - 1;
+    face_extents(p, face, tf.aF, tf.bF, tf.aS, tf.bS);
+    tf.off = p.off[face];
+  }
+  return t;
+}
+
+int tree_son(const Block &F, int index, Block *&S, Pair *&p) {
+  if (index < 0 || index >= (int)F.sons.size())
+    return fail("block %d has no son %d (%d sons)", F.id, index, (int)F.sons.size());
+  p = F.sons[index].get();
+  S = get_block(p->son);
+  return S ? 0 : 1;
+}
+
+int extract_layer(Block &F, int level) {
+  for (auto &pp : F.sons) {
+    Pair &p = *pp;
+    Block *S = get_block(p.son);
+    if (!S) return 1;
+    TreeArgs t = tree_args(F, *S, p);
+    int maxn = 0;
+    for (int face = 0; face < 6; ++face)
+      if (t.face[face].active) maxn = std::max(maxn, Q * t.face[face].aF * t.face[face].bF);
+    if (maxn == 0) continue;
+    hipLaunchKernelGGL(k_tree_extract, dim3((maxn + 255) / 256, 6), dim3(256), 0, F.stream, t, level);
+    HIPCHK(hipGetLastError());
+    p.extracted = level;
+  }
+  return 0;
+}
+
+int father_to_son(Block &F, Block &S, Pair &p, int n_timeStep) {
+  if (p.extracted != 2) return fail("father planes of block %d are not extracted at both levels", F.id);
+  TreeArgs t = tree_args(F, S, p);
+  // tau_all is the block's tau at every node for collision models 1-3, so the
+  // interpolated father tau is the father's tau exactly
+  t.coeff = (S.tau / F.tau) / 2.0;
+  for (int k = 0; k < 3; ++k) t.hv[k] = 0.5 * F.VF[k] * F.dh;
+  int maxn = 0;
+  for (int face = 0; face < 6; ++face)
+    if (t.face[face].active) maxn = std::max(maxn, t.face[face].aS * t.face[face].bS);
+  if (maxn == 0) return 0;
+  const dim3 grid((maxn + 255) / 256, 6);
+  const int level = n_timeStep == 0 ? 1 : 2;
+  if (p.scheme == 2) hipLaunchKernelGGL(k_tree_father_to_son<2>, grid, dim3(256), 0, F.stream, t, level);
+  else hipLaunchKernelGGL(k_tree_father_to_son<1>, grid, dim3(256), 0, F.stream, t, level);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int son_to_father(Block &F, Block &S, Pair &p) {
+  TreeArgs t = tree_args(F, S, p);
+  t.coeff = (F.tau / S.tau) * 2.0;
+  for (int k = 0; k < 3; ++k) t.hv[k] = 0.5 * S.VF[k] * S.dh;
+  int maxn = 0;
+  for (int face = 0; face < 6; ++face) {
+    if (!t.face[face].active) continue;
+    const int axis = face >> 1;
+    const int ea = axis == 0 ? 1 : 0, eb = axis == 2 ? 1 : 2;
+    maxn = std::max(maxn, t.sicnt[ea] * t.sicnt[eb]);
+  }
+  if (maxn == 0) return 0;
+  hipLaunchKernelGGL(k_tree_son_to_father, dim3((maxn + 255) / 256, 6), dim3(256), 0, F.stream, t);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void set_tree_time(Block &b, double t) {
+  b.blktime = t;
+  for (auto &p : b.sons) {
+    auto it = g_blocks.find(p->son);
+    if (it != g_blocks.end()) set_tree_time(*it->second, t);
+  }
+}
+
+// tree_collision_streaming_IBM_FEM for blocks without bodies
+int tree_step(Block &b) {
+  if (b.ibm.nbodies > 0 || b.fem.nbodies > 0)
+    return fail("block %d carries bodies: step it from the host with the IBM/FEM calls", b.id);
+  update_vf(b);
+  if (extract_layer(b, 1)) return 1;
+  if (collide_stream(b)) return 1;
+  if (launch_faces(b)) return 1;
+  b.ibm.active = false;
+  if (extract_layer(b, 2)) return 1;
+  for (auto &pp : b.sons) {
+    Block *S = get_block(pp->son);
+    if (!S) return 1;
+    for (int n = 0; n < 2; ++n) {
+      S->blktime = S->blktime + (double)n * S->dh;
+      if (tree_step(*S)) return 1;
+      if (father_to_son(b, *S, *pp, n)) return 1;
+    }
+    if (son_to_father(b, *S, *pp)) return 1;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsil_tree_add_son(int father, int son, int interpolateScheme, int *index) {
+  Block *F = get_block(father);
+  if (!F) return 1;
+  Block *S = get_block(son);
+  if (!S) return 1;
+  if (father == son) return fail("a block cannot be its own son");
+  if (S->father) return fail("block %d already has a father (%d)", son, S->father);
+  for (int a = father; a;) {
+    if (a == son) return fail("block %d is an ancestor of block %d", son, father);
+    auto it = g_blocks.find(a);
+    a = it == g_blocks.end() ? 0 : it->second->father;
+  }
+  if (F->precision != 0 || S->precision != 0) return fail("block trees use the FP64 plain form");
+  // check_blocks_params (LBMBlockComm.f90:508-544)
+  int r[3], per[3];
+  for (int a = 0; a < 3; ++a) {
+    per[a] = S->bc[2 * a] == BC_PERIODIC;
+    r[a] = per[a] ? 0 : 1;
+  }
+  const bool flag = std::fabs(F->dh - S->dh * 2.0) > 1e-8 || S->xd % 2 != r[0] || S->yd % 2 != r[1] ||
+                    S->zd % 2 != r[2];
+  double res1 = (S->xmin - F->xmin) / F->dh + (S->ymin - F->ymin) / F->dh + (S->zmin - F->zmin) / F->dh;
+  double res2 = (S->xmax - F->xmin) / F->dh + (S->ymax - F->ymin) / F->dh + (S->zmax - F->zmin) / F->dh;
+  res1 = std::fabs(res1 - std::nearbyint(res1));
+  res2 = std::fabs(res2 - std::nearbyint(res2));
+  if (flag || res1 + res2 > 1e-8)
+    return fail("grid points do not match between fluid blocks %d %d (%g %g); a son block needs an even number "
+                "of grid points along periodic axes and an odd number otherwise",
+                F->id, S->id, res1, res2);
+  // build_blocks_comunication (LBMBlockComm.f90:47-95)
+  auto pp = std::make_unique<Pair>();
+  Pair &p = *pp;
+  p.son = son;
+  p.scheme = interpolateScheme == 2 ? 2 : 1;
+  const int sd[3] = {S->xd - per[0], S->yd - per[1], S->zd - per[2]};
+  const double smin[3] = {S->xmin, S->ymin, S->zmin}, fmin[3] = {F->xmin, F->ymin, F->zmin};
+  const int ratio = (int)std::floor(F->dh / S->dh + 0.5);
+  for (int j = 0; j < 6; ++j) p.sds[j] = S->bc[j] == BC_FLUID ? ((j & 1) ? -1 : 1) : 0;
+  for (int a = 0; a < 3; ++a) {
+    p.s[2 * a] = 1;
+    p.s[2 * a + 1] = sd[a];
+    p.f[2 * a] = (int)std::floor((smin[a] - fmin[a]) / F->dh + 1.5);
+    p.f[2 * a + 1] = p.f[2 * a] + (sd[a] - 1) / ratio;
+  }
+  for (int j = 0; j < 6; ++j) {
+    p.si[j] = p.s[j] + p.sds[j] * ratio;
+    p.fi[j] = p.f[j] + p.sds[j];
+  }
+  p.dimS[0] = S->xd; p.dimS[1] = S->yd; p.dimS[2] = S->zd;
+  for (int a = 0; a < 3; ++a) p.dimF[a] = p.f[2 * a + 1] - p.f[2 * a] + 1;
+  // every index the exchange touches must lie inside both blocks
+  const int fdim[3] = {F->xd, F->yd, F->zd}, sdim[3] = {S->xd, S->yd, S->zd};
+  for (int a = 0; a < 3; ++a) {
+    if (p.f[2 * a] < 1 || p.f[2 * a + 1] > fdim[a])
+      return fail("son block %d does not lie inside father block %d along axis %d", S->id, F->id, a + 1);
+    const int cnt = p.si[2 * a + 1] >= p.si[2 * a] ? (p.si[2 * a + 1] - p.si[2 * a]) / 2 + 1 : 0;
+    if (cnt > 0 && (p.si[2 * a] < 1 || p.si[2 * a + 1] > sdim[a] || p.fi[2 * a] < 1 ||
+                    p.fi[2 * a] + cnt - 1 > fdim[a]))
+      return fail("son block %d: inner layers outside the blocks along axis %d", S->id, a + 1);
+  }
+  long long total = 0;
+  for (int face = 0; face < 6; ++face) {
+    if (!p.sds[face]) continue;
+    const int a = face >> 1;
+    if (p.s[face] < 1 || p.s[face] > sdim[a] || p.si[face] < 1 || p.si[face] > sdim[a] || p.f[face] < 1 ||
+        p.f[face] > fdim[a] || p.fi[face] < 1 || p.fi[face] > fdim[a])
+      return fail("son block %d: face %d layers outside the blocks", S->id, face + 1);
+    int aF, bF, aS, bS;
+    face_extents(p, face, aF, bF, aS, bS);
+    const int aT = (aS & 1) ? aS : aS - 1, bT = (bS & 1) ? bS : bS - 1;
+    if (p.scheme == 2 && (aT < 5 || bT < 5))
+      return fail("4th-order interpolation needs son faces of at least 5 nodes (face %d is %d x %d)", face + 1, aS,
+                  bS);
+    if ((aT + 1) / 2 > aF || (bT + 1) / 2 > bF)
+      return fail("son face %d does not match the father plane", face + 1);
+    p.off[face] = total;
+    total += 2LL * Q * aF * bF;
+  }
+  if (total > 0) HIPCHK(p.planes.ensure((size_t)total));
+  share_stream(*S, F->sh);
+  S->father = father;
+  F->sons.push_back(std::move(pp));
+  if (index) *index = (int)F->sons.size() - 1;
+  return 0;
+}
+
+int fsil_tree_pair_info(int father, int index, int *comm) {
+  Block *F = get_block(father);
+  if (!F) return 1;
+  Block *S;
+  Pair *p;
+  if (tree_son(*F, index, S, p)) return 1;
+  if (!comm) return fail("null comm");
+  for (int j = 0; j < 6; ++j) {
+    comm[j] = p->sds[j];
+    comm[6 + j] = p->s[j];
+    comm[12 + j] = p->f[j];
+    comm[18 + j] = p->si[j];
+    comm[24 + j] = p->fi[j];
+  }
+  for (int a = 0; a < 3; ++a) {
+    comm[30 + a] = p->dimS[a];
+    comm[33 + a] = p->dimF[a];
+  }
+  return 0;
+}
+
+int fsil_extract_interpolate_layer(int father, int level) {
+  Block *F = get_block(father);
+  if (!F) return 1;
+  if (level != 1 && level != 2) return fail("extract level must be 1 (old time) or 2 (new time)");
+  return extract_layer(*F, level);
+}
+
+int fsil_interpolation_father_to_son(int father, int index, int n_timeStep) {
+  Block *F = get_block(father);
+  if (!F) return 1;
+  Block *S;
+  Pair *p;
+  if (tree_son(*F, index, S, p)) return 1;
+  if (n_timeStep != 0 && n_timeStep != 1) return fail("n_timeStep must be 0 or 1");
+  return father_to_son(*F, *S, *p, n_timeStep);
+}
+
+int fsil_deliver_son_to_father(int father, int index) {
+  Block *F = get_block(father);
+  if (!F) return 1;
+  Block *S;
+  Pair *p;
+  if (tree_son(*F, index, S, p)) return 1;
+  return son_to_father(*F, *S, *p);
+}
+
+int fsil_tree_steps(int root, int nsteps, double t0, double dt) {
+  Block *b = get_block(root);
+  if (!b) return 1;
+  if (b->father) return fail("block %d is not a tree root", root);
+  double t = t0;
+  for (int s = 0; s < nsteps; ++s) {
+    t = t + dt;
+    set_tree_time(*b, t);  // main.f90:97
+    if (tree_step(*b)) return 1;
+  }
+  return 0;
+}
+
+int fsil_time_tree_steps(int root, int nsteps, double t0, double dt, double *total_ms) {
+  Block *b = get_block(root);
+  if (!b) return 1;
+  if (nsteps <= 0 || !total_ms) return fail("bad arguments to fsil_time_tree_steps");
+  HIPCHK(hipEventRecord(b->ev0, b->stream));
+  if (fsil_tree_steps(root, nsteps, t0, dt)) return 1;
+  HIPCHK(hipEventRecord(b->ev1, b->stream));
+  HIPCHK(hipEventSynchronize(b->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+  *total_ms = ms;
   return 0;
 }
 

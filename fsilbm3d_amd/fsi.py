@@ -105,11 +105,16 @@ class DeviceBodyBlock:
         _lib.check(self.lib.fsil_fem_info(self.block.handle, i, _lib.iptr(a), _lib.iptr(b), _lib.dptr(c)))
         return int(a[0]), int(b[0]), float(c[0])
 
+    def ibm_fem(self, time: float):
+        """IBM_FEM (LBMBlockComm.f90:320-338) at block time ``time``."""
+        blk = self.block
+        self.interaction_force(blk.dh)
+        _lib.check(self.lib.fsil_fem_substeps(blk.handle, float(time), self.numsubstep, blk.dh))
+
     def step(self, time: float):
         blk = self.block
         blk.update_volume_force(time)
-        self.interaction_force(blk.dh)
-        _lib.check(self.lib.fsil_fem_substeps(blk.handle, float(time), self.numsubstep, blk.dh))
+        self.ibm_fem(time)
         blk.collision_streaming_bc()
 
 
@@ -159,13 +164,18 @@ class FSIBlock:
         off = 6 * sum(b.nND for b in self.bodies[:i])
         return self.lodFlow[off:off + 6 * self.bodies[i].nND]
 
-    def step(self, time: float):
-        """One time step at block time ``time`` (main.f90:95-107)."""
+    def ibm_fem(self, time: float):
+        """IBM_FEM (LBMBlockComm.f90:320-338) at block time ``time``."""
         blk = self.block
-        blk.update_volume_force(time)
         self.interaction_force(blk.dh)
         dt_solid = blk.dh / float(self.numsubstep)
         for isub in range(1, self.numsubstep + 1):
             for b in self.bodies:
                 b.structure(time, isub, blk.dh, dt_solid)
+
+    def step(self, time: float):
+        """One time step at block time ``time`` (main.f90:95-107)."""
+        blk = self.block
+        blk.update_volume_force(time)
+        self.ibm_fem(time)
         blk.collision_streaming_bc()

@@ -160,6 +160,42 @@ int fsil_copy_bandwidth(long long nbytes, int reps, double *gbs);
 /* Band size and bounding box (x0,y0,z0,nx,ny,nz) of the last coupling call. */
 int fsil_ibm_band_info(int handle, int *nband, int *bbox);
 
+/* ---------------------------------------------------------------------------
+ * Block tree: refined son blocks (dh = father dh / 2) whose BCfluid (0) faces
+ * are fed from the father (LBMBlockComm.f90).  Tree blocks use the FP64 plain
+ * form; a son shares its father's stream. */
+
+/* Make `son` a son of `father`: check_blocks_params (LBMBlockComm.f90:508-544),
+ * the CommPair of build_blocks_comunication (:33-97) and the father planes of
+ * allocate_fIn_tau (:201-246).  interpolateScheme 2: 3rd/4th-order plane
+ * interpolation, otherwise linear (FlowCondition%interpolateScheme).
+ * *index receives the son's position among the father's sons (0-based). */
+int fsil_tree_add_son(int father, int son, int interpolateScheme, int *index);
+
+/* CommPair of son `index`: comm[0..35] = sds[6], s[6], f[6], si[6], fi[6]
+ * (1-based, as LBMBlockComm.f90:11-18), xDimS yDimS zDimS, xDimF yDimF zDimF. */
+int fsil_tree_pair_info(int father, int index, int *comm);
+
+/* extract_interpolate_layer (LBMBlockComm.f90:340-505) for every son of
+ * `father`: level 1 before the father's collision, level 2 after its boundary
+ * faces (level 2 also averages level 1 with it). */
+int fsil_extract_interpolate_layer(int father, int level);
+
+/* interpolation_father_to_son (LBMBlockComm.f90:655-806) after the son's
+ * sub-step n_timeStep (0 or 1). */
+int fsil_interpolation_father_to_son(int father, int index, int n_timeStep);
+
+/* deliver_son_to_father (LBMBlockComm.f90:546-653). */
+int fsil_deliver_son_to_father(int father, int index);
+
+/* nsteps root steps of tree_collision_streaming_IBM_FEM (LBMBlockComm.f90:279-318)
+ * for a tree whose blocks carry no bodies, time accumulated from t0 by dt and
+ * given to every block of the tree each step (main.f90:95-97). */
+int fsil_tree_steps(int root, int nsteps, double t0, double dt);
+
+/* fsil_tree_steps bracketed by device events on the tree's stream (ms). */
+int fsil_time_tree_steps(int root, int nsteps, double t0, double dt, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

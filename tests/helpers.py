@@ -206,6 +206,40 @@ def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None):
     return out
 
 
+def run_device_tree(pc, fs0=None, nsteps=None, per_call=False):
+    """Run a father/son ParityCase through the C-ABI; returns [father, sons...]
+    and the CommPair of every son.
+    per_call steps the recursion from the host mirror (one library call per
+    exchange); otherwise the whole tree step is one library call."""
+    from fsilbm3d_amd.block import LBMBlock
+    from fsilbm3d_amd.tree import BlockTree
+    case = pc.case
+    d = derived(case)
+    blocks = [LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params, block_id=1)]
+    for k, sb in enumerate(case.sons):
+        blocks.append(LBMBlock(sb.dims, sb.dh, sb.origin, sb.bc, sb.collide, d["nu"], sb.params, block_id=2 + k))
+    for b in blocks:
+        b.set_flow(**flow_kwargs(case))
+    fs = pc.initial_state() if fs0 is None else fs0
+    for b, f in zip(blocks, fs):
+        b.upload(np.asarray(f, dtype=np.float64))
+    tree = BlockTree(blocks, case.interpolateScheme)
+    tree.set_boundary_conditions()
+    n = pc.nsteps if nsteps is None else nsteps
+    if per_call:
+        time = 0.0
+        for _ in range(n):
+            time = time + case.dh
+            tree.step(time)
+    else:
+        tree.steps(n, 0.0, case.dh)
+    out = [b.download() for b in blocks]
+    pairs = {s: tree.comm_pair(s) for s in range(len(blocks)) if tree.father[s] >= 0}
+    for b in blocks[::-1]:
+        b.close()
+    return out, pairs
+
+
 def mismatch(a, b):
     d = np.abs(a - b)
     return int((a != b).sum()), float(d.max()) if d.size else 0.0

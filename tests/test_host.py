@@ -86,3 +86,28 @@ def test_pitch_rotation_matrix_is_orthonormal_and_rotates_marker_velocity():
     assert np.allclose(p.Evel, np.cross(W, r), atol=1e-12)
     # and every node shares the same angular velocity
     assert np.allclose(p.vel[:, 3:6], W, atol=0)
+
+
+def test_compare_blocks_containment():
+    """CompareBlocks (FluidDomain.f90:1845-1972): containment both ways,
+    separated blocks, and the father-adjacent son with one BCfluid face."""
+    from types import SimpleNamespace as NS
+    from fsilbm3d_amd.tree import compare_blocks
+
+    def blk(o, m, bc=(0,) * 6, dh=1.0):
+        return NS(origin=o, xmax=m[0], ymax=m[1], zmax=m[2], bc=bc, dh=dh)
+    big = blk((0, 0, 0), (15, 11, 11), bc=(101, 103, 302, 302, 302, 302))
+    son = blk((4, 3, 3), (10, 8, 8), dh=0.5)
+    far = blk((20, 20, 20), (25, 25, 25))
+    assert compare_blocks(big, son) == 1
+    assert compare_blocks(son, big) == -1
+    assert compare_blocks(son, far) == 0
+    # a son sharing the father's x-min plane is still contained (aligned side)
+    edge = blk((0, 3, 3), (6, 8, 8), dh=0.5)
+    assert compare_blocks(big, edge) == 1
+    # a son whose single BCfluid face lies one father spacing inside a
+    # BCfluid_father face is shrunk to that face and counts as contained
+    fat = blk((0, 0, 0), (15, 11, 11), bc=(101, 1, 302, 302, 302, 302))
+    over = blk((14, 3, 3), (20, 8, 8), bc=(0, 302, 302, 302, 302, 302), dh=0.5)
+    assert compare_blocks(fat, over) == 1
+    assert compare_blocks(big, over) == 0
