@@ -14,7 +14,11 @@ module fsil_bridge
               fsil_set_flow, fsil_set_time, fsil_get_volume_force, fsil_f_upload, fsil_f_download, &
               fsil_set_boundary_conditions, fsil_macro_download, fsil_collide_stream_bc, fsil_steps, &
               fsil_ibm_define_bodies, fsil_ibm_set_params, fsil_ibm_interaction_force, fsil_check, &
-              fsil_version
+              fsil_version, fsil_time_collide, fsil_time_steps, fsil_ibm_band_info, fsil_copy_bandwidth, &
+              fsil_fem_define_bodies, fsil_fem_set_solver, fsil_fem_update_markers, fsil_fem_substeps, &
+              fsil_fem_state, fsil_fem_info, fsil_tree_add_son, fsil_tree_pair_info, &
+              fsil_extract_interpolate_layer, fsil_interpolation_father_to_son, fsil_deliver_son_to_father, &
+              fsil_tree_steps, fsil_time_tree_steps
 
     interface
         integer(c_int) function fsil_version() bind(C, name='fsil_version')
@@ -112,6 +116,102 @@ module fsil_bridge
             real(c_double), intent(out) :: Eforce(*), lodFlow(*)
             integer(c_int), intent(out) :: iters
             real(c_double), intent(out) :: dmax
+        end function
+        integer(c_int) function fsil_time_collide(handle, nlaunch, avg_ms) bind(C, name='fsil_time_collide')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, nlaunch
+            real(c_double), intent(out) :: avg_ms
+        end function
+        integer(c_int) function fsil_time_steps(handle, nsteps, t0, dt, total_ms) bind(C, name='fsil_time_steps')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, nsteps
+            real(c_double), value :: t0, dt
+            real(c_double), intent(out) :: total_ms
+        end function
+        integer(c_int) function fsil_ibm_band_info(handle, nband, bbox) bind(C, name='fsil_ibm_band_info')
+            import :: c_int
+            integer(c_int), value :: handle
+            integer(c_int), intent(out) :: nband, bbox(6)
+        end function
+        integer(c_int) function fsil_copy_bandwidth(nbytes, reps, gbs) bind(C, name='fsil_copy_bandwidth')
+            import :: c_int, c_long_long, c_double
+            integer(c_long_long), value :: nbytes
+            integer(c_int), value :: reps
+            real(c_double), intent(out) :: gbs
+        end function
+        ! beam bodies on the device (SolidSolver.f90, Solidbody.f90)
+        integer(c_int) function fsil_fem_define_bodies(handle, nbodies, nND, nEL, xyz8, elems, cons, iBodyModel, &
+                                                       isMotionGiven, mat, kin, Lref, Uref, denIn, alpha) &
+                                                       bind(C, name='fsil_fem_define_bodies')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, nbodies
+            integer(c_int), intent(in) :: nND(*), nEL(*), elems(*), cons(*), iBodyModel(*), isMotionGiven(*)
+            real(c_double), intent(in) :: xyz8(*), mat(*), kin(*)
+            real(c_double), value :: Lref, Uref, denIn, alpha
+        end function
+        integer(c_int) function fsil_fem_set_solver(handle, dampK, dampM, GeoGamma, NewmarkGamma, NewmarkBeta, &
+                                                    dtolFEM, ntolFEM) bind(C, name='fsil_fem_set_solver')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, ntolFEM
+            real(c_double), value :: dampK, dampM, GeoGamma, NewmarkGamma, NewmarkBeta, dtolFEM
+        end function
+        integer(c_int) function fsil_fem_update_markers(handle) bind(C, name='fsil_fem_update_markers')
+            import :: c_int
+            integer(c_int), value :: handle
+        end function
+        integer(c_int) function fsil_fem_substeps(handle, time, numsubstep, deltat) bind(C, name='fsil_fem_substeps')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, numsubstep
+            real(c_double), value :: time, deltat
+        end function
+        integer(c_int) function fsil_fem_state(handle, body, pos, vel, acc, dsp) bind(C, name='fsil_fem_state')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, body
+            real(c_double), intent(out) :: pos(*), vel(*), acc(*), dsp(*)
+        end function
+        integer(c_int) function fsil_fem_info(handle, body, newton_iters, cg_iters, dnorm) bind(C, name='fsil_fem_info')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, body
+            integer(c_int), intent(out) :: newton_iters, cg_iters
+            real(c_double), intent(out) :: dnorm
+        end function
+        ! block tree (LBMBlockComm.f90)
+        integer(c_int) function fsil_tree_add_son(father, son, interpolateScheme, index) &
+                                                  bind(C, name='fsil_tree_add_son')
+            import :: c_int
+            integer(c_int), value :: father, son, interpolateScheme
+            integer(c_int), intent(out) :: index
+        end function
+        integer(c_int) function fsil_tree_pair_info(father, index, comm) bind(C, name='fsil_tree_pair_info')
+            import :: c_int
+            integer(c_int), value :: father, index
+            integer(c_int), intent(out) :: comm(36)
+        end function
+        integer(c_int) function fsil_extract_interpolate_layer(father, level) &
+                                                               bind(C, name='fsil_extract_interpolate_layer')
+            import :: c_int
+            integer(c_int), value :: father, level
+        end function
+        integer(c_int) function fsil_interpolation_father_to_son(father, index, n_timeStep) &
+                                                                 bind(C, name='fsil_interpolation_father_to_son')
+            import :: c_int
+            integer(c_int), value :: father, index, n_timeStep
+        end function
+        integer(c_int) function fsil_deliver_son_to_father(father, index) bind(C, name='fsil_deliver_son_to_father')
+            import :: c_int
+            integer(c_int), value :: father, index
+        end function
+        integer(c_int) function fsil_tree_steps(root, nsteps, t0, dt) bind(C, name='fsil_tree_steps')
+            import :: c_int, c_double
+            integer(c_int), value :: root, nsteps
+            real(c_double), value :: t0, dt
+        end function
+        integer(c_int) function fsil_time_tree_steps(root, nsteps, t0, dt, total_ms) &
+                                                     bind(C, name='fsil_time_tree_steps')
+            import :: c_int, c_double
+            integer(c_int), value :: root, nsteps
+            real(c_double), value :: t0, dt
+            real(c_double), intent(out) :: total_ms
         end function
     end interface
 

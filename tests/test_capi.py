@@ -57,3 +57,14 @@ def test_fortran_binding_compiles():
     out = subprocess.run([fc, "-c", "-ffree-form", src, "-module-dir", "/tmp", "-o", "/tmp/fsil_bridge.o"],
                          capture_output=True, text=True)
     assert out.returncode == 0, out.stderr
+
+
+def test_fortran_binding_covers_every_entry_point():
+    """Every function the header declares has a bind(C) interface in the
+    Fortran module, under the same name."""
+    import re
+    hdr = open(os.path.join(ROOT, "include", "fsilbm3d_amd.h")).read()
+    declared = set(re.findall(r"^(?:int|const char \*)\s*(fsil_\w+)\s*\(", hdr, re.M))
+    f90 = open(os.path.join(ROOT, "include", "fsil_bridge.f90")).read()
+    bound = set(re.findall(r"name='(fsil_\w+)'", f90))
+    assert declared and declared == bound, (declared - bound, bound - declared)

@@ -25,9 +25,9 @@ def test_device_rigid_bodies_match_reference_fixture(name):
 def test_device_elastic_plate_vs_reference_and_oracle(name):
     """Elastic plate (Newmark + Newton + PCG on the device).  The device libm
     (sin, cos, acos) may differ from the host C library in the last place, so
-    agreement is stated as a tolerance: fluid state relative 1e-9 of its range,
-    node positions 1e-9 (absolute, lattice units); Newton/CG iteration counts
-    equal on the well-conditioned case."""
+    agreement is stated as the plain form's tolerance, 1e-12 relative: fluid
+    state against the reference's fixture, node positions against the oracle;
+    Newton/CG iteration counts equal on the well-conditioned case."""
     pc = BY_NAME[name]
     print(f"[gpu] device-FEM elastic {name}", flush=True)
     ro, rd = [], []
@@ -37,9 +37,9 @@ def test_device_elastic_plate_vs_reference_and_oracle(name):
     n_ref, m_ref = mismatch(fd, fx)
     n_orc, m_orc = mismatch(fd, fo)
     print(f"[gpu] {name}: vs reference {n_ref} differ (max {m_ref:.3e}); vs oracle {n_orc} (max {m_orc:.3e})", flush=True)
-    assert m_ref <= 1e-9 * np.abs(fx).max()
+    assert m_ref <= 1e-12 * np.abs(fx).max()
     for so, sd in zip(ro, rd):
         po, pd_ = so["beam"][0][0], sd["beam"][0][0]
-        assert np.abs(po - pd_).max() <= 1e-9, np.abs(po - pd_).max()
+        assert np.abs(po - pd_).max() <= 1e-12 * np.abs(po).max(), np.abs(po - pd_).max()
         if name == "fem_flexible_plate":
             assert so["fem_info"][0][:2] == sd["fem_info"][0][:2]
