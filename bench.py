@@ -1,19 +1,26 @@
-"""Benchmark of the FSILBM3D fluid hot path on one MI355X.
+"""Benchmark of the FSILBM3D fluid/coupling hot path on one MI355X.
 
-Workload: the stencil benchmark configuration -- D3Q19 512x256x256 fully
-periodic block, SRT, tau 0.55, initial state the equilibrium of a seeded
-density 1+U(-1e-3,1e-3) and velocity U(-0.05,0.05) per cell (splitmix64, seed
-20261202).  One step = one fused collide+push-stream pass over every cell
-(the body of main.f90:93-147 for a block with no bodies and periodic faces).
+Headline workload (value): the flexible-plate FSI case -- D3Q19 384x192x192,
+uniform inflow U = 0.05 (101), outflow (103), slip lateral faces (302), Re 500
+on L = 48 cells; one flexible plate (49 chord nodes x 48 span markers = 2304
+markers, mass ratio 1, KB 0.1, KS 1000, clamped leading edge heaving 0.3 L at
+0.2 U/L), numsubstep 4, multi-direct forcing to 1e-6 with at most 10
+iterations.  One step = the body of main.f90:93-147: volume force, markers,
+immersed-boundary coupling, 4 beam sub-steps, fused collide+stream+faces.
+value = FSI-step MLUPS of the FP64 plain form (cells x steps / wall seconds,
+host-side around exactly --steps steps between device synchronisations, the
+initial state and the bodies already resident on the device).
 
-Prints ONE JSON line.  value = MLUPS of the FP64 plain form (whole job, one
-GPU), measured host-side around exactly --steps steps between device
-synchronisations; the FP32-deviation form is reported beside it, with its
-velocity L2 error against FP64.  roofline: the fused kernel's algorithmic
-bytes per launch (FP64: 19 reads + 19 writes x 8 B = 304 B per cell) over
-its average duration from HIP events on the block's stream.  cpu_baseline:
-the reference program itself (oracle/_ref, compiled from the reference
-sources) timed on this host's cores for a few steps of the same grid.
+Beside it: the stencil benchmark (D3Q19 512x256x256 fully periodic, SRT tau
+0.55, seeded equilibrium) in the FP64 plain and FP32-deviation forms, with the
+FP32 velocity L2 error against FP64.
+
+roofline: the fused collide+stream kernel of the FSI step (the band-overlay
+variant), algorithmic bytes per launch 19 reads + 19 writes x 8 B = 304 B per
+cell, over its average duration from HIP events on the block's stream.
+cpu_baseline: the reference program itself (oracle/_ref, compiled from the
+reference sources) timed on this host's cores for a few steps of the same
+FSI case.
 """
 import argparse
 import json
@@ -55,6 +62,45 @@ def run_form(prec, f0, steps, warmup, nlaunch):
                 kernel_ms=kern_ms, achieved_gbs=achieved)
 
 
+def run_fsi(w, steps, warmup, nlaunch):
+    """Flexible-plate FSI steps through the C-ABI (device beam solver)."""
+    import numpy as np
+    from fsilbm3d_amd.block import LBMBlock, synchronize
+    from fsilbm3d_amd.fsi import DeviceBodyBlock
+    N = int(np.prod(w.dims))
+    blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu)
+    blk.set_flow(**w.flow_kwargs())
+    blk.upload(w.initial_state())
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    fsi = DeviceBodyBlock(blk, [w.plate()], **w.solver_kwargs())
+    t = 0.0
+    for _ in range(warmup):
+        t = t + 1.0
+        fsi.step(t)
+    synchronize()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = t + 1.0
+        fsi.step(t)
+        its.append(fsi.iters)
+    synchronize()
+    secs = time.perf_counter() - t0
+    newton, cg, _ = fsi.info(0)
+    pos = fsi.state(0)[0]
+    lod = fsi.body_loads(0).reshape(-1, 6).sum(0)
+    # the fused kernel with the band overlay, timed with HIP events on the block's stream
+    fsi.interaction_force(blk.dh)
+    kern_ms = blk.time_collide(nlaunch)
+    synchronize()
+    blk.close()
+    return dict(mlups=N * steps / secs / 1e6, ms_per_step=secs / steps * 1e3, kernel_ms=kern_ms,
+                achieved_gbs=B_CELL[0] * N / (kern_ms * 1e-3) / 1e9, mdf_iters_mean=float(np.mean(its)),
+                newton_iters_last=newton, cg_iters_last=cg, tip_xyz=[float(v) for v in pos[-1, :3]],
+                force_xyz=[float(v) for v in lod[:3]], cells=N, time=t)
+
+
 def l2_fp32_vs_fp64(f0, nsteps):
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock
@@ -68,18 +114,31 @@ def l2_fp32_vs_fp64(f0, nsteps):
     return float(np.sqrt(((u[1] - u[0]) ** 2).sum() / (u[0] ** 2).sum()))
 
 
-def cpu_baseline(steps):
-    from oracle.refcase import Case, time_reference_steps
+def reference_fsi_case(w):
+    """The flexible-plate workload in the reference's own inputs."""
+    from oracle.refcase import Case, Group, plate_text
+    p = w.plate()
+    txt = plate_text([row[:3] for row in p.nodes8], float(p.nodes8[0, 3]), float(p.nodes8[0, 4]), (0.0, 0.0, 1.0),
+                     int(p.elements[0, 2]))
+    g = Group(plate=txt, iBodyModel=2, denR=p.denR, KB=p.KB, KS=p.KS, psR=p.psR, isMotionGiven=p.isMotionGiven,
+              firstXYZ=p.XYZo, XYZAmpl=p.XYZAmpl, freq=p.Freq)
+    sk = w.solver_kwargs()
+    return Case(dims=tuple(w.dims), bc=tuple(w.bc), Re=w.Re, uvwIn=(w.U, 0.0, 0.0), UrefType=0, Lref=w.L, LrefType=1,
+                TrefType=0, ntolLBM=sk["ntolLBM"], dtolLBM=sk["dtolLBM"], numsubstep=sk["numsubstep"],
+                dtolFEM=sk["dtolFEM"], ntolFEM=sk["ntolFEM"], groups=[g])
+
+
+def cpu_baseline(w, steps):
+    from oracle.refcase import time_reference_steps
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    case = Case(dims=DIMS, bc=(301,) * 6, Re=3.0, Uref=0.05, UrefType=9, Lref=1.0, LrefType=1, TrefType=0)
-    secs = time_reference_steps(case, steps, threads)
-    N = DIMS[0] * DIMS[1] * DIMS[2]
+    secs = time_reference_steps(reference_fsi_case(w), steps, threads)
+    N = w.dims[0] * w.dims[1] * w.dims[2]
     tot = sum(secs)
     return {"value": N * len(secs) / tot / 1e6 if tot > 0 else None, "unit": "MLUPS", "cores": threads,
             "kind": "reference",
-            "sample": f"{len(secs)} steps of the 512x256x256 periodic SRT block (reference program built with "
-                      f"amdflang -O3 -fopenmp, {threads} OpenMP threads, uniform initial state: per-step work is "
-                      f"value-independent); seconds per step {[round(s, 3) for s in secs]}"}
+            "sample": f"{len(secs)} FSI steps of the flexible-plate case (384x192x192, one elastic plate, "
+                      f"reference program built with amdflang -O3 -fopenmp, {threads} OpenMP threads, from its own "
+                      f"uniform initial state); seconds per step {[round(s, 3) for s in secs]}"}
 
 
 def traffic_from_profiles():
@@ -124,8 +183,9 @@ def pmc_driver():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--stencil-steps", type=int, default=200)
     ap.add_argument("--nlaunch", type=int, default=50)
     ap.add_argument("--l2-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=5)
@@ -139,61 +199,78 @@ def main():
               "decomposition is a design note, not a measured figure.", file=sys.stderr)
         sys.exit(2)
     from tests.golden.cases import seeded_state
+    from fsilbm3d_amd.workloads import flexible_plate
     if args.pmc_driver:
         pmc_driver()
         return
+    w = flexible_plate()
+    log("bench: flexible-plate FSI, FP64 plain form")
+    fsi = run_fsi(w, args.steps, args.warmup, args.nlaunch)
+    log(f"bench: FSI {fsi}")
     log("bench: seeding 512x256x256 state")
     f0 = seeded_state(DIMS, 20261202)
-    log("bench: FP64 plain form")
-    r64 = run_form(0, f0, args.steps, args.warmup, args.nlaunch)
+    log("bench: stencil FP64 plain form")
+    r64 = run_form(0, f0, args.stencil_steps, 20, args.nlaunch)
     log(f"bench: FP64 {r64}")
     r32 = None
     l2 = None
     if not args.no_fp32:
-        log("bench: FP32-deviation form")
-        r32 = run_form(1, f0, args.steps, args.warmup, args.nlaunch)
+        log("bench: stencil FP32-deviation form")
+        r32 = run_form(1, f0, args.stencil_steps, 20, args.nlaunch)
         log(f"bench: FP32 {r32}")
         if args.l2_steps > 0:
             log(f"bench: FP32 vs FP64 velocity L2 after {args.l2_steps} steps")
             l2 = l2_fp32_vs_fp64(f0, args.l2_steps)
     cpu = None
     if not args.no_cpu:
-        log(f"bench: reference CPU baseline, {args.cpu_steps} steps")
+        log(f"bench: reference CPU baseline, {args.cpu_steps} FSI steps")
         try:
-            cpu = cpu_baseline(args.cpu_steps)
+            cpu = cpu_baseline(w, args.cpu_steps)
         except Exception as e:  # reported, not fatal: the baseline is not the target
             cpu = {"value": None, "unit": "MLUPS", "cores": None, "kind": "reference", "sample": f"failed: {e}"}
     N = DIMS[0] * DIMS[1] * DIMS[2]
     cp = copy_gbs()
     out = {
         "metric": "FSI-step MLUPS, 1 MI355X: FP32+learned vs FP64 plain @drag/lift err<1e-3; %HBM",
-        "value": r64["mlups"],
+        "value": fsi["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": r64["ms_per_step"],
+        "ms_per_step": fsi["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded equilibrium state)",
-        "config": {"workload": "stencil benchmark: D3Q19 512x256x256 periodic, SRT tau 0.55, FP64 plain form",
-                   "cells": N, "collision": "SRT", "form": "fp64-plain"},
+        "data": "synthetic (uniform-flow initial state, prescribed plate kinematics)",
+        "config": {"workload": "flexible-plate FSI: D3Q19 384x192x192, inflow U 0.05, Re 500 on L 48, one elastic "
+                               "plate 49 chord nodes x 48 span markers, numsubstep 4, MDF to 1e-6 (max 10), "
+                               "FP64 plain form",
+                   "cells": fsi["cells"], "markers": 2304, "collision": "SRT", "form": "fp64-plain"},
+        "fsi": {k: fsi[k] for k in ("mdf_iters_mean", "newton_iters_last", "cg_iters_last", "tip_xyz", "force_xyz",
+                                   "kernel_ms")},
         "copy_kernel_gbs": cp,
-        "roofline": {"bound": "hbm", "achieved": r64["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": r64["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_collide_push<1,false,false,false>", "kernel_ms": r64["kernel_ms"],
+        "roofline": {"bound": "hbm", "achieved": fsi["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": fsi["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "k_collide_push<1,false,true,false> (band overlay)", "kernel_ms": fsi["kernel_ms"],
                      "bytes_per_cell": B_CELL[0]},
         "cpu_baseline": cpu,
+        "stencil": {"workload": "D3Q19 512x256x256 periodic, SRT tau 0.55, seeded equilibrium", "cells": N,
+                    "steps": args.stencil_steps, "value": r64["mlups"], "unit": "MLUPS",
+                    "ms_per_step": r64["ms_per_step"],
+                    "roofline": {"bound": "hbm", "achieved": r64["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                                 "unit": "GB/s", "frac": r64["achieved_gbs"] / HBM_PEAK_GBS,
+                                 "kernel": "k_collide_push<1,false,false,false>", "kernel_ms": r64["kernel_ms"],
+                                 "bytes_per_cell": B_CELL[0]}},
     }
     if r32 is not None:
-        out["fp32_deviation"] = {"value": r32["mlups"], "unit": "MLUPS", "ms_per_step": r32["ms_per_step"],
-                                 "speedup_vs_fp64": r32["mlups"] / r64["mlups"],
-                                 "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                                              "unit": "GB/s", "frac": r32["achieved_gbs"] / HBM_PEAK_GBS,
-                                              "kernel_ms": r32["kernel_ms"], "bytes_per_cell": B_CELL[1]},
-                                 "velocity_l2_rel_vs_fp64": l2, "l2_steps": args.l2_steps}
+        out["stencil"]["fp32_deviation"] = {
+            "value": r32["mlups"], "unit": "MLUPS", "ms_per_step": r32["ms_per_step"],
+            "speedup_vs_fp64": r32["mlups"] / r64["mlups"],
+            "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": r32["achieved_gbs"] / HBM_PEAK_GBS, "kernel_ms": r32["kernel_ms"],
+                         "bytes_per_cell": B_CELL[1]},
+            "velocity_l2_rel_vs_fp64": l2, "l2_steps": args.l2_steps}
     print(json.dumps(out), flush=True)
 
 

@@ -25,6 +25,7 @@ struct BeamElem {
   double tee[9], tn1[9], tn2[9];  // [row*3 + col]
   double prop[8];
   double mas[NED * NED], stf[NED * NED], geo[NED * NED], coef[NED * NED];
+  double stl[NED * NED];  // local stiffness: depends on the constant properties and len0 only
   double rot[9];
   double fint[NED], fload[NED], fdamp[NED];  // per-element contributions (internal force, loads)
 };
@@ -306,7 +307,21 @@ __host__ __device__ inline void el_rkr(const BeamElem &e, double *ek) {
     }
 }
 
-// Segment_BodyStress_D (SolidSolver.f90:750-839): local internal force -> e.fint
+// entry (I, J) of R^T ek R computed as el_rkr computes it (3x3 blocks,
+// kt = ek_block * r, then rt * kt, each sum in ascending order)
+__host__ __device__ inline double rkr_entry(const double *ek, const double *r, int I, int J) {
+  const int j1 = (I / 3) * 3, k = I - j1, j2 = (J / 3) * 3, ii = J - j2;
+  double s = 0.0;
+  for (int jj = 0; jj < 3; ++jj) {
+    double kt = 0.0;
+    for (int l = 0; l < 3; ++l) kt = kt + ek[(j1 + jj) * NED + j2 + l] * r[l * 3 + ii];
+    s = s + r[jj * 3 + k] * kt;
+  }
+  return s;
+}
+
+// Segment_BodyStress_D (SolidSolver.f90:750-839): local internal force -> e.fint,
+// with the local stiffness of Segment_FormStiffMatrix (e.stl, formed once)
 __host__ __device__ inline void el_body_stress(BeamElem &e) {
   const double du = e.dx1 - e.dx0, dv = e.dy1 - e.dy0, dw = e.dz1 - e.dz0;
   const double dl = ((e.dx0 + e.dx1) * du + (e.dy0 + e.dy1) * dv + (e.dz0 + e.dz1) * dw) / (e.len0 + e.len1);
@@ -325,7 +340,7 @@ __host__ __device__ inline void el_body_stress(BeamElem &e) {
   double fb[NED];
   for (int i = 0; i < NED; ++i) {
     double s = 0.0;
-    for (int k = 0; k < NED; ++k) s = s + e.stf[i * NED + k] * ub[k];
+    for (int k = 0; k < NED; ++k) s = s + e.stl[i * NED + k] * ub[k];
     fb[i] = s;
   }
   for (int i = 0; i < NED; ++i) e.fint[i] = 0.0;
@@ -452,6 +467,33 @@ __host__ __device__ inline int invert6(const double *A, double *Ai) {
   }
   for (int i = 0; i < 6; ++i)
     for (int j = 0; j < 6; ++j) Ai[i * 6 + j] = aug[i][6 + j];
+  return 0;
+}
+
+// Invert6x6 as invert6, on a caller-provided 6x12 work area (LDS) whose left
+// 6x6 half holds the matrix; the inverse is left in the right half.
+__device__ inline int invert6_inplace(double *aug) {
+#define AUG(i, j) aug[(i) * 12 + (j)]
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) AUG(i, 6 + j) = 0.0;
+  for (int i = 0; i < 6; ++i) AUG(i, 6 + i) = 1.0;
+  for (int i = 0; i < 6; ++i) {
+    int p = i;
+    double pivot = fabs(AUG(i, i));
+    for (int k = i + 1; k < 6; ++k)
+      if (fabs(AUG(k, i)) > pivot) { pivot = fabs(AUG(k, i)); p = k; }
+    if (pivot <= 1.0e-30) return i + 1;
+    if (p != i)
+      for (int j = 0; j < 12; ++j) { const double t = AUG(i, j); AUG(i, j) = AUG(p, j); AUG(p, j) = t; }
+    pivot = AUG(i, i);
+    for (int j = 0; j < 12; ++j) AUG(i, j) = AUG(i, j) / pivot;
+    for (int k = 0; k < 6; ++k)
+      if (k != i) {
+        const double factor = AUG(k, i);
+        for (int j = 0; j < 12; ++j) AUG(k, j) = AUG(k, j) - factor * AUG(i, j);
+      }
+  }
+#undef AUG
   return 0;
 }
 

@@ -3,6 +3,7 @@
 // Declarations and the reference routine each entry point replaces are in
 // include/fsilbm3d_amd.h.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -49,6 +50,18 @@ template <typename T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -73,23 +86,21 @@ struct Ibm {
   std::vector<int> marker_elem;              // global element id of each marker
   int ntol = 0;
   double dtol = 0.0, Uref = 1.0, denIn = 1.0;
-  // host scratch
-  std::vector<int> Ei;                       // 12*M (1-based grid indices)
-  std::vector<float> Ew;                     // 12*M
-  std::vector<int> bidx;                     // 64*M band index per entry
-  std::vector<std::pair<long long, int>> keyed;
-  std::vector<long long> band_cell;
-  std::vector<int> band_ptr, entry;
-  std::vector<int> bmap;
-  int bx0 = 0, by0 = 0, bz0 = 0, bnx = 0, bny = 0, bnz = 0;
-  int nband = 0;
-  // device
-  DevBuf<double> Exyz, Evel, Ea, Eforce, fT, tolpart, band_u, band_F, elem_xc, lod;
-  DevBuf<float> dEw;
-  DevBuf<int> dbidx, dptr, dentry, dbmap, dmarker_elem, dof_ptr, dof_list, dof_kind;
+  // device: markers and forces
+  DevBuf<double> Exyz, Evel, Ea, Eforce, fT, tolpart, elem_xc, lod;
+  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_kind, first_marker;
+  // device: stencils and band (built each call, see fsil_ibm_kernels.h)
+  int cap = 0;                               // 64*M, stride of the band arrays
+  int end_bit = 0;                           // key bits to sort
+  DevBuf<float> dEw;                         // 12*M
+  DevBuf<unsigned long long> keys, keys_sorted;
+  DevBuf<int> flags, scan, dentry, dbidx, dptr, ctl, cellmap;
   DevBuf<long long> dcell;
-  std::vector<double> h_tolpart;
-  std::vector<double> hExyz;                 // host copy of device-computed markers
+  DevBuf<double> band_u, band_F, dstate;
+  DevBuf<char> cub_tmp;
+  size_t cub_bytes = 0;
+  std::vector<int> h_ctl;
+  std::vector<double> h_dstate;
   int ndof = 0;
   bool active = false;  // band valid for the next collision
 };
@@ -102,8 +113,30 @@ struct FemBodyHost {
   double TTT0[9];
 };
 
+// The beam sub-steps run on a stream of their own: the fluid update of a step
+// does not read the beam state, so the sub-steps overlap the collide kernel.
+// ev_in orders them after the coupling call; ev_done orders the next marker
+// update (and the next coupling call) after them.  Descriptors and results
+// travel through pinned host memory.
+struct FemStream {
+  hipStream_t s = nullptr;
+  hipEvent_t ev_in = nullptr, ev_done = nullptr;
+  FemBodyDesc *h_stage = nullptr;  // pinned: numsubstep x nbodies descriptors
+  size_t cap = 0;
+  int n_staged = 0;                // descriptors of the last call (results after ev_done)
+  bool pending = false;            // a call whose results have not been checked
+  ~FemStream() {
+    if (s) (void)hipStreamSynchronize(s);
+    if (ev_in) (void)hipEventDestroy(ev_in);
+    if (ev_done) (void)hipEventDestroy(ev_done);
+    if (s) (void)hipStreamDestroy(s);
+    if (h_stage) (void)hipHostFree(h_stage);
+  }
+};
+
 struct Fem {
   int nbodies = 0, nEL = 0, nND = 0, gEQ = 0;
+  std::unique_ptr<FemStream> fs;
   std::vector<FemBodyHost> bh;
   std::vector<FemBodyDesc> desc;
   double dampK = 0, dampM = 0, GeoGamma = 1, NewmarkGamma = 0.5, NewmarkBeta = 0.25, dtolFEM = 1e-10;
@@ -291,14 +324,13 @@ int collide_stream(Block &b) {
   a.Mc = b.Mc.p;
   a.Mf = b.Mf.p;
   const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
-  const bool band = b.ibm.active && b.ibm.nband > 0;
+  const bool band = b.ibm.active && b.ibm.M > 0;
   if (band) {
-    a.bx0 = b.ibm.bx0; a.by0 = b.ibm.by0; a.bz0 = b.ibm.bz0;
-    a.bnx = b.ibm.bnx; a.bny = b.ibm.bny; a.bnz = b.ibm.bnz;
-    a.bmap = b.ibm.dbmap.p;
+    a.ctl = b.ibm.ctl.p;
+    a.cellmap = b.ibm.cellmap.p;
     a.band_u = b.ibm.band_u.p;
     a.band_F = b.ibm.band_F.p;
-    a.nband = b.ibm.nband;
+    a.cap = b.ibm.cap;
   }
   int m102 = 0, mhw = 0;
   for (int face = 0; face < 6; ++face) {
@@ -363,119 +395,6 @@ int macro32(Block &b, double *den, double *uuu) {
 }
 
 // Peskin 4-point kernel (Solidbody.f90:822-833)
-double peskin(double x) {
-  const double r = std::fabs(x);
-  if (r < 1.0) return (3.0 - 2.0 * r + std::sqrt(1.0 + 4.0 * r * (1.0 - r))) * 0.125;
-  if (r < 2.0) return (5.0 - 2.0 * r - std::sqrt(-7.0 + 4.0 * r * (3.0 - r))) * 0.125;
-  return 0.0;
-}
-
-// trimedindex (Solidbody.f90:834-866): 1-based indices, wrap/mirror by face code
-int trim_index(int i, int nd, int *out, int bclo, int bchi, std::string &why) {
-  for (int k = -1; k <= 2; ++k) {
-    int v = i + k;
-    if (v < 1) {
-      if (bclo == BC_PERIODIC) v += nd;
-      else if ((bclo == BC_SYMMETRIC || bclo == BC_WALL) && v == 0) v = 2;
-      else if (bclo == BC_WALL_HALFWAY && v == 0) v = 1;
-      else { why = "index out of min bound " + std::to_string(v); return 1; }
-    } else if (v > nd) {
-      if (bchi == BC_PERIODIC) v -= nd;
-      else if ((bchi == BC_SYMMETRIC || bchi == BC_WALL) && v == nd + 1) v = nd - 1;
-      else if (bchi == BC_WALL_HALFWAY && v == nd + 1) v = nd;
-      else { why = "index out of max bound " + std::to_string(v); return 1; }
-    }
-    out[k + 1] = v;
-  }
-  return 0;
-}
-
-// UpdateElmtInterp_ (Solidbody.f90:760-821) for every body + band assembly
-int build_stencil_and_band(Block &b, const double *Exyz) {
-  Ibm &I = b.ibm;
-  const int M = I.M;
-  I.Ei.resize(12 * (size_t)M);
-  I.Ew.resize(12 * (size_t)M);
-  const double invdh = 1.0 / b.dh;
-  const double org[3] = {b.xmin, b.ymin, b.zmin};
-  const int nd[3] = {b.xd, b.yd, b.zd};
-  for (int body = 0; body < I.nbodies; ++body) {
-    const int m0 = I.moff[body], m1 = I.moff[body + 1];
-    if (m1 == m0) continue;
-    int i0[3];
-    double x0[3];
-    for (int k = 0; k < 3; ++k) {
-      const int ii = (int)std::floor((Exyz[3 * (size_t)m0 + k] - org[k]) * invdh);
-      x0[k] = org[k] + (double)ii * b.dh;
-      i0[k] = ii + 1;
-    }
-    for (int m = m0; m < m1; ++m) {
-      for (int k = 0; k < 3; ++k) {
-        double off = (Exyz[3 * (size_t)m + k] - x0[k]) * invdh;
-        int idx = (int)std::floor(off);
-        off = off - (double)idx;
-        idx = idx + i0[k];
-        std::string why;
-        if (trim_index(idx, nd[k], &I.Ei[12 * (size_t)m + 4 * k], b.bc[2 * k], b.bc[2 * k + 1], why))
-          return fail("immersed-boundary stencil of marker %d (body %d): %s", m - m0 + 1, body + 1,
-                      why.c_str());
-        for (int j = -1; j <= 2; ++j)
-          I.Ew[12 * (size_t)m + 4 * k + j + 1] = (float)peskin((double)j - off);
-      }
-    }
-  }
-  // entries sorted by (cell, entry id): the reference's serial update order
-  const size_t E = 64 * (size_t)M;
-  I.keyed.resize(E);
-  for (int m = 0; m < M; ++m) {
-    const int *ix = &I.Ei[12 * (size_t)m];
-    for (int x = 0; x < 4; ++x)
-      for (int y = 0; y < 4; ++y)
-        for (int z = 0; z < 4; ++z) {
-          const long long cell =
-              ((long long)(ix[x] - 1) * b.yd + (ix[4 + y] - 1)) * b.zd + (ix[8 + z] - 1);
-          const int e = 64 * m + 16 * x + 4 * y + z;
-          I.keyed[e] = {cell, e};
-        }
-  }
-  std::sort(I.keyed.begin(), I.keyed.end());
-  I.band_cell.clear();
-  I.band_ptr.clear();
-  I.entry.resize(E);
-  I.bidx.resize(E);
-  for (size_t k = 0; k < E; ++k) {
-    if (k == 0 || I.keyed[k].first != I.keyed[k - 1].first) {
-      I.band_ptr.push_back((int)k);
-      I.band_cell.push_back(I.keyed[k].first);
-    }
-    I.entry[k] = I.keyed[k].second;
-    I.bidx[I.keyed[k].second] = (int)I.band_cell.size() - 1;
-  }
-  I.band_ptr.push_back((int)E);
-  I.nband = (int)I.band_cell.size();
-  // bounding box of the band and its local map
-  int lo[3] = {b.xd, b.yd, b.zd}, hi[3] = {-1, -1, -1};
-  for (long long c : I.band_cell) {
-    const int x = (int)(c / ((long long)b.yd * b.zd));
-    const int y = (int)((c / b.zd) % b.yd);
-    const int z = (int)(c % b.zd);
-    lo[0] = std::min(lo[0], x); hi[0] = std::max(hi[0], x);
-    lo[1] = std::min(lo[1], y); hi[1] = std::max(hi[1], y);
-    lo[2] = std::min(lo[2], z); hi[2] = std::max(hi[2], z);
-  }
-  I.bx0 = lo[0]; I.by0 = lo[1]; I.bz0 = lo[2];
-  I.bnx = hi[0] - lo[0] + 1; I.bny = hi[1] - lo[1] + 1; I.bnz = hi[2] - lo[2] + 1;
-  I.bmap.assign((size_t)I.bnx * I.bny * I.bnz, -1);
-  for (int k = 0; k < I.nband; ++k) {
-    const long long c = I.band_cell[k];
-    const int x = (int)(c / ((long long)b.yd * b.zd));
-    const int y = (int)((c / b.zd) % b.yd);
-    const int z = (int)(c % b.zd);
-    I.bmap[((size_t)(x - I.bx0) * I.bny + (y - I.by0)) * I.bnz + (z - I.bz0)] = k;
-  }
-  return 0;
-}
-
 template <typename T>
 int upload(DevBuf<T> &d, const T *h, size_t n, hipStream_t s) {
   HIPCHK(d.ensure(n));
@@ -587,6 +506,8 @@ void fem_init_body(const double *xyz8, int nND, int nEL, const int *elems, const
     el_form_mass(e);
     el_rotate(e);
     el_rkr(e, e.mas);
+    el_form_stiff(e);
+    for (int k = 0; k < NED * NED; ++k) e.stl[k] = e.stf[k];
   }
 }
 
@@ -625,6 +546,9 @@ FemArrays fem_arrays(Block &b) {
   A.node_ptr = F.node_ptr.p; A.node_e = F.node_e.p; A.node_h = F.node_h.p;
   return A;
 }
+
+int fem_collect(Block &b);
+int fem_order_after(Block &b);
 
 }  // namespace
 
@@ -1005,6 +929,44 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   HIPCHK(I.fT.ensure(3 * (size_t)I.M));
   HIPCHK(I.lod.ensure(I.ndof));
   HIPCHK(I.elem_xc.ensure(3 * (size_t)std::max(1, I.nelems)));
+  // device stencil and band structures
+  std::vector<int> first(std::max(1, I.M));
+  for (int body = 0; body < nbodies; ++body)
+    for (int m = I.moff[body]; m < I.moff[body + 1]; ++m) first[m] = I.moff[body];
+  if (upload(I.first_marker, first.data(), first.size(), b->stream)) return 1;
+  const int E = 64 * I.M;
+  I.cap = std::max(1, E);
+  int cbits = 1;
+  while (cbits < 63 && (1LL << cbits) < b->N) ++cbits;
+  I.end_bit = kKeyShift + cbits;
+  if (I.end_bit > 64) return fail("block too large for the band keys");
+  HIPCHK(I.dEw.ensure(12 * (size_t)std::max(1, I.M)));
+  HIPCHK(I.keys.ensure(I.cap));
+  HIPCHK(I.keys_sorted.ensure(I.cap));
+  HIPCHK(I.flags.ensure(I.cap));
+  HIPCHK(I.scan.ensure(I.cap));
+  HIPCHK(I.dentry.ensure(I.cap));
+  HIPCHK(I.dbidx.ensure(I.cap));
+  HIPCHK(I.dptr.ensure((size_t)I.cap + 1));
+  HIPCHK(I.dcell.ensure(I.cap));
+  HIPCHK(I.band_u.ensure(3 * (size_t)I.cap));
+  HIPCHK(I.band_F.ensure(3 * (size_t)I.cap));
+  HIPCHK(I.dstate.ensure(4));
+  HIPCHK(I.tolpart.ensure((I.M + 255) / 256 + 1));
+  HIPCHK(I.ctl.ensure(kCtlWords));
+  HIPCHK(I.cellmap.ensure((size_t)b->N));
+  HIPCHK(hipMemsetAsync(I.ctl.p, 0, kCtlWords * sizeof(int), b->stream));
+  HIPCHK(hipMemsetAsync(I.cellmap.p, 0xff, (size_t)b->N * sizeof(int), b->stream));  // -1
+  if (E > 0) {
+    size_t t1 = 0, t2 = 0;
+    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, t1, I.keys.p, I.keys_sorted.p, E, 0, I.end_bit,
+                                             b->stream));
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, t2, I.flags.p, I.scan.p, E, b->stream));
+    I.cub_bytes = std::max(t1, t2);
+    HIPCHK(I.cub_tmp.ensure(std::max<size_t>(1, I.cub_bytes)));
+  }
+  I.h_ctl.assign(kCtlWords, 0);
+  I.h_dstate.assign(4, 0.0);
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
@@ -1032,61 +994,73 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     return 0;
   }
   hipStream_t s = b->stream;
-  const int M = I.M;
+  const int M = I.M, E = 64 * M;
+  if (fem_order_after(*b)) return 1;  // k_loads rewrites the loads the beam sub-steps read
   if (!Exyz) {
     // markers computed on the device by fsil_fem_update_markers
     if (!b->fem.markers_on_device) return fail("no marker arrays given and no device markers computed");
-    if (build_stencil_and_band(*b, I.hExyz.data())) return 1;
   } else {
     if (!Evel || !Ea || !elem_xc) return fail("null marker arrays");
-    if (build_stencil_and_band(*b, Exyz)) return 1;
     if (upload(I.Exyz, Exyz, 3 * (size_t)M, s)) return 1;
     if (upload(I.Evel, Evel, 3 * (size_t)M, s)) return 1;
     if (upload(I.Ea, Ea, (size_t)M, s)) return 1;
     if (upload(I.elem_xc, elem_xc, 3 * (size_t)I.nelems, s)) return 1;
   }
-  if (upload(I.dEw, I.Ew.data(), I.Ew.size(), s)) return 1;
-  if (upload(I.dbidx, I.bidx.data(), I.bidx.size(), s)) return 1;
-  if (upload(I.dcell, I.band_cell.data(), I.band_cell.size(), s)) return 1;
-  if (upload(I.dptr, I.band_ptr.data(), I.band_ptr.size(), s)) return 1;
-  if (upload(I.dentry, I.entry.data(), I.entry.size(), s)) return 1;
-  if (upload(I.dbmap, I.bmap.data(), I.bmap.size(), s)) return 1;
-  HIPCHK(I.band_u.ensure(3 * (size_t)I.nband));
-  HIPCHK(I.band_F.ensure(3 * (size_t)I.nband));
+  // UpdateElmtInterp_ (Solidbody.f90:760-867) and the band, on the device
+  const int done0 = (I.ntol > 0 && 1e10 > I.dtol) ? 0 : 1;
+  hipLaunchKernelGGL(k_ibm_reset, dim3(std::min(1024, (E + 255) / 256)), dim3(256), 0, s, I.ctl.p, I.dcell.p,
+                     I.cellmap.p, I.dstate.p, done0);
+  HIPCHK(hipGetLastError());
+  StencilArgs sa{};
+  sa.M = M;
+  sa.Exyz = I.Exyz.p;
+  sa.first_marker = I.first_marker.p;
+  sa.org[0] = b->xmin; sa.org[1] = b->ymin; sa.org[2] = b->zmin;
+  sa.dh = b->dh;
+  sa.invdh = 1.0 / b->dh;
+  sa.nd[0] = b->xd; sa.nd[1] = b->yd; sa.nd[2] = b->zd;
+  for (int k = 0; k < 6; ++k) sa.bc[k] = b->bc[k];
+  sa.Ew = I.dEw.p;
+  sa.keys = I.keys.p;
+  sa.ctl = I.ctl.p;
+  hipLaunchKernelGGL(k_ibm_stencil, dim3((M + 255) / 256), dim3(256), 0, s, sa);
+  HIPCHK(hipGetLastError());
+  size_t tb = I.cub_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortKeys(I.cub_tmp.p, tb, I.keys.p, I.keys_sorted.p, E, 0, I.end_bit, s));
+  hipLaunchKernelGGL(k_ibm_heads, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys_sorted.p, I.flags.p);
+  HIPCHK(hipGetLastError());
+  tb = I.cub_bytes;
+  HIPCHK(hipcub::DeviceScan::InclusiveSum(I.cub_tmp.p, tb, I.flags.p, I.scan.p, E, s));
+  hipLaunchKernelGGL(k_ibm_band, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys_sorted.p, I.scan.p,
+                     I.dentry.p, I.dbidx.p, I.dcell.p, I.dptr.p, I.cellmap.p, I.ctl.p);
+  HIPCHK(hipGetLastError());
+  // calculate_interaction_force (Solidbody.f90:869-918)
   const int nblkM = (M + 255) / 256;
-  HIPCHK(I.tolpart.ensure(nblkM));
-  I.h_tolpart.resize(nblkM);
   const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
-  hipLaunchKernelGGL(k_band_macro, dim3((I.nband + 255) / 256), dim3(256), 0, s, b->fA.p, b->N,
-                     I.dcell.p, I.nband, hv0, hv1, hv2, I.band_u.p);
+  hipLaunchKernelGGL(k_band_macro, dim3((E + 255) / 256), dim3(256), 0, s, b->fA.p, b->N, I.dcell.p, I.ctl.p,
+                     I.cap, hv0, hv1, hv2, I.band_u.p);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_zero, dim3((3 * M + 255) / 256), dim3(256), 0, s, I.Eforce.p, 3LL * M);
   HIPCHK(hipGetLastError());
-  // MDF loop (Solidbody.f90:893-906)
+  // MDF loop (Solidbody.f90:893-906): ntol rounds are queued; the kernels of a
+  // round skip their work once k_mdf_check has found the loop finished
   const double invh3p = 0.5 * dt * ((1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh)) / I.denIn;
-  int it = 0;
-  double dm = 1e10;
   double dsum = 0.0;
   for (int body = 0; body < I.nbodies; ++body) dsum = dsum + (double)I.nmark[body];
-  while (it < I.ntol && dm > I.dtol) {
-    hipLaunchKernelGGL(k_interp, dim3(nblkM), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p,
-                       I.nband, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
+  for (int it = 0; it < I.ntol; ++it) {
+    hipLaunchKernelGGL(k_interp, dim3(nblkM), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
+                       I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_correct, dim3((I.nband + 255) / 256), dim3(256), 0, s, I.nband, I.dptr.p,
+    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p,
                        I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(I.h_tolpart.data(), I.tolpart.p, nblkM * sizeof(double),
-                          hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    double tol = 0.0;
-    for (int k = 0; k < nblkM; ++k) tol = tol + I.h_tolpart[k];
-    if (!std::isfinite(tol)) return fail("Nan found in PenaltyForce");
-    dm = tol / (dsum * I.Uref);
-    ++it;
+    hipLaunchKernelGGL(k_mdf_check, dim3(1), dim3(64), 0, s, I.ctl.p, I.dstate.p, I.tolpart.p, nblkM, dsum,
+                       I.Uref, I.ntol, I.dtol);
+    HIPCHK(hipGetLastError());
   }
   const double invh3s = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
-  hipLaunchKernelGGL(k_spread, dim3((I.nband + 255) / 256), dim3(256), 0, s, I.nband, I.dptr.p,
-                     I.dentry.p, I.dEw.p, I.Eforce.p, invh3s, b->VF[0], b->VF[1], b->VF[2], I.band_F.p);
+  hipLaunchKernelGGL(k_spread, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p, I.dentry.p,
+                     I.dEw.p, I.Eforce.p, invh3s, b->VF[0], b->VF[1], b->VF[2], I.band_F.p);
   HIPCHK(hipGetLastError());
   if (I.ndof > 0) {
     hipLaunchKernelGGL(k_loads, dim3((I.ndof + 255) / 256), dim3(256), 0, s, I.ndof, I.dof_ptr.p,
@@ -1094,14 +1068,62 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                        I.Eforce.p, I.lod.p);
     HIPCHK(hipGetLastError());
   }
+  HIPCHK(hipMemcpyAsync(I.h_ctl.data(), I.ctl.p, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipMemcpyAsync(I.h_dstate.data(), I.dstate.p, sizeof(double), hipMemcpyDeviceToHost, s));
   if (Eforce) HIPCHK(hipMemcpyAsync(Eforce, I.Eforce.p, 3 * (size_t)M * sizeof(double), hipMemcpyDeviceToHost, s));
   if (lodFlow && I.ndof > 0) HIPCHK(hipMemcpyAsync(lodFlow, I.lod.p, (size_t)I.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  if (fem_collect(*b)) return 1;  // results of the previous beam sub-steps
+  const int err = I.h_ctl[kErr];
+  if (err != 0x7fffffff) {
+    if (err == 3) return fail("Nan found in PenaltyForce");
+    const int m = err / 4 - 1, why = err % 4;
+    int body = 0;
+    while (body + 1 < I.nbodies && m >= I.moff[body + 1]) ++body;
+    return fail("immersed-boundary stencil of marker %d (body %d): index out of %s bound", m - I.moff[body] + 1,
+                body + 1, why == 1 ? "min" : "max");
+  }
   I.active = true;
-  if (iters) *iters = it;
-  if (dmax) *dmax = dm;
+  if (iters) *iters = I.h_ctl[kIters];
+  if (dmax) *dmax = I.h_dstate[0];
   return 0;
 }
+
+}  // extern "C"
+
+namespace {
+// Wait (host side) for the last beam sub-steps and check their results:
+// "CG solver breakdown" of any sub-step and body is an error.
+int fem_collect(Block &b) {
+  Fem &F = b.fem;
+  if (!F.fs || !F.fs->pending) return 0;
+  HIPCHK(hipEventSynchronize(F.fs->ev_done));
+  F.fs->pending = false;
+  const int nb = F.nbodies;
+  const int nsub = nb ? F.fs->n_staged / nb : 0;
+  for (int i = 0; i < F.fs->n_staged; ++i)
+    if (F.fs->h_stage[i].status)
+      return fail("ERROR: CG solver breakdown (body %d, sub-step %d)", i % nb + 1, i / nb + 1);
+  if (nsub > 0)
+    for (int bi = 0; bi < nb; ++bi) {
+      const FemBodyDesc &r = F.fs->h_stage[(nsub - 1) * nb + bi];
+      F.desc[bi].newton_iters = r.newton_iters;
+      F.desc[bi].cg_iters = r.cg_iters;
+      F.desc[bi].status = r.status;
+      F.desc[bi].dnorm = r.dnorm;
+    }
+  return 0;
+}
+
+// order work on the block's stream after the last beam sub-steps (device side)
+int fem_order_after(Block &b) {
+  Fem &F = b.fem;
+  if (F.fs && F.fs->pending) HIPCHK(hipStreamWaitEvent(b.stream, F.fs->ev_done, 0));
+  return 0;
+}
+}  // namespace
+
+extern "C" {
 
 int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, const double *xyz8,
                            const int *elems, const int *cons, const int *iBodyModel, const int *isMotionGiven,
@@ -1211,7 +1233,10 @@ int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, c
     HIPCHK(v->ensure(F.gEQ));
   HIPCHK(F.lbuf.ensure((size_t)F.nEL * NED));
   HIPCHK(F.blk.ensure((size_t)F.nND * 36));
-  HIPCHK(F.ddesc.ensure(nbodies));
+  F.fs = std::make_unique<FemStream>();
+  HIPCHK(hipStreamCreateWithFlags(&F.fs->s, hipStreamNonBlocking));
+  HIPCHK(hipEventCreateWithFlags(&F.fs->ev_in, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&F.fs->ev_done, hipEventDisableTiming));
   // lodFlow starts at zero (Beam_InitLoad)
   if (upload(b->ibm.lod, zeros.data(), zeros.size(), s)) return 1;
   HIPCHK(hipStreamSynchronize(s));
@@ -1234,6 +1259,7 @@ int fsil_fem_update_markers(int h) {
   Fem &F = b->fem;
   Ibm &I = b->ibm;
   if (F.nbodies == 0) return fail("no FEM bodies defined");
+  if (fem_order_after(*b)) return 1;
   FemMarkerArgs a{};
   a.el = F.el.p; a.pos = F.pos.p; a.vel = F.vel.p;
   a.body_of_elem = F.body_of_elem.p; a.elem_first_marker = F.elem_first_marker.p; a.body_goff = F.body_goff.p;
@@ -1241,9 +1267,6 @@ int fsil_fem_update_markers(int h) {
   a.Exyz = I.Exyz.p; a.Evel = I.Evel.p; a.Ea = I.Ea.p; a.elem_xc = I.elem_xc.p;
   hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 255) / 256), dim3(256), 0, b->stream, a);
   HIPCHK(hipGetLastError());
-  I.hExyz.resize(3 * (size_t)I.M);
-  HIPCHK(hipMemcpyAsync(I.hExyz.data(), I.Exyz.p, 3 * (size_t)I.M * sizeof(double), hipMemcpyDeviceToHost, b->stream));
-  HIPCHK(hipStreamSynchronize(b->stream));
   F.markers_on_device = true;
   return 0;
 }
@@ -1254,13 +1277,26 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   Fem &F = b->fem;
   if (F.nbodies == 0) return fail("no FEM bodies defined");
   if (numsubstep < 1) return fail("numsubstep must be >= 1");
+  FemStream &S = *F.fs;
+  if (fem_collect(*b)) return 1;  // the staging buffer is free once the previous call is done
   const double subdt = deltat / (double)numsubstep;
   FemArrays A = fem_arrays(*b);
-  std::vector<std::vector<FemBodyDesc>> staged(numsubstep, F.desc);
-  for (int isub = 1; isub <= numsubstep; ++isub) {
-    std::vector<FemBodyDesc> &D = staged[isub - 1];
+  int max_nd = 1;
+  for (const FemBodyDesc &d : F.desc) max_nd = std::max(max_nd, d.nND);
+  const int lds_nodes = std::min(max_nd, 512 / 6);  // 6 threads per node; 6x12 FP64 work areas (<= 48 KiB)
+  const size_t n = (size_t)numsubstep * F.nbodies;
+  if (n > S.cap) {
+    if (S.h_stage) HIPCHK(hipHostFree(S.h_stage));
+    S.h_stage = nullptr;
+    S.cap = 0;
+    HIPCHK(hipHostMalloc((void **)&S.h_stage, n * sizeof(FemBodyDesc), hipHostMallocDefault));
+    S.cap = n;
+    HIPCHK(F.ddesc.ensure(n));
+  }
+  for (int isub = 1; isub <= numsubstep; ++isub)
     for (int bi = 0; bi < F.nbodies; ++bi) {
-      FemBodyDesc &d = D[bi];
+      FemBodyDesc &d = S.h_stage[(size_t)(isub - 1) * F.nbodies + bi];
+      d = F.desc[bi];
       fem_kinematics(F.bh[bi], time, isub, deltat, subdt, d);
       double *c = d.coeffs;
       c[2] = 1.0 / (F.NewmarkBeta * subdt);
@@ -1272,15 +1308,22 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
       c[6] = subdt * (1.0 - F.NewmarkGamma);
       c[7] = subdt * F.NewmarkGamma;
       d.ntolFEM = F.ntolFEM; d.dtolFEM = F.dtolFEM; d.dampK = F.dampK; d.dampM = F.dampM; d.GeoGamma = F.GeoGamma;
+      d.newton_iters = 0; d.cg_iters = 0; d.status = 0; d.dnorm = 0.0;
     }
-    HIPCHK(hipMemcpyAsync(F.ddesc.p, D.data(), D.size() * sizeof(FemBodyDesc), hipMemcpyHostToDevice, b->stream));
-    hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(256), 0, b->stream, F.ddesc.p, A);
+  // after the coupling call that produced lodFlow (block's stream)
+  HIPCHK(hipEventRecord(S.ev_in, b->stream));
+  HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
+  HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
+  for (int isub = 0; isub < numsubstep; ++isub) {
+    hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512),
+                       (size_t)lds_nodes * 72 * sizeof(double) + (size_t)lds_nodes * sizeof(int), S.s,
+                       F.ddesc.p + (size_t)isub * F.nbodies, A, lds_nodes);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMemcpyAsync(F.desc.data(), F.ddesc.p, F.desc.size() * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, b->stream));
-  HIPCHK(hipStreamSynchronize(b->stream));
-  for (int bi = 0; bi < F.nbodies; ++bi)
-    if (F.desc[bi].status) return fail("ERROR: CG solver breakdown (body %d)", bi + 1);
+  HIPCHK(hipMemcpyAsync(S.h_stage, F.ddesc.p, n * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, S.s));
+  HIPCHK(hipEventRecord(S.ev_done, S.s));
+  S.n_staged = (int)n;
+  S.pending = true;
   return 0;
 }
 
@@ -1289,6 +1332,7 @@ int fsil_fem_state(int h, int body, double *pos, double *vel, double *acc, doubl
   if (!b) return 1;
   Fem &F = b->fem;
   if (body < 0 || body >= F.nbodies) return fail("body %d out of range", body);
+  if (fem_collect(*b)) return 1;
   const FemBodyDesc &d = F.desc[body];
   const size_t n = 6 * (size_t)d.nND;
   hipStream_t s = b->stream;
@@ -1305,6 +1349,7 @@ int fsil_fem_info(int h, int body, int *newton_iters, int *cg_iters, double *dno
   if (!b) return 1;
   Fem &F = b->fem;
   if (body < 0 || body >= F.nbodies) return fail("body %d out of range", body);
+  if (fem_collect(*b)) return 1;
   if (newton_iters) *newton_iters = F.desc[body].newton_iters;
   if (cg_iters) *cg_iters = F.desc[body].cg_iters;
   if (dnorm) *dnorm = F.desc[body].dnorm;
@@ -1340,10 +1385,19 @@ int fsil_copy_bandwidth(long long nbytes, int reps, double *gbs) {
 int fsil_ibm_band_info(int h, int *nband, int *bbox) {
   Block *b = get_block(h);
   if (!b) return 1;
-  if (nband) *nband = b->ibm.nband;
+  Ibm &I = b->ibm;
+  int c[kCtlWords] = {0};
+  if (I.ctl.p) {
+    HIPCHK(hipMemcpyAsync(c, I.ctl.p, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
+  const bool have = I.ctl.p && c[kNband] > 0;
+  if (nband) *nband = have ? c[kNband] : 0;
   if (bbox) {
-    bbox[0] = b->ibm.bx0; bbox[1] = b->ibm.by0; bbox[2] = b->ibm.bz0;
-    bbox[3] = b->ibm.bnx; bbox[4] = b->ibm.bny; bbox[5] = b->ibm.bnz;
+    for (int k = 0; k < 3; ++k) {
+      bbox[k] = have ? c[kBoxLo + k] : 0;
+      bbox[3 + k] = have ? c[kBoxHi + k] - c[kBoxLo + k] + 1 : 0;
+    }
   }
   return 0;
 }

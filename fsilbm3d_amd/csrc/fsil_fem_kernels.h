@@ -79,22 +79,42 @@ __device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D) {
   fem_sync();
 }
 
-// sequential dot product in DOF order, broadcast through LDS
-__device__ inline double fem_dot(const double *a, const double *b, int g0, int n, double *shared) {
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int i = 0; i < n; ++i) s = s + a[g0 + i] * b[g0 + i];
-    *shared = s;
+// sequential dot product in DOF order, broadcast through LDS.  The operands
+// are staged into LDS by the whole workgroup, chunk by chunk, so the one
+// thread that sums them in order reads LDS rather than waiting on global
+// memory for every term.
+__device__ inline double fem_dot(const double *a, const double *b, int g0, int n, double *shared, double *stage,
+                                 int cap) {
+  double s = 0.0;
+  for (int base = 0; base < n; base += cap) {
+    const int m = min(cap, n - base);
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+      stage[i] = a[g0 + base + i];
+      stage[cap + i] = b[g0 + base + i];
+    }
+    fem_sync();
+    if (threadIdx.x == 0) {
+      // unrolled so the LDS reads of a run of terms are in flight together;
+      // the additions stay in DOF order
+#pragma unroll 16
+      for (int i = 0; i < m; ++i) s = s + stage[i] * stage[cap + i];
+    }
+    fem_sync();
   }
+  if (threadIdx.x == 0) *shared = s;
   fem_sync();
   const double v = *shared;
   fem_sync();
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArrays A) {
+// Dynamic LDS: lds_nodes work areas of 6x12 doubles for the block inverses
+// (plus one int flag per node), reused as the staging area (2 x 36*lds_nodes
+// doubles) of the dot products.
+__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArrays A, int lds_nodes) {
   __shared__ double sh_val;
   __shared__ int sh_flag;
+  extern __shared__ double lds[];
   FemBodyDesc &D = descs[blockIdx.x];
   const int nE = D.nEL, nN = D.nND, nG = 6 * nN, g0 = D.goff;
   if (D.iBodyModel == 1) {
@@ -149,12 +169,32 @@ __global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArra
   int status = 0;
   double dnorm = 1.0;
   for (iter = 1; iter <= D.ntolFEM; ++iter) {
-    // UpdateMatrixANDLoad (SolidSolver.f90:1932-1947)
+    // UpdateMatrixANDLoad (SolidSolver.f90:1932-1947).  Per element: internal
+    // force from the local stiffness, local geometric stiffness, rotation.
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
       BeamElem &E = A.el[D.eoff + e];
-      el_form_stiff(E);
       el_body_stress(E);
-      el_update_matrix(E, c, D.GeoGamma, D.dampM, D.dampK);
+      el_form_geom(E);
+      el_rotate(E);
+    }
+    fem_sync();
+    // Per matrix entry: R^T K R, R^T G R and the Newmark tangent matrix
+    // (Segment_UpdateMatrix, SolidSolver.f90:131-157), entry by entry.
+    for (int t = threadIdx.x; t < nE * NED * NED; t += blockDim.x) {
+      const int e = t / (NED * NED), k = t - e * (NED * NED);
+      BeamElem &E = A.el[D.eoff + e];
+      const int I = k / NED, J = k - I * NED;
+      const double sR = rkr_entry(E.stl, E.rot, I, J);
+      const double gR = rkr_entry(E.geo, E.rot, I, J);
+      const double tanv = sR + D.GeoGamma * gR;
+      double cv = tanv + c[0] * E.mas[k] + c[1] * D.dampM * E.mas[k];
+      if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
+      E.stf[k] = sR;
+      E.coef[k] = cv;
+    }
+    fem_sync();
+    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+      BeamElem &E = A.el[D.eoff + e];
       el_update_load(E, c, D.dampM, D.dampK, A.dspO + g0, A.dsp + g0, A.vel + g0, A.acc + g0);
     }
     fem_sync();
@@ -175,25 +215,64 @@ __global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArra
       A.xF[g0 + g] = A.fixed[g0 + g] ? (iter == 1 ? A.vBC[g0 + g] : 0.0) : 0.0;
       A.x[g0 + g] = A.xF[g0 + g];
     }
-    for (int n = threadIdx.x; n < nN; n += blockDim.x) {
-      double B[36], Bi[36];
-      for (int k = 0; k < 36; ++k) B[k] = 0.0;
-      for (int k = A.node_ptr[D.noff + n]; k < A.node_ptr[D.noff + n + 1]; ++k) {
-        const BeamElem &E = A.el[A.node_e[k]];
-        const int o = A.node_h[k] ? 6 : 0;
-        for (int i = 0; i < 6; ++i)
-          for (int j = 0; j < 6; ++j) B[i * 6 + j] = B[i * 6 + j] + E.coef[(o + i) * NED + o + j];
-      }
-      for (int i = 0; i < 6; ++i)
-        if (A.fixed[g0 + 6 * n + i]) {
-          for (int j = 0; j < 6; ++j) { B[i * 6 + j] = 0.0; B[j * 6 + i] = 0.0; }
-          B[i * 6 + i] = 1.0;
+    // block-Jacobi preconditioner (SolidSolver.f90:1966-1975, Invert6x6
+    // :2234-2279): one 6x12 Gauss-Jordan work area per node in LDS, one thread
+    // per row; each row operation is the sequential one, the rows of a pivot
+    // step run side by side.
+    for (int base = 0; base < nN; base += lds_nodes) {
+      const int ln = (int)threadIdx.x / 6, row = (int)threadIdx.x - 6 * ln;
+      const int n = base + ln;
+      const bool act = ln < lds_nodes && n < nN;
+      double *aug = lds + 72 * ln;
+      int *bad = reinterpret_cast<int *>(lds + 72 * lds_nodes) + ln;
+      if (act) {
+        for (int j = 0; j < 12; ++j) aug[row * 12 + j] = 0.0;
+        for (int k = A.node_ptr[D.noff + n]; k < A.node_ptr[D.noff + n + 1]; ++k) {
+          const BeamElem &E = A.el[A.node_e[k]];
+          const int o = A.node_h[k] ? 6 : 0;
+          for (int j = 0; j < 6; ++j) aug[row * 12 + j] = aug[row * 12 + j] + E.coef[(o + row) * NED + o + j];
         }
-      if (invert6(B, Bi)) {
-        for (int k = 0; k < 36; ++k) Bi[k] = 0.0;
-        for (int i = 0; i < 6; ++i) Bi[i * 6 + i] = 1.0;
+        if (row == 0) *bad = 0;
       }
-      for (int k = 0; k < 36; ++k) A.blk[(size_t)(D.noff + n) * 36 + k] = Bi[k];
+      fem_sync();
+      if (act && row == 0) {
+        for (int i = 0; i < 6; ++i)
+          if (A.fixed[g0 + 6 * n + i]) {
+            for (int j = 0; j < 6; ++j) { aug[i * 12 + j] = 0.0; aug[j * 12 + i] = 0.0; }
+            aug[i * 12 + i] = 1.0;
+          }
+      }
+      fem_sync();
+      if (act) aug[row * 12 + 6 + row] = 1.0;
+      for (int i = 0; i < 6; ++i) {
+        if (act && row == 0 && !*bad) {
+          int p = i;
+          double pivot = fabs(aug[i * 12 + i]);
+          for (int k = i + 1; k < 6; ++k)
+            if (fabs(aug[k * 12 + i]) > pivot) { pivot = fabs(aug[k * 12 + i]); p = k; }
+          if (pivot <= 1.0e-30) *bad = i + 1;
+          else if (p != i)
+            for (int j = 0; j < 12; ++j) {
+              const double t = aug[i * 12 + j]; aug[i * 12 + j] = aug[p * 12 + j]; aug[p * 12 + j] = t;
+            }
+        }
+        fem_sync();
+        if (act && row == i && !*bad) {
+          const double pivot = aug[i * 12 + i];
+          for (int j = 0; j < 12; ++j) aug[i * 12 + j] = aug[i * 12 + j] / pivot;
+        }
+        fem_sync();
+        if (act && row != i && !*bad) {
+          const double factor = aug[row * 12 + i];
+          for (int j = 0; j < 12; ++j) aug[row * 12 + j] = aug[row * 12 + j] - factor * aug[i * 12 + j];
+        }
+        fem_sync();
+      }
+      if (act) {
+        double *out = A.blk + (size_t)(D.noff + n) * 36;
+        for (int j = 0; j < 6; ++j) out[row * 6 + j] = *bad ? (row == j ? 1.0 : 0.0) : aug[row * 12 + 6 + j];
+      }
+      fem_sync();
     }
     fem_sync();
     fem_matvec(A, D, A.x, A.Ap);
@@ -202,11 +281,11 @@ __global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArra
     fem_sync();
     fem_precond(A, D);
     const double err = 1.0e-6;
-    double rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val));
+    double rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val, lds, 36 * lds_nodes));
     if (!(rn <= err)) {
       for (int g = threadIdx.x; g < nG; g += blockDim.x) A.p[g0 + g] = A.z[g0 + g];
       fem_sync();
-      double rsold = fem_dot(A.r, A.z, g0, nG, &sh_val);
+      double rsold = fem_dot(A.r, A.z, g0, nG, &sh_val, lds, 36 * lds_nodes);
       if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
       for (int it = 1; it <= 10000; ++it) {
         ++cg_total;
@@ -214,7 +293,7 @@ __global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArra
         for (int g = threadIdx.x; g < nG; g += blockDim.x)
           if (A.fixed[g0 + g]) A.Ap[g0 + g] = 0.0;
         fem_sync();
-        const double pAp = fem_dot(A.p, A.Ap, g0, nG, &sh_val);
+        const double pAp = fem_dot(A.p, A.Ap, g0, nG, &sh_val, lds, 36 * lds_nodes);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
@@ -226,10 +305,10 @@ __global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArra
           A.r[g0 + g] = rv;
         }
         fem_sync();
-        rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val));
+        rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val, lds, 36 * lds_nodes));
         if (rn <= err) break;
         fem_precond(A, D);
-        const double rsnew = fem_dot(A.r, A.z, g0, nG, &sh_val);
+        const double rsnew = fem_dot(A.r, A.z, g0, nG, &sh_val, lds, 36 * lds_nodes);
         if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
         const double beta = rsnew / rsold;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
@@ -255,13 +334,26 @@ __global__ __launch_bounds__(256) void k_fem_substep(FemBodyDesc *descs, FemArra
       el_cptdxyz1(E);
       el_update_triads(E, A.x + g0);
     }
-    if (threadIdx.x == 0) {
+    // dnorm = |maxval(dsp_increment**2)| (SolidSolver.f90:2336): the maximum
+    // does not depend on the order, so a wave reduction gives the same value
+    {
       double mx = -INFINITY;
-      for (int g = 0; g < nG; ++g) {
+      for (int g = threadIdx.x; g < nG; g += blockDim.x) {
         const double v = A.x[g0 + g] * A.x[g0 + g];
         if (v > mx) mx = v;
       }
-      sh_val = fabs(mx);
+      for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(mx, off, 64);
+        if (o > mx) mx = o;
+      }
+      if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+      fem_sync();
+      if (threadIdx.x == 0) {
+        double m = lds[0];
+        for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w)
+          if (lds[w] > m) m = lds[w];
+        sh_val = fabs(m);
+      }
     }
     fem_sync();
     for (int n = threadIdx.x; n < nN; n += blockDim.x) {

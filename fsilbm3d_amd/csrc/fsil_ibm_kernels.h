@@ -7,20 +7,181 @@
 // serial loops over markers (Solidbody.f90:1034-1048, 968-977) update a cell;
 // a per-cell sequential reduction over its sorted entries therefore rounds
 // exactly as the reference does, with no atomics.
+//
+// The whole structure is built on the device each coupling call: stencils
+// (k_ibm_stencil), a radix sort of the 64-bit keys (cell << 26 | e), and the
+// band assembly (k_ibm_band).  Band arrays have a fixed stride `cap` = 64*M
+// (an upper bound of the band size) and the band size itself lives in device
+// memory (ctl[kNband]), so nothing is read back between the kernels.
 #pragma once
 #include "fsil_common.h"
 #include "fsil_lbm_kernels.h"
 
 namespace fsil {
 
+// control words of the coupling kept in device memory
+enum IbmCtl {
+  kBoxLo = 0,      // [0..2] lowest band cell coordinate per axis (atomic min)
+  kBoxHi = 3,      // [3..5] highest band cell coordinate per axis (atomic max)
+  kNband = 6,      // number of band cells
+  kErr = 7,        // first failing marker (1-based) * 4 + reason, INT_MAX when none
+  kDone = 8,       // multi-direct-forcing loop finished
+  kIters = 9,      // iterations run
+  kCtlWords = 16
+};
+constexpr int kKeyShift = 26;  // e = 64*m + s < 2^26
+
+// 4-point Peskin kernel (Solidbody.f90:822-833), FP64; sqrt is correctly rounded
+__host__ __device__ inline double peskin4(double x) {
+  const double r = fabs(x);
+  if (r < 1.0) return (3.0 - 2.0 * r + sqrt(1.0 + 4.0 * r * (1.0 - r))) * 0.125;
+  if (r < 2.0) return (5.0 - 2.0 * r - sqrt(-7.0 + 4.0 * r * (3.0 - r))) * 0.125;
+  return 0.0;
+}
+
+// trimedindex (Solidbody.f90:834-866): 1-based indices i-1..i+2 wrapped or
+// mirrored by the face codes; returns 0, or 1 (below the min face) / 2
+// (beyond the max face) for an index the face code cannot place, in which
+// case the index is clamped into the block so later kernels stay in bounds.
+__host__ __device__ inline int trim4(int i, int nd, int *out, int bclo, int bchi) {
+  int err = 0;
+  for (int k = -1; k <= 2; ++k) {
+    int v = i + k;
+    if (v < 1) {
+      if (bclo == BC_PERIODIC) v += nd;
+      else if ((bclo == BC_SYMMETRIC || bclo == BC_WALL) && v == 0) v = 2;
+      else if (bclo == BC_WALL_HALFWAY && v == 0) v = 1;
+      else { err = err ? err : 1; v = 1; }
+    } else if (v > nd) {
+      if (bchi == BC_PERIODIC) v -= nd;
+      else if ((bchi == BC_SYMMETRIC || bchi == BC_WALL) && v == nd + 1) v = nd - 1;
+      else if (bchi == BC_WALL_HALFWAY && v == nd + 1) v = nd;
+      else { err = err ? err : 2; v = nd; }
+    }
+    if (v < 1 || v > nd) { err = err ? err : (v < 1 ? 1 : 2); v = v < 1 ? 1 : nd; }
+    out[k + 1] = v;
+  }
+  return err;
+}
+
+// reset of the control words; clears the previous band from the cell map
+__global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const long long *__restrict__ prev_cell,
+                                                   int *__restrict__ cellmap, double *__restrict__ dstate,
+                                                   int done0) {
+  const int nprev = ctl[kNband];
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nprev; i += gridDim.x * blockDim.x)
+    cellmap[prev_cell[i]] = -1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctl[kBoxLo] = ctl[kBoxLo + 1] = ctl[kBoxLo + 2] = 0x7fffffff;
+    ctl[kBoxHi] = ctl[kBoxHi + 1] = ctl[kBoxHi + 2] = -1;
+    ctl[kErr] = 0x7fffffff;
+    ctl[kDone] = done0;
+    ctl[kIters] = 0;
+    dstate[0] = 1e10;  // dmax before the first iteration (Solidbody.f90:893)
+  }
+}
+
+struct StencilArgs {
+  int M;
+  const double *__restrict__ Exyz;       // 3*M
+  const int *__restrict__ first_marker;  // per marker: first marker of its body
+  double org[3];
+  double dh, invdh;
+  int nd[3];
+  int bc[6];
+  float *__restrict__ Ew;                // 12*M
+  unsigned long long *__restrict__ keys; // 64*M
+  int *__restrict__ ctl;
+};
+
+// UpdateElmtInterp_ (Solidbody.f90:760-821) per marker: anchor from the
+// body's first marker, trimmed 4x3 indices, FP32 weights, and the 64 keys.
+__global__ __launch_bounds__(256) void k_ibm_stencil(StencilArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.M) return;
+  const int m0 = a.first_marker[m];
+  int ix[12];
+  int err = 0;
+  int lo[3], hi[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const int ii = (int)floor((a.Exyz[3LL * m0 + k] - a.org[k]) * a.invdh);
+    const double x0 = a.org[k] + (double)ii * a.dh;
+    double off = (a.Exyz[3LL * m + k] - x0) * a.invdh;
+    int idx = (int)floor(off);
+    off = off - (double)idx;
+    idx = idx + ii + 1;
+    const int e = trim4(idx, a.nd[k], &ix[4 * k], a.bc[2 * k], a.bc[2 * k + 1]);
+    if (e && !err) err = e;
+    lo[k] = hi[k] = ix[4 * k];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      a.Ew[12LL * m + 4 * k + j] = (float)peskin4((double)(j - 1) - off);
+      lo[k] = min(lo[k], ix[4 * k + j]);
+      hi[k] = max(hi[k], ix[4 * k + j]);
+    }
+  }
+  if (err) atomicMin(&a.ctl[kErr], (m + 1) * 4 + err);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    atomicMin(&a.ctl[kBoxLo + k], lo[k] - 1);
+    atomicMax(&a.ctl[kBoxHi + k], hi[k] - 1);
+  }
+  unsigned long long *K = a.keys + 64LL * m;
+  for (int x = 0; x < 4; ++x)
+    for (int y = 0; y < 4; ++y)
+#pragma unroll
+      for (int z = 0; z < 4; ++z) {
+        const unsigned long long cell =
+            ((unsigned long long)(ix[x] - 1) * a.nd[1] + (unsigned long long)(ix[4 + y] - 1)) * a.nd[2] +
+            (unsigned long long)(ix[8 + z] - 1);
+        const int e = 64 * m + 16 * x + 4 * y + z;
+        K[16 * x + 4 * y + z] = (cell << kKeyShift) | (unsigned long long)e;
+      }
+}
+
+// head flags of the sorted keys (first entry of each band cell)
+__global__ __launch_bounds__(256) void k_ibm_heads(int E, const unsigned long long *__restrict__ keys,
+                                                   int *__restrict__ flag) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  flag[k] = (k == 0 || (keys[k] >> kKeyShift) != (keys[k - 1] >> kKeyShift)) ? 1 : 0;
+}
+
+// band assembly from the sorted keys and the inclusive scan of the heads
+__global__ __launch_bounds__(256) void k_ibm_band(int E, const unsigned long long *__restrict__ keys,
+                                                  const int *__restrict__ scan, int *__restrict__ entry,
+                                                  int *__restrict__ bidx, long long *__restrict__ band_cell,
+                                                  int *__restrict__ band_ptr, int *__restrict__ cellmap,
+                                                  int *__restrict__ ctl) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= E) return;
+  const unsigned long long key = keys[k];
+  const long long cell = (long long)(key >> kKeyShift);
+  const int e = (int)(key & ((1ULL << kKeyShift) - 1));
+  const int b = scan[k] - 1;
+  entry[k] = e;
+  bidx[e] = b;
+  if (k == 0 || (keys[k - 1] >> kKeyShift) != (key >> kKeyShift)) {
+    band_cell[b] = cell;
+    band_ptr[b] = k;
+    cellmap[cell] = b;
+  }
+  if (k == E - 1) {
+    band_ptr[b + 1] = E;
+    ctl[kNband] = b + 1;
+  }
+}
+
 // band macros: den/u of every band cell from the stored state, same
 // expressions as the collision kernel (FluidDomain.f90:1136-1139)
 __global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f, long long N,
-                                                    const long long *__restrict__ cell, int nband,
+                                                    const long long *__restrict__ cell,
+                                                    const int *__restrict__ ctl, int cap,
                                                     double hv0, double hv1, double hv2,
                                                     double *__restrict__ band_u) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nband) return;
+  if (b >= ctl[kNband]) return;
   const long long n = cell[b];
   double g[Q];
 #pragma unroll
@@ -28,8 +189,8 @@ __global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f
   double s, m1, m2, m3;
   moments(g, s, m1, m2, m3);
   band_u[b] = (m1 + hv0) / s;
-  band_u[nband + b] = (m2 + hv1) / s;
-  band_u[2 * nband + b] = (m3 + hv2) / s;
+  band_u[cap + b] = (m2 + hv1) / s;
+  band_u[2LL * cap + b] = (m3 + hv2) / s;
 }
 
 __global__ void k_zero(double *__restrict__ p, long long n) {
@@ -56,12 +217,14 @@ __device__ __forceinline__ double wave_sum(double v) {
 // Per-block partial sums of |dV|_1 go to tolpart[blockIdx.x].
 __global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ bidx,
                                                 const float *__restrict__ Ew,
-                                                const double *__restrict__ band_u, int nband,
+                                                const double *__restrict__ band_u, int cap,
+                                                const int *__restrict__ ctl,
                                                 const double *__restrict__ Evel,
                                                 const double *__restrict__ Ea, double invh3,
                                                 double *__restrict__ Eforce,
                                                 double *__restrict__ fT,
                                                 double *__restrict__ tolpart) {
+  if (ctl[kDone]) return;  // uniform over the grid
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   double tl = 0.0;
   if (m < M) {
@@ -77,8 +240,8 @@ __global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ b
           const double rz = (double)W[8 + z];
           const int b = I[16 * x + 4 * y + z];
           v1 = v1 + ((band_u[b] * rx) * ry) * rz;
-          v2 = v2 + ((band_u[nband + b] * rx) * ry) * rz;
-          v3 = v3 + ((band_u[2 * nband + b] * rx) * ry) * rz;
+          v2 = v2 + ((band_u[cap + b] * rx) * ry) * rz;
+          v3 = v3 + ((band_u[2LL * cap + b] * rx) * ry) * rz;
         }
       }
     }
@@ -107,14 +270,16 @@ __global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ b
 
 // velocity correction of every band cell: u -= fT*rx*ry*rz over its entries
 // in (marker, stencil) order (Solidbody.f90:1034-1048)
-__global__ __launch_bounds__(256) void k_correct(int nband, const int *__restrict__ ptr,
+__global__ __launch_bounds__(256) void k_correct(const int *__restrict__ ctl, int cap,
+                                                 const int *__restrict__ ptr,
                                                  const int *__restrict__ entry,
                                                  const float *__restrict__ Ew,
                                                  const double *__restrict__ fT,
                                                  double *__restrict__ band_u) {
+  if (ctl[kDone]) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nband) return;
-  double u1 = band_u[b], u2 = band_u[nband + b], u3 = band_u[2 * nband + b];
+  if (b >= ctl[kNband]) return;
+  double u1 = band_u[b], u2 = band_u[cap + b], u3 = band_u[2LL * cap + b];
   for (int k = ptr[b]; k < ptr[b + 1]; ++k) {
     const int e = entry[k];
     const int m = e >> 6, s = e & 63;
@@ -126,21 +291,43 @@ __global__ __launch_bounds__(256) void k_correct(int nband, const int *__restric
     u3 = u3 - ((fT[3LL * m + 2] * rx) * ry) * rz;
   }
   band_u[b] = u1;
-  band_u[nband + b] = u2;
-  band_u[2 * nband + b] = u3;
+  band_u[cap + b] = u2;
+  band_u[2LL * cap + b] = u3;
+}
+
+// stop test of the multi-direct-forcing loop (Solidbody.f90:895-906): the
+// block partials summed in block order, dmax = tol / (sum M_b * Uref), one
+// iteration counted; sets the done word when the loop condition fails.
+__global__ void k_mdf_check(int *__restrict__ ctl, double *__restrict__ dstate,
+                            const double *__restrict__ tolpart, int nblk, double dsum, double Uref,
+                            int ntol, double dtol) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || ctl[kDone]) return;
+  double tol = 0.0;
+  for (int k = 0; k < nblk; ++k) tol = tol + tolpart[k];
+  const int it = ctl[kIters] + 1;
+  ctl[kIters] = it;
+  if (!isfinite(tol)) {
+    atomicMin(&ctl[kErr], 3);  // "Nan found in PenaltyForce" (Solidbody.f90:1028-1031)
+    ctl[kDone] = 1;
+    return;
+  }
+  const double dm = tol / (dsum * Uref);
+  dstate[0] = dm;
+  if (!(it < ntol && dm > dtol)) ctl[kDone] = 1;
 }
 
 // Eulerian force of every band cell: (0 - sum F/h^3*rx*ry*rz) + VF
 // (ResetVolumeForce_ FluidDomain.f90:1195-1206, FluidVolumeForce_
 // Solidbody.f90:968-977, add_volume_force_ FluidDomain.f90:1182-1193)
-__global__ __launch_bounds__(256) void k_spread(int nband, const int *__restrict__ ptr,
+__global__ __launch_bounds__(256) void k_spread(const int *__restrict__ ctl, int cap,
+                                                const int *__restrict__ ptr,
                                                 const int *__restrict__ entry,
                                                 const float *__restrict__ Ew,
                                                 const double *__restrict__ Eforce, double invh3,
                                                 double VF1, double VF2, double VF3,
                                                 double *__restrict__ band_F) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nband) return;
+  if (b >= ctl[kNband]) return;
   double g1 = 0.0, g2 = 0.0, g3 = 0.0;
   for (int k = ptr[b]; k < ptr[b + 1]; ++k) {
     const int e = entry[k];
@@ -155,8 +342,8 @@ __global__ __launch_bounds__(256) void k_spread(int nband, const int *__restrict
     g3 = g3 + (-(((h3 * rx) * ry) * rz));
   }
   band_F[b] = g1 + VF1;
-  band_F[nband + b] = g2 + VF2;
-  band_F[2 * nband + b] = g3 + VF3;
+  band_F[cap + b] = g2 + VF2;
+  band_F[2LL * cap + b] = g3 + VF3;
 }
 
 // Beam nodal loads lodFlow: per degree of freedom, the sum over its markers in
@@ -175,6 +362,9 @@ __global__ __launch_bounds__(256) void k_loads(int ndof, const int *__restrict__
   if (d >= ndof) return;
   const int k = dof_kind[d];
   double acc = 0.0;
+  // unrolled: the marker ids and values of a run of terms are loaded
+  // together; the sum stays in marker order
+#pragma unroll 8
   for (int j = dof_ptr[d]; j < dof_ptr[d + 1]; ++j) {
     const int m = dof_list[j];
     double val;

@@ -84,11 +84,11 @@ struct CollideArgs {
   const double *__restrict__ Mc;  // MRT M_COLLID, Fortran column-major 19x19
   const double *__restrict__ Mf;  // MRT M_FORCE
   // immersed-boundary band overlay (cells whose velocity/force come from IBM)
-  int bx0, by0, bz0, bnx, bny, bnz;
-  const int *__restrict__ bmap;       // bbox-local map -> band index, -1 outside band
-  const double *__restrict__ band_u;  // [3][nband] corrected velocity
-  const double *__restrict__ band_F;  // [3][nband] spread force (+VF)
-  int nband;
+  const int *__restrict__ ctl;        // band bounding box in ctl[0..5] (device memory)
+  const int *__restrict__ cellmap;    // cell -> band index, -1 outside the band
+  const double *__restrict__ band_u;  // [3][cap] corrected velocity
+  const double *__restrict__ band_F;  // [3][cap] spread force (+VF)
+  int cap;
   // auxiliary plane outputs for boundary codes 102 (den/u of the first
   // interior plane) and 203/204 (post-collision boundary plane)
   int aux_face_mask102, aux_face_mask_hw;
@@ -129,17 +129,16 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
   bool forced = FORCE;
   double F1 = a.VF[0], F2 = a.VF[1], F3 = a.VF[2];
   if constexpr (BAND) {
-    const int lx = x - a.bx0, ly = y - a.by0, lz = z - a.bz0;
-    if ((unsigned)lx < (unsigned)a.bnx && (unsigned)ly < (unsigned)a.bny &&
-        (unsigned)lz < (unsigned)a.bnz) {
-      const int b = a.bmap[((long long)lx * a.bny + ly) * a.bnz + lz];
+    if (x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] &&
+        z <= a.ctl[5]) {
+      const int b = a.cellmap[n];
       if (b >= 0) {
         u1 = a.band_u[b];
-        u2 = a.band_u[a.nband + b];
-        u3 = a.band_u[2 * a.nband + b];
+        u2 = a.band_u[a.cap + b];
+        u3 = a.band_u[2 * a.cap + b];
         F1 = a.band_F[b];
-        F2 = a.band_F[a.nband + b];
-        F3 = a.band_F[2 * a.nband + b];
+        F2 = a.band_F[a.cap + b];
+        F3 = a.band_F[2 * a.cap + b];
         forced = true;
       }
     }

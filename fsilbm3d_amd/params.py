@@ -117,3 +117,17 @@ def fortran_order(m: np.ndarray) -> np.ndarray:
 def volume_force(VFin, amp, freq, phi, t):
     """update_volume_force_ (FluidDomain.f90:1174-1180)."""
     return (VFin[0] + amp * math.sin(2.0 * PI * freq * t + phi / 180.0 * PI), VFin[1], VFin[2])
+
+
+def equilibrium_point(rho: float, u) -> np.ndarray:
+    """calculate_distribution_funcion (FluidDomain.f90:1827-1834) of one node,
+    in its association order: wt*rho*(((1+3ux)+4.5ux*ux)-1.5u^2)."""
+    u1, u2, u3 = (float(v) for v in u)
+    uSqr = u1 * u1
+    uSqr = uSqr + u2 * u2
+    uSqr = uSqr + u3 * u3
+    out = np.empty(19)
+    for q in range(19):
+        ux = (u1 * EE[q, 0] + u2 * EE[q, 1]) + u3 * EE[q, 2]
+        out[q] = (WT[q] * rho) * (((1.0 + 3.0 * ux) + (4.5 * ux) * ux) - 1.5 * uSqr)
+    return out

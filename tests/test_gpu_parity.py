@@ -113,8 +113,9 @@ def test_macro_quantities_match_oracle():
 
 
 def test_stencil_out_of_bounds_is_an_error():
-    """A marker whose stencil leaves a non-periodic, non-mirrored face is
-    rejected on the host before any kernel runs (Solidbody.f90:850-862)."""
+    """A marker whose stencil leaves a non-periodic, non-mirrored face is an
+    error of the coupling call (Solidbody.f90:850-862); the stencil kernel
+    clamps such indices, so no kernel leaves the block meanwhile."""
     from fsilbm3d_amd._lib import FsilError
     from oracle.refcase import Case
     case = Case(dims=(12, 12, 12), bc=(101, 103, 302, 302, 302, 302), Re=10.0, uvwIn=(0.05, 0, 0), UrefType=9,
