@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(HERE, "libfsil.so")
 # every exported symbol with (restype, argtypes); the CPU test checks this
 # list against include/fsilbm3d_amd.h
 _i, _d, _pi, _pd = C.c_int, C.c_double, C.POINTER(C.c_int), C.POINTER(C.c_double)
+_pf = C.POINTER(C.c_float)
 SIGNATURES = {
     "fsil_version": (_i, []),
     "fsil_last_error": (C.c_char_p, []),
@@ -48,6 +49,13 @@ SIGNATURES = {
     "fsil_deliver_son_to_father": (_i, [_i, _i]),
     "fsil_tree_steps": (_i, [_i, _i, _d, _d]),
     "fsil_time_tree_steps": (_i, [_i, _i, _d, _d, _pd]),
+    "fsil_mlp_create": (_i, [_i, _pi]),
+    "fsil_mlp_destroy": (_i, [_i]),
+    "fsil_mlp_get_params": (_i, [_i, _pf]),
+    "fsil_mlp_set_params": (_i, [_i, _pf]),
+    "fsil_mlp_forward": (_i, [_i, _pf, _i, _pf, _i]),
+    "fsil_mlp_loss_grad": (_i, [_i, _pf, _pf, _i, _pd, _pf]),
+    "fsil_mlp_train": (_i, [_i, _pf, _pf, _i, _i, _i, _d, _i, _pd, _pd]),
     "fsil_fem_define_bodies": (_i, [_i, _i, _pi, _pi, _pd, _pi, _pi, _pi, _pi, _pd, _pd, _d, _d, _d, _d]),
     "fsil_fem_set_solver": (_i, [_i, _d, _d, _d, _d, _d, _d, _i]),
     "fsil_fem_update_markers": (_i, [_i]),
@@ -95,6 +103,11 @@ def dptr(a: np.ndarray):
 def iptr(a: np.ndarray):
     assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
     return a.ctypes.data_as(_pi)
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], (a.dtype, a.flags)
+    return a.ctypes.data_as(_pf)
 
 
 _initialised = False

@@ -22,6 +22,7 @@
 #include "fsil_lbm32_kernels.h"
 #include "fsil_fem_kernels.h"
 #include "fsil_tree_kernels.h"
+#include "fsil_mlp_kernels.h"
 
 using namespace fsil;
 
@@ -88,7 +89,9 @@ struct Ibm {
   double dtol = 0.0, Uref = 1.0, denIn = 1.0;
   // device: markers and forces
   DevBuf<double> Exyz, Evel, Ea, Eforce, fT, tolpart, elem_xc, lod;
-  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_kind, first_marker;
+  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_kind, dof_of_entry, first_marker;
+  DevBuf<double> load_term;                  // one term per dof_list entry
+  int nlist = 0;
   // device: stencils and band (built each call, see fsil_ibm_kernels.h)
   int cap = 0;                               // 64*M, stride of the band arrays
   int end_bit = 0;                           // key bits to sort
@@ -103,6 +106,7 @@ struct Ibm {
   std::vector<double> h_dstate;
   int ndof = 0;
   bool active = false;  // band valid for the next collision
+  int warm_mlp = 0;     // network handle of the learned warm start (0: off)
 };
 
 // Device FEM state of the bodies carried by a block.
@@ -259,6 +263,7 @@ int launch_faces(Block &b) {
     if (!is_active_face(code)) continue;
     FaceArgs a{};
     a.f = b.fA.p;
+    a.g = b.gA.p;
     a.xd = b.xd; a.yd = b.yd; a.zd = b.zd; a.N = b.N;
     a.face = face; a.code = code;
     a.dh = b.dh; a.xmin = b.xmin; a.ymin = b.ymin; a.zmin = b.zmin;
@@ -276,13 +281,24 @@ int launch_faces(Block &b) {
     a.hw = b.hw_valid[face] ? b.hw[face].p : nullptr;
     const int np = plane_size(b, face);
     const dim3 grid((np + 255) / 256), block(256);
-    switch (face) {
-      case 0: hipLaunchKernelGGL(k_face<0>, grid, block, 0, b.stream, a); break;
-      case 1: hipLaunchKernelGGL(k_face<1>, grid, block, 0, b.stream, a); break;
-      case 2: hipLaunchKernelGGL(k_face<2>, grid, block, 0, b.stream, a); break;
-      case 3: hipLaunchKernelGGL(k_face<3>, grid, block, 0, b.stream, a); break;
-      case 4: hipLaunchKernelGGL(k_face<4>, grid, block, 0, b.stream, a); break;
-      case 5: hipLaunchKernelGGL(k_face<5>, grid, block, 0, b.stream, a); break;
+    if (b.precision == 0) {
+      switch (face) {
+        case 0: hipLaunchKernelGGL((k_face<0, false>), grid, block, 0, b.stream, a); break;
+        case 1: hipLaunchKernelGGL((k_face<1, false>), grid, block, 0, b.stream, a); break;
+        case 2: hipLaunchKernelGGL((k_face<2, false>), grid, block, 0, b.stream, a); break;
+        case 3: hipLaunchKernelGGL((k_face<3, false>), grid, block, 0, b.stream, a); break;
+        case 4: hipLaunchKernelGGL((k_face<4, false>), grid, block, 0, b.stream, a); break;
+        case 5: hipLaunchKernelGGL((k_face<5, false>), grid, block, 0, b.stream, a); break;
+      }
+    } else {
+      switch (face) {
+        case 0: hipLaunchKernelGGL((k_face<0, true>), grid, block, 0, b.stream, a); break;
+        case 1: hipLaunchKernelGGL((k_face<1, true>), grid, block, 0, b.stream, a); break;
+        case 2: hipLaunchKernelGGL((k_face<2, true>), grid, block, 0, b.stream, a); break;
+        case 3: hipLaunchKernelGGL((k_face<3, true>), grid, block, 0, b.stream, a); break;
+        case 4: hipLaunchKernelGGL((k_face<4, true>), grid, block, 0, b.stream, a); break;
+        case 5: hipLaunchKernelGGL((k_face<5, true>), grid, block, 0, b.stream, a); break;
+      }
     }
     HIPCHK(hipGetLastError());
   }
@@ -370,10 +386,20 @@ int collide_stream32(Block &b) {
     a.VF[k] = (float)b.VF[k];
   }
   const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
+  const bool band = b.ibm.active && b.ibm.M > 0;
+  if (band) {
+    a.ctl = b.ibm.ctl.p;
+    a.cellmap = b.ibm.cellmap.p;
+    a.band_u = b.ibm.band_u.p;
+    a.band_F = b.ibm.band_F.p;
+    a.cap = b.ibm.cap;
+  }
   const dim3 blk = cell_block(b);
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
-  if (force) hipLaunchKernelGGL(k_collide_push32<true>, grid, blk, 0, b.stream, a);
-  else hipLaunchKernelGGL(k_collide_push32<false>, grid, blk, 0, b.stream, a);
+  if (force && band) hipLaunchKernelGGL((k_collide_push32<true, true>), grid, blk, 0, b.stream, a);
+  else if (force) hipLaunchKernelGGL((k_collide_push32<true, false>), grid, blk, 0, b.stream, a);
+  else if (band) hipLaunchKernelGGL((k_collide_push32<false, true>), grid, blk, 0, b.stream, a);
+  else hipLaunchKernelGGL((k_collide_push32<false, false>), grid, blk, 0, b.stream, a);
   HIPCHK(hipGetLastError());
   std::swap(b.gA.p, b.gB.p);
   std::swap(b.gA.n, b.gB.n);
@@ -600,7 +626,8 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   if (precision == 1 && iCollidModel != 1) return fail("FP32-deviation form supports SRT only");
   if (precision == 1)
     for (int k = 0; k < 6; ++k)
-      if (bc[k] != BC_PERIODIC) return fail("FP32-deviation form supports periodic blocks only");
+      if (bc[k] == BC_NEQ_DIRICHLET || bc[k] == BC_WALL_HALFWAY || bc[k] == BC_MOVING_WALL_HALFWAY)
+        return fail("FP32-deviation form does not support boundary code %d (face %d)", bc[k], k + 1);
   for (int k = 0; k < 6; ++k) {
     const int c = bc[k];
     const bool ok = c == 0 || c == 1 || c == 101 || c == 102 || c == 103 || c == 104 || c == 201 ||
@@ -754,7 +781,11 @@ int fsil_f_download(int h, double *fIn) {
 int fsil_set_boundary_conditions(int h) {
   Block *b = get_block(h);
   if (!b) return 1;
-  if (b->precision != 0) return 0;  // FP32 form: periodic blocks only (checked at creation)
+  if (b->precision != 0) {  // FP32 form: no code-102 planes (refused at creation)
+    if (launch_faces(*b)) return 1;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+  }
   // Code 102 reads den/uuu of the first interior plane.  At start-up the
   // reference calls this right after initialise_, where den = denIn and uuu is
   // the inflow profile at every node (FluidDomain.f90:533-545), so those fields
@@ -815,6 +846,7 @@ int fsil_collide_stream_bc(int h) {
     if (launch_faces(*b)) return 1;
   } else {
     if (collide_stream32(*b)) return 1;
+    if (launch_faces(*b)) return 1;
   }
   b->ibm.active = false;
   return 0;
@@ -873,7 +905,6 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
                            const int *nelems, const int *marker_elem, const int *elem_nodes) {
   Block *b = get_block(h);
   if (!b) return 1;
-  if (b->precision != 0) return fail("immersed boundary coupling runs on the FP64 form");
   if (nbodies < 0) return fail("negative body count");
   Ibm &I = b->ibm;
   I = Ibm();
@@ -920,6 +951,12 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   }
   if (upload(I.dof_ptr, ptr.data(), ptr.size(), b->stream)) return 1;
   if (upload(I.dof_list, lst.data(), lst.size(), b->stream)) return 1;
+  std::vector<int> owner(lst.size());
+  for (int d = 0; d < I.ndof; ++d)
+    for (int j = ptr[d]; j < ptr[d + 1]; ++j) owner[j] = d;
+  if (upload(I.dof_of_entry, owner.data(), owner.size(), b->stream)) return 1;
+  I.nlist = (int)lst.size();
+  HIPCHK(I.load_term.ensure(std::max<size_t>(1, lst.size())));
   if (upload(I.dof_kind, kind.data(), kind.size(), b->stream)) return 1;
   if (upload(I.dmarker_elem, I.marker_elem.data(), I.marker_elem.size(), b->stream)) return 1;
   HIPCHK(I.Exyz.ensure(3 * (size_t)I.M));
@@ -952,7 +989,7 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   HIPCHK(I.band_u.ensure(3 * (size_t)I.cap));
   HIPCHK(I.band_F.ensure(3 * (size_t)I.cap));
   HIPCHK(I.dstate.ensure(4));
-  HIPCHK(I.tolpart.ensure((I.M + 255) / 256 + 1));
+  HIPCHK(I.tolpart.ensure(std::max(1, I.M)));  // per-marker |dV|_1
   HIPCHK(I.ctl.ensure(kCtlWords));
   HIPCHK(I.cellmap.ensure((size_t)b->N));
   HIPCHK(hipMemsetAsync(I.ctl.p, 0, kCtlWords * sizeof(int), b->stream));
@@ -1035,10 +1072,13 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                      I.dentry.p, I.dbidx.p, I.dcell.p, I.dptr.p, I.cellmap.p, I.ctl.p);
   HIPCHK(hipGetLastError());
   // calculate_interaction_force (Solidbody.f90:869-918)
-  const int nblkM = (M + 255) / 256;
   const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
-  hipLaunchKernelGGL(k_band_macro, dim3((E + 255) / 256), dim3(256), 0, s, b->fA.p, b->N, I.dcell.p, I.ctl.p,
-                     I.cap, hv0, hv1, hv2, I.band_u.p);
+  if (b->precision == 0)
+    hipLaunchKernelGGL(k_band_macro, dim3((E + 255) / 256), dim3(256), 0, s, b->fA.p, b->N, I.dcell.p, I.ctl.p,
+                       I.cap, hv0, hv1, hv2, I.band_u.p);
+  else
+    hipLaunchKernelGGL(k_band_macro32, dim3((E + 255) / 256), dim3(256), 0, s, b->gA.p, b->N, I.dcell.p,
+                       I.ctl.p, I.cap, hv0, hv1, hv2, I.band_u.p);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_zero, dim3((3 * M + 255) / 256), dim3(256), 0, s, I.Eforce.p, 3LL * M);
   HIPCHK(hipGetLastError());
@@ -1048,13 +1088,13 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   double dsum = 0.0;
   for (int body = 0; body < I.nbodies; ++body) dsum = dsum + (double)I.nmark[body];
   for (int it = 0; it < I.ntol; ++it) {
-    hipLaunchKernelGGL(k_interp, dim3(nblkM), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
+    hipLaunchKernelGGL(k_interp, dim3((M + 3) / 4), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
                        I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p,
                        I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mdf_check, dim3(1), dim3(64), 0, s, I.ctl.p, I.dstate.p, I.tolpart.p, nblkM, dsum,
+    hipLaunchKernelGGL(k_mdf_check, dim3(1), dim3(256), 0, s, I.ctl.p, I.dstate.p, I.tolpart.p, M, dsum,
                        I.Uref, I.ntol, I.dtol);
     HIPCHK(hipGetLastError());
   }
@@ -1063,9 +1103,14 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                      I.dEw.p, I.Eforce.p, invh3s, b->VF[0], b->VF[1], b->VF[2], I.band_F.p);
   HIPCHK(hipGetLastError());
   if (I.ndof > 0) {
-    hipLaunchKernelGGL(k_loads, dim3((I.ndof + 255) / 256), dim3(256), 0, s, I.ndof, I.dof_ptr.p,
-                       I.dof_list.p, I.dof_kind.p, I.dmarker_elem.p, I.elem_xc.p, I.Exyz.p,
-                       I.Eforce.p, I.lod.p);
+    if (I.nlist > 0) {
+      hipLaunchKernelGGL(k_load_terms, dim3((I.nlist + 255) / 256), dim3(256), 0, s, I.nlist, I.dof_of_entry.p,
+                         I.dof_list.p, I.dof_kind.p, I.dmarker_elem.p, I.elem_xc.p, I.Exyz.p, I.Eforce.p,
+                         I.load_term.p);
+      HIPCHK(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_loads, dim3((I.ndof + 255) / 256), dim3(256), 0, s, I.ndof, I.dof_ptr.p, I.load_term.p,
+                       I.lod.p);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(I.h_ctl.data(), I.ctl.p, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s));
@@ -1131,7 +1176,6 @@ int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, c
                            double alpha) {
   Block *b = get_block(h);
   if (!b) return 1;
-  if (b->precision != 0) return fail("immersed boundary coupling runs on the FP64 form");
   if (nbodies <= 0) return fail("fsil_fem_define_bodies needs at least one body");
   Fem &F = b->fem;
   F = Fem();
@@ -1284,6 +1328,8 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   int max_nd = 1;
   for (const FemBodyDesc &d : F.desc) max_nd = std::max(max_nd, d.nND);
   const int lds_nodes = std::min(max_nd, 512 / 6);  // 6 threads per node; 6x12 FP64 work areas (<= 48 KiB)
+  const int vec_lds = fem_lds_bytes(lds_nodes, 1, max_nd) <= 64 * 1024 ? 1 : 0;
+  const size_t lds_bytes = fem_lds_bytes(lds_nodes, vec_lds, max_nd);
   const size_t n = (size_t)numsubstep * F.nbodies;
   if (n > S.cap) {
     if (S.h_stage) HIPCHK(hipHostFree(S.h_stage));
@@ -1315,9 +1361,8 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
   HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
   for (int isub = 0; isub < numsubstep; ++isub) {
-    hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512),
-                       (size_t)lds_nodes * 72 * sizeof(double) + (size_t)lds_nodes * sizeof(int), S.s,
-                       F.ddesc.p + (size_t)isub * F.nbodies, A, lds_nodes);
+    hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s,
+                       F.ddesc.p + (size_t)isub * F.nbodies, A, lds_nodes, vec_lds);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(S.h_stage, F.ddesc.p, n * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, S.s));
@@ -1702,6 +1747,264 @@ int fsil_time_tree_steps(int root, int nsteps, double t0, double dt, double *tot
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
   *total_ms = ms;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned warm start: the network (fsil_mlp_kernels.h)
+namespace {
+
+struct Mlp {
+  DevBuf<float> th, g, m, v;                         // parameters, gradient, Adam moments
+  DevBuf<float> X, Y, bx, by, H1, H2, H3, Yp, dY, dZ3, dZ2, dZ1;
+  DevBuf<double> part;
+  DevBuf<int> idx;
+  int adam_t = 0;
+  hipStream_t s = nullptr;
+  ~Mlp() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+std::map<int, std::unique_ptr<Mlp>> g_mlps;
+int g_next_mlp = 1;
+
+Mlp *get_mlp(int h) {
+  auto it = g_mlps.find(h);
+  if (it == g_mlps.end()) {
+    fail("invalid network handle %d", h);
+    return nullptr;
+  }
+  return it->second.get();
+}
+
+uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + k * 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+double splitmix64_unit(uint64_t seed, uint64_t k) { return (double)(splitmix64_at(seed, k) >> 11) * (1.0 / 9007199254740992.0); }
+
+int mlp_workspace(Mlp &M, int n) {
+  HIPCHK(M.H1.ensure((size_t)n * kMlpH));
+  HIPCHK(M.H2.ensure((size_t)n * kMlpH));
+  HIPCHK(M.H3.ensure((size_t)n * kMlpH));
+  HIPCHK(M.dZ3.ensure((size_t)n * kMlpH));
+  HIPCHK(M.dZ2.ensure((size_t)n * kMlpH));
+  HIPCHK(M.dZ1.ensure((size_t)n * kMlpH));
+  HIPCHK(M.Yp.ensure((size_t)n * kMlpOut));
+  HIPCHK(M.dY.ensure((size_t)n * kMlpOut));
+  HIPCHK(M.part.ensure(((size_t)n * kMlpOut + 255) / 256));
+  return 0;
+}
+
+inline dim3 g256(long long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+// FP32 forward of n samples into M.H1..H3, M.Yp
+int mlp_forward32(Mlp &M, const float *X, int n) {
+  const float *th = M.th.p;
+  hipLaunchKernelGGL(k_dense_fwd, g256((long long)n * kMlpH), dim3(256), 0, M.s, X, n, kMlpIn, th + kMlpOffW1,
+                     th + kMlpOffB1, kMlpH, 1, M.H1.p);
+  hipLaunchKernelGGL(k_dense_fwd, g256((long long)n * kMlpH), dim3(256), 0, M.s, M.H1.p, n, kMlpH, th + kMlpOffW2,
+                     th + kMlpOffB2, kMlpH, 1, M.H2.p);
+  hipLaunchKernelGGL(k_dense_fwd, g256((long long)n * kMlpH), dim3(256), 0, M.s, M.H2.p, n, kMlpH, th + kMlpOffW3,
+                     th + kMlpOffB3, kMlpH, 1, M.H3.p);
+  hipLaunchKernelGGL(k_dense_fwd, g256((long long)n * kMlpOut), dim3(256), 0, M.s, M.H3.p, n, kMlpH,
+                     th + kMlpOffW4, th + kMlpOffB4, kMlpOut, 0, M.Yp.p);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// forward + backward of n samples; gradient into M.g, squared-error block partials into M.part
+int mlp_fwd_bwd(Mlp &M, const float *X, const float *Y, int n) {
+  if (mlp_forward32(M, X, n)) return 1;
+  const float *th = M.th.p;
+  float *g = M.g.p;
+  const int cnt = n * kMlpOut;
+  hipLaunchKernelGGL(k_mse_grad, g256(cnt), dim3(256), 0, M.s, M.Yp.p, Y, cnt, M.dY.p, M.part.p);
+  hipLaunchKernelGGL(k_dense_bwd_w, g256(kMlpOut * kMlpH + kMlpOut), dim3(256), 0, M.s, M.dY.p, M.H3.p, n, kMlpOut,
+                     kMlpH, g + kMlpOffW4, g + kMlpOffB4);
+  hipLaunchKernelGGL(k_dense_bwd_in, g256((long long)n * kMlpH), dim3(256), 0, M.s, M.dY.p, n, kMlpOut,
+                     th + kMlpOffW4, kMlpH, M.H3.p, M.dZ3.p);
+  hipLaunchKernelGGL(k_dense_bwd_w, g256(kMlpH * kMlpH + kMlpH), dim3(256), 0, M.s, M.dZ3.p, M.H2.p, n, kMlpH,
+                     kMlpH, g + kMlpOffW3, g + kMlpOffB3);
+  hipLaunchKernelGGL(k_dense_bwd_in, g256((long long)n * kMlpH), dim3(256), 0, M.s, M.dZ3.p, n, kMlpH,
+                     th + kMlpOffW3, kMlpH, M.H2.p, M.dZ2.p);
+  hipLaunchKernelGGL(k_dense_bwd_w, g256(kMlpH * kMlpH + kMlpH), dim3(256), 0, M.s, M.dZ2.p, M.H1.p, n, kMlpH,
+                     kMlpH, g + kMlpOffW2, g + kMlpOffB2);
+  hipLaunchKernelGGL(k_dense_bwd_in, g256((long long)n * kMlpH), dim3(256), 0, M.s, M.dZ2.p, n, kMlpH,
+                     th + kMlpOffW2, kMlpH, M.H1.p, M.dZ1.p);
+  hipLaunchKernelGGL(k_dense_bwd_w, g256(kMlpH * kMlpIn + kMlpH), dim3(256), 0, M.s, M.dZ1.p, X, n, kMlpH, kMlpIn,
+                     g + kMlpOffW1, g + kMlpOffB1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int mlp_loss(Mlp &M, int n, double *loss) {
+  const int cnt = n * kMlpOut, nb = (cnt + 255) / 256;
+  std::vector<double> p(nb);
+  HIPCHK(hipMemcpyAsync(p.data(), M.part.p, nb * sizeof(double), hipMemcpyDeviceToHost, M.s));
+  HIPCHK(hipStreamSynchronize(M.s));
+  double t = 0.0;
+  for (double v : p) t = t + v;
+  *loss = t / (double)cnt;
+  return 0;
+}
+
+// rows idx[0..nb) of X/Y into the batch buffers
+__global__ void k_gather_rows(const float *__restrict__ src, const int *__restrict__ idx, int nb, int width,
+                              float *__restrict__ dst) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * width) return;
+  const int r = t / width, c = t - r * width;
+  dst[t] = src[(size_t)idx[r] * width + c];
+}
+
+}  // namespace
+
+extern "C" {
+
+int fsil_mlp_create(int seed, int *handle) {
+  if (g_device < 0) return fail("fsil_init has not been called");
+  if (!handle) return fail("null handle");
+  auto mp = std::make_unique<Mlp>();
+  Mlp &M = *mp;
+  HIPCHK(hipStreamCreateWithFlags(&M.s, hipStreamNonBlocking));
+  std::vector<float> th(kMlpParams, 0.0f);
+  const int sizes[4][2] = {{kMlpH, kMlpIn}, {kMlpH, kMlpH}, {kMlpH, kMlpH}, {kMlpOut, kMlpH}};
+  int k = 0;
+  for (auto &sz : sizes) {
+    const double lim = std::sqrt(6.0 / (double)(sz[0] + sz[1]));
+    for (int i = 0; i < sz[0] * sz[1]; ++i, ++k) th[k] = (float)(lim * (2.0 * splitmix64_unit((uint64_t)seed, k + 1) - 1.0));
+    k += sz[0];  // zero biases
+  }
+  for (DevBuf<float> *b : {&M.th, &M.g, &M.m, &M.v}) HIPCHK(b->ensure(kMlpParams));
+  HIPCHK(hipMemcpy(M.th.p, th.data(), kMlpParams * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(M.m.p, 0, kMlpParams * sizeof(float)));
+  HIPCHK(hipMemset(M.v.p, 0, kMlpParams * sizeof(float)));
+  const int h = g_next_mlp++;
+  g_mlps[h] = std::move(mp);
+  *handle = h;
+  return 0;
+}
+
+int fsil_mlp_destroy(int handle) {
+  if (!get_mlp(handle)) return 1;
+  for (auto &kv : g_blocks)
+    if (kv.second->ibm.warm_mlp == handle) kv.second->ibm.warm_mlp = 0;
+  g_mlps.erase(handle);
+  return 0;
+}
+
+int fsil_mlp_get_params(int handle, float *theta) {
+  Mlp *M = get_mlp(handle);
+  if (!M) return 1;
+  if (!theta) return fail("null theta");
+  HIPCHK(hipMemcpyAsync(theta, M->th.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, M->s));
+  HIPCHK(hipStreamSynchronize(M->s));
+  return 0;
+}
+
+int fsil_mlp_set_params(int handle, const float *theta) {
+  Mlp *M = get_mlp(handle);
+  if (!M) return 1;
+  if (!theta) return fail("null theta");
+  HIPCHK(hipMemcpyAsync(M->th.p, theta, kMlpParams * sizeof(float), hipMemcpyHostToDevice, M->s));
+  HIPCHK(hipMemsetAsync(M->m.p, 0, kMlpParams * sizeof(float), M->s));
+  HIPCHK(hipMemsetAsync(M->v.p, 0, kMlpParams * sizeof(float), M->s));
+  HIPCHK(hipStreamSynchronize(M->s));
+  M->adam_t = 0;
+  return 0;
+}
+
+int fsil_mlp_forward(int handle, const float *X, int n, float *Y, int bf16) {
+  Mlp *M = get_mlp(handle);
+  if (!M) return 1;
+  if (n <= 0 || !X || !Y) return fail("bad arguments to fsil_mlp_forward");
+  HIPCHK(M->X.ensure((size_t)n * kMlpIn));
+  HIPCHK(hipMemcpyAsync(M->X.p, X, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, M->s));
+  if (mlp_workspace(*M, n)) return 1;
+  if (bf16) {
+    hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((n + 63) / 64), dim3(256), 0, M->s, M->X.p, n, M->th.p, M->Yp.p);
+    HIPCHK(hipGetLastError());
+  } else {
+    if (mlp_forward32(*M, M->X.p, n)) return 1;
+  }
+  HIPCHK(hipMemcpyAsync(Y, M->Yp.p, (size_t)n * kMlpOut * sizeof(float), hipMemcpyDeviceToHost, M->s));
+  HIPCHK(hipStreamSynchronize(M->s));
+  return 0;
+}
+
+int fsil_mlp_loss_grad(int handle, const float *X, const float *Y, int n, double *loss, float *grad) {
+  Mlp *M = get_mlp(handle);
+  if (!M) return 1;
+  if (n <= 0 || !X || !Y) return fail("bad arguments to fsil_mlp_loss_grad");
+  HIPCHK(M->X.ensure((size_t)n * kMlpIn));
+  HIPCHK(M->Y.ensure((size_t)n * kMlpOut));
+  HIPCHK(hipMemcpyAsync(M->X.p, X, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, M->s));
+  HIPCHK(hipMemcpyAsync(M->Y.p, Y, (size_t)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice, M->s));
+  if (mlp_workspace(*M, n)) return 1;
+  if (mlp_fwd_bwd(*M, M->X.p, M->Y.p, n)) return 1;
+  if (grad) HIPCHK(hipMemcpyAsync(grad, M->g.p, kMlpParams * sizeof(float), hipMemcpyDeviceToHost, M->s));
+  double l = 0.0;
+  if (mlp_loss(*M, n, &l)) return 1;
+  if (loss) *loss = l;
+  return 0;
+}
+
+int fsil_mlp_train(int handle, const float *X, const float *Y, int n, int epochs, int batch, double lr, int seed,
+                   double *loss_before, double *loss_after) {
+  Mlp *M = get_mlp(handle);
+  if (!M) return 1;
+  if (n <= 0 || !X || !Y || epochs < 0 || batch <= 0) return fail("bad arguments to fsil_mlp_train");
+  batch = std::min(batch, n);
+  HIPCHK(M->X.ensure((size_t)n * kMlpIn));
+  HIPCHK(M->Y.ensure((size_t)n * kMlpOut));
+  HIPCHK(M->bx.ensure((size_t)batch * kMlpIn));
+  HIPCHK(M->by.ensure((size_t)batch * kMlpOut));
+  HIPCHK(M->idx.ensure((size_t)n));
+  HIPCHK(hipMemcpyAsync(M->X.p, X, (size_t)n * kMlpIn * sizeof(float), hipMemcpyHostToDevice, M->s));
+  HIPCHK(hipMemcpyAsync(M->Y.p, Y, (size_t)n * kMlpOut * sizeof(float), hipMemcpyHostToDevice, M->s));
+  if (mlp_workspace(*M, std::max(n, batch))) return 1;
+  if (loss_before) {
+    if (mlp_fwd_bwd(*M, M->X.p, M->Y.p, n)) return 1;
+    if (mlp_loss(*M, n, loss_before)) return 1;
+  }
+  std::vector<int> perm(n);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int i = n - 1; i > 0; --i) {  // Fisher-Yates from the splitmix64 stream
+      const int j = (int)(splitmix64_at((uint64_t)seed + 7919ULL * (uint64_t)ep, (uint64_t)i) % (uint64_t)(i + 1));
+      std::swap(perm[i], perm[j]);
+    }
+    HIPCHK(hipMemcpyAsync(M->idx.p, perm.data(), n * sizeof(int), hipMemcpyHostToDevice, M->s));
+    HIPCHK(hipStreamSynchronize(M->s));  // perm is reused by the next epoch
+    for (int b0 = 0; b0 + batch <= n; b0 += batch) {
+      hipLaunchKernelGGL(k_gather_rows, g256((long long)batch * kMlpIn), dim3(256), 0, M->s, M->X.p, M->idx.p + b0,
+                         batch, kMlpIn, M->bx.p);
+      hipLaunchKernelGGL(k_gather_rows, g256((long long)batch * kMlpOut), dim3(256), 0, M->s, M->Y.p,
+                         M->idx.p + b0, batch, kMlpOut, M->by.p);
+      if (mlp_fwd_bwd(*M, M->bx.p, M->by.p, batch)) return 1;
+      M->adam_t += 1;
+      const float bc1 = 1.0f - std::pow(b1, (float)M->adam_t), bc2 = 1.0f - std::pow(b2, (float)M->adam_t);
+      hipLaunchKernelGGL(k_adam, g256(kMlpParams), dim3(256), 0, M->s, M->th.p, M->g.p, M->m.p, M->v.p, kMlpParams,
+                         (float)lr, b1, b2, eps, bc1, bc2);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if (loss_after) {
+    if (mlp_fwd_bwd(*M, M->X.p, M->Y.p, n)) return 1;
+    if (mlp_loss(*M, n, loss_after)) return 1;
+  }
+  HIPCHK(hipStreamSynchronize(M->s));
   return 0;
 }
 

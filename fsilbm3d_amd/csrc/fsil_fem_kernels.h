@@ -43,6 +43,12 @@ struct FemArrays {
 
 __device__ inline void fem_sync() { __syncthreads(); }
 
+// The conjugate-gradient vectors of one body: in LDS when they fit (the usual
+// case), else in the global work arrays; indexed by the body's DOF number.
+struct FemVecs {
+  double *x, *r, *p, *z, *Ap, *blk;  // blk: the 6x6 inverse blocks, 36 per node
+};
+
 // element-by-element product out = A*v for one body (Beam_MatrixMultipy,
 // SolidSolver.f90:2035-2049)
 __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const double *v, double *out) {
@@ -50,10 +56,12 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
   for (int t = threadIdx.x; t < nE * NED; t += blockDim.x) {
     const int e = t / NED, i = t - e * NED;
     const BeamElem &E = A.el[D.eoff + e];
+    const int n0 = 6 * E.node0, n1 = 6 * E.node1;
     double s = 0.0;
+#pragma unroll
     for (int k = 0; k < NED; ++k) {
-      const int gk = k < 6 ? 6 * E.node0 + k : 6 * E.node1 + (k - 6);
-      s = s + E.coef[i * NED + k] * v[g0 + gk];
+      const int gk = k < 6 ? n0 + k : n1 + (k - 6);
+      s = s + E.coef[i * NED + k] * v[gk];
     }
     A.lbuf[(size_t)(D.eoff + e) * NED + i] = s;
   }
@@ -62,61 +70,92 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
     double s = 0.0;
     for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
       s = s + A.lbuf[(size_t)A.dof_e[k] * NED + A.dof_i[k]];
-    out[g0 + g] = s;
+    out[g] = s;
   }
   fem_sync();
 }
 
-__device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D) {
+__device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, const FemVecs &V) {
   const int g0 = D.goff;
   for (int g = threadIdx.x; g < 6 * D.nND; g += blockDim.x) {
     const int node = g / 6, i = g - node * 6;
-    const double *B = A.blk + (size_t)(D.noff + node) * 36;
+    const double *B = V.blk + (size_t)node * 36;
     double s = 0.0;
-    for (int k = 0; k < 6; ++k) s = s + B[i * 6 + k] * A.r[g0 + 6 * node + k];
-    A.z[g0 + g] = A.fixed[g0 + g] ? 0.0 : s;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s = s + B[i * 6 + k] * V.r[6 * node + k];
+    V.z[g] = A.fixed[g0 + g] ? 0.0 : s;
   }
   fem_sync();
 }
 
-// sequential dot product in DOF order, broadcast through LDS.  The operands
-// are staged into LDS by the whole workgroup, chunk by chunk, so the one
-// thread that sums them in order reads LDS rather than waiting on global
-// memory for every term.
-__device__ inline double fem_dot(const double *a, const double *b, int g0, int n, double *shared, double *stage,
-                                 int cap) {
-  double s = 0.0;
+// Dot products in DOF order, broadcast through LDS.  The products a_i*b_i
+// (each one rounding, as in the sequential loop) are formed by the whole
+// workgroup into LDS; one thread then adds them in order, so the serial part
+// is only the chain of additions.  fem_dot2 runs two such chains side by side
+// on two waves.
+__device__ inline void fem_dot_chunks(const double *a, const double *b, const double *c, const double *d, int n,
+                                      double *shared, double *stage, int cap, double &r1, double &r2) {
+  double s1 = 0.0, s2 = 0.0;
   for (int base = 0; base < n; base += cap) {
     const int m = min(cap, n - base);
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
-      stage[i] = a[g0 + base + i];
-      stage[cap + i] = b[g0 + base + i];
+      stage[i] = a[base + i] * b[base + i];
+      if (c) stage[cap + i] = c[base + i] * d[base + i];
     }
     fem_sync();
     if (threadIdx.x == 0) {
-      // unrolled so the LDS reads of a run of terms are in flight together;
-      // the additions stay in DOF order
 #pragma unroll 16
-      for (int i = 0; i < m; ++i) s = s + stage[i] * stage[cap + i];
+      for (int i = 0; i < m; ++i) s1 = s1 + stage[i];
+    } else if (c && threadIdx.x == 64) {
+#pragma unroll 16
+      for (int i = 0; i < m; ++i) s2 = s2 + stage[cap + i];
     }
     fem_sync();
   }
-  if (threadIdx.x == 0) *shared = s;
+  if (threadIdx.x == 0) shared[0] = s1;
+  if (threadIdx.x == 64) shared[1] = s2;
   fem_sync();
-  const double v = *shared;
+  r1 = shared[0];
+  r2 = shared[1];
   fem_sync();
-  return v;
 }
 
-// Dynamic LDS: lds_nodes work areas of 6x12 doubles for the block inverses
-// (plus one int flag per node), reused as the staging area (2 x 36*lds_nodes
-// doubles) of the dot products.
-__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArrays A, int lds_nodes) {
+__device__ inline double fem_dot(const double *a, const double *b, int n, double *shared, double *stage, int cap) {
+  double r1, r2;
+  fem_dot_chunks(a, b, nullptr, nullptr, n, shared, stage, cap, r1, r2);
+  return r1;
+}
+
+__device__ inline void fem_dot2(const double *a, const double *b, const double *c, const double *d, int n,
+                                double *shared, double *stage, int cap, double &r1, double &r2) {
+  fem_dot_chunks(a, b, c, d, n, shared, stage, cap, r1, r2);
+}
+
+// LDS bytes of k_fem_substep: lds_nodes work areas of 6x12 doubles for the
+// block inverses (also the staging area of the dot products) and one int
+// flag per node; then, when vec_lds, the five CG vectors and the inverse
+// blocks of the largest body.
+__host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int max_nd) {
+  size_t d = 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
+  if (vec_lds) d += 5 * 6 * (size_t)max_nd + 36 * (size_t)max_nd;
+  return d * sizeof(double);
+}
+
+__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArrays A, int lds_nodes, int vec_lds) {
   __shared__ double sh_val;
+  __shared__ double sh_pair[2];
   __shared__ int sh_flag;
   extern __shared__ double lds[];
   FemBodyDesc &D = descs[blockIdx.x];
   const int nE = D.nEL, nN = D.nND, nG = 6 * nN, g0 = D.goff;
+  FemVecs V;
+  if (vec_lds) {
+    double *vb = lds + 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
+    V.x = vb; V.r = vb + nG; V.p = vb + 2 * nG; V.z = vb + 3 * nG; V.Ap = vb + 4 * nG; V.blk = vb + 5 * nG;
+  } else {
+    V.x = A.x + g0; V.r = A.r + g0; V.p = A.p + g0; V.z = A.z + g0; V.Ap = A.Ap + g0;
+    V.blk = A.blk + (size_t)D.noff * 36;
+  }
   if (D.iBodyModel == 1) {
     // rigid prescribed motion (SolidSolver.f90:1826-1858)
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
@@ -213,7 +252,7 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
     // CG_Solve (SolidSolver.f90:1949-2020), block-Jacobi preconditioner
     for (int g = threadIdx.x; g < nG; g += blockDim.x) {
       A.xF[g0 + g] = A.fixed[g0 + g] ? (iter == 1 ? A.vBC[g0 + g] : 0.0) : 0.0;
-      A.x[g0 + g] = A.xF[g0 + g];
+      V.x[g] = A.xF[g0 + g];
     }
     // block-Jacobi preconditioner (SolidSolver.f90:1966-1975, Invert6x6
     // :2234-2279): one 6x12 Gauss-Jordan work area per node in LDS, one thread
@@ -269,52 +308,58 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
         fem_sync();
       }
       if (act) {
-        double *out = A.blk + (size_t)(D.noff + n) * 36;
+        double *out = V.blk + (size_t)n * 36;
         for (int j = 0; j < 6; ++j) out[row * 6 + j] = *bad ? (row == j ? 1.0 : 0.0) : aug[row * 12 + 6 + j];
       }
       fem_sync();
     }
     fem_sync();
-    fem_matvec(A, D, A.x, A.Ap);
+    fem_matvec(A, D, V.x, V.Ap);
     for (int g = threadIdx.x; g < nG; g += blockDim.x)
-      A.r[g0 + g] = A.fixed[g0 + g] ? 0.0 : A.lodEffe[g0 + g] - A.Ap[g0 + g];
+      V.r[g] = A.fixed[g0 + g] ? 0.0 : A.lodEffe[g0 + g] - V.Ap[g];
     fem_sync();
-    fem_precond(A, D);
+    fem_precond(A, D, V);
     const double err = 1.0e-6;
-    double rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val, lds, 36 * lds_nodes));
+    const int cap = 36 * lds_nodes;
+    double rr, rz;
+    fem_dot2(V.r, V.r, V.r, V.z, nG, sh_pair, lds, cap, rr, rz);
+    double rn = sqrt(rr);
     if (!(rn <= err)) {
-      for (int g = threadIdx.x; g < nG; g += blockDim.x) A.p[g0 + g] = A.z[g0 + g];
+      for (int g = threadIdx.x; g < nG; g += blockDim.x) V.p[g] = V.z[g];
       fem_sync();
-      double rsold = fem_dot(A.r, A.z, g0, nG, &sh_val, lds, 36 * lds_nodes);
+      double rsold = rz;
       if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
       for (int it = 1; it <= 10000; ++it) {
         ++cg_total;
-        fem_matvec(A, D, A.p, A.Ap);
+        fem_matvec(A, D, V.p, V.Ap);
         for (int g = threadIdx.x; g < nG; g += blockDim.x)
-          if (A.fixed[g0 + g]) A.Ap[g0 + g] = 0.0;
+          if (A.fixed[g0 + g]) V.Ap[g] = 0.0;
         fem_sync();
-        const double pAp = fem_dot(A.p, A.Ap, g0, nG, &sh_val, lds, 36 * lds_nodes);
+        const double pAp = fem_dot(V.p, V.Ap, nG, sh_pair, lds, cap);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
-          double xv = A.x[g0 + g] + alpha * A.p[g0 + g];
+          double xv = V.x[g] + alpha * V.p[g];
           if (A.fixed[g0 + g]) xv = A.xF[g0 + g];
-          A.x[g0 + g] = xv;
-          double rv = A.r[g0 + g] - alpha * A.Ap[g0 + g];
+          V.x[g] = xv;
+          double rv = V.r[g] - alpha * V.Ap[g];
           if (A.fixed[g0 + g]) rv = 0.0;
-          A.r[g0 + g] = rv;
+          V.r[g] = rv;
         }
         fem_sync();
-        rn = sqrt(fem_dot(A.r, A.r, g0, nG, &sh_val, lds, 36 * lds_nodes));
+        // z = M^-1 r is formed before the convergence test (unused when the
+        // loop stops there) so that r.r and r.z are summed side by side
+        fem_precond(A, D, V);
+        double rsnew;
+        fem_dot2(V.r, V.r, V.r, V.z, nG, sh_pair, lds, cap, rr, rsnew);
+        rn = sqrt(rr);
         if (rn <= err) break;
-        fem_precond(A, D);
-        const double rsnew = fem_dot(A.r, A.z, g0, nG, &sh_val, lds, 36 * lds_nodes);
         if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
         const double beta = rsnew / rsold;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
-          double pv = A.z[g0 + g] + beta * A.p[g0 + g];
+          double pv = V.z[g] + beta * V.p[g];
           if (A.fixed[g0 + g]) pv = 0.0;
-          A.p[g0 + g] = pv;
+          V.p[g] = pv;
         }
         fem_sync();
         rsold = rsnew;
@@ -322,7 +367,7 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
       if (status) break;
     }
     // UpdateDspANDTride (SolidSolver.f90:2304-2338); the ramp factor is 1
-    for (int g = threadIdx.x; g < nG; g += blockDim.x) A.dsp[g0 + g] = A.dsp[g0 + g] + A.x[g0 + g];
+    for (int g = threadIdx.x; g < nG; g += blockDim.x) A.dsp[g0 + g] = A.dsp[g0 + g] + V.x[g];
     if (threadIdx.x == 0) sh_val = 0.0;
     fem_sync();
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
@@ -332,14 +377,14 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
         E.x1[6 + k] = E.x0[6 + k] + A.dsp[g0 + 6 * E.node1 + k];
       }
       el_cptdxyz1(E);
-      el_update_triads(E, A.x + g0);
+      el_update_triads(E, V.x);
     }
     // dnorm = |maxval(dsp_increment**2)| (SolidSolver.f90:2336): the maximum
     // does not depend on the order, so a wave reduction gives the same value
     {
       double mx = -INFINITY;
       for (int g = threadIdx.x; g < nG; g += blockDim.x) {
-        const double v = A.x[g0 + g] * A.x[g0 + g];
+        const double v = V.x[g] * V.x[g];
         if (v > mx) mx = v;
       }
       for (int off = 32; off > 0; off >>= 1) {

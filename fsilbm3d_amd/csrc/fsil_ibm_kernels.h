@@ -212,9 +212,11 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// One marker per lane: 64-point interpolation of the band velocity, velocity
-// defect, penalty force, force accumulation (PenaltyForce_, Solidbody.f90:1000-1026).
-// Per-block partial sums of |dV|_1 go to tolpart[blockIdx.x].
+// One marker per wave: each lane forms one of the 64 interpolation terms
+// ((u*rx)*ry)*rz of the marker's stencil (Solidbody.f90:1009-1015), lane 0
+// adds them in the reference's x, y, z order, then the velocity defect,
+// penalty force and force accumulation (PenaltyForce_, Solidbody.f90:1016-1026).
+// The marker's |dV|_1 goes to tl[m] for the residual.
 __global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ bidx,
                                                 const float *__restrict__ Ew,
                                                 const double *__restrict__ band_u, int cap,
@@ -223,48 +225,38 @@ __global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ b
                                                 const double *__restrict__ Ea, double invh3,
                                                 double *__restrict__ Eforce,
                                                 double *__restrict__ fT,
-                                                double *__restrict__ tolpart) {
+                                                double *__restrict__ tl) {
   if (ctl[kDone]) return;  // uniform over the grid
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  double tl = 0.0;
+  __shared__ double t[4][3][64];
+  const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + w;
   if (m < M) {
-    double v1 = 0.0, v2 = 0.0, v3 = 0.0;
-    const int *I = bidx + 64LL * m;
+    const int b = bidx[64LL * m + s];
     const float *W = Ew + 12LL * m;
-    for (int x = 0; x < 4; ++x) {
-      const double rx = (double)W[x];
-      for (int y = 0; y < 4; ++y) {
-        const double ry = (double)W[4 + y];
-#pragma unroll
-        for (int z = 0; z < 4; ++z) {
-          const double rz = (double)W[8 + z];
-          const int b = I[16 * x + 4 * y + z];
-          v1 = v1 + ((band_u[b] * rx) * ry) * rz;
-          v2 = v2 + ((band_u[cap + b] * rx) * ry) * rz;
-          v3 = v3 + ((band_u[2LL * cap + b] * rx) * ry) * rz;
-        }
-      }
+    const double rx = (double)W[s >> 4], ry = (double)W[4 + ((s >> 2) & 3)], rz = (double)W[8 + (s & 3)];
+    t[w][0][s] = ((band_u[b] * rx) * ry) * rz;
+    t[w][1][s] = ((band_u[cap + b] * rx) * ry) * rz;
+    t[w][2][s] = ((band_u[2LL * cap + b] * rx) * ry) * rz;
+  }
+  __syncthreads();
+  if (m < M && s == 0) {
+    double v1 = 0.0, v2 = 0.0, v3 = 0.0;
+#pragma unroll 16
+    for (int k = 0; k < 64; ++k) {
+      v1 = v1 + t[w][0][k];
+      v2 = v2 + t[w][1][k];
+      v3 = v3 + t[w][2][k];
     }
     const double d1 = Evel[3LL * m] - v1, d2 = Evel[3LL * m + 1] - v2, d3 = Evel[3LL * m + 2] - v3;
     const double a = Ea[m];
     const double F1 = d1 * a, F2 = d2 * a, F3 = d3 * a;
-    tl = (fabs(d1) + fabs(d2)) + fabs(d3);
+    tl[m] = (fabs(d1) + fabs(d2)) + fabs(d3);
     Eforce[3LL * m] = Eforce[3LL * m] + F1;
     Eforce[3LL * m + 1] = Eforce[3LL * m + 1] + F2;
     Eforce[3LL * m + 2] = Eforce[3LL * m + 2] + F3;
     fT[3LL * m] = F1 * invh3;
     fT[3LL * m + 1] = F2 * invh3;
     fT[3LL * m + 2] = F3 * invh3;
-  }
-  __shared__ double part[4];
-  tl = wave_sum(tl);
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (lane == 0) part[w] = tl;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) s = s + part[k];
-    tolpart[blockIdx.x] = s;
   }
 }
 
@@ -296,14 +288,22 @@ __global__ __launch_bounds__(256) void k_correct(const int *__restrict__ ctl, in
 }
 
 // stop test of the multi-direct-forcing loop (Solidbody.f90:895-906): the
-// block partials summed in block order, dmax = tol / (sum M_b * Uref), one
-// iteration counted; sets the done word when the loop condition fails.
-__global__ void k_mdf_check(int *__restrict__ ctl, double *__restrict__ dstate,
-                            const double *__restrict__ tolpart, int nblk, double dsum, double Uref,
-                            int ntol, double dtol) {
-  if (threadIdx.x != 0 || blockIdx.x != 0 || ctl[kDone]) return;
+// markers' |dV|_1 summed in a fixed order (one workgroup: strided partial sums,
+// then a wave and a block tree), dmax = tol / (sum M_b * Uref), one iteration
+// counted; sets the done word when the loop condition fails.
+__global__ __launch_bounds__(256) void k_mdf_check(int *__restrict__ ctl, double *__restrict__ dstate,
+                                                   const double *__restrict__ tl, int M, double dsum, double Uref,
+                                                   int ntol, double dtol) {
+  if (ctl[kDone]) return;
+  double p = 0.0;
+  for (int m = threadIdx.x; m < M; m += blockDim.x) p = p + tl[m];
+  __shared__ double part[4];
+  p = wave_sum(p);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = p;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   double tol = 0.0;
-  for (int k = 0; k < nblk; ++k) tol = tol + tolpart[k];
+  for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) tol = tol + part[k];
   const int it = ctl[kIters] + 1;
   ctl[kIters] = it;
   if (!isfinite(tol)) {
@@ -350,38 +350,43 @@ __global__ __launch_bounds__(256) void k_spread(const int *__restrict__ ctl, int
 // marker order of 0.5*F (translational) or 0.5*(r x F) (rotational) with r the
 // arm from the element axis midpoint (Solidbody.f90:944-967).
 // dof_list entry: marker id; dof_kind[d] = component (0..2 force, 3..5 moment).
+// k_load_terms forms every term (one thread per list entry), k_loads adds
+// each DOF's terms in list order.
+__global__ __launch_bounds__(256) void k_load_terms(int nlist, const int *__restrict__ dof_of_entry,
+                                                    const int *__restrict__ dof_list,
+                                                    const int *__restrict__ dof_kind,
+                                                    const int *__restrict__ marker_elem,
+                                                    const double *__restrict__ elem_xc,
+                                                    const double *__restrict__ Exyz,
+                                                    const double *__restrict__ Eforce,
+                                                    double *__restrict__ term) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nlist) return;
+  const int k = dof_kind[dof_of_entry[j]];
+  const int m = dof_list[j];
+  double val;
+  if (k < 3) {
+    val = Eforce[3LL * m + k];
+  } else {
+    const int el = marker_elem[m];
+    const double r1 = Exyz[3LL * m] - elem_xc[3LL * el];
+    const double r2 = Exyz[3LL * m + 1] - elem_xc[3LL * el + 1];
+    const double r3 = Exyz[3LL * m + 2] - elem_xc[3LL * el + 2];
+    const double F1 = Eforce[3LL * m], F2 = Eforce[3LL * m + 1], F3 = Eforce[3LL * m + 2];
+    if (k == 3) val = r2 * F3 - r3 * F2;
+    else if (k == 4) val = r3 * F1 - r1 * F3;
+    else val = r1 * F2 - r2 * F1;
+  }
+  term[j] = 0.5 * val;
+}
+
 __global__ __launch_bounds__(256) void k_loads(int ndof, const int *__restrict__ dof_ptr,
-                                               const int *__restrict__ dof_list,
-                                               const int *__restrict__ dof_kind,
-                                               const int *__restrict__ marker_elem,
-                                               const double *__restrict__ elem_xc,
-                                               const double *__restrict__ Exyz,
-                                               const double *__restrict__ Eforce,
-                                               double *__restrict__ lod) {
+                                               const double *__restrict__ term, double *__restrict__ lod) {
   const int d = blockIdx.x * blockDim.x + threadIdx.x;
   if (d >= ndof) return;
-  const int k = dof_kind[d];
   double acc = 0.0;
-  // unrolled: the marker ids and values of a run of terms are loaded
-  // together; the sum stays in marker order
 #pragma unroll 8
-  for (int j = dof_ptr[d]; j < dof_ptr[d + 1]; ++j) {
-    const int m = dof_list[j];
-    double val;
-    if (k < 3) {
-      val = Eforce[3LL * m + k];
-    } else {
-      const int el = marker_elem[m];
-      const double r1 = Exyz[3LL * m] - elem_xc[3LL * el];
-      const double r2 = Exyz[3LL * m + 1] - elem_xc[3LL * el + 1];
-      const double r3 = Exyz[3LL * m + 2] - elem_xc[3LL * el + 2];
-      const double F1 = Eforce[3LL * m], F2 = Eforce[3LL * m + 1], F3 = Eforce[3LL * m + 2];
-      if (k == 3) val = r2 * F3 - r3 * F2;
-      else if (k == 4) val = r3 * F1 - r1 * F3;
-      else val = r1 * F2 - r2 * F1;
-    }
-    acc = acc + 0.5 * val;
-  }
+  for (int j = dof_ptr[d]; j < dof_ptr[d + 1]; ++j) acc = acc + term[j];
   lod[d] = acc;
 }
 

@@ -8,7 +8,8 @@
 //   feq_q - w_q = w_q * (drho + rho*(3 u.e + 4.5 (u.e)^2 - 1.5 u.u))
 //   g'_q = g_q + Omega*((feq_q - w_q) - g_q) + (1 - Omega/2) * Flb_q
 // which is the reference's SRT + Guo update (FluidDomain.f90:1217-1227)
-// rearranged around w_q.
+// rearranged around w_q.  Cells of an immersed-boundary band take the
+// coupling's corrected velocity and force (BAND), as in the FP64 form.
 #pragma once
 #include "fsil_common.h"
 
@@ -22,6 +23,12 @@ struct Collide32Args {
   float Omega, c1, dt3;
   float hv[3];
   float VF[3];
+  // immersed-boundary band overlay (corrected velocity and force, FP64)
+  const int *__restrict__ ctl;        // band bounding box in ctl[0..5]
+  const int *__restrict__ cellmap;    // cell -> band index, -1 outside
+  const double *__restrict__ band_u;  // [3][cap]
+  const double *__restrict__ band_F;  // [3][cap]
+  int cap;
 };
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -36,7 +43,7 @@ __device__ __forceinline__ float dotu32(float u1, float u2, float u3) {
   return (float)ex(q) * u1 + (float)ey(q) * u2 + (float)ez(q) * u3;
 }
 
-template <bool FORCE>
+template <bool FORCE, bool BAND>
 __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -60,7 +67,23 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   }
   const float rho = 1.0f + dr;
   const float inv = 1.0f / rho;
-  const float u1 = (j1 + a.hv[0]) * inv, u2 = (j2 + a.hv[1]) * inv, u3 = (j3 + a.hv[2]) * inv;
+  float u1 = (j1 + a.hv[0]) * inv, u2 = (j2 + a.hv[1]) * inv, u3 = (j3 + a.hv[2]) * inv;
+  bool forced = FORCE;
+  float F1 = a.VF[0], F2 = a.VF[1], F3 = a.VF[2];
+  if constexpr (BAND) {
+    if (x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] && z <= a.ctl[5]) {
+      const int b = a.cellmap[n];
+      if (b >= 0) {
+        u1 = (float)a.band_u[b];
+        u2 = (float)a.band_u[a.cap + b];
+        u3 = (float)a.band_u[2 * a.cap + b];
+        F1 = (float)a.band_F[b];
+        F2 = (float)a.band_F[a.cap + b];
+        F3 = (float)a.band_F[2 * a.cap + b];
+        forced = true;
+      }
+    }
+  }
   const float uSqr = u1 * u1 + u2 * u2 + u3 * u3;
   float post[Q];
 #pragma unroll
@@ -74,12 +97,14 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
     const float w = (float)wt(q);
     const float feqd = w * (dr + rho * (ux * (3.0f + 4.5f * ux) - 1.5f * uSqr));
     float t = g[q] + a.Omega * (feqd - g[q]);
-    if constexpr (FORCE) {
-      const float e1 = (float)ex(q), e2 = (float)ey(q), e3 = (float)ez(q);
-      const float fl = a.dt3 * w *
-                       ((e1 - u1 + 3.0f * ux * e1) * a.VF[0] + (e2 - u2 + 3.0f * ux * e2) * a.VF[1] +
-                        (e3 - u3 + 3.0f * ux * e3) * a.VF[2]);
-      t = t + a.c1 * fl;
+    if (FORCE || BAND) {
+      if (forced) {
+        const float e1 = (float)ex(q), e2 = (float)ey(q), e3 = (float)ez(q);
+        const float fl = a.dt3 * w *
+                         ((e1 - u1 + 3.0f * ux * e1) * F1 + (e2 - u2 + 3.0f * ux * e2) * F2 +
+                          (e3 - u3 + 3.0f * ux * e3) * F3);
+        t = t + a.c1 * fl;
+      }
     }
     post[q] = t;
   }
@@ -99,6 +124,31 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
     const unsigned d = n + sx + sy + sz;
     __builtin_nontemporal_store(post[q], reinterpret_cast<float *>(gout + q * plane + (d << 2)));
   }
+}
+
+// band macros of the FP32-deviation state (the immersed-boundary coupling
+// runs in FP64): rho = 1 + sum g, j = sum e g, summed in FP64 from the stored
+// deviations
+__global__ __launch_bounds__(256) void k_band_macro32(const float *__restrict__ g, long long N,
+                                                      const long long *__restrict__ cell,
+                                                      const int *__restrict__ ctl, int cap, double hv0,
+                                                      double hv1, double hv2, double *__restrict__ band_u) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ctl[6]) return;
+  const long long n = cell[b];
+  double dr = 0.0, j1 = 0.0, j2 = 0.0, j3 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double v = (double)g[q * N + n];
+    dr = dr + v;
+    if (ex(q) != 0) j1 = ex(q) > 0 ? j1 + v : j1 - v;
+    if (ey(q) != 0) j2 = ey(q) > 0 ? j2 + v : j2 - v;
+    if (ez(q) != 0) j3 = ez(q) > 0 ? j3 + v : j3 - v;
+  }
+  const double rho = 1.0 + dr;
+  band_u[b] = (j1 + hv0) / rho;
+  band_u[cap + b] = (j2 + hv1) / rho;
+  band_u[2LL * cap + b] = (j3 + hv2) / rho;
 }
 
 // den/u of the FP32-deviation state, in FP64 output arrays

@@ -289,7 +289,8 @@ __global__ __launch_bounds__(256) void k_macro(const double *__restrict__ f, lon
 // Boundary faces (FluidDomain.f90:616-1126), one launch per face in the order
 // x-, x+, y-, y+, z-, z+ so that edges and corners see earlier faces' values.
 struct FaceArgs {
-  double *__restrict__ f;
+  double *__restrict__ f;   // FP64 plain state, or
+  float *__restrict__ g;    // FP32-deviation state (g = f - w)
   int xd, yd, zd;
   long long N;
   int face, code;
@@ -364,8 +365,24 @@ __device__ __forceinline__ double mw_q(int q, double rho, double v1, double v2, 
   return 0.0;
 }
 
-template <int FACE>
+// Population access of the face kernel in either storage form; the
+// deviation form converts through f = g + w, so the copies of the wall,
+// symmetry and extrapolation codes return the stored deviations unchanged.
+template <bool F32>
+struct FaceIO {
+  __device__ static __forceinline__ double ld(const FaceArgs &a, int q, long long i) {
+    if constexpr (F32) return (double)a.g[i] + wt(q);
+    else return a.f[i];
+  }
+  __device__ static __forceinline__ void st(const FaceArgs &a, int q, long long i, double v) {
+    if constexpr (F32) a.g[i] = (float)(v - wt(q));
+    else a.f[i] = v;
+  }
+};
+
+template <int FACE, bool F32>
 __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
+  using IO = FaceIO<F32>;
   constexpr int axis = FACE >> 1, hi = FACE & 1;
   const int nd0 = axis == 0 ? a.yd : a.xd;             // outer in-plane extent
   const int nd1 = axis == 2 ? a.yd : a.zd;             // inner in-plane extent
@@ -395,7 +412,7 @@ __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
       uSqr = uSqr + v[1] * v[1];
       uSqr = uSqr + v[2] * v[2];
 #pragma unroll
-      for (int q = 0; q < Q; ++q) a.f[q * N + n] = feq_q(q, a.denIn, v[0], v[1], v[2], uSqr);
+      for (int q = 0; q < Q; ++q) IO::st(a, q, q * N + n, feq_q(q, a.denIn, v[0], v[1], v[2], uSqr));
       break;
     }
     case BC_NEQ_DIRICHLET: {
@@ -415,25 +432,25 @@ __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
         const int q = S[k];
         const double fe = feq_q(q, a.denIn, v[0], v[1], v[2], uSqr);
         const double fi = feq_q(q, r2, w1, w2, w3, wSqr);
-        a.f[q * N + n] = fe + (a.f[q * N + n2] - fi);
+        IO::st(a, q, q * N + n, fe + (IO::ld(a, q, q * N + n2) - fi));
       }
       break;
     }
     case BC_EXTRAP1:
 #pragma unroll
-      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = a.f[S[k] * N + n2];
+      for (int k = 0; k < 5; ++k) IO::st(a, S[k], S[k] * N + n, IO::ld(a, S[k], S[k] * N + n2));
       break;
     case BC_EXTRAP2:
 #pragma unroll
       for (int k = 0; k < 5; ++k)
-        a.f[S[k] * N + n] = 2.0 * a.f[S[k] * N + n2] - a.f[S[k] * N + n3];
+        IO::st(a, S[k], S[k] * N + n, 2.0 * IO::ld(a, S[k], S[k] * N + n2) - IO::ld(a, S[k], S[k] * N + n3));
       break;
     case BC_WALL: {
       double tmp[5];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) tmp[k] = a.f[oppo(S[k]) * N + n];
+      for (int k = 0; k < 5; ++k) tmp[k] = IO::ld(a, oppo(S[k]), oppo(S[k]) * N + n);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      for (int k = 0; k < 5; ++k) IO::st(a, S[k], S[k] * N + n, tmp[k]);
       break;
     }
     case BC_WALL_HALFWAY: {
@@ -443,7 +460,7 @@ __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
 #pragma unroll
       for (int k = 0; k < 5; ++k) tmp[k] = a.hw[(long long)oppo(S[k]) * np + t];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      for (int k = 0; k < 5; ++k) IO::st(a, S[k], S[k] * N + n, tmp[k]);
       break;
     }
     case BC_MOVING_WALL: {
@@ -452,9 +469,9 @@ __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
       double tmp[5];
 #pragma unroll
       for (int k = 0; k < 5; ++k)
-        tmp[k] = a.f[oppo(S[k]) * N + n] + mw_q(S[k], a.denIn, v[0], v[1], v[2]);
+        tmp[k] = IO::ld(a, oppo(S[k]), oppo(S[k]) * N + n) + mw_q(S[k], a.denIn, v[0], v[1], v[2]);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      for (int k = 0; k < 5; ++k) IO::st(a, S[k], S[k] * N + n, tmp[k]);
       break;
     }
     case BC_MOVING_WALL_HALFWAY: {
@@ -468,16 +485,16 @@ __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
       for (int k = 0; k < 5; ++k)
         tmp[k] = a.hw[(long long)oppo(S[k]) * np + t] + mw_q(S[k], a.denIn, v[0], v[1], v[2]);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      for (int k = 0; k < 5; ++k) IO::st(a, S[k], S[k] * N + n, tmp[k]);
       break;
     }
     case BC_SYMMETRIC: {
       constexpr const int *M = FaceSets<FACE>::sym;
       double tmp[5];
 #pragma unroll
-      for (int k = 0; k < 5; ++k) tmp[k] = a.f[M[k] * N + n];
+      for (int k = 0; k < 5; ++k) tmp[k] = IO::ld(a, M[k], M[k] * N + n);
 #pragma unroll
-      for (int k = 0; k < 5; ++k) a.f[S[k] * N + n] = tmp[k];
+      for (int k = 0; k < 5; ++k) IO::st(a, S[k], S[k] * N + n, tmp[k]);
       break;
     }
     default:

@@ -18,7 +18,8 @@ module fsil_bridge
               fsil_fem_define_bodies, fsil_fem_set_solver, fsil_fem_update_markers, fsil_fem_substeps, &
               fsil_fem_state, fsil_fem_info, fsil_tree_add_son, fsil_tree_pair_info, &
               fsil_extract_interpolate_layer, fsil_interpolation_father_to_son, fsil_deliver_son_to_father, &
-              fsil_tree_steps, fsil_time_tree_steps
+              fsil_tree_steps, fsil_time_tree_steps, fsil_mlp_create, fsil_mlp_destroy, fsil_mlp_get_params, &
+              fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train
 
     interface
         integer(c_int) function fsil_version() bind(C, name='fsil_version')
@@ -212,6 +213,47 @@ module fsil_bridge
             integer(c_int), value :: root, nsteps
             real(c_double), value :: t0, dt
             real(c_double), intent(out) :: total_ms
+        end function
+        ! learned warm start of the direct-forcing loop (no reference counterpart)
+        integer(c_int) function fsil_mlp_create(seed, handle) bind(C, name='fsil_mlp_create')
+            import :: c_int
+            integer(c_int), value :: seed
+            integer(c_int), intent(out) :: handle
+        end function
+        integer(c_int) function fsil_mlp_destroy(handle) bind(C, name='fsil_mlp_destroy')
+            import :: c_int
+            integer(c_int), value :: handle
+        end function
+        integer(c_int) function fsil_mlp_get_params(handle, theta) bind(C, name='fsil_mlp_get_params')
+            import :: c_int, c_float
+            integer(c_int), value :: handle
+            real(c_float), intent(out) :: theta(*)
+        end function
+        integer(c_int) function fsil_mlp_set_params(handle, theta) bind(C, name='fsil_mlp_set_params')
+            import :: c_int, c_float
+            integer(c_int), value :: handle
+            real(c_float), intent(in) :: theta(*)
+        end function
+        integer(c_int) function fsil_mlp_forward(handle, X, n, Y, bf16) bind(C, name='fsil_mlp_forward')
+            import :: c_int, c_float
+            integer(c_int), value :: handle, n, bf16
+            real(c_float), intent(in) :: X(*)
+            real(c_float), intent(out) :: Y(*)
+        end function
+        integer(c_int) function fsil_mlp_loss_grad(handle, X, Y, n, loss, grad) bind(C, name='fsil_mlp_loss_grad')
+            import :: c_int, c_float, c_double
+            integer(c_int), value :: handle, n
+            real(c_float), intent(in) :: X(*), Y(*)
+            real(c_double), intent(out) :: loss
+            real(c_float), intent(out) :: grad(*)
+        end function
+        integer(c_int) function fsil_mlp_train(handle, X, Y, n, epochs, batch, lr, seed, loss_before, loss_after) &
+                                               bind(C, name='fsil_mlp_train')
+            import :: c_int, c_float, c_double
+            integer(c_int), value :: handle, n, epochs, batch, seed
+            real(c_double), value :: lr
+            real(c_float), intent(in) :: X(*), Y(*)
+            real(c_double), intent(out) :: loss_before, loss_after
         end function
     end interface
 

@@ -196,6 +196,30 @@ int fsil_tree_steps(int root, int nsteps, double t0, double dt);
 /* fsil_tree_steps bracketed by device events on the tree's stream (ms). */
 int fsil_time_tree_steps(int root, int nsteps, double t0, double dt, double *total_ms);
 
+/* ---------------------------------------------------------------------------
+ * Learned warm start of the multi-direct-forcing loop.  No counterpart in the
+ * reference: it extends calculate_interaction_force (Solidbody.f90:869-918)
+ * with a per-marker network 12 -> 64 -> 64 -> 64 -> 3 (tanh hidden layers)
+ * that guesses the converged marker force; the loop still runs to the same
+ * tolerance.  Parameters: 9347 floats, W1[64][12] b1 W2[64][64] b2 W3[64][64]
+ * b3 W4[3][64] b4. */
+
+/* A network with Glorot-uniform weights from the splitmix64 stream of seed. */
+int fsil_mlp_create(int seed, int *handle);
+int fsil_mlp_destroy(int handle);
+int fsil_mlp_get_params(int handle, float *theta);
+/* Also resets the Adam moments. */
+int fsil_mlp_set_params(int handle, const float *theta);
+/* Y[n][3] = net(X[n][12]); bf16 != 0: BF16 matrix-core inference (FP32
+ * accumulation), else the FP32 training forward. */
+int fsil_mlp_forward(int handle, const float *X, int n, float *Y, int bf16);
+/* Mean squared error over the n x 3 outputs and its gradient (FP32). */
+int fsil_mlp_loss_grad(int handle, const float *X, const float *Y, int n, double *loss, float *grad);
+/* Adam on mini-batches of `batch` rows, `epochs` passes in a splitmix64
+ * shuffled order; full-data loss before and after (either may be NULL). */
+int fsil_mlp_train(int handle, const float *X, const float *Y, int n, int epochs, int batch, double lr, int seed,
+                   double *loss_before, double *loss_after);
+
 #ifdef __cplusplus
 }
 #endif

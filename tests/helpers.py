@@ -243,3 +243,30 @@ def run_device_tree(pc, fs0=None, nsteps=None, per_call=False):
 def mismatch(a, b):
     d = np.abs(a - b)
     return int((a != b).sum()), float(d.max()) if d.size else 0.0
+
+
+def run_plate_fsi(w, precision, nsteps, record_every=1):
+    """A flexible-plate workload (fsilbm3d_amd.workloads.FlexiblePlate) on the
+    device in the given storage form; returns per-step total fluid force on
+    the body (sum of lodFlow x, y, z), MDF iterations, and the final node
+    positions."""
+    from fsilbm3d_amd.block import LBMBlock
+    from fsilbm3d_amd.fsi import DeviceBodyBlock
+    blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu, precision=precision)
+    blk.set_flow(**w.flow_kwargs())
+    blk.upload(w.initial_state())
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    fsi = DeviceBodyBlock(blk, [w.plate()], **w.solver_kwargs())
+    forces, iters = [], []
+    t = 0.0
+    for k in range(nsteps):
+        t = t + 1.0
+        fsi.step(t)
+        if (k + 1) % record_every == 0:
+            forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
+            iters.append(fsi.iters)
+    pos = fsi.state(0)[0].copy()
+    den, uuu = blk.calculate_macro_quantities()
+    blk.close()
+    return dict(forces=np.array(forces), iters=iters, pos=pos, u=uuu)
