@@ -107,6 +107,10 @@ struct Ibm {
   int ndof = 0;
   bool active = false;  // band valid for the next collision
   int warm_mlp = 0;     // network handle of the learned warm start (0: off)
+  int record = 0;       // compute the network's features and targets every call
+  DevBuf<float> wsX, wsY, wsP;               // features (12 per marker), targets, guesses (3 per marker)
+  DevBuf<double> Eprev, Enrm, tl0;           // previous force, normals, zero-guess residual terms
+  int last_warm = 0, last_fallback = 0;
 };
 
 // Device FEM state of the bodies carried by a block.
@@ -215,6 +219,34 @@ Block *get_block(int h) {
   auto it = g_blocks.find(h);
   if (it == g_blocks.end()) {
     fail("invalid block handle %d", h);
+    return nullptr;
+  }
+  return it->second.get();
+}
+
+// the warm-start network (fsil_mlp_kernels.h), one per handle
+struct Mlp {
+  DevBuf<float> th, g, m, v;                         // parameters, gradient, Adam moments
+  DevBuf<float> X, Y, bx, by, H1, H2, H3, Yp, dY, dZ3, dZ2, dZ1;
+  DevBuf<double> part;
+  DevBuf<int> idx;
+  int adam_t = 0;
+  hipStream_t s = nullptr;
+  ~Mlp() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  }
+};
+
+std::map<int, std::unique_ptr<Mlp>> g_mlps;
+int g_next_mlp = 1;
+
+Mlp *get_mlp(int h) {
+  auto it = g_mlps.find(h);
+  if (it == g_mlps.end()) {
+    fail("invalid network handle %d", h);
     return nullptr;
   }
   return it->second.get();
@@ -992,6 +1024,17 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   HIPCHK(I.tolpart.ensure(std::max(1, I.M)));  // per-marker |dV|_1
   HIPCHK(I.ctl.ensure(kCtlWords));
   HIPCHK(I.cellmap.ensure((size_t)b->N));
+  {
+    const size_t m1 = std::max(1, I.M);
+    HIPCHK(I.wsX.ensure(12 * m1));
+    HIPCHK(I.wsY.ensure(3 * m1));
+    HIPCHK(I.wsP.ensure(3 * m1));
+    HIPCHK(I.Eprev.ensure(3 * m1));
+    HIPCHK(I.Enrm.ensure(3 * m1));
+    HIPCHK(I.tl0.ensure(m1));
+    HIPCHK(hipMemsetAsync(I.Eprev.p, 0, 3 * m1 * sizeof(double), b->stream));
+    HIPCHK(hipMemsetAsync(I.Enrm.p, 0, 3 * m1 * sizeof(double), b->stream));
+  }
   HIPCHK(hipMemsetAsync(I.ctl.p, 0, kCtlWords * sizeof(int), b->stream));
   HIPCHK(hipMemsetAsync(I.cellmap.p, 0xff, (size_t)b->N * sizeof(int), b->stream));  // -1
   if (E > 0) {
@@ -1045,8 +1088,17 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   // UpdateElmtInterp_ (Solidbody.f90:760-867) and the band, on the device
   const int done0 = (I.ntol > 0 && 1e10 > I.dtol) ? 0 : 1;
+  Mlp *net = nullptr;
+  if (I.warm_mlp) {
+    auto it = g_mlps.find(I.warm_mlp);
+    if (it == g_mlps.end()) return fail("warm-start network %d no longer exists", I.warm_mlp);
+    net = it->second.get();
+    // the network's parameters are written on its own stream (training)
+    HIPCHK(hipStreamSynchronize(net->s));
+  }
+  const int warm = (net && !done0) ? 1 : 0;
   hipLaunchKernelGGL(k_ibm_reset, dim3(std::min(1024, (E + 255) / 256)), dim3(256), 0, s, I.ctl.p, I.dcell.p,
-                     I.cellmap.p, I.dstate.p, done0);
+                     I.cellmap.p, I.dstate.p, done0, warm);
   HIPCHK(hipGetLastError());
   StencilArgs sa{};
   sa.M = M;
@@ -1082,12 +1134,29 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_zero, dim3((3 * M + 255) / 256), dim3(256), 0, s, I.Eforce.p, 3LL * M);
   HIPCHK(hipGetLastError());
-  // MDF loop (Solidbody.f90:893-906): ntol rounds are queued; the kernels of a
-  // round skip their work once k_mdf_check has found the loop finished
   const double invh3p = 0.5 * dt * ((1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh)) / I.denIn;
   double dsum = 0.0;
   for (int body = 0; body < I.nbodies; ++body) dsum = dsum + (double)I.nmark[body];
-  for (int it = 0; it < I.ntol; ++it) {
+  if (warm || I.record) {
+    hipLaunchKernelGGL(k_ws_features, dim3((M + 3) / 4), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
+                       I.Evel.p, I.Ea.p, I.Eprev.p, I.Enrm.p, I.Uref, I.wsX.p, I.tl0.p);
+    HIPCHK(hipGetLastError());
+  }
+  if (warm) {
+    // the learned starting force, applied to the band as one forcing round
+    hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((M + 63) / 64), dim3(256), 0, s, I.wsX.p, M, net->th.p, I.wsP.p);
+    hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Uref, invh3p,
+                       I.Eforce.p, I.fT.p);
+    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p, I.dentry.p,
+                       I.dEw.p, I.fT.p, I.band_u.p);
+    hipLaunchKernelGGL(k_ws_dm0, dim3(1), dim3(256), 0, s, I.tl0.p, M, dsum, I.Uref, I.dstate.p);
+    HIPCHK(hipGetLastError());
+  }
+  // MDF loop (Solidbody.f90:893-906): ntol rounds are queued (one more with
+  // the warm start, in case it falls back); the kernels of a round skip their
+  // work once k_mdf_check has found the loop finished
+  const int rounds = I.ntol + warm;
+  for (int it = 0; it < rounds; ++it) {
     hipLaunchKernelGGL(k_interp, dim3((M + 3) / 4), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
                        I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
     HIPCHK(hipGetLastError());
@@ -1096,6 +1165,23 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_mdf_check, dim3(1), dim3(256), 0, s, I.ctl.p, I.dstate.p, I.tolpart.p, M, dsum,
                        I.Uref, I.ntol, I.dtol);
+    HIPCHK(hipGetLastError());
+    if (warm && it == 0) {
+      // restart from the zero guess when the first round asked for it
+      hipLaunchKernelGGL(k_ws_fallback, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.ctl.p, I.Eforce.p);
+      if (b->precision == 0)
+        hipLaunchKernelGGL(k_band_macro, dim3((E + 255) / 256), dim3(256), 0, s, b->fA.p, b->N, I.dcell.p,
+                           I.ctl.p, I.cap, hv0, hv1, hv2, I.band_u.p, 1);
+      else
+        hipLaunchKernelGGL(k_band_macro32, dim3((E + 255) / 256), dim3(256), 0, s, b->gA.p, b->N, I.dcell.p,
+                           I.ctl.p, I.cap, hv0, hv1, hv2, I.band_u.p, 1);
+      hipLaunchKernelGGL(k_ws_fallback_end, dim3(1), dim3(64), 0, s, I.ctl.p);
+      HIPCHK(hipGetLastError());
+    }
+  }
+  if (warm || I.record) {
+    hipLaunchKernelGGL(k_ws_targets, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.Eforce.p, I.Ea.p, I.Uref,
+                       I.wsY.p, I.Eprev.p);
     HIPCHK(hipGetLastError());
   }
   const double invh3s = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
@@ -1129,8 +1215,34 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                 body + 1, why == 1 ? "min" : "max");
   }
   I.active = true;
+  I.last_warm = I.h_ctl[kWarm];
+  I.last_fallback = I.h_ctl[kFallback] != 0;
   if (iters) *iters = I.h_ctl[kIters];
   if (dmax) *dmax = I.h_dstate[0];
+  return 0;
+}
+
+int fsil_ibm_set_warm_start(int h, int mlp, int record) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (mlp && !get_mlp(mlp)) return 1;
+  b->ibm.warm_mlp = mlp;
+  b->ibm.record = record ? 1 : 0;
+  return 0;
+}
+
+int fsil_ibm_warm_start_data(int h, float *X, float *Y, int *info) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Ibm &I = b->ibm;
+  if (!(I.record || I.warm_mlp)) return fail("warm-start features are computed only with a network or recording on");
+  if (X) HIPCHK(hipMemcpyAsync(X, I.wsX.p, 12 * (size_t)I.M * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+  if (Y) HIPCHK(hipMemcpyAsync(Y, I.wsY.p, 3 * (size_t)I.M * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (info) {
+    info[0] = I.last_warm;
+    info[1] = I.last_fallback;
+  }
   return 0;
 }
 
@@ -1193,6 +1305,8 @@ int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, c
   F.bh.assign(nbodies, FemBodyHost{});
   for (int bi = 0; bi < nbodies; ++bi) {
     if (nND[bi] < 2 || nEL[bi] < 1) return fail("body %d: needs nodes and elements", bi + 1);
+    if (nND[bi] > 510)  // the solver keeps a body's CG products in one LDS staging area
+      return fail("body %d: the device beam solver takes bodies of at most 510 nodes (%d given)", bi + 1, nND[bi]);
     for (int e = 0; e < nEL[bi]; ++e)
       for (int k = 0; k < 2; ++k) {
         const int nd = elems[3 * (eo + e) + k];
@@ -1308,7 +1422,7 @@ int fsil_fem_update_markers(int h) {
   a.el = F.el.p; a.pos = F.pos.p; a.vel = F.vel.p;
   a.body_of_elem = F.body_of_elem.p; a.elem_first_marker = F.elem_first_marker.p; a.body_goff = F.body_goff.p;
   a.beta = F.beta.p; a.nEL_total = F.nEL;
-  a.Exyz = I.Exyz.p; a.Evel = I.Evel.p; a.Ea = I.Ea.p; a.elem_xc = I.elem_xc.p;
+  a.Exyz = I.Exyz.p; a.Evel = I.Evel.p; a.Ea = I.Ea.p; a.elem_xc = I.elem_xc.p; a.Enrm = I.Enrm.p;
   hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 255) / 256), dim3(256), 0, b->stream, a);
   HIPCHK(hipGetLastError());
   F.markers_on_device = true;
@@ -1755,33 +1869,6 @@ int fsil_time_tree_steps(int root, int nsteps, double t0, double dt, double *tot
 // ---------------------------------------------------------------------------
 // learned warm start: the network (fsil_mlp_kernels.h)
 namespace {
-
-struct Mlp {
-  DevBuf<float> th, g, m, v;                         // parameters, gradient, Adam moments
-  DevBuf<float> X, Y, bx, by, H1, H2, H3, Yp, dY, dZ3, dZ2, dZ1;
-  DevBuf<double> part;
-  DevBuf<int> idx;
-  int adam_t = 0;
-  hipStream_t s = nullptr;
-  ~Mlp() {
-    if (s) {
-      (void)hipStreamSynchronize(s);
-      (void)hipStreamDestroy(s);
-    }
-  }
-};
-
-std::map<int, std::unique_ptr<Mlp>> g_mlps;
-int g_next_mlp = 1;
-
-Mlp *get_mlp(int h) {
-  auto it = g_mlps.find(h);
-  if (it == g_mlps.end()) {
-    fail("invalid network handle %d", h);
-    return nullptr;
-  }
-  return it->second.get();
-}
 
 uint64_t splitmix64_at(uint64_t seed, uint64_t k) {
   uint64_t z = seed + k * 0x9E3779B97F4A7C15ULL;

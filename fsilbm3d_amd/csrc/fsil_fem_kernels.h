@@ -50,8 +50,10 @@ struct FemVecs {
 };
 
 // element-by-element product out = A*v for one body (Beam_MatrixMultipy,
-// SolidSolver.f90:2035-2049)
-__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const double *v, double *out) {
+// SolidSolver.f90:2035-2049); optionally zeroes the fixed DOFs of the result
+// and leaves the products v_g*out_g in prod (for the next dot product)
+__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const double *v, double *out,
+                                  bool zero_fixed = false, double *prod = nullptr) {
   const int nE = D.nEL, g0 = D.goff;
   for (int t = threadIdx.x; t < nE * NED; t += blockDim.x) {
     const int e = t / NED, i = t - e * NED;
@@ -70,12 +72,31 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
     double s = 0.0;
     for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
       s = s + A.lbuf[(size_t)A.dof_e[k] * NED + A.dof_i[k]];
+    if (zero_fixed && A.fixed[g0 + g]) s = 0.0;
     out[g] = s;
+    if (prod) prod[g] = v[g] * s;
   }
   fem_sync();
 }
 
-__device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, const FemVecs &V) {
+// sum of terms[0..n) in order by one thread, broadcast
+__device__ inline double fem_serial_sum(const double *terms, int n, double *shared) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < n; ++i) s = s + terms[i];
+    *shared = s;
+  }
+  fem_sync();
+  const double v = *shared;
+  fem_sync();
+  return v;
+}
+
+// z = M^-1 r (block Jacobi); optionally leaves the products r_g r_g in
+// prod[g] and r_g z_g in prod[cap + g]
+__device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, const FemVecs &V,
+                                   double *prod = nullptr, int cap = 0) {
   const int g0 = D.goff;
   for (int g = threadIdx.x; g < 6 * D.nND; g += blockDim.x) {
     const int node = g / 6, i = g - node * 6;
@@ -83,8 +104,33 @@ __device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, con
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) s = s + B[i * 6 + k] * V.r[6 * node + k];
-    V.z[g] = A.fixed[g0 + g] ? 0.0 : s;
+    const double z = A.fixed[g0 + g] ? 0.0 : s;
+    V.z[g] = z;
+    if (prod) {
+      const double r = V.r[g];
+      prod[g] = r * r;
+      prod[cap + g] = r * z;
+    }
   }
+  fem_sync();
+}
+
+// the two ordered sums of prod[0..n) and prod[cap..cap+n) on two waves
+__device__ inline void fem_pair_sums(const double *prod, int cap, int n, double *shared, double &r1, double &r2) {
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < n; ++i) s = s + prod[i];
+    shared[0] = s;
+  } else if (threadIdx.x == 64) {
+    double s = 0.0;
+#pragma unroll 16
+    for (int i = 0; i < n; ++i) s = s + prod[cap + i];
+    shared[1] = s;
+  }
+  fem_sync();
+  r1 = shared[0];
+  r2 = shared[1];
   fem_sync();
 }
 
@@ -217,19 +263,46 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
       el_rotate(E);
     }
     fem_sync();
-    // Per matrix entry: R^T K R, R^T G R and the Newmark tangent matrix
-    // (Segment_UpdateMatrix, SolidSolver.f90:131-157), entry by entry.
-    for (int t = threadIdx.x; t < nE * NED * NED; t += blockDim.x) {
-      const int e = t / (NED * NED), k = t - e * (NED * NED);
+    // Per 3x3 block of the element matrices: R^T K R, R^T G R and the Newmark
+    // tangent matrix (Segment_UpdateMatrix, SolidSolver.f90:131-157); each
+    // entry is summed as el_rkr sums it.
+    for (int t = threadIdx.x; t < nE * 16; t += blockDim.x) {
+      const int e = t >> 4, blkij = t & 15;
       BeamElem &E = A.el[D.eoff + e];
-      const int I = k / NED, J = k - I * NED;
-      const double sR = rkr_entry(E.stl, E.rot, I, J);
-      const double gR = rkr_entry(E.geo, E.rot, I, J);
-      const double tanv = sR + D.GeoGamma * gR;
-      double cv = tanv + c[0] * E.mas[k] + c[1] * D.dampM * E.mas[k];
-      if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
-      E.stf[k] = sR;
-      E.coef[k] = cv;
+      const int j1 = (blkij >> 2) * 3, j2 = (blkij & 3) * 3;
+      double r[9], ks[9], kg[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) r[q] = E.rot[q];
+#pragma unroll
+      for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+        for (int b2 = 0; b2 < 3; ++b2) {
+          ks[a2 * 3 + b2] = E.stl[(j1 + a2) * NED + j2 + b2];
+          kg[a2 * 3 + b2] = E.geo[(j1 + a2) * NED + j2 + b2];
+        }
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int ii = 0; ii < 3; ++ii) {
+          double sR = 0.0, gR = 0.0;
+#pragma unroll
+          for (int jj = 0; jj < 3; ++jj) {
+            double kts = 0.0, ktg = 0.0;
+#pragma unroll
+            for (int l = 0; l < 3; ++l) {
+              kts = kts + ks[jj * 3 + l] * r[l * 3 + ii];
+              ktg = ktg + kg[jj * 3 + l] * r[l * 3 + ii];
+            }
+            sR = sR + r[jj * 3 + k] * kts;
+            gR = gR + r[jj * 3 + k] * ktg;
+          }
+          const int idx = (j1 + k) * NED + j2 + ii;
+          const double tanv = sR + D.GeoGamma * gR;
+          double cv = tanv + c[0] * E.mas[idx] + c[1] * D.dampM * E.mas[idx];
+          if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
+          E.stf[idx] = sR;
+          E.coef[idx] = cv;
+        }
     }
     fem_sync();
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
@@ -331,11 +404,9 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
       if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
       for (int it = 1; it <= 10000; ++it) {
         ++cg_total;
-        fem_matvec(A, D, V.p, V.Ap);
-        for (int g = threadIdx.x; g < nG; g += blockDim.x)
-          if (A.fixed[g0 + g]) V.Ap[g] = 0.0;
-        fem_sync();
-        const double pAp = fem_dot(V.p, V.Ap, nG, sh_pair, lds, cap);
+        // A p with the fixed DOFs zeroed, and the products p_g (A p)_g
+        fem_matvec(A, D, V.p, V.Ap, true, lds);
+        const double pAp = fem_serial_sum(lds, nG, sh_pair);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
@@ -349,9 +420,9 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
         fem_sync();
         // z = M^-1 r is formed before the convergence test (unused when the
         // loop stops there) so that r.r and r.z are summed side by side
-        fem_precond(A, D, V);
+        fem_precond(A, D, V, lds, cap);
         double rsnew;
-        fem_dot2(V.r, V.r, V.r, V.z, nG, sh_pair, lds, cap, rr, rsnew);
+        fem_pair_sums(lds, cap, nG, sh_pair, rr, rsnew);
         rn = sqrt(rr);
         if (rn <= err) break;
         if (fabs(rsold) <= 1.0e-30) { status = 2; break; }
@@ -435,6 +506,7 @@ struct FemMarkerArgs {
   const double *beta;   // per body: -(alpha*2*denIn)
   int nEL_total;
   double *Exyz, *Evel, *Ea, *elem_xc;
+  double *Enrm;         // marker normals (chord x span direction), for the warm start; may be null
 };
 
 __global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
@@ -469,6 +541,16 @@ __global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
       a.Evel[3LL * (m0 + s) + k] = tvel[k] + w[k];
     }
     a.Ea[m0 + s] = area;
+  }
+  if (a.Enrm) {
+    double ax[3], nv[3];
+    for (int k = 0; k < 3; ++k) ax[k] = E.x1[6 + k] - E.x1[k];
+    nv[0] = ax[1] * dirc[2] - ax[2] * dirc[1];
+    nv[1] = ax[2] * dirc[0] - ax[0] * dirc[2];
+    nv[2] = ax[0] * dirc[1] - ax[1] * dirc[0];
+    const double nn = sqrt(bdot3(nv, nv));
+    for (int s = 0; s < E.Nspan; ++s)
+      for (int k = 0; k < 3; ++k) a.Enrm[3LL * (m0 + s) + k] = nn > 0.0 ? nv[k] / nn : 0.0;
   }
   for (int k = 0; k < 3; ++k) a.elem_xc[3LL * e + k] = 0.5 * (E.x1[k] + E.x1[6 + k]);
 }

@@ -27,6 +27,9 @@ enum IbmCtl {
   kErr = 7,        // first failing marker (1-based) * 4 + reason, INT_MAX when none
   kDone = 8,       // multi-direct-forcing loop finished
   kIters = 9,      // iterations run
+  kWarm = 10,      // learned warm start in use for this call
+  kFallback = 11,  // 1: the warm start did not lower the residual, restart from zero; 2: restarted
+  kDone0 = 12,     // the done word at the start of a loop (no iteration at all)
   kCtlWords = 16
 };
 constexpr int kKeyShift = 26;  // e = 64*m + s < 2^26
@@ -67,7 +70,7 @@ __host__ __device__ inline int trim4(int i, int nd, int *out, int bclo, int bchi
 // reset of the control words; clears the previous band from the cell map
 __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const long long *__restrict__ prev_cell,
                                                    int *__restrict__ cellmap, double *__restrict__ dstate,
-                                                   int done0) {
+                                                   int done0, int warm) {
   const int nprev = ctl[kNband];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nprev; i += gridDim.x * blockDim.x)
     cellmap[prev_cell[i]] = -1;
@@ -77,6 +80,9 @@ __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const 
     ctl[kErr] = 0x7fffffff;
     ctl[kDone] = done0;
     ctl[kIters] = 0;
+    ctl[kWarm] = warm;
+    ctl[kFallback] = 0;
+    ctl[kDone0] = done0;
     dstate[0] = 1e10;  // dmax before the first iteration (Solidbody.f90:893)
   }
 }
@@ -179,7 +185,8 @@ __global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f
                                                     const long long *__restrict__ cell,
                                                     const int *__restrict__ ctl, int cap,
                                                     double hv0, double hv1, double hv2,
-                                                    double *__restrict__ band_u) {
+                                                    double *__restrict__ band_u, int only_on_fallback = 0) {
+  if (only_on_fallback && ctl[kFallback] != 1) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ctl[kNband]) return;
   const long long n = cell[b];
@@ -191,6 +198,33 @@ __global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f
   band_u[b] = (m1 + hv0) / s;
   band_u[cap + b] = (m2 + hv1) / s;
   band_u[2LL * cap + b] = (m3 + hv2) / s;
+}
+
+// band macros of the FP32-deviation state (the immersed-boundary coupling
+// runs in FP64): rho = 1 + sum g, j = sum e g, summed in FP64 from the stored
+// deviations
+__global__ __launch_bounds__(256) void k_band_macro32(const float *__restrict__ g, long long N,
+                                                      const long long *__restrict__ cell,
+                                                      const int *__restrict__ ctl, int cap, double hv0,
+                                                      double hv1, double hv2, double *__restrict__ band_u,
+                                                      int only_on_fallback = 0) {
+  if (only_on_fallback && ctl[kFallback] != 1) return;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ctl[kNband]) return;
+  const long long n = cell[b];
+  double dr = 0.0, j1 = 0.0, j2 = 0.0, j3 = 0.0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const double v = (double)g[q * N + n];
+    dr = dr + v;
+    if (ex(q) != 0) j1 = ex(q) > 0 ? j1 + v : j1 - v;
+    if (ey(q) != 0) j2 = ey(q) > 0 ? j2 + v : j2 - v;
+    if (ez(q) != 0) j3 = ez(q) > 0 ? j3 + v : j3 - v;
+  }
+  const double rho = 1.0 + dr;
+  band_u[b] = (j1 + hv0) / rho;
+  band_u[cap + b] = (j2 + hv1) / rho;
+  band_u[2LL * cap + b] = (j3 + hv2) / rho;
 }
 
 __global__ void k_zero(double *__restrict__ p, long long n) {
@@ -313,6 +347,16 @@ __global__ __launch_bounds__(256) void k_mdf_check(int *__restrict__ ctl, double
   }
   const double dm = tol / (dsum * Uref);
   dstate[0] = dm;
+  if (it == 1 && ctl[kWarm] == 1 && !(dm < dstate[1])) {
+    // the guessed force did not lower the residual below the zero guess's:
+    // restart the loop from zero (k_ws_fallback, k_band_macro on fallback)
+    ctl[kWarm] = 0;
+    ctl[kFallback] = 1;
+    ctl[kIters] = 0;
+    ctl[kDone] = ctl[kDone0];
+    dstate[0] = 1e10;
+    return;
+  }
   if (!(it < ntol && dm > dtol)) ctl[kDone] = 1;
 }
 
@@ -388,6 +432,104 @@ __global__ __launch_bounds__(256) void k_loads(int ndof, const int *__restrict__
 #pragma unroll 8
   for (int j = dof_ptr[d]; j < dof_ptr[d + 1]; ++j) acc = acc + term[j];
   lod[d] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Learned warm start (no counterpart in the reference)
+
+// Features of every marker for the network, one wave per marker: the
+// interpolated velocity of the uncorrected band (as k_interp forms it), the
+// marker velocity, the marker's force of the previous call and its normal:
+//   X = [U/Uref, V/Uref, Fprev/(Uref |Ea|), n]
+// and the zero guess's |V - U|_1 (the residual a plain loop starts from).
+__global__ __launch_bounds__(256) void k_ws_features(int M, const int *__restrict__ bidx,
+                                                     const float *__restrict__ Ew,
+                                                     const double *__restrict__ band_u, int cap,
+                                                     const double *__restrict__ Evel,
+                                                     const double *__restrict__ Ea,
+                                                     const double *__restrict__ Eprev,
+                                                     const double *__restrict__ Enrm, double Uref,
+                                                     float *__restrict__ X, double *__restrict__ tl0) {
+  __shared__ double t[4][3][64];
+  const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + w;
+  if (m < M) {
+    const int b = bidx[64LL * m + s];
+    const float *W = Ew + 12LL * m;
+    const double rx = (double)W[s >> 4], ry = (double)W[4 + ((s >> 2) & 3)], rz = (double)W[8 + (s & 3)];
+    t[w][0][s] = ((band_u[b] * rx) * ry) * rz;
+    t[w][1][s] = ((band_u[cap + b] * rx) * ry) * rz;
+    t[w][2][s] = ((band_u[2LL * cap + b] * rx) * ry) * rz;
+  }
+  __syncthreads();
+  if (m < M && s == 0) {
+    double v[3] = {0.0, 0.0, 0.0};
+    for (int k = 0; k < 64; ++k) {
+      v[0] = v[0] + t[w][0][k];
+      v[1] = v[1] + t[w][1][k];
+      v[2] = v[2] + t[w][2][k];
+    }
+    const double fs = Uref * fabs(Ea[m]);
+    float *x = X + 12LL * m;
+    double r = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      x[c] = (float)(v[c] / Uref);
+      x[3 + c] = (float)(Evel[3LL * m + c] / Uref);
+      x[6 + c] = (float)(fs > 0.0 ? Eprev[3LL * m + c] / fs : 0.0);
+      x[9 + c] = Enrm ? (float)Enrm[3LL * m + c] : 0.f;
+      r = r + fabs(Evel[3LL * m + c] - v[c]);
+    }
+    tl0[m] = r;
+  }
+}
+
+// dstate[1] = the zero guess's residual, reduced as k_mdf_check reduces
+__global__ __launch_bounds__(256) void k_ws_dm0(const double *__restrict__ tl0, int M, double dsum, double Uref,
+                                                double *__restrict__ dstate) {
+  double p = 0.0;
+  for (int m = threadIdx.x; m < M; m += blockDim.x) p = p + tl0[m];
+  __shared__ double part[4];
+  p = wave_sum(p);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = p;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double tol = 0.0;
+  for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) tol = tol + part[k];
+  dstate[1] = tol / (dsum * Uref);
+}
+
+// the network's guess as the starting force: Eforce = F0, fT = F0 * invh3
+// (applied to the band by k_correct as one forcing round)
+__global__ __launch_bounds__(256) void k_ws_apply(int M, const float *__restrict__ Y, const double *__restrict__ Ea,
+                                                  double Uref, double invh3, double *__restrict__ Eforce,
+                                                  double *__restrict__ fT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * M) return;
+  const double F0 = (double)Y[i] * (Uref * fabs(Ea[i / 3]));
+  Eforce[i] = F0;
+  fT[i] = F0 * invh3;
+}
+
+// restart from the zero guess when k_mdf_check asked for it
+__global__ __launch_bounds__(256) void k_ws_fallback(int M, int *__restrict__ ctl, double *__restrict__ Eforce) {
+  if (ctl[kFallback] != 1) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * M) Eforce[i] = 0.0;
+}
+__global__ void k_ws_fallback_end(int *__restrict__ ctl) {
+  if (threadIdx.x == 0 && ctl[kFallback] == 1) ctl[kFallback] = 2;
+}
+
+// training targets (the converged force over Uref |Ea|) and the previous
+// force of the next call
+__global__ __launch_bounds__(256) void k_ws_targets(int M, const double *__restrict__ Eforce,
+                                                    const double *__restrict__ Ea, double Uref,
+                                                    float *__restrict__ Y, double *__restrict__ Eprev) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * M) return;
+  const double fs = Uref * fabs(Ea[i / 3]);
+  Y[i] = (float)(fs > 0.0 ? Eforce[i] / fs : 0.0);
+  Eprev[i] = Eforce[i];
 }
 
 }  // namespace fsil

@@ -126,31 +126,6 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   }
 }
 
-// band macros of the FP32-deviation state (the immersed-boundary coupling
-// runs in FP64): rho = 1 + sum g, j = sum e g, summed in FP64 from the stored
-// deviations
-__global__ __launch_bounds__(256) void k_band_macro32(const float *__restrict__ g, long long N,
-                                                      const long long *__restrict__ cell,
-                                                      const int *__restrict__ ctl, int cap, double hv0,
-                                                      double hv1, double hv2, double *__restrict__ band_u) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ctl[6]) return;
-  const long long n = cell[b];
-  double dr = 0.0, j1 = 0.0, j2 = 0.0, j3 = 0.0;
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const double v = (double)g[q * N + n];
-    dr = dr + v;
-    if (ex(q) != 0) j1 = ex(q) > 0 ? j1 + v : j1 - v;
-    if (ey(q) != 0) j2 = ey(q) > 0 ? j2 + v : j2 - v;
-    if (ez(q) != 0) j3 = ez(q) > 0 ? j3 + v : j3 - v;
-  }
-  const double rho = 1.0 + dr;
-  band_u[b] = (j1 + hv0) / rho;
-  band_u[cap + b] = (j2 + hv1) / rho;
-  band_u[2LL * cap + b] = (j3 + hv2) / rho;
-}
-
 // den/u of the FP32-deviation state, in FP64 output arrays
 __global__ __launch_bounds__(256) void k_macro32(const float *__restrict__ g, long long N, float hv0,
                                                  float hv1, float hv2, double *__restrict__ den,

@@ -88,6 +88,19 @@ class DeviceBodyBlock:
                                                        _lib.iptr(it), _lib.dptr(dm)))
         self.iters, self.dmax = int(it[0]), float(dm[0])
 
+    def set_warm_start(self, net=None, record: bool = False):
+        """Learned warm start of the direct-forcing loop (fsilbm3d_amd.mlp.WarmStartNet)."""
+        _lib.check(self.lib.fsil_ibm_set_warm_start(self.block.handle, net.handle if net is not None else 0,
+                                                    int(bool(record))))
+
+    def warm_start_data(self):
+        """(features [M,12], targets [M,3], guess used, fell back) of the last step."""
+        X = np.zeros((self.M, 12), np.float32)
+        Y = np.zeros((self.M, 3), np.float32)
+        info = np.zeros(2, np.int32)
+        _lib.check(self.lib.fsil_ibm_warm_start_data(self.block.handle, _lib.fptr(X), _lib.fptr(Y), _lib.iptr(info)))
+        return X, Y, bool(info[0]), bool(info[1])
+
     def body_loads(self, i: int) -> np.ndarray:
         off = 6 * int(self.nND[:i].sum())
         return self.lodFlow[off:off + 6 * int(self.nND[i])]

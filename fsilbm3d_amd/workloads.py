@@ -36,6 +36,7 @@ class FlexiblePlate:
     KB: float = 0.1
     KS: float = 1000.0
     denR: float = 1.0
+    body_model: int = 2                                    # 2 flexible beam, 1 rigid prescribed motion
     bc: Tuple[int, ...] = (101, 103, 302, 302, 302, 302)
 
     @property
@@ -54,7 +55,7 @@ class FlexiblePlate:
         cons = np.zeros((n, 6), np.int32)
         cons[0] = 1                                        # clamped leading edge
         xyzo = (self.le[0] + half, self.le[1], self.le[2])
-        return PlateSpec(nodes8=nodes, elements=elements, constraints=cons, iBodyModel=2,
+        return PlateSpec(nodes8=nodes, elements=elements, constraints=cons, iBodyModel=self.body_model,
                          isMotionGiven=(1, 1, 1, 1, 1, 1), denR=self.denR, KB=self.KB, KS=self.KS, psR=0.3,
                          Freq=self.freq * self.U / self.L, XYZo=xyzo, XYZAmpl=(0.0, self.amp * self.L, 0.0))
 

@@ -19,7 +19,8 @@ module fsil_bridge
               fsil_fem_state, fsil_fem_info, fsil_tree_add_son, fsil_tree_pair_info, &
               fsil_extract_interpolate_layer, fsil_interpolation_father_to_son, fsil_deliver_son_to_father, &
               fsil_tree_steps, fsil_time_tree_steps, fsil_mlp_create, fsil_mlp_destroy, fsil_mlp_get_params, &
-              fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train
+              fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train, fsil_ibm_set_warm_start, &
+              fsil_ibm_warm_start_data
 
     interface
         integer(c_int) function fsil_version() bind(C, name='fsil_version')
@@ -254,6 +255,16 @@ module fsil_bridge
             real(c_double), value :: lr
             real(c_float), intent(in) :: X(*), Y(*)
             real(c_double), intent(out) :: loss_before, loss_after
+        end function
+        integer(c_int) function fsil_ibm_set_warm_start(handle, mlp, record) bind(C, name='fsil_ibm_set_warm_start')
+            import :: c_int
+            integer(c_int), value :: handle, mlp, record
+        end function
+        integer(c_int) function fsil_ibm_warm_start_data(handle, X, Y, info) bind(C, name='fsil_ibm_warm_start_data')
+            import :: c_int, c_float
+            integer(c_int), value :: handle
+            real(c_float), intent(out) :: X(*), Y(*)
+            integer(c_int), intent(out) :: info(2)
         end function
     end interface
 

@@ -220,6 +220,19 @@ int fsil_mlp_loss_grad(int handle, const float *X, const float *Y, int n, double
 int fsil_mlp_train(int handle, const float *X, const float *Y, int n, int epochs, int batch, double lr, int seed,
                    double *loss_before, double *loss_after);
 
+/* Use network `mlp` (0: none) to guess the starting marker force of every
+ * coupling call of block `handle`; record != 0 computes the network's features
+ * and training targets on every call even without a network.  When the guess
+ * does not lower the first residual below the zero guess's, the call restarts
+ * from zero (the plain loop's result). */
+int fsil_ibm_set_warm_start(int handle, int mlp, int record);
+
+/* Features X[M][12] (interpolated velocity / Uref, marker velocity / Uref,
+ * previous force / (Uref |Ea|), normal) and targets Y[M][3] (converged force /
+ * (Uref |Ea|)) of the last coupling call; info[0] = guess used, info[1] =
+ * fell back to zero. */
+int fsil_ibm_warm_start_data(int handle, float *X, float *Y, int *info);
+
 #ifdef __cplusplus
 }
 #endif

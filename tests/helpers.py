@@ -245,11 +245,12 @@ def mismatch(a, b):
     return int((a != b).sum()), float(d.max()) if d.size else 0.0
 
 
-def run_plate_fsi(w, precision, nsteps, record_every=1):
+def run_plate_fsi(w, precision, nsteps, record_every=1, net=None, warm_from=None, collect=0, solver=None):
     """A flexible-plate workload (fsilbm3d_amd.workloads.FlexiblePlate) on the
     device in the given storage form; returns per-step total fluid force on
-    the body (sum of lodFlow x, y, z), MDF iterations, and the final node
-    positions."""
+    the body (sum of lodFlow x, y, z), MDF iterations and residuals, and the
+    final node positions.  net/warm_from: learned warm start from that step
+    on; collect: record the network's features/targets of the first steps."""
     from fsilbm3d_amd.block import LBMBlock
     from fsilbm3d_amd.fsi import DeviceBodyBlock
     blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu, precision=precision)
@@ -257,16 +258,33 @@ def run_plate_fsi(w, precision, nsteps, record_every=1):
     blk.upload(w.initial_state())
     blk.update_volume_force(0.0)
     blk.set_boundary_conditions()
-    fsi = DeviceBodyBlock(blk, [w.plate()], **w.solver_kwargs())
-    forces, iters = [], []
+    kw = w.solver_kwargs()
+    kw.update(solver or {})
+    fsi = DeviceBodyBlock(blk, [w.plate()], **kw)
+    forces, iters, dmax, warm, fell = [], [], [], [], []
+    XS, YS = [], []
     t = 0.0
     for k in range(nsteps):
+        use = net is not None and warm_from is not None and k >= warm_from
+        fsi.set_warm_start(net if use else None, record=(k < collect) or use)
         t = t + 1.0
         fsi.step(t)
         if (k + 1) % record_every == 0:
             forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
-            iters.append(fsi.iters)
+        iters.append(fsi.iters)
+        dmax.append(fsi.dmax)
+        if k < collect or use:
+            X, Y, used, fb = fsi.warm_start_data()
+            if k < collect:
+                XS.append(X)
+                YS.append(Y)
+            warm.append(used)
+            fell.append(fb)
     pos = fsi.state(0)[0].copy()
     den, uuu = blk.calculate_macro_quantities()
     blk.close()
-    return dict(forces=np.array(forces), iters=iters, pos=pos, u=uuu)
+    out = dict(forces=np.array(forces), iters=iters, dmax=dmax, pos=pos, u=uuu, warm=warm, fell=fell)
+    if collect:
+        out["X"] = np.concatenate(XS)
+        out["Y"] = np.concatenate(YS)
+    return out
