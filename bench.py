@@ -1,5 +1,15 @@
 """Benchmark of the FSILBM3D fluid/coupling hot path on one MI355X.
 
+Forms of the FSI step (all through the C-ABI, same steps, same inputs):
+fp64-plain (the reference's arithmetic), fp32-deviation (fluid stored and
+collided in FP32 as f - w; solid, coupling and reductions FP64), and
+fp32-deviation+learned (the same with the learned warm start of the
+direct-forcing loop, trained first on two seeded FP32 runs).  `value` is the
+FSI-step MLUPS of the first of fp32-deviation+learned, fp32-deviation whose
+drag and lift stay within 1e-3 (relative to the largest |drag|, |lift|) of
+the fp64-plain run over the timed steps, else of fp64-plain; every form is
+reported under "forms" with its force error and speed-up.
+
 Headline workload (value): the flexible-plate FSI case -- D3Q19 384x192x192,
 uniform inflow U = 0.05 (101), outflow (103), slip lateral faces (302), Re 500
 on L = 48 cells; one flexible plate (49 chord nodes x 48 span markers = 2304
@@ -7,17 +17,18 @@ markers, mass ratio 1, KB 0.1, KS 1000, clamped leading edge heaving 0.3 L at
 0.2 U/L), numsubstep 4, multi-direct forcing to 1e-6 with at most 10
 iterations.  One step = the body of main.f90:93-147: volume force, markers,
 immersed-boundary coupling, 4 beam sub-steps, fused collide+stream+faces.
-value = FSI-step MLUPS of the FP64 plain form (cells x steps / wall seconds,
-host-side around exactly --steps steps between device synchronisations, the
-initial state and the bodies already resident on the device).
+MLUPS = cells x steps / wall seconds, host-side around exactly --steps steps
+(after --warmup steps) between device synchronisations, the initial state and
+the bodies already resident on the device.
 
 Beside it: the stencil benchmark (D3Q19 512x256x256 fully periodic, SRT tau
 0.55, seeded equilibrium) in the FP64 plain and FP32-deviation forms, with the
 FP32 velocity L2 error against FP64.
 
-roofline: the fused collide+stream kernel of the FSI step (the band-overlay
-variant), algorithmic bytes per launch 19 reads + 19 writes x 8 B = 304 B per
-cell, over its average duration from HIP events on the block's stream.
+roofline: the fused collide+stream kernel of the headline form's FSI step
+(the band-overlay variant), algorithmic bytes per launch 19 reads + 19 writes
+per cell (304 B FP64, 152 B FP32-deviation), over its average duration from
+HIP events on the block's stream.
 cpu_baseline: the reference program itself (oracle/_ref, compiled from the
 reference sources) timed on this host's cores for a few steps of the same
 FSI case.
@@ -62,43 +73,90 @@ def run_form(prec, f0, steps, warmup, nlaunch):
                 kernel_ms=kern_ms, achieved_gbs=achieved)
 
 
-def run_fsi(w, steps, warmup, nlaunch):
-    """Flexible-plate FSI steps through the C-ABI (device beam solver)."""
+def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0):
+    """Flexible-plate FSI steps through the C-ABI (device beam solver).
+    precision 0: FP64 plain, 1: FP32-deviation fluid (FP64 solid and
+    coupling).  net: learned warm start during the timed steps.  collect:
+    record the network's features/targets of the first `collect` steps.
+    Returns timings and the per-step force on the body over the timed steps."""
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock, synchronize
     from fsilbm3d_amd.fsi import DeviceBodyBlock
     N = int(np.prod(w.dims))
-    blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu)
+    blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu, precision=precision)
     blk.set_flow(**w.flow_kwargs())
     blk.upload(w.initial_state())
     blk.update_volume_force(0.0)
     blk.set_boundary_conditions()
     fsi = DeviceBodyBlock(blk, [w.plate()], **w.solver_kwargs())
+    XS, YS = [], []
     t = 0.0
-    for _ in range(warmup):
+    for k in range(warmup):
+        fsi.set_warm_start(None, record=k < collect)
         t = t + 1.0
         fsi.step(t)
+        if k < collect:
+            X, Y, _, _ = fsi.warm_start_data()
+            XS.append(X)
+            YS.append(Y)
+    fsi.set_warm_start(net, record=False)
     synchronize()
-    its = []
+    its, dm = [], []
+    forces = []
     t0 = time.perf_counter()
     for _ in range(steps):
         t = t + 1.0
         fsi.step(t)
         its.append(fsi.iters)
+        dm.append(fsi.dmax)
+        forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
     synchronize()
     secs = time.perf_counter() - t0
     newton, cg, _ = fsi.info(0)
     pos = fsi.state(0)[0]
-    lod = fsi.body_loads(0).reshape(-1, 6).sum(0)
     # the fused kernel with the band overlay, timed with HIP events on the block's stream
+    fsi.set_warm_start(None)
     fsi.interaction_force(blk.dh)
     kern_ms = blk.time_collide(nlaunch)
     synchronize()
     blk.close()
-    return dict(mlups=N * steps / secs / 1e6, ms_per_step=secs / steps * 1e3, kernel_ms=kern_ms,
-                achieved_gbs=B_CELL[0] * N / (kern_ms * 1e-3) / 1e9, mdf_iters_mean=float(np.mean(its)),
-                newton_iters_last=newton, cg_iters_last=cg, tip_xyz=[float(v) for v in pos[-1, :3]],
-                force_xyz=[float(v) for v in lod[:3]], cells=N, time=t)
+    out = dict(mlups=N * steps / secs / 1e6, ms_per_step=secs / steps * 1e3, kernel_ms=kern_ms,
+               achieved_gbs=B_CELL[precision] * N / (kern_ms * 1e-3) / 1e9, mdf_iters_mean=float(np.mean(its)),
+               mdf_residual_mean=float(np.mean(dm)), newton_iters_last=newton, cg_iters_last=cg,
+               tip_xyz=[float(v) for v in pos[-1, :3]], forces=np.array(forces), cells=N, time=t)
+    if collect:
+        out["X"] = np.concatenate(XS)
+        out["Y"] = np.concatenate(YS)
+    return out
+
+
+def force_error(a, b):
+    """max over steps of |drag_a - drag_b| and |lift_a - lift_b|, relative to
+    the largest |drag| / |lift| of b (the FP64 plain run)."""
+    import numpy as np
+    Fa, Fb = a["forces"][:, :2], b["forces"][:, :2]
+    rel = np.abs(Fa - Fb).max(axis=0) / np.abs(Fb).max(axis=0)
+    return {"drag": float(rel[0]), "lift": float(rel[1])}
+
+
+def train_warm_start(w, steps, epochs, seed=20261202):
+    """The learned warm start trained on the solver's own trajectories: two
+    seeded FP32 runs of the workload with heave amplitude in U(0.2, 0.4) L and
+    frequency in U(0.15, 0.3) U/L, every marker of every step a sample."""
+    import dataclasses
+    import numpy as np
+    from fsilbm3d_amd.mlp import WarmStartNet
+    u = np.random.default_rng(seed).random(4)
+    XS, YS = [], []
+    for r in range(2):
+        wr = dataclasses.replace(w, amp=0.2 + 0.2 * u[2 * r], freq=0.15 + 0.15 * u[2 * r + 1])
+        d = run_fsi(wr, 1, steps, 1, precision=1, collect=steps)
+        XS.append(d["X"])
+        YS.append(d["Y"])
+    X, Y = np.concatenate(XS), np.concatenate(YS)
+    net = WarmStartNet(seed & 0x7fffffff)
+    before, after = net.train(X, Y, epochs=epochs, batch=1024, lr=2e-3, seed=seed & 0x7fffffff)
+    return net, {"samples": int(X.shape[0]), "epochs": epochs, "loss_before": before, "loss_after": after}
 
 
 def l2_fp32_vs_fp64(f0, nsteps):
@@ -141,11 +199,11 @@ def cpu_baseline(w, steps):
                       f"uniform initial state); seconds per step {[round(s, 3) for s in secs]}"}
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the FP64 fused kernel from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel
-    calibration), or None when it has not been measured."""
-    p = os.path.join(ROOT, "profiles", "collide_push_fp64_512x256x256_pmc.json")
+def traffic_from_profiles(name="collide_push_fp64_512x256x256_pmc.json"):
+    """HBM bytes per launch of a fused kernel from a committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when it has not been measured."""
+    p = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(p):
         return None
     try:
@@ -163,12 +221,21 @@ def copy_gbs(nbytes=2 << 30, reps=10):
     return float(out[0])
 
 
-def pmc_driver():
+def pmc_driver(kind="stencil"):
     """Kernels for the HBM-counter runs: k_copy over 2 GiB (calibration: 4 GiB
-    of traffic per launch) and 3 launches of the FP64 fused kernel."""
+    of traffic per launch), then either 3 launches of the FP64 fused kernel on
+    the stencil workload, or 3 flexible-plate FSI steps in each storage form
+    (the band-overlay fused kernels)."""
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock, synchronize
     copy_gbs(2 << 30, 1)
+    if kind == "fsi":
+        from fsilbm3d_amd.workloads import flexible_plate
+        w = flexible_plate()
+        for prec in (0, 1):
+            run_fsi(w, 3, 0, 1, precision=prec)
+        log("pmc driver done")
+        return
     blk = LBMBlock(DIMS, bc=(301,) * 6, nu=NU, precision=0)
     f = np.empty((19,) + DIMS)
     w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
@@ -186,13 +253,17 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--stencil-steps", type=int, default=200)
+    ap.add_argument("--learned-steps", type=int, default=50,
+                    help="steps of each of the two training runs of the warm start (0: no learned form)")
+    ap.add_argument("--learned-epochs", type=int, default=20)
     ap.add_argument("--nlaunch", type=int, default=50)
     ap.add_argument("--l2-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--pmc-driver", action="store_true",
-                    help="profiling driver: one 2 GiB copy kernel, then 3 launches of the FP64 fused kernel")
+    ap.add_argument("--pmc-driver", nargs="?", const="stencil", default=None, choices=["stencil", "fsi"],
+                    help="profiling driver: one 2 GiB copy kernel, then 3 launches of the FP64 fused kernel "
+                         "(stencil) or 3 FSI steps in each storage form (fsi)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
@@ -201,12 +272,28 @@ def main():
     from tests.golden.cases import seeded_state
     from fsilbm3d_amd.workloads import flexible_plate
     if args.pmc_driver:
-        pmc_driver()
+        pmc_driver(args.pmc_driver)
         return
     w = flexible_plate()
     log("bench: flexible-plate FSI, FP64 plain form")
-    fsi = run_fsi(w, args.steps, args.warmup, args.nlaunch)
-    log(f"bench: FSI {fsi}")
+    fsi = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=0)
+    log(f"bench: FSI FP64 {fsi['mlups']:.1f} MLUPS, {fsi['ms_per_step']:.3f} ms/step")
+    log("bench: flexible-plate FSI, FP32-deviation fluid")
+    fsi32 = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1)
+    err32 = force_error(fsi32, fsi)
+    log(f"bench: FSI FP32 {fsi32['mlups']:.1f} MLUPS, force error {err32}")
+    learned = None
+    if args.learned_steps > 0:
+        log(f"bench: training the warm start on {2 * args.learned_steps} FP32 steps")
+        t0 = time.perf_counter()
+        net, tinfo = train_warm_start(w, args.learned_steps, args.learned_epochs)
+        tinfo["seconds"] = time.perf_counter() - t0
+        log(f"bench: warm start trained {tinfo}")
+        fsiL = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1, net=net)
+        errL = force_error(fsiL, fsi)
+        log(f"bench: FSI FP32+learned {fsiL['mlups']:.1f} MLUPS, force error {errL}")
+        learned = (fsiL, errL, tinfo)
+        net.close()
     log("bench: seeding 512x256x256 state")
     f0 = seeded_state(DIMS, 20261202)
     log("bench: stencil FP64 plain form")
@@ -230,36 +317,58 @@ def main():
             cpu = {"value": None, "unit": "MLUPS", "cores": None, "kind": "reference", "sample": f"failed: {e}"}
     N = DIMS[0] * DIMS[1] * DIMS[2]
     cp = copy_gbs()
+    # headline form: FP32+learned when its drag/lift stay within 1e-3 of the
+    # FP64 plain run, else FP32 alone under the same test, else FP64 plain
+    forms = {"fp64-plain": (fsi, {"drag": 0.0, "lift": 0.0})}
+    forms["fp32-deviation"] = (fsi32, err32)
+    if learned is not None:
+        forms["fp32-deviation+learned"] = (learned[0], learned[1])
+    head = "fp64-plain"
+    for name in ("fp32-deviation+learned", "fp32-deviation"):
+        if name in forms and max(forms[name][1].values()) < 1e-3:
+            head = name
+            break
+    H = forms[head][0]
+    f32 = head != "fp64-plain"
+    roofline = {"bound": "hbm", "achieved": H["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": H["achieved_gbs"] / HBM_PEAK_GBS,
+                "traffic": traffic_from_profiles("collide_push32_band_384x192x192_pmc.json" if f32 else
+                                                 "collide_push_band_384x192x192_pmc.json"),
+                "kernel": ("k_collide_push32<false,true>" if f32 else "k_collide_push<1,false,true,false>") +
+                          " (band overlay) of the FSI step",
+                "kernel_ms": H["kernel_ms"], "bytes_per_cell": B_CELL[1 if f32 else 0]}
     out = {
         "metric": "FSI-step MLUPS, 1 MI355X: FP32+learned vs FP64 plain @drag/lift err<1e-3; %HBM",
-        "value": fsi["mlups"],
+        "value": H["mlups"],
         "unit": "MLUPS",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": fsi["ms_per_step"],
+        "ms_per_step": H["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "fp64-plain" else "f32 (fluid storage and collision; FP64 solid and coupling)",
         "data": "synthetic (uniform-flow initial state, prescribed plate kinematics)",
         "config": {"workload": "flexible-plate FSI: D3Q19 384x192x192, inflow U 0.05, Re 500 on L 48, one elastic "
-                               "plate 49 chord nodes x 48 span markers, numsubstep 4, MDF to 1e-6 (max 10), "
-                               "FP64 plain form",
-                   "cells": fsi["cells"], "markers": 2304, "collision": "SRT", "form": "fp64-plain"},
-        "fsi": {k: fsi[k] for k in ("mdf_iters_mean", "newton_iters_last", "cg_iters_last", "tip_xyz", "force_xyz",
-                                   "kernel_ms")},
+                               "plate 49 chord nodes x 48 span markers, numsubstep 4, MDF to 1e-6 (max 10)",
+                   "cells": fsi["cells"], "markers": 2304, "collision": "SRT", "form": head},
+        "forms": {name: {"value": r["mlups"], "ms_per_step": r["ms_per_step"],
+                         "speedup_vs_fp64_plain": r["mlups"] / fsi["mlups"], "force_error_vs_fp64_plain": e,
+                         "mdf_iters_mean": r["mdf_iters_mean"], "mdf_residual_mean": r["mdf_residual_mean"],
+                         "newton_iters_last": r["newton_iters_last"], "cg_iters_last": r["cg_iters_last"],
+                         "tip_xyz": r["tip_xyz"], "collide_kernel_ms": r["kernel_ms"]}
+                  for name, (r, e) in forms.items()},
+        "warm_start_training": learned[2] if learned is not None else None,
         "copy_kernel_gbs": cp,
-        "roofline": {"bound": "hbm", "achieved": fsi["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": fsi["achieved_gbs"] / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_collide_push<1,false,true,false> (band overlay)", "kernel_ms": fsi["kernel_ms"],
-                     "bytes_per_cell": B_CELL[0]},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "stencil": {"workload": "D3Q19 512x256x256 periodic, SRT tau 0.55, seeded equilibrium", "cells": N,
                     "steps": args.stencil_steps, "value": r64["mlups"], "unit": "MLUPS",
                     "ms_per_step": r64["ms_per_step"],
                     "roofline": {"bound": "hbm", "achieved": r64["achieved_gbs"], "peak": HBM_PEAK_GBS,
                                  "unit": "GB/s", "frac": r64["achieved_gbs"] / HBM_PEAK_GBS,
+                                 "traffic": traffic_from_profiles(),
                                  "kernel": "k_collide_push<1,false,false,false>", "kernel_ms": r64["kernel_ms"],
                                  "bytes_per_cell": B_CELL[0]}},
     }
