@@ -1444,6 +1444,13 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   const int lds_nodes = std::min(max_nd, 512 / 6);  // 6 threads per node; 6x12 FP64 work areas (<= 48 KiB)
   const int vec_lds = fem_lds_bytes(lds_nodes, 1, max_nd) <= 64 * 1024 ? 1 : 0;
   const size_t lds_bytes = fem_lds_bytes(lds_nodes, vec_lds, max_nd);
+  // CG assembly lists in the block-inverse area: 2 nG staging + 12 nEL products + ints
+  int cg_lds = 1;
+  for (const FemBodyDesc &d : F.desc) {
+    const int nG = 6 * d.nND, nent = NED * d.nEL;
+    const size_t need = 2 * (size_t)nG + NED * (size_t)d.nEL + ((size_t)nG + 1 + nent + 1) / 2;
+    if (need > 72 * (size_t)lds_nodes) cg_lds = 0;
+  }
   const size_t n = (size_t)numsubstep * F.nbodies;
   if (n > S.cap) {
     if (S.h_stage) HIPCHK(hipHostFree(S.h_stage));
@@ -1476,7 +1483,7 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
   for (int isub = 0; isub < numsubstep; ++isub) {
     hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s,
-                       F.ddesc.p + (size_t)isub * F.nbodies, A, lds_nodes, vec_lds);
+                       F.ddesc.p + (size_t)isub * F.nbodies, A, lds_nodes, vec_lds, cg_lds);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipMemcpyAsync(S.h_stage, F.ddesc.p, n * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, S.s));

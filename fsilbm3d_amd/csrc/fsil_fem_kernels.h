@@ -43,35 +43,69 @@ struct FemArrays {
 
 __device__ inline void fem_sync() { __syncthreads(); }
 
+// s + t[0] + t[1] + ... + t[n-1], in that order; the terms are read sixteen
+// at a time into registers first, so the reads overlap and only the
+// additions form the serial chain
+__device__ inline double ordered_sum(double s, const double *t, int n) {
+  int i = 0;
+  for (; i + 16 <= n; i += 16) {
+    double b[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) b[j] = t[i + j];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) s = s + b[j];
+  }
+  for (; i < n; ++i) s = s + t[i];
+  return s;
+}
+
+
 // The conjugate-gradient vectors of one body: in LDS when they fit (the usual
 // case), else in the global work arrays; indexed by the body's DOF number.
 struct FemVecs {
   double *x, *r, *p, *z, *Ap, *blk;  // blk: the 6x6 inverse blocks, 36 per node
 };
 
+// LDS copy of a body's assembly lists for the CG: per DOF the range of its
+// (element, local index) entries, each packed as element*12 + index, and the
+// element products of the mat-vec (lbuf)
+struct CgLds {
+  double *lbuf;
+  const int *ptr, *ent;
+  bool on;
+};
+
 // element-by-element product out = A*v for one body (Beam_MatrixMultipy,
 // SolidSolver.f90:2035-2049); optionally zeroes the fixed DOFs of the result
 // and leaves the products v_g*out_g in prod (for the next dot product)
-__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const double *v, double *out,
-                                  bool zero_fixed = false, double *prod = nullptr) {
+__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const CgLds &C, const double *v,
+                                  double *out, bool zero_fixed = false, double *prod = nullptr) {
   const int nE = D.nEL, g0 = D.goff;
   for (int t = threadIdx.x; t < nE * NED; t += blockDim.x) {
     const int e = t / NED, i = t - e * NED;
     const BeamElem &E = A.el[D.eoff + e];
     const int n0 = 6 * E.node0, n1 = 6 * E.node1;
-    double s = 0.0;
+    double vv[NED], cc[NED];
 #pragma unroll
     for (int k = 0; k < NED; ++k) {
-      const int gk = k < 6 ? n0 + k : n1 + (k - 6);
-      s = s + E.coef[i * NED + k] * v[gk];
+      vv[k] = v[k < 6 ? n0 + k : n1 + (k - 6)];
+      cc[k] = E.coef[i * NED + k];
     }
-    A.lbuf[(size_t)(D.eoff + e) * NED + i] = s;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NED; ++k) s = s + cc[k] * vv[k];
+    if (C.on) C.lbuf[t] = s;
+    else A.lbuf[(size_t)(D.eoff + e) * NED + i] = s;
   }
   fem_sync();
   for (int g = threadIdx.x; g < 6 * D.nND; g += blockDim.x) {
     double s = 0.0;
-    for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
-      s = s + A.lbuf[(size_t)A.dof_e[k] * NED + A.dof_i[k]];
+    if (C.on) {
+      for (int k = C.ptr[g]; k < C.ptr[g + 1]; ++k) s = s + C.lbuf[C.ent[k]];
+    } else {
+      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
+        s = s + A.lbuf[(size_t)A.dof_e[k] * NED + A.dof_i[k]];
+    }
     if (zero_fixed && A.fixed[g0 + g]) s = 0.0;
     out[g] = s;
     if (prod) prod[g] = v[g] * s;
@@ -79,14 +113,17 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
   fem_sync();
 }
 
+// barrier of the lanes of one wave (they run in lockstep; the fences keep the
+// compiler from moving LDS accesses across it)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // sum of terms[0..n) in order by one thread, broadcast
 __device__ inline double fem_serial_sum(const double *terms, int n, double *shared) {
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < n; ++i) s = s + terms[i];
-    *shared = s;
-  }
+  if (threadIdx.x == 0) *shared = ordered_sum(0.0, terms, n);
   fem_sync();
   const double v = *shared;
   fem_sync();
@@ -117,17 +154,8 @@ __device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, con
 
 // the two ordered sums of prod[0..n) and prod[cap..cap+n) on two waves
 __device__ inline void fem_pair_sums(const double *prod, int cap, int n, double *shared, double &r1, double &r2) {
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < n; ++i) s = s + prod[i];
-    shared[0] = s;
-  } else if (threadIdx.x == 64) {
-    double s = 0.0;
-#pragma unroll 16
-    for (int i = 0; i < n; ++i) s = s + prod[cap + i];
-    shared[1] = s;
-  }
+  if (threadIdx.x == 0) shared[0] = ordered_sum(0.0, prod, n);
+  else if (threadIdx.x == 64) shared[1] = ordered_sum(0.0, prod + cap, n);
   fem_sync();
   r1 = shared[0];
   r2 = shared[1];
@@ -149,13 +177,8 @@ __device__ inline void fem_dot_chunks(const double *a, const double *b, const do
       if (c) stage[cap + i] = c[base + i] * d[base + i];
     }
     fem_sync();
-    if (threadIdx.x == 0) {
-#pragma unroll 16
-      for (int i = 0; i < m; ++i) s1 = s1 + stage[i];
-    } else if (c && threadIdx.x == 64) {
-#pragma unroll 16
-      for (int i = 0; i < m; ++i) s2 = s2 + stage[cap + i];
-    }
+    if (threadIdx.x == 0) s1 = ordered_sum(s1, stage, m);
+    else if (c && threadIdx.x == 64) s2 = ordered_sum(s2, stage + cap, m);
     fem_sync();
   }
   if (threadIdx.x == 0) shared[0] = s1;
@@ -187,7 +210,8 @@ __host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int 
   return d * sizeof(double);
 }
 
-__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArrays A, int lds_nodes, int vec_lds) {
+__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArrays A, int lds_nodes, int vec_lds,
+                                                   int cg_lds) {
   __shared__ double sh_val;
   __shared__ double sh_pair[2];
   __shared__ int sh_flag;
@@ -331,10 +355,14 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
     // :2234-2279): one 6x12 Gauss-Jordan work area per node in LDS, one thread
     // per row; each row operation is the sequential one, the rows of a pivot
     // step run side by side.
-    for (int base = 0; base < nN; base += lds_nodes) {
-      const int ln = (int)threadIdx.x / 6, row = (int)threadIdx.x - 6 * ln;
+    // ten nodes per wave (lanes 0..59), so the six row threads of a node are
+    // in one wave and step in lockstep: wave barriers between the phases
+    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int slots = min(((int)blockDim.x >> 6) * 10, lds_nodes);
+    for (int base = 0; base < nN; base += slots) {
+      const int ln = wv * 10 + lane / 6, row = lane - 6 * (lane / 6);
       const int n = base + ln;
-      const bool act = ln < lds_nodes && n < nN;
+      const bool act = lane < 60 && ln < slots && n < nN;
       double *aug = lds + 72 * ln;
       int *bad = reinterpret_cast<int *>(lds + 72 * lds_nodes) + ln;
       if (act) {
@@ -346,7 +374,7 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
         }
         if (row == 0) *bad = 0;
       }
-      fem_sync();
+      wave_sync();
       if (act && row == 0) {
         for (int i = 0; i < 6; ++i)
           if (A.fixed[g0 + 6 * n + i]) {
@@ -354,8 +382,9 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
             aug[i * 12 + i] = 1.0;
           }
       }
-      fem_sync();
+      wave_sync();
       if (act) aug[row * 12 + 6 + row] = 1.0;
+      wave_sync();
       for (int i = 0; i < 6; ++i) {
         if (act && row == 0 && !*bad) {
           int p = i;
@@ -368,32 +397,47 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
               const double t = aug[i * 12 + j]; aug[i * 12 + j] = aug[p * 12 + j]; aug[p * 12 + j] = t;
             }
         }
-        fem_sync();
+        wave_sync();
         if (act && row == i && !*bad) {
           const double pivot = aug[i * 12 + i];
           for (int j = 0; j < 12; ++j) aug[i * 12 + j] = aug[i * 12 + j] / pivot;
         }
-        fem_sync();
+        wave_sync();
         if (act && row != i && !*bad) {
           const double factor = aug[row * 12 + i];
           for (int j = 0; j < 12; ++j) aug[row * 12 + j] = aug[row * 12 + j] - factor * aug[i * 12 + j];
         }
-        fem_sync();
+        wave_sync();
       }
       if (act) {
         double *out = V.blk + (size_t)n * 36;
         for (int j = 0; j < 6; ++j) out[row * 6 + j] = *bad ? (row == j ? 1.0 : 0.0) : aug[row * 12 + 6 + j];
       }
-      fem_sync();
+      wave_sync();
     }
     fem_sync();
-    fem_matvec(A, D, V.x, V.Ap);
+    // the body's assembly lists and the mat-vec products into LDS (behind the
+    // dot-product staging area [0, 2 nG)) when they fit
+    CgLds C{};
+    if (cg_lds) {
+      C.lbuf = lds + 2 * nG;
+      int *ip = reinterpret_cast<int *>(lds + 2 * nG + NED * nE);
+      const int p0 = A.dof_ptr[g0], nent = A.dof_ptr[g0 + nG] - p0;
+      int *ep = ip + nG + 1;
+      for (int g = threadIdx.x; g <= nG; g += blockDim.x) ip[g] = A.dof_ptr[g0 + g] - p0;
+      for (int k = threadIdx.x; k < nent; k += blockDim.x) ep[k] = (A.dof_e[p0 + k] - D.eoff) * NED + A.dof_i[p0 + k];
+      C.ptr = ip;
+      C.ent = ep;
+      C.on = true;
+      fem_sync();
+    }
+    fem_matvec(A, D, C, V.x, V.Ap);
     for (int g = threadIdx.x; g < nG; g += blockDim.x)
       V.r[g] = A.fixed[g0 + g] ? 0.0 : A.lodEffe[g0 + g] - V.Ap[g];
     fem_sync();
     fem_precond(A, D, V);
     const double err = 1.0e-6;
-    const int cap = 36 * lds_nodes;
+    const int cap = nG;  // product staging: [0, nG) and [nG, 2 nG)
     double rr, rz;
     fem_dot2(V.r, V.r, V.r, V.z, nG, sh_pair, lds, cap, rr, rz);
     double rn = sqrt(rr);
@@ -405,7 +449,7 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
       for (int it = 1; it <= 10000; ++it) {
         ++cg_total;
         // A p with the fixed DOFs zeroed, and the products p_g (A p)_g
-        fem_matvec(A, D, V.p, V.Ap, true, lds);
+        fem_matvec(A, D, C, V.p, V.Ap, true, lds);
         const double pAp = fem_serial_sum(lds, nG, sh_pair);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
