@@ -79,6 +79,26 @@ struct DevBuf {
   }
 };
 
+// Pinned host copy of a coupling call's control words, written by k_ibm_report.
+struct PinnedReport {
+  IbmReport *h = nullptr, *d = nullptr;  // host pointer and its device address
+  PinnedReport() = default;
+  PinnedReport(const PinnedReport &) = delete;
+  PinnedReport &operator=(const PinnedReport &) = delete;
+  ~PinnedReport() {
+    if (h) (void)hipHostFree(h);
+  }
+  hipError_t ensure() {
+    if (h) return hipSuccess;
+    hipError_t e = hipHostMalloc((void **)&h, sizeof(IbmReport), hipHostMallocDefault);
+    if (e != hipSuccess) { h = nullptr; return e; }
+    memset(h, 0, sizeof(IbmReport));
+    e = hipHostGetDevicePointer((void **)&d, h, 0);
+    if (e != hipSuccess) { (void)hipHostFree(h); h = nullptr; d = nullptr; }
+    return e;
+  }
+};
+
 struct Ibm {
   int nbodies = 0;
   std::vector<int> nmark, nnode, nelem;      // per body
@@ -102,8 +122,7 @@ struct Ibm {
   DevBuf<double> band_u, band_F, dstate;
   DevBuf<char> cub_tmp;
   size_t cub_bytes = 0;
-  std::vector<int> h_ctl;
-  std::vector<double> h_dstate;
+  std::unique_ptr<PinnedReport> rep = std::make_unique<PinnedReport>();
   int ndof = 0;
   bool active = false;  // band valid for the next collision
   int warm_mlp = 0;     // network handle of the learned warm start (0: off)
@@ -1045,8 +1064,7 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
     I.cub_bytes = std::max(t1, t2);
     HIPCHK(I.cub_tmp.ensure(std::max<size_t>(1, I.cub_bytes)));
   }
-  I.h_ctl.assign(kCtlWords, 0);
-  I.h_dstate.assign(4, 0.0);
+  HIPCHK(I.rep->ensure());
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
@@ -1199,13 +1217,14 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                        I.lod.p);
     HIPCHK(hipGetLastError());
   }
-  HIPCHK(hipMemcpyAsync(I.h_ctl.data(), I.ctl.p, kCtlWords * sizeof(int), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipMemcpyAsync(I.h_dstate.data(), I.dstate.p, sizeof(double), hipMemcpyDeviceToHost, s));
+  hipLaunchKernelGGL(k_ibm_report, dim3(1), dim3(64), 0, s, I.ctl.p, I.dstate.p, I.rep->d);
+  HIPCHK(hipGetLastError());
   if (Eforce) HIPCHK(hipMemcpyAsync(Eforce, I.Eforce.p, 3 * (size_t)M * sizeof(double), hipMemcpyDeviceToHost, s));
   if (lodFlow && I.ndof > 0) HIPCHK(hipMemcpyAsync(lodFlow, I.lod.p, (size_t)I.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (fem_collect(*b)) return 1;  // results of the previous beam sub-steps
-  const int err = I.h_ctl[kErr];
+  const IbmReport &R = *I.rep->h;
+  const int err = R.ctl[kErr];
   if (err != 0x7fffffff) {
     if (err == 3) return fail("Nan found in PenaltyForce");
     const int m = err / 4 - 1, why = err % 4;
@@ -1215,10 +1234,22 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                 body + 1, why == 1 ? "min" : "max");
   }
   I.active = true;
-  I.last_warm = I.h_ctl[kWarm];
-  I.last_fallback = I.h_ctl[kFallback] != 0;
-  if (iters) *iters = I.h_ctl[kIters];
-  if (dmax) *dmax = I.h_dstate[0];
+  I.last_warm = R.ctl[kWarm];
+  I.last_fallback = R.ctl[kFallback] != 0;
+  if (iters) *iters = R.ctl[kIters];
+  if (dmax) *dmax = R.dmax;
+  return 0;
+}
+
+int fsil_ibm_forces(int h, double *Eforce, double *lodFlow) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Ibm &I = b->ibm;
+  if (I.M == 0) return 0;
+  if (Eforce) HIPCHK(hipMemcpyAsync(Eforce, I.Eforce.p, 3 * (size_t)I.M * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  if (lodFlow && I.ndof > 0)
+    HIPCHK(hipMemcpyAsync(lodFlow, I.lod.p, (size_t)I.ndof * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 
@@ -1435,6 +1466,7 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   Fem &F = b->fem;
   if (F.nbodies == 0) return fail("no FEM bodies defined");
   if (numsubstep < 1) return fail("numsubstep must be >= 1");
+  if (!F.fs || !F.fs->s) return fail("FEM bodies not fully defined (fsil_fem_define_bodies failed)");
   FemStream &S = *F.fs;
   if (fem_collect(*b)) return 1;  // the staging buffer is free once the previous call is done
   const double subdt = deltat / (double)numsubstep;
@@ -1481,11 +1513,9 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   HIPCHK(hipEventRecord(S.ev_in, b->stream));
   HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
   HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
-  for (int isub = 0; isub < numsubstep; ++isub) {
-    hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s,
-                       F.ddesc.p + (size_t)isub * F.nbodies, A, lds_nodes, vec_lds, cg_lds);
-    HIPCHK(hipGetLastError());
-  }
+  hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, F.ddesc.p, F.nbodies, numsubstep, A,
+                     lds_nodes, vec_lds, cg_lds);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(S.h_stage, F.ddesc.p, n * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, S.s));
   HIPCHK(hipEventRecord(S.ev_done, S.s));
   S.n_staged = (int)n;

@@ -210,13 +210,12 @@ __host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int 
   return d * sizeof(double);
 }
 
-__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArrays A, int lds_nodes, int vec_lds,
-                                                   int cg_lds) {
+// One sub-step of one body by the whole workgroup.
+__device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int lds_nodes, int vec_lds, int cg_lds) {
   __shared__ double sh_val;
   __shared__ double sh_pair[2];
   __shared__ int sh_flag;
   extern __shared__ double lds[];
-  FemBodyDesc &D = descs[blockIdx.x];
   const int nE = D.nEL, nN = D.nND, nG = 6 * nN, g0 = D.goff;
   FemVecs V;
   if (vec_lds) {
@@ -539,6 +538,20 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, FemArra
     D.dnorm = dnorm;
   }
   (void)sh_flag;
+}
+
+// numsubstep sub-steps of every body in one launch: workgroup b runs body b's
+// sub-steps in order, sub-step s from descs[s * nbodies + b].  The waves issue
+// at raised priority (s_setprio): the collide kernel's waves that share the CU
+// are HBM-bound and lose little, while this chain of dependent reductions
+// would otherwise wait behind them for every instruction.
+__global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, int nbodies, int nsub, FemArrays A,
+                                                   int lds_nodes, int vec_lds, int cg_lds) {
+  __builtin_amdgcn_s_setprio(3);
+  for (int isub = 0; isub < nsub; ++isub) {
+    fem_substep_body(descs[(size_t)isub * nbodies + blockIdx.x], A, lds_nodes, vec_lds, cg_lds);
+    fem_sync();
+  }
 }
 
 // PlateUpdatePosVelArea_ (Solidbody.f90:604-646): one thread per element,

@@ -434,6 +434,20 @@ __global__ __launch_bounds__(256) void k_loads(int ndof, const int *__restrict__
   lod[d] = acc;
 }
 
+// The control words and the last residual of a coupling call, written into
+// pinned host memory (read by the host after the stream's synchronisation).
+struct IbmReport {
+  int ctl[kCtlWords];
+  double dmax;
+};
+
+__global__ __launch_bounds__(64) void k_ibm_report(const int *__restrict__ ctl, const double *__restrict__ dstate,
+                                                   IbmReport *__restrict__ out) {
+  const int t = threadIdx.x;
+  if (t < kCtlWords) out->ctl[t] = ctl[t];
+  if (t == 0) out->dmax = dstate[0];
+}
+
 // ---------------------------------------------------------------------------
 // Learned warm start (no counterpart in the reference)
 

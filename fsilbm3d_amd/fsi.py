@@ -74,8 +74,9 @@ class DeviceBodyBlock:
         _lib.check(lib.fsil_ibm_set_params(block.handle, int(ntolLBM), float(dtolLBM), float(Uref), float(denIn)))
         self.nND = nND
         self.M = int(sum(int(np.asarray(p.elements)[:, 2].sum()) for p in plates))
-        self.Eforce = np.zeros((max(self.M, 1), 3))
-        self.lodFlow = np.zeros(6 * int(nND.sum()))
+        self._Eforce = np.zeros((max(self.M, 1), 3))
+        self._lodFlow = np.zeros(6 * int(nND.sum()))
+        self._stale = False
         self.iters = 0
         self.dmax = 0.0
 
@@ -84,9 +85,26 @@ class DeviceBodyBlock:
         it = np.zeros(1, dtype=np.int32)
         dm = np.zeros(1)
         _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
-                                                       _lib.dptr(self.Eforce), _lib.dptr(self.lodFlow),
-                                                       _lib.iptr(it), _lib.dptr(dm)))
+                                                       None, None, _lib.iptr(it), _lib.dptr(dm)))
         self.iters, self.dmax = int(it[0]), float(dm[0])
+        self._stale = True
+
+    def _fetch(self):
+        if self._stale:
+            _lib.check(self.lib.fsil_ibm_forces(self.block.handle, _lib.dptr(self._Eforce), _lib.dptr(self._lodFlow)))
+            self._stale = False
+
+    @property
+    def Eforce(self) -> np.ndarray:
+        """Marker forces of the last coupling call (read from the device on first use)."""
+        self._fetch()
+        return self._Eforce
+
+    @property
+    def lodFlow(self) -> np.ndarray:
+        """Beam loads of the last coupling call (read from the device on first use)."""
+        self._fetch()
+        return self._lodFlow
 
     def set_warm_start(self, net=None, record: bool = False):
         """Learned warm start of the direct-forcing loop (fsilbm3d_amd.mlp.WarmStartNet)."""

@@ -120,6 +120,12 @@ int fsil_ibm_interaction_force(int handle, double dt, const double *Exyz, const 
                                const double *Ea, const double *elem_xc, double *Eforce,
                                double *lodFlow, int *iters, double *dmax);
 
+/* v_Eforce and lodFlow of the last fsil_ibm_interaction_force (either may be
+ * NULL): the host reads of the bodies' marker forces and beam loads that the
+ * reference's writers make (Solidbody.f90:920-979 leaves them in the body
+ * records); the coupling call itself need not copy them out every step. */
+int fsil_ibm_forces(int handle, double *Eforce, double *lodFlow);
+
 /* Plate bodies on the device beam solver, carried by the block (read_solid_files,
  * Beam_ReadBuild, Beam_Initialise, calculate_angle_material with isKB = 1:
  * Solidbody.f90:73-216, SolidSolver.f90:1270-1446, 1533-1616).  Per body b:
