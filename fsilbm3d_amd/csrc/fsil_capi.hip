@@ -114,9 +114,10 @@ struct Ibm {
   int nlist = 0;
   // device: stencils and band (built each call, see fsil_ibm_kernels.h)
   int cap = 0;                               // 64*M, stride of the band arrays
-  int end_bit = 0;                           // key bits to sort
   DevBuf<float> dEw;                         // 12*M
-  DevBuf<unsigned long long> keys, keys_sorted;
+  DevBuf<unsigned> keys;                      // cell of each stencil entry
+  DevBuf<int> tmp;                            // entries in arrival order per band cell
+  DevBuf<int> blkbox;                         // per claim workgroup: its part of the band's box
   DevBuf<int> flags, scan, dentry, dbidx, dptr, ctl, cellmap;
   DevBuf<long long> dcell;
   DevBuf<double> band_u, band_F, dstate;
@@ -1024,13 +1025,12 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   if (upload(I.first_marker, first.data(), first.size(), b->stream)) return 1;
   const int E = 64 * I.M;
   I.cap = std::max(1, E);
-  int cbits = 1;
-  while (cbits < 63 && (1LL << cbits) < b->N) ++cbits;
-  I.end_bit = kKeyShift + cbits;
-  if (I.end_bit > 64) return fail("block too large for the band keys");
+  if (b->N > (1LL << 32)) return fail("block too large for the band keys (at most 2^32 cells)");
+  if (64LL * I.M >= (1LL << 31)) return fail("too many markers for the band (64 M must be below 2^31)");
   HIPCHK(I.dEw.ensure(12 * (size_t)std::max(1, I.M)));
   HIPCHK(I.keys.ensure(I.cap));
-  HIPCHK(I.keys_sorted.ensure(I.cap));
+  HIPCHK(I.tmp.ensure(I.cap));
+  HIPCHK(I.blkbox.ensure(6 * (size_t)((I.cap + kClaimThreads - 1) / kClaimThreads)));
   HIPCHK(I.flags.ensure(I.cap));
   HIPCHK(I.scan.ensure(I.cap));
   HIPCHK(I.dentry.ensure(I.cap));
@@ -1057,11 +1057,9 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   HIPCHK(hipMemsetAsync(I.ctl.p, 0, kCtlWords * sizeof(int), b->stream));
   HIPCHK(hipMemsetAsync(I.cellmap.p, 0xff, (size_t)b->N * sizeof(int), b->stream));  // -1
   if (E > 0) {
-    size_t t1 = 0, t2 = 0;
-    HIPCHK(hipcub::DeviceRadixSort::SortKeys(nullptr, t1, I.keys.p, I.keys_sorted.p, E, 0, I.end_bit,
-                                             b->stream));
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, t2, I.flags.p, I.scan.p, E, b->stream));
-    I.cub_bytes = std::max(t1, t2);
+    size_t t2 = 0;
+    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, t2, I.scan.p, I.dptr.p + 1, E, b->stream));
+    I.cub_bytes = t2;
     HIPCHK(I.cub_tmp.ensure(std::max<size_t>(1, I.cub_bytes)));
   }
   HIPCHK(I.rep->ensure());
@@ -1116,7 +1114,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   const int warm = (net && !done0) ? 1 : 0;
   hipLaunchKernelGGL(k_ibm_reset, dim3(std::min(1024, (E + 255) / 256)), dim3(256), 0, s, I.ctl.p, I.dcell.p,
-                     I.cellmap.p, I.dstate.p, done0, warm);
+                     I.cellmap.p, I.dstate.p, done0, warm, E, I.scan.p, I.dptr.p);
   HIPCHK(hipGetLastError());
   StencilArgs sa{};
   sa.M = M;
@@ -1130,16 +1128,24 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   sa.Ew = I.dEw.p;
   sa.keys = I.keys.p;
   sa.ctl = I.ctl.p;
-  hipLaunchKernelGGL(k_ibm_stencil, dim3((M + 255) / 256), dim3(256), 0, s, sa);
+  // band: cells bid for by the stencils and claimed, per-cell counts (I.scan) and slots
+  // (I.flags), CSR pointers by an inclusive scan into dptr[1..], entries
+  // placed in arrival order (I.tmp), then at their ranks by entry number
+  hipLaunchKernelGGL(k_ibm_stencil, dim3((M + 3) / 4), dim3(256), 0, s, sa);
+  hipLaunchKernelGGL(k_ibm_bid, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys.p, I.cellmap.p);
+  const int nclaim = (E + kClaimThreads - 1) / kClaimThreads;
+  hipLaunchKernelGGL(k_ibm_claim, dim3(nclaim), dim3(kClaimThreads), 0, s, E, I.keys.p, I.cellmap.p, I.dcell.p,
+                     b->yd, b->zd, I.blkbox.p, I.ctl.p);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_ibm_count, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys.p, I.cellmap.p, I.dbidx.p,
+                     I.scan.p, I.flags.p, I.ctl.p);
   HIPCHK(hipGetLastError());
   size_t tb = I.cub_bytes;
-  HIPCHK(hipcub::DeviceRadixSort::SortKeys(I.cub_tmp.p, tb, I.keys.p, I.keys_sorted.p, E, 0, I.end_bit, s));
-  hipLaunchKernelGGL(k_ibm_heads, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys_sorted.p, I.flags.p);
-  HIPCHK(hipGetLastError());
-  tb = I.cub_bytes;
-  HIPCHK(hipcub::DeviceScan::InclusiveSum(I.cub_tmp.p, tb, I.flags.p, I.scan.p, E, s));
-  hipLaunchKernelGGL(k_ibm_band, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys_sorted.p, I.scan.p,
-                     I.dentry.p, I.dbidx.p, I.dcell.p, I.dptr.p, I.cellmap.p, I.ctl.p);
+  HIPCHK(hipcub::DeviceScan::InclusiveSum(I.cub_tmp.p, tb, I.scan.p, I.dptr.p + 1, E, s));
+  hipLaunchKernelGGL(k_ibm_fill, dim3((E + 255) / 256), dim3(256), 0, s, E, I.dbidx.p, I.flags.p, I.dptr.p,
+                     I.tmp.p);
+  hipLaunchKernelGGL(k_ibm_rank, dim3((E + 255) / 256), dim3(256), 0, s, E, I.ctl.p, I.dbidx.p, I.dptr.p, I.tmp.p,
+                     I.dentry.p, I.blkbox.p, nclaim);
   HIPCHK(hipGetLastError());
   // calculate_interaction_force (Solidbody.f90:869-918)
   const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
@@ -1227,6 +1233,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   const int err = R.ctl[kErr];
   if (err != 0x7fffffff) {
     if (err == 3) return fail("Nan found in PenaltyForce");
+    if (err == 4) return fail("immersed-boundary band: a stencil cell without a band number");
     const int m = err / 4 - 1, why = err % 4;
     int body = 0;
     while (body + 1 < I.nbodies && m >= I.moff[body + 1]) ++body;
@@ -1454,7 +1461,7 @@ int fsil_fem_update_markers(int h) {
   a.body_of_elem = F.body_of_elem.p; a.elem_first_marker = F.elem_first_marker.p; a.body_goff = F.body_goff.p;
   a.beta = F.beta.p; a.nEL_total = F.nEL;
   a.Exyz = I.Exyz.p; a.Evel = I.Evel.p; a.Ea = I.Ea.p; a.elem_xc = I.elem_xc.p; a.Enrm = I.Enrm.p;
-  hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 255) / 256), dim3(256), 0, b->stream, a);
+  hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 3) / 4), dim3(256), 0, b->stream, a);
   HIPCHK(hipGetLastError());
   F.markers_on_device = true;
   return 0;

@@ -566,8 +566,11 @@ struct FemMarkerArgs {
   double *Enrm;         // marker normals (chord x span direction), for the warm start; may be null
 };
 
+// one wave per element: every lane forms the element's mid-axis state, lane
+// l writes span markers l, l+64, ...
 __global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int e = blockIdx.x * 4 + ((int)threadIdx.x >> 6);
   if (e >= a.nEL_total) return;
   const BeamElem &E = a.el[e];
   const int body = a.body_of_elem[e];
@@ -586,7 +589,7 @@ __global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
   if (nrm > 1.0e-12) for (int k = 0; k < 3; ++k) dirc[k] = dirc[k] / nrm;
   else for (int k = 0; k < 3; ++k) dirc[k] = E.tee[k * 3 + 1];
   const int m0 = a.elem_first_marker[e];
-  for (int s = 0; s < E.Nspan; ++s) {
+  for (int s = lane; s < E.Nspan; s += 64) {
     const double ls = dl * (0.5 + (double)s) - E.Lspan;
     double r[3], w[3];
     for (int k = 0; k < 3; ++k) r[k] = dirc[k] * ls;
@@ -606,10 +609,11 @@ __global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
     nv[1] = ax[2] * dirc[0] - ax[0] * dirc[2];
     nv[2] = ax[0] * dirc[1] - ax[1] * dirc[0];
     const double nn = sqrt(bdot3(nv, nv));
-    for (int s = 0; s < E.Nspan; ++s)
+    for (int s = lane; s < E.Nspan; s += 64)
       for (int k = 0; k < 3; ++k) a.Enrm[3LL * (m0 + s) + k] = nn > 0.0 ? nv[k] / nn : 0.0;
   }
-  for (int k = 0; k < 3; ++k) a.elem_xc[3LL * e + k] = 0.5 * (E.x1[k] + E.x1[6 + k]);
+  if (lane == 0)
+    for (int k = 0; k < 3; ++k) a.elem_xc[3LL * e + k] = 0.5 * (E.x1[k] + E.x1[6 + k]);
 }
 
 }  // namespace fsil

@@ -3,16 +3,23 @@
 // Data model.  Every marker m owns 64 stencil entries e = 64*m + s with
 // s = 16*ix + 4*iy + iz in the reference's x-outer, y, z-inner loop order
 // (Solidbody.f90:1009-1015).  The cells those entries touch form the "band".
-// Entries are sorted by (cell, e), which is the order in which the reference's
-// serial loops over markers (Solidbody.f90:1034-1048, 968-977) update a cell;
-// a per-cell sequential reduction over its sorted entries therefore rounds
-// exactly as the reference does, with no atomics.
+// Each band cell keeps its entries in ascending e, which is the order in
+// which the reference's serial loops over markers (Solidbody.f90:1034-1048,
+// 968-977) update a cell; a per-cell sequential reduction over its entries
+// therefore rounds exactly as the reference does, with no atomics on values.
 //
-// The whole structure is built on the device each coupling call: stencils
-// (k_ibm_stencil), a radix sort of the 64-bit keys (cell << 26 | e), and the
-// band assembly (k_ibm_band).  Band arrays have a fixed stride `cap` = 64*M
-// (an upper bound of the band size) and the band size itself lives in device
-// memory (ctl[kNband]), so nothing is read back between the kernels.
+// The whole structure is built on the device each coupling call, with no
+// global sort: k_ibm_stencil (one wave per marker) forms the stencil and
+// every entry bids for its cell in the cell map; k_ibm_claim gives each
+// cell's winning entry a band number (from an atomic counter, so the band's
+// order varies from call to call while every per-cell result does not);
+// k_ibm_count gives every entry its band cell and a slot; an inclusive scan
+// of the per-cell counts makes the CSR pointers; k_ibm_fill places the
+// entries in arrival order and k_ibm_rank moves each to its rank by e among
+// its cell's entries.  Band arrays
+// have a fixed stride `cap` = 64*M (an upper bound of the band size) and the
+// band size itself lives in device memory (ctl[kNband]), so nothing is read
+// back between the kernels.
 #pragma once
 #include "fsil_common.h"
 #include "fsil_lbm_kernels.h"
@@ -21,8 +28,8 @@ namespace fsil {
 
 // control words of the coupling kept in device memory
 enum IbmCtl {
-  kBoxLo = 0,      // [0..2] lowest band cell coordinate per axis (atomic min)
-  kBoxHi = 3,      // [3..5] highest band cell coordinate per axis (atomic max)
+  kBoxLo = 0,      // [0..2] lowest band cell coordinate per axis (the fluid kernels look
+  kBoxHi = 3,      // [3..5] highest       up the cell map only inside this box)
   kNband = 6,      // number of band cells
   kErr = 7,        // first failing marker (1-based) * 4 + reason, INT_MAX when none
   kDone = 8,       // multi-direct-forcing loop finished
@@ -30,9 +37,9 @@ enum IbmCtl {
   kWarm = 10,      // learned warm start in use for this call
   kFallback = 11,  // 1: the warm start did not lower the residual, restart from zero; 2: restarted
   kDone0 = 12,     // the done word at the start of a loop (no iteration at all)
+  kNew = 13,       // band cells claimed so far in this call (becomes kNband)
   kCtlWords = 16
 };
-constexpr int kKeyShift = 26;  // e = 64*m + s < 2^26
 
 // 4-point Peskin kernel (Solidbody.f90:822-833), FP64; sqrt is correctly rounded
 __host__ __device__ inline double peskin4(double x) {
@@ -67,16 +74,19 @@ __host__ __device__ inline int trim4(int i, int nd, int *out, int bclo, int bchi
   return err;
 }
 
-// reset of the control words; clears the previous band from the cell map
+// reset of the control words; clears the previous band from the cell map and
+// the per-cell entry counts (E of them) of this call
 __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const long long *__restrict__ prev_cell,
                                                    int *__restrict__ cellmap, double *__restrict__ dstate,
-                                                   int done0, int warm) {
+                                                   int done0, int warm, int E, int *__restrict__ count,
+                                                   int *__restrict__ band_ptr) {
   const int nprev = ctl[kNband];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nprev; i += gridDim.x * blockDim.x)
     cellmap[prev_cell[i]] = -1;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) count[i] = 0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctl[kBoxLo] = ctl[kBoxLo + 1] = ctl[kBoxLo + 2] = 0x7fffffff;
-    ctl[kBoxHi] = ctl[kBoxHi + 1] = ctl[kBoxHi + 2] = -1;
+    band_ptr[0] = 0;
+    ctl[kNew] = 0;
     ctl[kErr] = 0x7fffffff;
     ctl[kDone] = done0;
     ctl[kIters] = 0;
@@ -96,19 +106,21 @@ struct StencilArgs {
   int nd[3];
   int bc[6];
   float *__restrict__ Ew;                // 12*M
-  unsigned long long *__restrict__ keys; // 64*M
+  unsigned *__restrict__ keys;           // 64*M: cell of entry e = 64*m + s
   int *__restrict__ ctl;
 };
 
-// UpdateElmtInterp_ (Solidbody.f90:760-821) per marker: anchor from the
-// body's first marker, trimmed 4x3 indices, FP32 weights, and the 64 keys.
+// UpdateElmtInterp_ (Solidbody.f90:760-821), one wave per marker: every lane
+// forms the marker's anchor (from its body's first marker) and trimmed 4x3
+// indices, lanes 0..11 write the FP32 weights, lane s the cell of entry s.
 __global__ __launch_bounds__(256) void k_ibm_stencil(StencilArgs a) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = threadIdx.x & 63;
+  const int m = blockIdx.x * 4 + ((int)threadIdx.x >> 6);
   if (m >= a.M) return;
   const int m0 = a.first_marker[m];
-  int ix[12];
+  const int sel[3] = {s >> 4, (s >> 2) & 3, s & 3};
+  int cidx[3];
   int err = 0;
-  int lo[3], hi[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const int ii = (int)floor((a.Exyz[3LL * m0 + k] - a.org[k]) * a.invdh);
@@ -117,66 +129,155 @@ __global__ __launch_bounds__(256) void k_ibm_stencil(StencilArgs a) {
     int idx = (int)floor(off);
     off = off - (double)idx;
     idx = idx + ii + 1;
-    const int e = trim4(idx, a.nd[k], &ix[4 * k], a.bc[2 * k], a.bc[2 * k + 1]);
+    int ix[4];
+    const int e = trim4(idx, a.nd[k], ix, a.bc[2 * k], a.bc[2 * k + 1]);
     if (e && !err) err = e;
-    lo[k] = hi[k] = ix[4 * k];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      a.Ew[12LL * m + 4 * k + j] = (float)peskin4((double)(j - 1) - off);
-      lo[k] = min(lo[k], ix[4 * k + j]);
-      hi[k] = max(hi[k], ix[4 * k + j]);
-    }
+    cidx[k] = ix[sel[k]];
+    if ((s >> 2) == k) a.Ew[12LL * m + s] = (float)peskin4((double)((s & 3) - 1) - off);
   }
-  if (err) atomicMin(&a.ctl[kErr], (m + 1) * 4 + err);
+  if (s == 0 && err) atomicMin(&a.ctl[kErr], (m + 1) * 4 + err);
+  a.keys[64LL * m + s] =
+      ((unsigned)(cidx[0] - 1) * (unsigned)a.nd[1] + (unsigned)(cidx[1] - 1)) * (unsigned)a.nd[2] +
+      (unsigned)(cidx[2] - 1);
+}
+
+// Cell-map accesses of the band build go through device-scope atomics (the
+// workgroups that bid for, claim and look up one cell run on different XCDs).
+__device__ __forceinline__ int cellmap_load(const int *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// every entry bids -(e+2) for its cell: the cell map keeps the lowest bid, so
+// the cell's highest entry wins (cells outside the band hold -1)
+__global__ __launch_bounds__(256) void k_ibm_bid(int E, const unsigned *__restrict__ keys, int *__restrict__ cellmap) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < E) atomicMin(&cellmap[keys[e]], -(e + 2));
+}
+
+// the winning entry of each cell takes a band number and writes it into the
+// cell map.  One 1024-thread workgroup makes one atomic add on the band
+// counter (atomics on one line from many workgroups serialize), and leaves
+// its part of the band's bounding box in blkbox[6 * workgroup ..] for
+// k_ibm_rank.
+constexpr int kClaimThreads = 1024;
+__global__ __launch_bounds__(kClaimThreads) void k_ibm_claim(int E, const unsigned *__restrict__ keys,
+                                                             int *__restrict__ cellmap,
+                                                             long long *__restrict__ band_cell, int ny, int nz,
+                                                             int *__restrict__ blkbox, int *__restrict__ ctl) {
+  constexpr int NW = kClaimThreads / 64;
+  __shared__ int wave_won[NW], wave_off[NW], blk_base, wbox[NW][6];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = threadIdx.x & 63, w = (int)threadIdx.x >> 6;
+  const unsigned cell = e < E ? keys[e] : 0u;
+  const bool won = e < E && cellmap_load(&cellmap[cell]) == -(e + 2);
+  const unsigned long long mask = __ballot(won);
+  int v[3];
+  {
+    const unsigned plane = (unsigned)ny * (unsigned)nz;
+    const unsigned x = cell / plane, r = cell - x * plane, y = r / (unsigned)nz;
+    v[0] = (int)x;
+    v[1] = (int)y;
+    v[2] = (int)(r - y * (unsigned)nz);
+  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    atomicMin(&a.ctl[kBoxLo + k], lo[k] - 1);
-    atomicMax(&a.ctl[kBoxHi + k], hi[k] - 1);
+    int lo = won ? v[k] : 0x7fffffff, hi = won ? v[k] : -1;
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = min(lo, __shfl_xor(lo, off, 64));
+      hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if (s == 0) {
+      wbox[w][k] = lo;
+      wbox[w][3 + k] = hi;
+    }
   }
-  unsigned long long *K = a.keys + 64LL * m;
-  for (int x = 0; x < 4; ++x)
-    for (int y = 0; y < 4; ++y)
-#pragma unroll
-      for (int z = 0; z < 4; ++z) {
-        const unsigned long long cell =
-            ((unsigned long long)(ix[x] - 1) * a.nd[1] + (unsigned long long)(ix[4 + y] - 1)) * a.nd[2] +
-            (unsigned long long)(ix[8 + z] - 1);
-        const int e = 64 * m + 16 * x + 4 * y + z;
-        K[16 * x + 4 * y + z] = (cell << kKeyShift) | (unsigned long long)e;
-      }
+  if (s == 0) wave_won[w] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int k = 0; k < NW; ++k) {
+      wave_off[k] = t;
+      t += wave_won[k];
+    }
+    blk_base = t ? atomicAdd(&ctl[kNew], t) : 0;
+  }
+  if (threadIdx.x < 6) {
+    const int k = threadIdx.x;
+    int r = wbox[0][k];
+    for (int q = 1; q < NW; ++q) r = k < 3 ? min(r, wbox[q][k]) : max(r, wbox[q][k]);
+    blkbox[6 * blockIdx.x + k] = r;
+  }
+  __syncthreads();
+  if (won) {
+    const int b = blk_base + wave_off[w] + __popcll(mask & ((1ULL << s) - 1ULL));
+    band_cell[b] = (long long)cell;
+    atomicExch(&cellmap[cell], b);
+  }
 }
 
-// head flags of the sorted keys (first entry of each band cell)
-__global__ __launch_bounds__(256) void k_ibm_heads(int E, const unsigned long long *__restrict__ keys,
-                                                   int *__restrict__ flag) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= E) return;
-  flag[k] = (k == 0 || (keys[k] >> kKeyShift) != (keys[k - 1] >> kKeyShift)) ? 1 : 0;
-}
-
-// band assembly from the sorted keys and the inclusive scan of the heads
-__global__ __launch_bounds__(256) void k_ibm_band(int E, const unsigned long long *__restrict__ keys,
-                                                  const int *__restrict__ scan, int *__restrict__ entry,
-                                                  int *__restrict__ bidx, long long *__restrict__ band_cell,
-                                                  int *__restrict__ band_ptr, int *__restrict__ cellmap,
-                                                  int *__restrict__ ctl) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= E) return;
-  const unsigned long long key = keys[k];
-  const long long cell = (long long)(key >> kKeyShift);
-  const int e = (int)(key & ((1ULL << kKeyShift) - 1));
-  const int b = scan[k] - 1;
-  entry[k] = e;
+// every entry: its band cell and a slot among the cell's entries (the slots
+// are in arrival order; k_ibm_rank restores entry order).  A cell without a
+// band number would be an inconsistent build: reported in ctl[kErr] (code 4),
+// the entry skipped.
+__global__ __launch_bounds__(256) void k_ibm_count(int E, const unsigned *__restrict__ keys,
+                                                   const int *__restrict__ cellmap, int *__restrict__ bidx,
+                                                   int *__restrict__ count, int *__restrict__ slot,
+                                                   int *__restrict__ ctl) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const int b = cellmap_load(&cellmap[keys[e]]);
+  if (b < 0 || b >= E) {
+    atomicMin(&ctl[kErr], 4);
+    bidx[e] = 0;
+    slot[e] = 0;
+    return;
+  }
   bidx[e] = b;
-  if (k == 0 || (keys[k - 1] >> kKeyShift) != (key >> kKeyShift)) {
-    band_cell[b] = cell;
-    band_ptr[b] = k;
-    cellmap[cell] = b;
+  slot[e] = atomicAdd(&count[b], 1);
+}
+
+// entries into their cells' CSR ranges in slot order (band_ptr: inclusive
+// scan of the counts, shifted by one)
+__global__ __launch_bounds__(256) void k_ibm_fill(int E, const int *__restrict__ bidx,
+                                                  const int *__restrict__ slot, const int *__restrict__ band_ptr,
+                                                  int *__restrict__ tmp) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  tmp[band_ptr[bidx[e]] + slot[e]] = e;
+}
+
+// every entry at its rank in its cell's range (the number of the cell's
+// entries below it), so each cell lists its entries in ascending e; the band
+// size is published, and workgroup 0 reduces the claim workgroups' boxes
+// (nbox of them) into ctl[kBoxLo..kBoxHi+2]
+__global__ __launch_bounds__(256) void k_ibm_rank(int E, int *__restrict__ ctl, const int *__restrict__ bidx,
+                                                  const int *__restrict__ band_ptr, const int *__restrict__ tmp,
+                                                  int *__restrict__ entry, const int *__restrict__ blkbox,
+                                                  int nbox) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e == 0) ctl[kNband] = ctl[kNew];
+  if (blockIdx.x == 0) {
+    __shared__ int red[6][256];
+    int r[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
+    for (int q = threadIdx.x; q < nbox; q += blockDim.x)
+      for (int k = 0; k < 6; ++k) r[k] = k < 3 ? min(r[k], blkbox[6 * q + k]) : max(r[k], blkbox[6 * q + k]);
+    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = r[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if ((int)threadIdx.x < h)
+        for (int k = 0; k < 6; ++k)
+          red[k][threadIdx.x] = k < 3 ? min(red[k][threadIdx.x], red[k][threadIdx.x + h])
+                                      : max(red[k][threadIdx.x], red[k][threadIdx.x + h]);
+      __syncthreads();
+    }
+    if (threadIdx.x < 6) ctl[kBoxLo + threadIdx.x] = red[threadIdx.x][0];
   }
-  if (k == E - 1) {
-    band_ptr[b + 1] = E;
-    ctl[kNband] = b + 1;
-  }
+  if (e >= E) return;
+  const int b = bidx[e];
+  const int p0 = band_ptr[b], n = band_ptr[b + 1] - p0;
+  int r = 0;
+  for (int j = 0; j < n; ++j) r += tmp[p0 + j] < e ? 1 : 0;
+  entry[p0 + r] = e;
 }
 
 // band macros: den/u of every band cell from the stored state, same
@@ -244,6 +345,47 @@ __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = v + __shfl_down(v, off, 64);
   return v;
+}
+
+// sum of tl[0..M) in a fixed order by one 256-thread workgroup: strided
+// partial sums, a wave tree, then the four wave sums in order
+__device__ inline double mdf_tl_sum(const double *tl, int M) {
+  double p = 0.0;
+  for (int m = threadIdx.x; m < M; m += blockDim.x) p = p + tl[m];
+  __shared__ double part[4];
+  p = wave_sum(p);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = p;
+  __syncthreads();
+  double tol = 0.0;
+  for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) tol = tol + part[k];
+  return tol;
+}
+
+// stop test of the multi-direct-forcing loop (Solidbody.f90:895-906) from the
+// summed |dV|_1 of the markers: dmax = tol / (sum M_b * Uref), one iteration
+// counted; sets the done word when the loop condition fails
+__device__ inline void mdf_decide(int *ctl, double *dstate, double tol, double dsum, double Uref, int ntol,
+                                  double dtol) {
+  const int it = ctl[kIters] + 1;
+  ctl[kIters] = it;
+  if (!isfinite(tol)) {
+    atomicMin(&ctl[kErr], 3);  // "Nan found in PenaltyForce" (Solidbody.f90:1028-1031)
+    ctl[kDone] = 1;
+    return;
+  }
+  const double dm = tol / (dsum * Uref);
+  dstate[0] = dm;
+  if (it == 1 && ctl[kWarm] == 1 && !(dm < dstate[1])) {
+    // the guessed force did not lower the residual below the zero guess's:
+    // restart the loop from zero (k_ws_fallback, k_band_macro on fallback)
+    ctl[kWarm] = 0;
+    ctl[kFallback] = 1;
+    ctl[kIters] = 0;
+    ctl[kDone] = ctl[kDone0];
+    dstate[0] = 1e10;
+    return;
+  }
+  if (!(it < ntol && dm > dtol)) ctl[kDone] = 1;
 }
 
 // One marker per wave: each lane forms one of the 64 interpolation terms
@@ -321,43 +463,14 @@ __global__ __launch_bounds__(256) void k_correct(const int *__restrict__ ctl, in
   band_u[2LL * cap + b] = u3;
 }
 
-// stop test of the multi-direct-forcing loop (Solidbody.f90:895-906): the
-// markers' |dV|_1 summed in a fixed order (one workgroup: strided partial sums,
-// then a wave and a block tree), dmax = tol / (sum M_b * Uref), one iteration
-// counted; sets the done word when the loop condition fails.
+// stop test of a forcing round: the markers' |dV|_1 summed in a fixed order
+// by one workgroup, then mdf_decide
 __global__ __launch_bounds__(256) void k_mdf_check(int *__restrict__ ctl, double *__restrict__ dstate,
                                                    const double *__restrict__ tl, int M, double dsum, double Uref,
                                                    int ntol, double dtol) {
   if (ctl[kDone]) return;
-  double p = 0.0;
-  for (int m = threadIdx.x; m < M; m += blockDim.x) p = p + tl[m];
-  __shared__ double part[4];
-  p = wave_sum(p);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = p;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double tol = 0.0;
-  for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) tol = tol + part[k];
-  const int it = ctl[kIters] + 1;
-  ctl[kIters] = it;
-  if (!isfinite(tol)) {
-    atomicMin(&ctl[kErr], 3);  // "Nan found in PenaltyForce" (Solidbody.f90:1028-1031)
-    ctl[kDone] = 1;
-    return;
-  }
-  const double dm = tol / (dsum * Uref);
-  dstate[0] = dm;
-  if (it == 1 && ctl[kWarm] == 1 && !(dm < dstate[1])) {
-    // the guessed force did not lower the residual below the zero guess's:
-    // restart the loop from zero (k_ws_fallback, k_band_macro on fallback)
-    ctl[kWarm] = 0;
-    ctl[kFallback] = 1;
-    ctl[kIters] = 0;
-    ctl[kDone] = ctl[kDone0];
-    dstate[0] = 1e10;
-    return;
-  }
-  if (!(it < ntol && dm > dtol)) ctl[kDone] = 1;
+  const double tol = mdf_tl_sum(tl, M);
+  if (threadIdx.x == 0) mdf_decide(ctl, dstate, tol, dsum, Uref, ntol, dtol);
 }
 
 // Eulerian force of every band cell: (0 - sum F/h^3*rx*ry*rz) + VF
@@ -500,16 +613,8 @@ __global__ __launch_bounds__(256) void k_ws_features(int M, const int *__restric
 // dstate[1] = the zero guess's residual, reduced as k_mdf_check reduces
 __global__ __launch_bounds__(256) void k_ws_dm0(const double *__restrict__ tl0, int M, double dsum, double Uref,
                                                 double *__restrict__ dstate) {
-  double p = 0.0;
-  for (int m = threadIdx.x; m < M; m += blockDim.x) p = p + tl0[m];
-  __shared__ double part[4];
-  p = wave_sum(p);
-  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = p;
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  double tol = 0.0;
-  for (int k = 0; k < (int)(blockDim.x + 63) / 64; ++k) tol = tol + part[k];
-  dstate[1] = tol / (dsum * Uref);
+  const double tol = mdf_tl_sum(tl0, M);
+  if (threadIdx.x == 0) dstate[1] = tol / (dsum * Uref);
 }
 
 // the network's guess as the starting force: Eforce = F0, fT = F0 * invh3

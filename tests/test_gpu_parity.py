@@ -124,3 +124,45 @@ def test_stencil_out_of_bounds_is_an_error():
                     plates=[Plate(3, -1.0, 1.0, 1.0, 2, XYZo=(0.6, 6.0, 6.0))])
     with pytest.raises(FsilError, match="out of min bound"):
         run_device(pc)
+
+
+@pytest.mark.parametrize("name", ["ibm_heaving_plate", "ibm_pitching_two_plates"])
+def test_band_size_and_box_match_oracle_stencils(name):
+    """fsil_ibm_band_info after a coupling call: the band is the set of cells
+    the markers' trimmed 4x4x4 stencils touch (UpdateElmtInterp_,
+    Solidbody.f90:760-821), counted once each, and its bounding box."""
+    from fsilbm3d_amd import _lib
+    from fsilbm3d_amd.block import LBMBlock
+    from fsilbm3d_amd.fsi import FSIBlock
+    from oracle import oracle as O
+    from tests.helpers import derived, flow_kwargs, make_bodies
+    pc = BY_NAME[name]
+    case = pc.case
+    d = derived(case)
+    _say(f"[gpu] band info {name}")
+    blk = LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params)
+    blk.set_flow(**flow_kwargs(case))
+    blk.upload(np.array(pc.initial_state(), dtype=np.float64))
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    bodies = make_bodies(pc, case.denIn)
+    fsi = FSIBlock(blk, bodies, ntolLBM=case.ntolLBM, dtolLBM=case.dtolLBM, Uref=d["Uref"], denIn=case.denIn)
+    fsi.interaction_force(case.dh)
+    nb = np.zeros(1, np.int32)
+    box = np.zeros(6, np.int32)
+    _lib.check(blk.lib.fsil_ibm_band_info(blk.handle, _lib.iptr(nb), _lib.iptr(box)))
+    cells = set()
+    geom4 = (case.dh,) + tuple(case.origin)
+    for b in bodies:
+        Ei, _ = O.stencil(b.Exyz, case.dims, geom4, case.bc)
+        Ei = Ei.reshape(-1, 3, 4) - 1
+        for row in Ei:
+            for x in row[0]:
+                for y in row[1]:
+                    for z in row[2]:
+                        cells.add((int(x), int(y), int(z)))
+    blk.close()
+    c = np.array(sorted(cells))
+    lo, hi = c.min(axis=0), c.max(axis=0)
+    assert int(nb[0]) == len(cells)
+    assert list(box) == [*lo.tolist(), *(hi - lo + 1).tolist()]
