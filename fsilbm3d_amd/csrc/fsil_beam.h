@@ -28,6 +28,7 @@ struct BeamElem {
   double stl[NED * NED];  // local stiffness: depends on the constant properties and len0 only
   double rot[9];
   double fint[NED], fload[NED], fdamp[NED];  // per-element contributions (internal force, loads)
+  double gsc[9];  // geometric-stiffness coefficients gy1..gy4, gz1..gz4, gt (el_geom_scalars)
 };
 
 __host__ __device__ inline double bpow(double x, int n) {
@@ -243,39 +244,61 @@ __host__ __device__ inline void el_form_stiff(BeamElem &e) {
 #undef KK
 }
 
-// Segment_FormGeomMatrix (SolidSolver.f90:480-600)
-__host__ __device__ inline void el_form_geom(BeamElem &e) {
+// Segment_FormGeomMatrix (SolidSolver.f90:480-600): the nine distinct
+// coefficients of the geometric stiffness, into e.gsc
+__host__ __device__ inline void el_geom_scalars(BeamElem &e) {
   const double s = e.geoFRM;
   const double emod = e.prop[0], gmod = e.prop[1], area = e.prop[2], zix = e.prop[5], ziy = e.prop[6],
                ziz = e.prop[7];
   const double L = e.len0;
-  double *G = e.geo;
-  for (int k = 0; k < NED * NED; ++k) G[k] = 0.0;
   const double ksy = 5.0 / 6.0, ksz = 5.0 / 6.0;
   const double phiy = 12.0 * emod * ziz / (ksy * gmod * area * L * L);
   const double phiz = 12.0 * emod * ziy / (ksz * gmod * area * L * L);
   const double py2 = bpow(1.0 + phiy, 2), pz2 = bpow(1.0 + phiz, 2);
-  const double gy1 = s / L * (6.0 / 5.0 + 2.0 * phiy + phiy * phiy) / py2;
-  const double gy2 = s / L * (L / 10.0) / py2;
-  const double gy3 = s / L * (2.0 * L * L / 15.0 + phiy * L * L / 6.0 + phiy * phiy * L * L / 12.0) / py2;
-  const double gy4 = s / L * (-(L * L / 30.0) - phiy * L * L / 6.0 - phiy * phiy * L * L / 12.0) / py2;
-#define GG(i, j) G[(i) * NED + (j)]
-  GG(1, 1) = gy1; GG(5, 5) = gy3; GG(7, 7) = gy1; GG(11, 11) = gy3;
-  GG(1, 5) = gy2; GG(1, 7) = -gy1; GG(1, 11) = gy2; GG(5, 7) = -gy2; GG(5, 11) = gy4; GG(7, 11) = -gy2;
-  const double gz1 = s / L * (6.0 / 5.0 + 2.0 * phiz + phiz * phiz) / pz2;
-  const double gz2 = s / L * (L / 10.0) / pz2;
-  const double gz3 = s / L * (2.0 * L * L / 15.0 + phiz * L * L / 6.0 + phiz * phiz * L * L / 12.0) / pz2;
-  const double gz4 = s / L * (-(L * L / 30.0) - phiz * L * L / 6.0 - phiz * phiz * L * L / 12.0) / pz2;
-  GG(2, 2) = gz1; GG(4, 4) = gz3; GG(8, 8) = gz1; GG(10, 10) = gz3;
-  GG(2, 4) = -gz2; GG(2, 8) = -gz1; GG(2, 10) = -gz2; GG(4, 8) = gz2; GG(4, 10) = gz4; GG(8, 10) = gz2;
-  const double gt = s * zix / (area * L);
-  GG(3, 3) = gt; GG(3, 9) = -gt; GG(9, 9) = gt;
-  GG(5, 1) = GG(1, 5); GG(7, 1) = GG(1, 7); GG(11, 1) = GG(1, 11); GG(7, 5) = GG(5, 7); GG(11, 5) = GG(5, 11);
-  GG(11, 7) = GG(7, 11);
-  GG(4, 2) = GG(2, 4); GG(8, 2) = GG(2, 8); GG(10, 2) = GG(2, 10); GG(8, 4) = GG(4, 8); GG(10, 4) = GG(4, 10);
-  GG(10, 8) = GG(8, 10);
-  GG(9, 3) = GG(3, 9);
-#undef GG
+  double *g = e.gsc;
+  g[0] = s / L * (6.0 / 5.0 + 2.0 * phiy + phiy * phiy) / py2;
+  g[1] = s / L * (L / 10.0) / py2;
+  g[2] = s / L * (2.0 * L * L / 15.0 + phiy * L * L / 6.0 + phiy * phiy * L * L / 12.0) / py2;
+  g[3] = s / L * (-(L * L / 30.0) - phiy * L * L / 6.0 - phiy * phiy * L * L / 12.0) / py2;
+  g[4] = s / L * (6.0 / 5.0 + 2.0 * phiz + phiz * phiz) / pz2;
+  g[5] = s / L * (L / 10.0) / pz2;
+  g[6] = s / L * (2.0 * L * L / 15.0 + phiz * L * L / 6.0 + phiz * phiz * L * L / 12.0) / pz2;
+  g[7] = s / L * (-(L * L / 30.0) - phiz * L * L / 6.0 - phiz * phiz * L * L / 12.0) / pz2;
+  g[8] = s * zix / (area * L);
+}
+
+// Pattern of Segment_FormGeomMatrix: entry (i, j) is 0 (zero), k (coefficient
+// k-1) or -k (its negative); symmetric.
+struct GeoPattern {
+  signed char c[NED * NED];
+  constexpr GeoPattern() : c() {
+    const int e[23][3] = {{1, 1, 1},   {7, 7, 1},   {5, 5, 3},  {11, 11, 3}, {1, 5, 2},  {1, 11, 2},
+                          {1, 7, -1},  {5, 7, -2},  {7, 11, -2}, {5, 11, 4}, {2, 2, 5},  {8, 8, 5},
+                          {4, 4, 7},   {10, 10, 7}, {2, 4, -6},  {2, 10, -6}, {2, 8, -5}, {4, 8, 6},
+                          {8, 10, 6},  {4, 10, 8},  {3, 3, 9},   {9, 9, 9},   {3, 9, -9}};
+    for (int q = 0; q < 23; ++q) {
+      c[e[q][0] * NED + e[q][1]] = (signed char)e[q][2];
+      c[e[q][1] * NED + e[q][0]] = (signed char)e[q][2];
+    }
+  }
+};
+__device__ constexpr GeoPattern kGeoPatternD{};
+constexpr GeoPattern kGeoPatternH{};
+
+// entry (i, j) of the geometric stiffness from its coefficients g
+__host__ __device__ inline double geo_entry(const double *g, int i, int j) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const int k = kGeoPatternD.c[i * NED + j];
+#else
+  const int k = kGeoPatternH.c[i * NED + j];
+#endif
+  return k == 0 ? 0.0 : (k > 0 ? g[k - 1] : -g[-k - 1]);
+}
+
+__host__ __device__ inline void el_form_geom(BeamElem &e) {
+  el_geom_scalars(e);
+  for (int i = 0; i < NED; ++i)
+    for (int j = 0; j < NED; ++j) e.geo[i * NED + j] = geo_entry(e.gsc, i, j);
 }
 
 // RotateMatrix + RKR (SolidSolver.f90:699-748): ek <- R^T ek R by 3x3 blocks

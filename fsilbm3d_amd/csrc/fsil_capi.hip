@@ -175,6 +175,8 @@ struct Fem {
       beta;
   DevBuf<int> fixed, dof_ptr, dof_e, dof_i, node_ptr, node_e, node_h, body_of_elem, elem_first_marker, body_goff;
   bool markers_on_device = false;
+  size_t lds_limit = 0;  // the device's LDS bytes per workgroup
+  size_t lds_set = 64 * 1024;  // dynamic LDS bytes k_fem_substep may take
 };
 
 // A stream shared by the blocks of one tree (father and sons run in order).
@@ -1431,6 +1433,12 @@ int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, c
   HIPCHK(F.blk.ensure((size_t)F.nND * 36));
   F.fs = std::make_unique<FemStream>();
   HIPCHK(hipStreamCreateWithFlags(&F.fs->s, hipStreamNonBlocking));
+  {
+    int dev = 0, lim = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    F.lds_limit = (size_t)lim;
+  }
   HIPCHK(hipEventCreateWithFlags(&F.fs->ev_in, hipEventDisableTiming));
   HIPCHK(hipEventCreateWithFlags(&F.fs->ev_done, hipEventDisableTiming));
   // lodFlow starts at zero (Beam_InitLoad)
@@ -1481,8 +1489,36 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   int max_nd = 1;
   for (const FemBodyDesc &d : F.desc) max_nd = std::max(max_nd, d.nND);
   const int lds_nodes = std::min(max_nd, 512 / 6);  // 6 threads per node; 6x12 FP64 work areas (<= 48 KiB)
-  const int vec_lds = fem_lds_bytes(lds_nodes, 1, max_nd) <= 64 * 1024 ? 1 : 0;
-  const size_t lds_bytes = fem_lds_bytes(lds_nodes, vec_lds, max_nd);
+  int max_ne = 1;
+  for (const FemBodyDesc &d : F.desc) max_ne = std::max(max_ne, d.nEL);
+  // LDS: the CG vectors and flags, then the element matrices, as far as the
+  // device's workgroup LDS holds them (beyond 64 KiB after raising the
+  // kernel's dynamic-LDS limit)
+  const size_t lds_lim = F.lds_limit > 1024 ? F.lds_limit - 1024 : 64 * 1024;
+  auto allow = [&](size_t bytes) {
+    if (bytes <= F.lds_set) return true;
+    if (hipFuncSetAttribute((const void *)k_fem_substep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
+        hipSuccess) {
+      F.lds_set = bytes;
+      return true;
+    }
+    (void)hipGetLastError();
+    return false;
+  };
+  int vec_lds = 0, coef_off = 0;
+  size_t lds_bytes = fem_lds_bytes(lds_nodes, 0, max_nd, max_ne);
+  {
+    const size_t b1 = fem_lds_bytes(lds_nodes, 1, max_nd, max_ne);
+    if (b1 <= lds_lim && allow(b1)) {
+      vec_lds = 1;
+      lds_bytes = b1;
+      const size_t b2 = b1 + (size_t)max_ne * NED * NED * sizeof(double);
+      if (b2 <= lds_lim && allow(b2)) {
+        coef_off = (int)(b1 / sizeof(double));
+        lds_bytes = b2;
+      }
+    }
+  }
   // CG assembly lists in the block-inverse area: 2 nG staging + 12 nEL products + ints
   int cg_lds = 1;
   for (const FemBodyDesc &d : F.desc) {
@@ -1521,7 +1557,7 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
   HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
   hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, F.ddesc.p, F.nbodies, numsubstep, A,
-                     lds_nodes, vec_lds, cg_lds);
+                     lds_nodes, vec_lds, cg_lds, coef_off);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(S.h_stage, F.ddesc.p, n * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, S.s));
   HIPCHK(hipEventRecord(S.ev_done, S.s));

@@ -43,17 +43,37 @@ struct FemArrays {
 
 __device__ inline void fem_sync() { __syncthreads(); }
 
-// s + t[0] + t[1] + ... + t[n-1], in that order; the terms are read sixteen
-// at a time into registers first, so the reads overlap and only the
-// additions form the serial chain
+// s + t[0] + t[1] + ... + t[n-1], in that order.  The terms are read sixteen
+// at a time into registers, the next sixteen while the current ones are
+// added, so only the chain of additions is serial.
 __device__ inline double ordered_sum(double s, const double *t, int n) {
+  constexpr int B = 16;
   int i = 0;
-  for (; i + 16 <= n; i += 16) {
-    double b[16];
+  if (n >= B) {
+    double a[B], b[B];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) b[j] = t[i + j];
+    for (int j = 0; j < B; ++j) a[j] = t[j];
+    i = B;  // first term not yet read
+    for (;;) {
+      const bool more = i + B <= n;
+      if (more) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) s = s + b[j];
+        for (int j = 0; j < B; ++j) b[j] = t[i + j];
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j) s = s + a[j];
+      if (!more) break;
+      i += B;
+      const bool more2 = i + B <= n;
+      if (more2) {
+#pragma unroll
+        for (int j = 0; j < B; ++j) a[j] = t[i + j];
+      }
+#pragma unroll
+      for (int j = 0; j < B; ++j) s = s + b[j];
+      if (!more2) break;
+      i += B;
+    }
   }
   for (; i < n; ++i) s = s + t[i];
   return s;
@@ -64,6 +84,9 @@ __device__ inline double ordered_sum(double s, const double *t, int n) {
 // case), else in the global work arrays; indexed by the body's DOF number.
 struct FemVecs {
   double *x, *r, *p, *z, *Ap, *blk;  // blk: the 6x6 inverse blocks, 36 per node
+  double *xF;                        // lifted fixed-DOF values of the CG
+  const int *fixed;                  // fixed-DOF flags
+  const int *en;                     // node0, node1 of each element (null: read the elements)
 };
 
 // LDS copy of a body's assembly lists for the CG: per DOF the range of its
@@ -73,23 +96,24 @@ struct CgLds {
   double *lbuf;
   const int *ptr, *ent;
   bool on;
+  const double *coef;  // LDS copy of the body's element matrices (transposed), or null
 };
 
 // element-by-element product out = A*v for one body (Beam_MatrixMultipy,
 // SolidSolver.f90:2035-2049); optionally zeroes the fixed DOFs of the result
 // and leaves the products v_g*out_g in prod (for the next dot product)
-__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const CgLds &C, const double *v,
-                                  double *out, bool zero_fixed = false, double *prod = nullptr) {
+__device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, const FemVecs &V, const CgLds &C,
+                                  const double *v, double *out, bool zero_fixed = false, double *prod = nullptr) {
   const int nE = D.nEL, g0 = D.goff;
   for (int t = threadIdx.x; t < nE * NED; t += blockDim.x) {
     const int e = t / NED, i = t - e * NED;
     const BeamElem &E = A.el[D.eoff + e];
-    const int n0 = 6 * E.node0, n1 = 6 * E.node1;
+    const int n0 = 6 * (V.en ? V.en[2 * e] : E.node0), n1 = 6 * (V.en ? V.en[2 * e + 1] : E.node1);
     double vv[NED], cc[NED];
 #pragma unroll
     for (int k = 0; k < NED; ++k) {
       vv[k] = v[k < 6 ? n0 + k : n1 + (k - 6)];
-      cc[k] = E.coef[i * NED + k];
+      cc[k] = C.coef ? C.coef[e * NED * NED + k * NED + i] : E.coef[i * NED + k];
     }
     double s = 0.0;
 #pragma unroll
@@ -106,7 +130,7 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
       for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
         s = s + A.lbuf[(size_t)A.dof_e[k] * NED + A.dof_i[k]];
     }
-    if (zero_fixed && A.fixed[g0 + g]) s = 0.0;
+    if (zero_fixed && V.fixed[g]) s = 0.0;
     out[g] = s;
     if (prod) prod[g] = v[g] * s;
   }
@@ -141,7 +165,7 @@ __device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, con
     double s = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) s = s + B[i * 6 + k] * V.r[6 * node + k];
-    const double z = A.fixed[g0 + g] ? 0.0 : s;
+    const double z = V.fixed[g] ? 0.0 : s;
     V.z[g] = z;
     if (prod) {
       const double r = V.r[g];
@@ -202,28 +226,49 @@ __device__ inline void fem_dot2(const double *a, const double *b, const double *
 
 // LDS bytes of k_fem_substep: lds_nodes work areas of 6x12 doubles for the
 // block inverses (also the staging area of the dot products) and one int
-// flag per node; then, when vec_lds, the five CG vectors and the inverse
-// blocks of the largest body.
-__host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int max_nd) {
+// flag per node; then, when vec_lds, the five CG vectors, the inverse blocks,
+// the lifted fixed values, the fixed flags and the element node pairs of the
+// largest body; then, optionally, 144 doubles per element of the largest
+// body (the element matrices, at coef_off doubles).
+__host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int max_nd, int max_ne) {
   size_t d = 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
-  if (vec_lds) d += 5 * 6 * (size_t)max_nd + 36 * (size_t)max_nd;
+  // five CG vectors, inverse blocks, xF; the fixed flags and element node pairs (ints)
+  if (vec_lds) d += 6 * 6 * (size_t)max_nd + 36 * (size_t)max_nd + (6 * (size_t)max_nd + 2 * (size_t)max_ne + 1) / 2;
   return d * sizeof(double);
 }
 
 // One sub-step of one body by the whole workgroup.
-__device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int lds_nodes, int vec_lds, int cg_lds) {
+__device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int lds_nodes, int vec_lds, int cg_lds,
+                                        int coef_off) {
   __shared__ double sh_val;
   __shared__ double sh_pair[2];
   __shared__ int sh_flag;
   extern __shared__ double lds[];
   const int nE = D.nEL, nN = D.nND, nG = 6 * nN, g0 = D.goff;
+  // the element matrices (coef) in LDS, transposed per element ([k][i]), when
+  // they fit behind the other areas
+  double *coefL = coef_off ? lds + coef_off : nullptr;
   FemVecs V;
   if (vec_lds) {
     double *vb = lds + 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
     V.x = vb; V.r = vb + nG; V.p = vb + 2 * nG; V.z = vb + 3 * nG; V.Ap = vb + 4 * nG; V.blk = vb + 5 * nG;
+    V.xF = V.blk + 36 * (size_t)nN;
+    int *fx = reinterpret_cast<int *>(V.xF + nG);
+    int *en = fx + nG;
+    for (int g = threadIdx.x; g < nG; g += blockDim.x) fx[g] = A.fixed[g0 + g];
+    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+      en[2 * e] = A.el[D.eoff + e].node0;
+      en[2 * e + 1] = A.el[D.eoff + e].node1;
+    }
+    V.fixed = fx;
+    V.en = en;
+    fem_sync();
   } else {
     V.x = A.x + g0; V.r = A.r + g0; V.p = A.p + g0; V.z = A.z + g0; V.Ap = A.Ap + g0;
     V.blk = A.blk + (size_t)D.noff * 36;
+    V.xF = A.xF + g0;
+    V.fixed = A.fixed + g0;
+    V.en = nullptr;
   }
   if (D.iBodyModel == 1) {
     // rigid prescribed motion (SolidSolver.f90:1826-1858)
@@ -282,7 +327,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
       BeamElem &E = A.el[D.eoff + e];
       el_body_stress(E);
-      el_form_geom(E);
+      el_geom_scalars(E);
       el_rotate(E);
     }
     fem_sync();
@@ -294,14 +339,18 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       BeamElem &E = A.el[D.eoff + e];
       const int j1 = (blkij >> 2) * 3, j2 = (blkij & 3) * 3;
       double r[9], ks[9], kg[9];
+      double gs[9];
 #pragma unroll
-      for (int q = 0; q < 9; ++q) r[q] = E.rot[q];
+      for (int q = 0; q < 9; ++q) {
+        r[q] = E.rot[q];
+        gs[q] = E.gsc[q];
+      }
 #pragma unroll
       for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
         for (int b2 = 0; b2 < 3; ++b2) {
           ks[a2 * 3 + b2] = E.stl[(j1 + a2) * NED + j2 + b2];
-          kg[a2 * 3 + b2] = E.geo[(j1 + a2) * NED + j2 + b2];
+          kg[a2 * 3 + b2] = geo_entry(gs, j1 + a2, j2 + b2);
         }
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -323,8 +372,9 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
           const double tanv = sR + D.GeoGamma * gR;
           double cv = tanv + c[0] * E.mas[idx] + c[1] * D.dampM * E.mas[idx];
           if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
-          E.stf[idx] = sR;
-          E.coef[idx] = cv;
+          if (D.dampK > 0.0) E.stf[idx] = sR;  // read only by the stiffness-damping load
+          if (coefL) coefL[e * NED * NED + (j2 + ii) * NED + j1 + k] = cv;
+          else E.coef[idx] = cv;
         }
     }
     fem_sync();
@@ -347,77 +397,101 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     fem_sync();
     // CG_Solve (SolidSolver.f90:1949-2020), block-Jacobi preconditioner
     for (int g = threadIdx.x; g < nG; g += blockDim.x) {
-      A.xF[g0 + g] = A.fixed[g0 + g] ? (iter == 1 ? A.vBC[g0 + g] : 0.0) : 0.0;
-      V.x[g] = A.xF[g0 + g];
+      const double xf = V.fixed[g] ? (iter == 1 ? A.vBC[g0 + g] : 0.0) : 0.0;
+      V.xF[g] = xf;
+      V.x[g] = xf;
     }
     // block-Jacobi preconditioner (SolidSolver.f90:1966-1975, Invert6x6
-    // :2234-2279): one 6x12 Gauss-Jordan work area per node in LDS, one thread
-    // per row; each row operation is the sequential one, the rows of a pivot
-    // step run side by side.
-    // ten nodes per wave (lanes 0..59), so the six row threads of a node are
-    // in one wave and step in lockstep: wave barriers between the phases
-    const int wv = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-    const int slots = min(((int)blockDim.x >> 6) * 10, lds_nodes);
-    for (int base = 0; base < nN; base += slots) {
-      const int ln = wv * 10 + lane / 6, row = lane - 6 * (lane / 6);
-      const int n = base + ln;
-      const bool act = lane < 60 && ln < slots && n < nN;
-      double *aug = lds + 72 * ln;
-      int *bad = reinterpret_cast<int *>(lds + 72 * lds_nodes) + ln;
-      if (act) {
-        for (int j = 0; j < 12; ++j) aug[row * 12 + j] = 0.0;
-        for (int k = A.node_ptr[D.noff + n]; k < A.node_ptr[D.noff + n + 1]; ++k) {
-          const BeamElem &E = A.el[A.node_e[k]];
-          const int o = A.node_h[k] ? 6 : 0;
-          for (int j = 0; j < 6; ++j) aug[row * 12 + j] = aug[row * 12 + j] + E.coef[(o + row) * NED + o + j];
-        }
-        if (row == 0) *bad = 0;
-      }
-      wave_sync();
-      if (act && row == 0) {
-        for (int i = 0; i < 6; ++i)
-          if (A.fixed[g0 + 6 * n + i]) {
-            for (int j = 0; j < 6; ++j) { aug[i * 12 + j] = 0.0; aug[j * 12 + i] = 0.0; }
-            aug[i * 12 + i] = 1.0;
-          }
-      }
-      wave_sync();
-      if (act) aug[row * 12 + 6 + row] = 1.0;
-      wave_sync();
-      for (int i = 0; i < 6; ++i) {
-        if (act && row == 0 && !*bad) {
-          int p = i;
-          double pivot = fabs(aug[i * 12 + i]);
-          for (int k = i + 1; k < 6; ++k)
-            if (fabs(aug[k * 12 + i]) > pivot) { pivot = fabs(aug[k * 12 + i]); p = k; }
-          if (pivot <= 1.0e-30) *bad = i + 1;
-          else if (p != i)
-            for (int j = 0; j < 12; ++j) {
-              const double t = aug[i * 12 + j]; aug[i * 12 + j] = aug[p * 12 + j]; aug[p * 12 + j] = t;
+    // :2234-2279): Gauss-Jordan on each node's 6x12 block with one lane per
+    // row holding its row in registers, ten nodes per wave (lanes 0..59);
+    // pivot columns, swapped rows and the pivot row move between the node's
+    // lanes by shuffles.  Every row operation is the sequential one.
+    {
+      const int lane = (int)threadIdx.x & 63;
+      const int wv = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+      const int ln = lane / 6, row = lane - 6 * ln;
+      const int lbase = 6 * ln;
+      for (int base = wv * 10; base < nN; base += nw * 10) {
+        const int n = base + ln;
+        const bool act = lane < 60 && n < nN;
+        double a[12];
+#pragma unroll
+        for (int j = 0; j < 12; ++j) a[j] = 0.0;
+        int fxr = 0, fxc[6];
+#pragma unroll
+        for (int j = 0; j < 6; ++j) fxc[j] = 0;
+        if (act) {
+          for (int k = A.node_ptr[D.noff + n]; k < A.node_ptr[D.noff + n + 1]; ++k) {
+            const int o = A.node_h[k] ? 6 : 0;
+            if (coefL) {
+              const double *cl = coefL + (A.node_e[k] - D.eoff) * NED * NED;
+#pragma unroll
+              for (int j = 0; j < 6; ++j) a[j] = a[j] + cl[(o + j) * NED + o + row];
+            } else {
+              const BeamElem &E = A.el[A.node_e[k]];
+#pragma unroll
+              for (int j = 0; j < 6; ++j) a[j] = a[j] + E.coef[(o + row) * NED + o + j];
             }
+          }
+#pragma unroll
+          for (int j = 0; j < 6; ++j) fxc[j] = V.fixed[6 * n + j];
+          fxr = V.fixed[6 * n + row];
         }
-        wave_sync();
-        if (act && row == i && !*bad) {
-          const double pivot = aug[i * 12 + i];
-          for (int j = 0; j < 12; ++j) aug[i * 12 + j] = aug[i * 12 + j] / pivot;
+        // fixed DOFs: their rows and columns zeroed, unit diagonal
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+          if (fxr || fxc[j]) a[j] = (j == row && fxr) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = 0; j < 6; ++j) a[6 + j] = j == row ? 1.0 : 0.0;
+        int bad = 0;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+          // column i of the node's rows, pivot as the sequential search picks it
+          double col[6];
+#pragma unroll
+          for (int k = 0; k < 6; ++k) col[k] = __shfl(a[i], lbase + k, 64);
+          int p = i;
+          double pivot = fabs(col[i]);
+#pragma unroll
+          for (int k = i + 1; k < 6; ++k)
+            if (fabs(col[k]) > pivot) {
+              pivot = fabs(col[k]);
+              p = k;
+            }
+          if (!bad && pivot <= 1.0e-30) bad = i + 1;
+          // rows i and p exchanged
+          const int src = lbase + (row == i ? p : (row == p ? i : row));
+#pragma unroll
+          for (int j = 0; j < 12; ++j) {
+            const double v = __shfl(a[j], src, 64);
+            if (!bad) a[j] = v;
+          }
+          if (!bad && row == i) {
+            const double pv = a[i];
+#pragma unroll
+            for (int j = 0; j < 12; ++j) a[j] = a[j] / pv;
+          }
+          double prow[12];
+#pragma unroll
+          for (int j = 0; j < 12; ++j) prow[j] = __shfl(a[j], lbase + i, 64);
+          if (!bad && row != i) {
+            const double factor = a[i];
+#pragma unroll
+            for (int j = 0; j < 12; ++j) a[j] = a[j] - factor * prow[j];
+          }
         }
-        wave_sync();
-        if (act && row != i && !*bad) {
-          const double factor = aug[row * 12 + i];
-          for (int j = 0; j < 12; ++j) aug[row * 12 + j] = aug[row * 12 + j] - factor * aug[i * 12 + j];
+        if (act) {
+          double *out = V.blk + (size_t)n * 36;
+#pragma unroll
+          for (int j = 0; j < 6; ++j) out[row * 6 + j] = bad ? (row == j ? 1.0 : 0.0) : a[6 + j];
         }
-        wave_sync();
       }
-      if (act) {
-        double *out = V.blk + (size_t)n * 36;
-        for (int j = 0; j < 6; ++j) out[row * 6 + j] = *bad ? (row == j ? 1.0 : 0.0) : aug[row * 12 + 6 + j];
-      }
-      wave_sync();
     }
     fem_sync();
     // the body's assembly lists and the mat-vec products into LDS (behind the
     // dot-product staging area [0, 2 nG)) when they fit
     CgLds C{};
+    C.coef = coefL;
     if (cg_lds) {
       C.lbuf = lds + 2 * nG;
       int *ip = reinterpret_cast<int *>(lds + 2 * nG + NED * nE);
@@ -430,9 +504,9 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       C.on = true;
       fem_sync();
     }
-    fem_matvec(A, D, C, V.x, V.Ap);
+    fem_matvec(A, D, V, C, V.x, V.Ap);
     for (int g = threadIdx.x; g < nG; g += blockDim.x)
-      V.r[g] = A.fixed[g0 + g] ? 0.0 : A.lodEffe[g0 + g] - V.Ap[g];
+      V.r[g] = V.fixed[g] ? 0.0 : A.lodEffe[g0 + g] - V.Ap[g];
     fem_sync();
     fem_precond(A, D, V);
     const double err = 1.0e-6;
@@ -448,16 +522,16 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       for (int it = 1; it <= 10000; ++it) {
         ++cg_total;
         // A p with the fixed DOFs zeroed, and the products p_g (A p)_g
-        fem_matvec(A, D, C, V.p, V.Ap, true, lds);
+        fem_matvec(A, D, V, C, V.p, V.Ap, true, lds);
         const double pAp = fem_serial_sum(lds, nG, sh_pair);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
           double xv = V.x[g] + alpha * V.p[g];
-          if (A.fixed[g0 + g]) xv = A.xF[g0 + g];
+          if (V.fixed[g]) xv = V.xF[g];
           V.x[g] = xv;
           double rv = V.r[g] - alpha * V.Ap[g];
-          if (A.fixed[g0 + g]) rv = 0.0;
+          if (V.fixed[g]) rv = 0.0;
           V.r[g] = rv;
         }
         fem_sync();
@@ -472,7 +546,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
         const double beta = rsnew / rsold;
         for (int g = threadIdx.x; g < nG; g += blockDim.x) {
           double pv = V.z[g] + beta * V.p[g];
-          if (A.fixed[g0 + g]) pv = 0.0;
+          if (V.fixed[g]) pv = 0.0;
           V.p[g] = pv;
         }
         fem_sync();
@@ -546,10 +620,10 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
 // are HBM-bound and lose little, while this chain of dependent reductions
 // would otherwise wait behind them for every instruction.
 __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, int nbodies, int nsub, FemArrays A,
-                                                   int lds_nodes, int vec_lds, int cg_lds) {
+                                                   int lds_nodes, int vec_lds, int cg_lds, int coef_off) {
   __builtin_amdgcn_s_setprio(3);
   for (int isub = 0; isub < nsub; ++isub) {
-    fem_substep_body(descs[(size_t)isub * nbodies + blockIdx.x], A, lds_nodes, vec_lds, cg_lds);
+    fem_substep_body(descs[(size_t)isub * nbodies + blockIdx.x], A, lds_nodes, vec_lds, cg_lds, coef_off);
     fem_sync();
   }
 }
