@@ -1173,24 +1173,25 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((M + 63) / 64), dim3(256), 0, s, I.wsX.p, M, net->th.p, I.wsP.p);
     hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Uref, invh3p,
                        I.Eforce.p, I.fT.p);
-    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p, I.dentry.p,
-                       I.dEw.p, I.fT.p, I.band_u.p);
+    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, -1, MdfTest{}, I.cap, I.dptr.p,
+                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
     hipLaunchKernelGGL(k_ws_dm0, dim3(1), dim3(256), 0, s, I.tl0.p, M, dsum, I.Uref, I.dstate.p);
     HIPCHK(hipGetLastError());
   }
   // MDF loop (Solidbody.f90:893-906): ntol rounds are queued (one more with
-  // the warm start, in case it falls back); the kernels of a round skip their
-  // work once k_mdf_check has found the loop finished
+  // the warm start, in case it falls back); a round is k_interp and
+  // k_correct, whose workgroup 0 makes the stop test, and the kernels of the
+  // rounds after the stop return at once
   const int rounds = I.ntol + warm;
+  MdfTest mt{};
+  mt.tl = I.tolpart.p; mt.dstate = I.dstate.p; mt.M = M; mt.ntol = I.ntol;
+  mt.dsum = dsum; mt.Uref = I.Uref; mt.dtol = I.dtol;
   for (int it = 0; it < rounds; ++it) {
-    hipLaunchKernelGGL(k_interp, dim3((M + 3) / 4), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
+    hipLaunchKernelGGL(k_interp, dim3((M + 3) / 4), dim3(256), 0, s, M, it, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
                        I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p,
+    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, it, mt, I.cap, I.dptr.p,
                        I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_mdf_check, dim3(1), dim3(256), 0, s, I.ctl.p, I.dstate.p, I.tolpart.p, M, dsum,
-                       I.Uref, I.ntol, I.dtol);
     HIPCHK(hipGetLastError());
     if (warm && it == 0) {
       // restart from the zero guess when the first round asked for it
@@ -1512,7 +1513,7 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
     if (b1 <= lds_lim && allow(b1)) {
       vec_lds = 1;
       lds_bytes = b1;
-      const size_t b2 = b1 + (size_t)max_ne * NED * NED * sizeof(double);
+      const size_t b2 = b1 + (size_t)max_ne * kCoefStride * sizeof(double);
       if (b2 <= lds_lim && allow(b2)) {
         coef_off = (int)(b1 / sizeof(double));
         lds_bytes = b2;

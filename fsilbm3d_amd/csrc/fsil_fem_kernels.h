@@ -92,6 +92,11 @@ struct FemVecs {
 // LDS copy of a body's assembly lists for the CG: per DOF the range of its
 // (element, local index) entries, each packed as element*12 + index, and the
 // element products of the mat-vec (lbuf)
+// LDS stride (doubles) of one element's matrix: 144 entries padded so that
+// the five elements a wave's mat-vec lanes touch fall on distinct LDS banks
+// (172 doubles = 24 banks mod 64)
+constexpr int kCoefStride = 172;
+
 struct CgLds {
   double *lbuf;
   const int *ptr, *ent;
@@ -113,7 +118,7 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
 #pragma unroll
     for (int k = 0; k < NED; ++k) {
       vv[k] = v[k < 6 ? n0 + k : n1 + (k - 6)];
-      cc[k] = C.coef ? C.coef[e * NED * NED + k * NED + i] : E.coef[i * NED + k];
+      cc[k] = C.coef ? C.coef[e * kCoefStride + k * NED + i] : E.coef[i * NED + k];
     }
     double s = 0.0;
 #pragma unroll
@@ -228,8 +233,8 @@ __device__ inline void fem_dot2(const double *a, const double *b, const double *
 // block inverses (also the staging area of the dot products) and one int
 // flag per node; then, when vec_lds, the five CG vectors, the inverse blocks,
 // the lifted fixed values, the fixed flags and the element node pairs of the
-// largest body; then, optionally, 144 doubles per element of the largest
-// body (the element matrices, at coef_off doubles).
+// largest body; then, optionally, kCoefStride doubles per element of the
+// largest body (the element matrices, at coef_off doubles).
 __host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int max_nd, int max_ne) {
   size_t d = 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
   // five CG vectors, inverse blocks, xF; the fixed flags and element node pairs (ints)
@@ -373,7 +378,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
           double cv = tanv + c[0] * E.mas[idx] + c[1] * D.dampM * E.mas[idx];
           if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
           if (D.dampK > 0.0) E.stf[idx] = sR;  // read only by the stiffness-damping load
-          if (coefL) coefL[e * NED * NED + (j2 + ii) * NED + j1 + k] = cv;
+          if (coefL) coefL[e * kCoefStride + (j2 + ii) * NED + j1 + k] = cv;
           else E.coef[idx] = cv;
         }
     }
@@ -424,7 +429,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
           for (int k = A.node_ptr[D.noff + n]; k < A.node_ptr[D.noff + n + 1]; ++k) {
             const int o = A.node_h[k] ? 6 : 0;
             if (coefL) {
-              const double *cl = coefL + (A.node_e[k] - D.eoff) * NED * NED;
+              const double *cl = coefL + (A.node_e[k] - D.eoff) * kCoefStride;
 #pragma unroll
               for (int j = 0; j < 6; ++j) a[j] = a[j] + cl[(o + j) * NED + o + row];
             } else {

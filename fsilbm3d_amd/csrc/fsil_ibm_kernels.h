@@ -38,6 +38,7 @@ enum IbmCtl {
   kFallback = 11,  // 1: the warm start did not lower the residual, restart from zero; 2: restarted
   kDone0 = 12,     // the done word at the start of a loop (no iteration at all)
   kNew = 13,       // band cells claimed so far in this call (becomes kNband)
+  kRan = 14,       // the last forcing round whose interpolation ran (-1: none)
   kCtlWords = 16
 };
 
@@ -87,6 +88,7 @@ __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const 
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     band_ptr[0] = 0;
     ctl[kNew] = 0;
+    ctl[kRan] = -1;
     ctl[kErr] = 0x7fffffff;
     ctl[kDone] = done0;
     ctl[kIters] = 0;
@@ -392,17 +394,19 @@ __device__ inline void mdf_decide(int *ctl, double *dstate, double tol, double d
 // ((u*rx)*ry)*rz of the marker's stencil (Solidbody.f90:1009-1015), lane 0
 // adds them in the reference's x, y, z order, then the velocity defect,
 // penalty force and force accumulation (PenaltyForce_, Solidbody.f90:1016-1026).
-// The marker's |dV|_1 goes to tl[m] for the residual.
-__global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ bidx,
+// The marker's |dV|_1 goes to tl[m] for the residual; ctl[kRan] = round
+// records that this forcing round ran.
+__global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__restrict__ bidx,
                                                 const float *__restrict__ Ew,
                                                 const double *__restrict__ band_u, int cap,
-                                                const int *__restrict__ ctl,
+                                                int *__restrict__ ctl,
                                                 const double *__restrict__ Evel,
                                                 const double *__restrict__ Ea, double invh3,
                                                 double *__restrict__ Eforce,
                                                 double *__restrict__ fT,
                                                 double *__restrict__ tl) {
   if (ctl[kDone]) return;  // uniform over the grid
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kRan] = round;
   __shared__ double t[4][3][64];
   const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + w;
@@ -436,41 +440,80 @@ __global__ __launch_bounds__(256) void k_interp(int M, const int *__restrict__ b
   }
 }
 
+// Reads of a band cell's entries, eight at a time: the entry numbers, then
+// their weights and marker values, all issued before any is used.
+struct BandChunk {
+  double rx[8], ry[8], rz[8], v1[8], v2[8], v3[8];
+};
+__device__ __forceinline__ int band_chunk(int k, int k1, const int *__restrict__ entry,
+                                          const float *__restrict__ Ew, const double *__restrict__ val,
+                                          BandChunk &c) {
+  const int n = min(8, k1 - k);
+  int mm[8], ss[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const int e = j < n ? entry[k + j] : 0;
+    mm[j] = e >> 6;
+    ss[j] = e & 63;
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (j < n) {
+      const float *W = Ew + 12LL * mm[j];
+      c.rx[j] = (double)W[ss[j] >> 4];
+      c.ry[j] = (double)W[4 + ((ss[j] >> 2) & 3)];
+      c.rz[j] = (double)W[8 + (ss[j] & 3)];
+      c.v1[j] = val[3LL * mm[j]];
+      c.v2[j] = val[3LL * mm[j] + 1];
+      c.v3[j] = val[3LL * mm[j] + 2];
+    }
+  }
+  return n;
+}
+
+struct MdfTest {
+  const double *tl;
+  double *dstate;
+  int M, ntol;
+  double dsum, Uref, dtol;
+};
+
 // velocity correction of every band cell: u -= fT*rx*ry*rz over its entries
-// in (marker, stencil) order (Solidbody.f90:1034-1048)
-__global__ __launch_bounds__(256) void k_correct(const int *__restrict__ ctl, int cap,
+// in (marker, stencil) order (Solidbody.f90:1034-1048).  round >= 0: the
+// correction of that forcing round (runs when its k_interp ran), and
+// workgroup 0 makes the round's stop test (the markers' |dV|_1 summed in a
+// fixed order, then mdf_decide); round < 0 (the warm-start guess): runs
+// unless the loop is done, no stop test.
+__global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int round, MdfTest mt, int cap,
                                                  const int *__restrict__ ptr,
                                                  const int *__restrict__ entry,
                                                  const float *__restrict__ Ew,
                                                  const double *__restrict__ fT,
                                                  double *__restrict__ band_u) {
-  if (ctl[kDone]) return;
+  if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
+  if (round >= 0 && blockIdx.x == 0) {
+    const double tol = mdf_tl_sum(mt.tl, mt.M);
+    if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol);
+  }
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ctl[kNband]) return;
   double u1 = band_u[b], u2 = band_u[cap + b], u3 = band_u[2LL * cap + b];
-  for (int k = ptr[b]; k < ptr[b + 1]; ++k) {
-    const int e = entry[k];
-    const int m = e >> 6, s = e & 63;
-    const float *W = Ew + 12LL * m;
-    const double rx = (double)W[s >> 4], ry = (double)W[4 + ((s >> 2) & 3)],
-                 rz = (double)W[8 + (s & 3)];
-    u1 = u1 - ((fT[3LL * m] * rx) * ry) * rz;
-    u2 = u2 - ((fT[3LL * m + 1] * rx) * ry) * rz;
-    u3 = u3 - ((fT[3LL * m + 2] * rx) * ry) * rz;
+  const int k1 = ptr[b + 1];
+  for (int k = ptr[b]; k < k1;) {
+    BandChunk c;
+    const int n = band_chunk(k, k1, entry, Ew, fT, c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < n) {
+        u1 = u1 - ((c.v1[j] * c.rx[j]) * c.ry[j]) * c.rz[j];
+        u2 = u2 - ((c.v2[j] * c.rx[j]) * c.ry[j]) * c.rz[j];
+        u3 = u3 - ((c.v3[j] * c.rx[j]) * c.ry[j]) * c.rz[j];
+      }
+    k += n;
   }
   band_u[b] = u1;
   band_u[cap + b] = u2;
   band_u[2LL * cap + b] = u3;
-}
-
-// stop test of a forcing round: the markers' |dV|_1 summed in a fixed order
-// by one workgroup, then mdf_decide
-__global__ __launch_bounds__(256) void k_mdf_check(int *__restrict__ ctl, double *__restrict__ dstate,
-                                                   const double *__restrict__ tl, int M, double dsum, double Uref,
-                                                   int ntol, double dtol) {
-  if (ctl[kDone]) return;
-  const double tol = mdf_tl_sum(tl, M);
-  if (threadIdx.x == 0) mdf_decide(ctl, dstate, tol, dsum, Uref, ntol, dtol);
 }
 
 // Eulerian force of every band cell: (0 - sum F/h^3*rx*ry*rz) + VF
@@ -486,17 +529,19 @@ __global__ __launch_bounds__(256) void k_spread(const int *__restrict__ ctl, int
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ctl[kNband]) return;
   double g1 = 0.0, g2 = 0.0, g3 = 0.0;
-  for (int k = ptr[b]; k < ptr[b + 1]; ++k) {
-    const int e = entry[k];
-    const int m = e >> 6, s = e & 63;
-    const float *W = Ew + 12LL * m;
-    const double rx = (double)W[s >> 4], ry = (double)W[4 + ((s >> 2) & 3)],
-                 rz = (double)W[8 + (s & 3)];
-    const double h1 = Eforce[3LL * m] * invh3, h2 = Eforce[3LL * m + 1] * invh3,
-                 h3 = Eforce[3LL * m + 2] * invh3;
-    g1 = g1 + (-(((h1 * rx) * ry) * rz));
-    g2 = g2 + (-(((h2 * rx) * ry) * rz));
-    g3 = g3 + (-(((h3 * rx) * ry) * rz));
+  const int k1 = ptr[b + 1];
+  for (int k = ptr[b]; k < k1;) {
+    BandChunk c;
+    const int n = band_chunk(k, k1, entry, Ew, Eforce, c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < n) {
+        const double h1 = c.v1[j] * invh3, h2 = c.v2[j] * invh3, h3 = c.v3[j] * invh3;
+        g1 = g1 + (-(((h1 * c.rx[j]) * c.ry[j]) * c.rz[j]));
+        g2 = g2 + (-(((h2 * c.rx[j]) * c.ry[j]) * c.rz[j]));
+        g3 = g3 + (-(((h3 * c.rx[j]) * c.ry[j]) * c.rz[j]));
+      }
+    k += n;
   }
   band_F[b] = g1 + VF1;
   band_F[cap + b] = g2 + VF2;
@@ -610,7 +655,7 @@ __global__ __launch_bounds__(256) void k_ws_features(int M, const int *__restric
   }
 }
 
-// dstate[1] = the zero guess's residual, reduced as k_mdf_check reduces
+// dstate[1] = the zero guess's residual, reduced as the stop test reduces
 __global__ __launch_bounds__(256) void k_ws_dm0(const double *__restrict__ tl0, int M, double dsum, double Uref,
                                                 double *__restrict__ dstate) {
   const double tol = mdf_tl_sum(tl0, M);
