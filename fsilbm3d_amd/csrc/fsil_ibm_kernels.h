@@ -410,6 +410,13 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
   __shared__ double t[4][3][64];
   const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + w;
+  // the marker's own values, read while the stencil gathers
+  double ev1 = 0.0, ev2 = 0.0, ev3 = 0.0, am = 0.0, ef1 = 0.0, ef2 = 0.0, ef3 = 0.0;
+  if (m < M && s == 0) {
+    ev1 = Evel[3LL * m]; ev2 = Evel[3LL * m + 1]; ev3 = Evel[3LL * m + 2];
+    am = Ea[m];
+    ef1 = Eforce[3LL * m]; ef2 = Eforce[3LL * m + 1]; ef3 = Eforce[3LL * m + 2];
+  }
   if (m < M) {
     const int b = bidx[64LL * m + s];
     const float *W = Ew + 12LL * m;
@@ -427,13 +434,12 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
       v2 = v2 + t[w][1][k];
       v3 = v3 + t[w][2][k];
     }
-    const double d1 = Evel[3LL * m] - v1, d2 = Evel[3LL * m + 1] - v2, d3 = Evel[3LL * m + 2] - v3;
-    const double a = Ea[m];
-    const double F1 = d1 * a, F2 = d2 * a, F3 = d3 * a;
+    const double d1 = ev1 - v1, d2 = ev2 - v2, d3 = ev3 - v3;
+    const double F1 = d1 * am, F2 = d2 * am, F3 = d3 * am;
     tl[m] = (fabs(d1) + fabs(d2)) + fabs(d3);
-    Eforce[3LL * m] = Eforce[3LL * m] + F1;
-    Eforce[3LL * m + 1] = Eforce[3LL * m + 1] + F2;
-    Eforce[3LL * m + 2] = Eforce[3LL * m + 2] + F3;
+    Eforce[3LL * m] = ef1 + F1;
+    Eforce[3LL * m + 1] = ef2 + F2;
+    Eforce[3LL * m + 2] = ef3 + F3;
     fT[3LL * m] = F1 * invh3;
     fT[3LL * m + 1] = F2 * invh3;
     fT[3LL * m + 2] = F3 * invh3;
@@ -480,9 +486,10 @@ struct MdfTest {
 
 // velocity correction of every band cell: u -= fT*rx*ry*rz over its entries
 // in (marker, stencil) order (Solidbody.f90:1034-1048).  round >= 0: the
-// correction of that forcing round (runs when its k_interp ran), and
-// workgroup 0 makes the round's stop test (the markers' |dV|_1 summed in a
-// fixed order, then mdf_decide); round < 0 (the warm-start guess): runs
+// correction of that forcing round (runs when its k_interp ran), and the
+// last workgroup (the grid covers 64 M cells, the band usually far fewer, so
+// it is mostly idle) makes the round's stop test (the markers' |dV|_1 summed
+// in a fixed order, then mdf_decide); round < 0 (the warm-start guess): runs
 // unless the loop is done, no stop test.
 __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int round, MdfTest mt, int cap,
                                                  const int *__restrict__ ptr,
@@ -491,7 +498,7 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
                                                  const double *__restrict__ fT,
                                                  double *__restrict__ band_u) {
   if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
-  if (round >= 0 && blockIdx.x == 0) {
+  if (round >= 0 && blockIdx.x == gridDim.x - 1) {
     const double tol = mdf_tl_sum(mt.tl, mt.M);
     if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol);
   }
