@@ -73,12 +73,13 @@ def run_form(prec, f0, steps, warmup, nlaunch):
                 kernel_ms=kern_ms, achieved_gbs=achieved)
 
 
-def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0):
+def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_forces=False):
     """Flexible-plate FSI steps through the C-ABI (device beam solver).
     precision 0: FP64 plain, 1: FP32-deviation fluid (FP64 solid and
     coupling).  net: learned warm start during the timed steps.  collect:
     record the network's features/targets of the first `collect` steps.
-    Returns timings and the per-step force on the body over the timed steps."""
+    Returns timings; with record_forces the per-step force on the body over
+    the timed steps (a host read each step, so such a run is not timed)."""
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock, synchronize
     from fsilbm3d_amd.fsi import DeviceBodyBlock
@@ -106,10 +107,14 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0):
     t0 = time.perf_counter()
     for _ in range(steps):
         t = t + 1.0
+        # the coupling call waits for its own end (so the beam sub-steps are
+        # dispatched before the collide kernel fills the CUs); the timed run
+        # reads no forces back
         fsi.step(t)
         its.append(fsi.iters)
         dm.append(fsi.dmax)
-        forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
+        if record_forces:
+            forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
     synchronize()
     secs = time.perf_counter() - t0
     newton, cg, _ = fsi.info(0)
@@ -121,8 +126,9 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0):
     synchronize()
     blk.close()
     out = dict(mlups=N * steps / secs / 1e6, ms_per_step=secs / steps * 1e3, kernel_ms=kern_ms,
-               achieved_gbs=B_CELL[precision] * N / (kern_ms * 1e-3) / 1e9, mdf_iters_mean=float(np.mean(its)),
-               mdf_residual_mean=float(np.mean(dm)), newton_iters_last=newton, cg_iters_last=cg,
+               achieved_gbs=B_CELL[precision] * N / (kern_ms * 1e-3) / 1e9,
+               mdf_iters_mean=float(np.mean(its)), mdf_residual_mean=float(np.mean(dm)),
+               newton_iters_last=newton, cg_iters_last=cg,
                tip_xyz=[float(v) for v in pos[-1, :3]], forces=np.array(forces), cells=N, time=t)
     if collect:
         out["X"] = np.concatenate(XS)
@@ -275,11 +281,19 @@ def main():
         pmc_driver(args.pmc_driver)
         return
     w = flexible_plate()
+
+    def forces_of(r, precision, net=None):
+        # the same steps again, untimed, reading the body force after each one
+        r["forces"] = run_fsi(w, args.steps, args.warmup, 1, precision=precision, net=net,
+                              record_forces=True)["forces"]
+
     log("bench: flexible-plate FSI, FP64 plain form")
     fsi = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=0)
+    forces_of(fsi, 0)
     log(f"bench: FSI FP64 {fsi['mlups']:.1f} MLUPS, {fsi['ms_per_step']:.3f} ms/step")
     log("bench: flexible-plate FSI, FP32-deviation fluid")
     fsi32 = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1)
+    forces_of(fsi32, 1)
     err32 = force_error(fsi32, fsi)
     log(f"bench: FSI FP32 {fsi32['mlups']:.1f} MLUPS, force error {err32}")
     learned = None
@@ -290,6 +304,7 @@ def main():
         tinfo["seconds"] = time.perf_counter() - t0
         log(f"bench: warm start trained {tinfo}")
         fsiL = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1, net=net)
+        forces_of(fsiL, 1, net)
         errL = force_error(fsiL, fsi)
         log(f"bench: FSI FP32+learned {fsiL['mlups']:.1f} MLUPS, force error {errL}")
         learned = (fsiL, errL, tinfo)

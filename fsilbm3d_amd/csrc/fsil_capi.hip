@@ -82,15 +82,21 @@ struct DevBuf {
 // Pinned host copy of a coupling call's control words, written by k_ibm_report.
 struct PinnedReport {
   IbmReport *h = nullptr, *d = nullptr;  // host pointer and its device address
+  hipEvent_t ev = nullptr;               // recorded after k_ibm_report
+  bool pending = false;                  // a report the host has not checked yet
   PinnedReport() = default;
   PinnedReport(const PinnedReport &) = delete;
   PinnedReport &operator=(const PinnedReport &) = delete;
   ~PinnedReport() {
+    if (ev) (void)hipEventSynchronize(ev);
+    if (ev) (void)hipEventDestroy(ev);
     if (h) (void)hipHostFree(h);
   }
   hipError_t ensure() {
     if (h) return hipSuccess;
-    hipError_t e = hipHostMalloc((void **)&h, sizeof(IbmReport), hipHostMallocDefault);
+    hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+    if (e != hipSuccess) { ev = nullptr; return e; }
+    e = hipHostMalloc((void **)&h, sizeof(IbmReport), hipHostMallocDefault);
     if (e != hipSuccess) { h = nullptr; return e; }
     memset(h, 0, sizeof(IbmReport));
     e = hipHostGetDevicePointer((void **)&d, h, 0);
@@ -131,6 +137,8 @@ struct Ibm {
   DevBuf<float> wsX, wsY, wsP;               // features (12 per marker), targets, guesses (3 per marker)
   DevBuf<double> Eprev, Enrm, tl0;           // previous force, normals, zero-guess residual terms
   int last_warm = 0, last_fallback = 0;
+  int last_iters = 0;
+  double last_dmax = 0.0;
 };
 
 // Device FEM state of the bodies carried by a block.
@@ -629,6 +637,7 @@ FemArrays fem_arrays(Block &b) {
 
 int fem_collect(Block &b);
 int fem_order_after(Block &b);
+int ibm_check_report(Block &b);
 
 }  // namespace
 
@@ -1085,6 +1094,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   Block *b = get_block(h);
   if (!b) return 1;
   Ibm &I = b->ibm;
+  if (ibm_check_report(*b)) return 1;  // an earlier call that did not wait
   I.active = false;
   if (I.nbodies == 0 || I.M == 0) {
     if (iters) *iters = 0;
@@ -1228,26 +1238,22 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   hipLaunchKernelGGL(k_ibm_report, dim3(1), dim3(64), 0, s, I.ctl.p, I.dstate.p, I.rep->d);
   HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(I.rep->ev, s));
+  I.rep->pending = true;
+  if (!(Eforce || lodFlow || iters || dmax)) {
+    // nothing asked back: no host wait; the report is checked by the next
+    // synchronising coupling call (an error then surfaces there)
+    I.active = true;
+    return 0;
+  }
   if (Eforce) HIPCHK(hipMemcpyAsync(Eforce, I.Eforce.p, 3 * (size_t)M * sizeof(double), hipMemcpyDeviceToHost, s));
   if (lodFlow && I.ndof > 0) HIPCHK(hipMemcpyAsync(lodFlow, I.lod.p, (size_t)I.ndof * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
   if (fem_collect(*b)) return 1;  // results of the previous beam sub-steps
-  const IbmReport &R = *I.rep->h;
-  const int err = R.ctl[kErr];
-  if (err != 0x7fffffff) {
-    if (err == 3) return fail("Nan found in PenaltyForce");
-    if (err == 4) return fail("immersed-boundary band: a stencil cell without a band number");
-    const int m = err / 4 - 1, why = err % 4;
-    int body = 0;
-    while (body + 1 < I.nbodies && m >= I.moff[body + 1]) ++body;
-    return fail("immersed-boundary stencil of marker %d (body %d): index out of %s bound", m - I.moff[body] + 1,
-                body + 1, why == 1 ? "min" : "max");
-  }
+  if (ibm_check_report(*b)) return 1;
   I.active = true;
-  I.last_warm = R.ctl[kWarm];
-  I.last_fallback = R.ctl[kFallback] != 0;
-  if (iters) *iters = R.ctl[kIters];
-  if (dmax) *dmax = R.dmax;
+  if (iters) *iters = I.last_iters;
+  if (dmax) *dmax = I.last_dmax;
   return 0;
 }
 
@@ -1260,7 +1266,7 @@ int fsil_ibm_forces(int h, double *Eforce, double *lodFlow) {
   if (lodFlow && I.ndof > 0)
     HIPCHK(hipMemcpyAsync(lodFlow, I.lod.p, (size_t)I.ndof * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
-  return 0;
+  return ibm_check_report(*b);
 }
 
 int fsil_ibm_set_warm_start(int h, int mlp, int record) {
@@ -1280,6 +1286,7 @@ int fsil_ibm_warm_start_data(int h, float *X, float *Y, int *info) {
   if (X) HIPCHK(hipMemcpyAsync(X, I.wsX.p, 12 * (size_t)I.M * sizeof(float), hipMemcpyDeviceToHost, b->stream));
   if (Y) HIPCHK(hipMemcpyAsync(Y, I.wsY.p, 3 * (size_t)I.M * sizeof(float), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
+  if (ibm_check_report(*b)) return 1;
   if (info) {
     info[0] = I.last_warm;
     info[1] = I.last_fallback;
@@ -1290,6 +1297,33 @@ int fsil_ibm_warm_start_data(int h, float *X, float *Y, int *info) {
 }  // extern "C"
 
 namespace {
+// The host's check of a coupling call's report (waits for it if needed):
+// the stencil, NaN and band errors of the reference, then the iteration
+// count and residual of the call.
+int ibm_check_report(Block &b) {
+  Ibm &I = b.ibm;
+  if (!I.rep || !I.rep->pending) return 0;
+  I.rep->pending = false;
+  HIPCHK(hipEventSynchronize(I.rep->ev));
+  const IbmReport &R = *I.rep->h;
+  const int err = R.ctl[kErr];
+  if (err != 0x7fffffff) {
+    I.active = false;
+    if (err == 3) return fail("Nan found in PenaltyForce");
+    if (err == 4) return fail("immersed-boundary band: a stencil cell without a band number");
+    const int m = err / 4 - 1, why = err % 4;
+    int body = 0;
+    while (body + 1 < I.nbodies && m >= I.moff[body + 1]) ++body;
+    return fail("immersed-boundary stencil of marker %d (body %d): index out of %s bound", m - I.moff[body] + 1,
+                body + 1, why == 1 ? "min" : "max");
+  }
+  I.last_warm = R.ctl[kWarm];
+  I.last_fallback = R.ctl[kFallback] != 0;
+  I.last_iters = R.ctl[kIters];
+  I.last_dmax = R.dmax;
+  return 0;
+}
+
 // Wait (host side) for the last beam sub-steps and check their results:
 // "CG solver breakdown" of any sub-step and body is an error.
 int fem_collect(Block &b) {

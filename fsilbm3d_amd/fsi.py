@@ -80,13 +80,20 @@ class DeviceBodyBlock:
         self.iters = 0
         self.dmax = 0.0
 
-    def interaction_force(self, dt: float):
+    def interaction_force(self, dt: float, wait: bool = True):
+        """The coupling call; with wait=False nothing is read back (no host
+        wait; iters/dmax stay None and an error surfaces at the next call)."""
         _lib.check(self.lib.fsil_fem_update_markers(self.block.handle))
-        it = np.zeros(1, dtype=np.int32)
-        dm = np.zeros(1)
-        _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
-                                                       None, None, _lib.iptr(it), _lib.dptr(dm)))
-        self.iters, self.dmax = int(it[0]), float(dm[0])
+        if wait:
+            it = np.zeros(1, dtype=np.int32)
+            dm = np.zeros(1)
+            _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
+                                                           None, None, _lib.iptr(it), _lib.dptr(dm)))
+            self.iters, self.dmax = int(it[0]), float(dm[0])
+        else:
+            _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
+                                                           None, None, None, None))
+            self.iters, self.dmax = None, None
         self._stale = True
 
     def _fetch(self):
@@ -136,16 +143,16 @@ class DeviceBodyBlock:
         _lib.check(self.lib.fsil_fem_info(self.block.handle, i, _lib.iptr(a), _lib.iptr(b), _lib.dptr(c)))
         return int(a[0]), int(b[0]), float(c[0])
 
-    def ibm_fem(self, time: float):
+    def ibm_fem(self, time: float, wait: bool = True):
         """IBM_FEM (LBMBlockComm.f90:320-338) at block time ``time``."""
         blk = self.block
-        self.interaction_force(blk.dh)
+        self.interaction_force(blk.dh, wait)
         _lib.check(self.lib.fsil_fem_substeps(blk.handle, float(time), self.numsubstep, blk.dh))
 
-    def step(self, time: float):
+    def step(self, time: float, wait: bool = True):
         blk = self.block
         blk.update_volume_force(time)
-        self.ibm_fem(time)
+        self.ibm_fem(time, wait)
         blk.collision_streaming_bc()
 
 

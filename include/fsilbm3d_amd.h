@@ -115,7 +115,11 @@ int fsil_ibm_set_params(int handle, int ntolLBM, double dtolLBM, double Uref, do
  *   Exyz[3M], Evel[3M], Ea[M]  markers of all bodies, carried order
  *   elem_xc[3*nelems]          beam axis midpoints (moment arm origin)
  *   Eforce[3M] out             v_Eforce;  lodFlow[6*sum(nnodes)] out
- *   iters, dmax out            iterations and final residual */
+ *   iters, dmax out            iterations and final residual
+ * With Eforce, lodFlow, iters and dmax all NULL the call only queues its work
+ * on the block's stream and returns (the fluid update and beam sub-steps that
+ * follow are ordered after it on the device); its errors are then reported
+ * by the next coupling call or fsil_ibm_forces. */
 int fsil_ibm_interaction_force(int handle, double dt, const double *Exyz, const double *Evel,
                                const double *Ea, const double *elem_xc, double *Eforce,
                                double *lodFlow, int *iters, double *dmax);

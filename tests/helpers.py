@@ -162,10 +162,11 @@ def plate_specs(pc):
     return out
 
 
-def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None):
+def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None, wait=True):
     """Run a ParityCase through the C-ABI on the GPU; returns the final state.
     Bodies run on the device beam solver when device_fem (default: when any
-    plate is elastic), else with the host rigid kinematics."""
+    plate is elastic), else with the host rigid kinematics.  wait=False: the
+    device-body coupling calls read nothing back (no host wait per step)."""
     from fsilbm3d_amd.block import LBMBlock
     from fsilbm3d_amd.fsi import DeviceBodyBlock, FSIBlock
     case = pc.case
@@ -193,7 +194,10 @@ def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None):
     time = 0.0
     for _ in range(n):
         time = time + case.dh
-        fsi.step(time)
+        if isinstance(fsi, DeviceBodyBlock):
+            fsi.step(time, wait=wait)
+        else:
+            fsi.step(time)
         if record is not None and bodies:
             rec = dict(iters=fsi.iters, Eforce=fsi.Eforce[:fsi.M].copy(),
                        lod=[fsi.body_loads(i).copy() for i in range(len(bodies))])

@@ -43,3 +43,15 @@ def test_device_elastic_plate_vs_reference_and_oracle(name):
         assert np.abs(po - pd_).max() <= 1e-12 * np.abs(po).max(), np.abs(po - pd_).max()
         if name == "fem_flexible_plate":
             assert so["fem_info"][0][:2] == sd["fem_info"][0][:2]
+
+
+@pytest.mark.parametrize("name", ["fem_flexible_plate", "ibm_heaving_plate"])
+def test_coupling_without_read_backs_matches(name):
+    """Coupling calls that ask nothing back (no host wait per step) give the
+    same state bit for bit as the waiting calls."""
+    pc = BY_NAME[name]
+    print(f"[gpu] no-wait coupling {name}", flush=True)
+    fa = run_device(pc, device_fem=True)
+    fb = run_device(pc, device_fem=True, wait=False)
+    n, m = mismatch(fb, fa)
+    assert n == 0, f"{n} differ, max {m:.3e}"

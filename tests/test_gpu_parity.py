@@ -124,6 +124,10 @@ def test_stencil_out_of_bounds_is_an_error():
                     plates=[Plate(3, -1.0, 1.0, 1.0, 2, XYZo=(0.6, 6.0, 6.0))])
     with pytest.raises(FsilError, match="out of min bound"):
         run_device(pc)
+    # a coupling call that does not wait reports it at the next call
+    pc2 = dataclasses.replace(pc, nsteps=2)
+    with pytest.raises(FsilError, match="out of min bound"):
+        run_device(pc2, device_fem=True, wait=False)
 
 
 @pytest.mark.parametrize("name", ["ibm_heaving_plate", "ibm_pitching_two_plates"])
