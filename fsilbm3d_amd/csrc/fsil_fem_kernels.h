@@ -150,13 +150,12 @@ __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// sum of terms[0..n) in order by one thread, broadcast
+// sum of terms[0..n) in order by one thread, broadcast.  The caller passes
+// a block barrier before *shared is written again.
 __device__ inline double fem_serial_sum(const double *terms, int n, double *shared) {
   if (threadIdx.x == 0) *shared = ordered_sum(0.0, terms, n);
   fem_sync();
-  const double v = *shared;
-  fem_sync();
-  return v;
+  return *shared;
 }
 
 // z = M^-1 r (block Jacobi); optionally leaves the products r_g r_g in
@@ -181,14 +180,14 @@ __device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, con
   fem_sync();
 }
 
-// the two ordered sums of prod[0..n) and prod[cap..cap+n) on two waves
+// the two ordered sums of prod[0..n) and prod[cap..cap+n) on two waves,
+// broadcast.  The caller passes a block barrier before shared is written again.
 __device__ inline void fem_pair_sums(const double *prod, int cap, int n, double *shared, double &r1, double &r2) {
   if (threadIdx.x == 0) shared[0] = ordered_sum(0.0, prod, n);
   else if (threadIdx.x == 64) shared[1] = ordered_sum(0.0, prod + cap, n);
   fem_sync();
   r1 = shared[0];
   r2 = shared[1];
-  fem_sync();
 }
 
 // Dot products in DOF order, broadcast through LDS.  The products a_i*b_i
@@ -531,18 +530,39 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
         const double pAp = fem_serial_sum(lds, nG, sh_pair);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
-        for (int g = threadIdx.x; g < nG; g += blockDim.x) {
-          double xv = V.x[g] + alpha * V.p[g];
-          if (V.fixed[g]) xv = V.xF[g];
-          V.x[g] = xv;
-          double rv = V.r[g] - alpha * V.Ap[g];
-          if (V.fixed[g]) rv = 0.0;
-          V.r[g] = rv;
+        // x, r update, then z = M^-1 r (formed before the convergence test,
+        // unused when the loop stops there, so that r.r and r.z are summed
+        // side by side), with sixty DOFs (ten whole nodes) per wave: a node's
+        // block-Jacobi row reads only r values of its own wave
+        {
+          const int nw = (int)blockDim.x >> 6, wv = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+          for (int base = 0; base < nG; base += 60 * nw) {
+            const int g = base + wv * 60 + ln;
+            const bool on = ln < 60 && g < nG;
+            if (on) {
+              double xv = V.x[g] + alpha * V.p[g];
+              if (V.fixed[g]) xv = V.xF[g];
+              V.x[g] = xv;
+              double rv = V.r[g] - alpha * V.Ap[g];
+              if (V.fixed[g]) rv = 0.0;
+              V.r[g] = rv;
+            }
+            wave_sync();
+            if (on) {
+              const int node = g / 6, i = g - node * 6;
+              const double *B = V.blk + (size_t)node * 36;
+              double sz = 0.0;
+#pragma unroll
+              for (int k = 0; k < 6; ++k) sz = sz + B[i * 6 + k] * V.r[6 * node + k];
+              const double z = V.fixed[g] ? 0.0 : sz;
+              V.z[g] = z;
+              const double r = V.r[g];
+              lds[g] = r * r;
+              lds[cap + g] = r * z;
+            }
+          }
         }
         fem_sync();
-        // z = M^-1 r is formed before the convergence test (unused when the
-        // loop stops there) so that r.r and r.z are summed side by side
-        fem_precond(A, D, V, lds, cap);
         double rsnew;
         fem_pair_sums(lds, cap, nG, sh_pair, rr, rsnew);
         rn = sqrt(rr);
