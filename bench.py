@@ -23,7 +23,9 @@ the bodies already resident on the device.
 
 Beside it: the stencil benchmark (D3Q19 512x256x256 fully periodic, SRT tau
 0.55, seeded equilibrium) in the FP64 plain and FP32-deviation forms, with the
-FP32 velocity L2 error against FP64.
+FP32 velocity L2 error against FP64; and the school (1024x384x384, 16 plates
+of the flexible-plate case on a 4x4 diamond lattice) in both forms, with the
+per-body force error of FP32 against FP64 after the same steps.
 
 roofline: the fused collide+stream kernel of the headline form's FSI step
 (the band-overlay variant), algorithmic bytes per launch 19 reads + 19 writes
@@ -86,7 +88,7 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_
     N = int(np.prod(w.dims))
     blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu, precision=precision)
     blk.set_flow(**w.flow_kwargs())
-    blk.upload(w.initial_state())
+    blk.fill(w.equilibrium())
     blk.update_volume_force(0.0)
     blk.set_boundary_conditions()
     fsi = DeviceBodyBlock(blk, [w.plate()], **w.solver_kwargs())
@@ -134,6 +136,45 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_
         out["X"] = np.concatenate(XS)
         out["Y"] = np.concatenate(YS)
     return out
+
+
+def run_school(steps, warmup, precision):
+    """The school configuration (1024x384x384, 16 flexible plates): timed
+    steps, then each body's force (drag, lift, side) after the last step and
+    the band-overlay fused kernel's duration."""
+    import numpy as np
+    from fsilbm3d_amd.block import LBMBlock, synchronize
+    from fsilbm3d_amd.fsi import DeviceBodyBlock
+    from fsilbm3d_amd.workloads import school
+    w = school()
+    N = int(np.prod(w.dims))
+    blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu, precision=precision)
+    blk.set_flow(**w.flow_kwargs())
+    blk.fill(w.equilibrium())
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    fsi = DeviceBodyBlock(blk, w.plates(), **w.solver_kwargs())
+    t = 0.0
+    for _ in range(warmup):
+        t = t + 1.0
+        fsi.step(t)
+    synchronize()
+    its = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t = t + 1.0
+        fsi.step(t)
+        its.append(fsi.iters)
+    synchronize()
+    secs = time.perf_counter() - t0
+    loads = np.array([fsi.body_loads(i).reshape(-1, 6).sum(0)[:3] for i in range(len(fsi.nND))])
+    fsi.interaction_force(blk.dh)
+    kern_ms = blk.time_collide(5)
+    synchronize()
+    blk.close()
+    return dict(mlups=N * steps / secs / 1e6, ms_per_step=secs / steps * 1e3, kernel_ms=kern_ms,
+                achieved_gbs=B_CELL[precision] * N / (kern_ms * 1e-3) / 1e9, loads=loads, cells=N,
+                mdf_iters_mean=float(np.mean(its)), bodies=len(loads))
 
 
 def force_error(a, b):
@@ -265,6 +306,8 @@ def main():
     ap.add_argument("--nlaunch", type=int, default=50)
     ap.add_argument("--l2-steps", type=int, default=1000)
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--school-steps", type=int, default=20,
+                    help="timed steps of the 16-plate 1024x384x384 school in each form (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--pmc-driver", nargs="?", const="stencil", default=None, choices=["stencil", "fsi"],
@@ -275,6 +318,7 @@ def main():
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
               "decomposition is a design note, not a measured figure.", file=sys.stderr)
         sys.exit(2)
+    import numpy as np
     from tests.golden.cases import seeded_state
     from fsilbm3d_amd.workloads import flexible_plate
     if args.pmc_driver:
@@ -309,6 +353,27 @@ def main():
         log(f"bench: FSI FP32+learned {fsiL['mlups']:.1f} MLUPS, force error {errL}")
         learned = (fsiL, errL, tinfo)
         net.close()
+    school = None
+    if args.school_steps > 0:
+        log(f"bench: school 1024x384x384, 16 plates, {args.school_steps} steps per form")
+        s64 = run_school(args.school_steps, 2, 0)
+        s32 = run_school(args.school_steps, 2, 1)
+        scale = np.abs(s64["loads"][:, :2]).max(axis=0)
+        serr = np.abs(s32["loads"][:, :2] - s64["loads"][:, :2]).max(axis=0) / scale
+        school = {"workload": "school: D3Q19 1024x384x384, 16 flexible plates as the flexible-plate case on a 4x4 "
+                              "diamond lattice (2 L x 1.5 L), per-plate heave phase",
+                  "cells": s32["cells"], "bodies": s32["bodies"], "markers": 16 * 2304, "steps": args.school_steps,
+                  "form": "fp32-deviation", "value": s32["mlups"], "unit": "MLUPS", "ms_per_step": s32["ms_per_step"],
+                  "mdf_iters_mean": s32["mdf_iters_mean"],
+                  "roofline": {"bound": "hbm", "achieved": s32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": s32["achieved_gbs"] / HBM_PEAK_GBS, "kernel": "k_collide_push32<false,true>",
+                               "kernel_ms": s32["kernel_ms"], "bytes_per_cell": B_CELL[1]},
+                  "fp64_plain": {"value": s64["mlups"], "ms_per_step": s64["ms_per_step"],
+                                 "kernel_ms": s64["kernel_ms"]},
+                  "speedup_vs_fp64_plain": s32["mlups"] / s64["mlups"],
+                  "per_body_force_error_vs_fp64_plain": {"drag": float(serr[0]), "lift": float(serr[1]),
+                                                         "after_steps": args.school_steps + 2}}
+        log(f"bench: school FP32 {s32['mlups']:.1f} MLUPS, FP64 {s64['mlups']:.1f} MLUPS, per-body error {serr}")
     log("bench: seeding 512x256x256 state")
     f0 = seeded_state(DIMS, 20261202)
     log("bench: stencil FP64 plain form")
@@ -387,6 +452,7 @@ def main():
                                  "kernel": "k_collide_push<1,false,false,false>", "kernel_ms": r64["kernel_ms"],
                                  "bytes_per_cell": B_CELL[0]}},
     }
+    out["school"] = school
     if r32 is not None:
         out["stencil"]["fp32_deviation"] = {
             "value": r32["mlups"], "unit": "MLUPS", "ms_per_step": r32["ms_per_step"],

@@ -30,6 +30,7 @@ SIGNATURES = {
     "fsil_set_time": (_i, [_i, _d]),
     "fsil_get_volume_force": (_i, [_i, _pd]),
     "fsil_f_upload": (_i, [_i, _pd]),
+    "fsil_f_fill": (_i, [_i, _pd]),
     "fsil_f_download": (_i, [_i, _pd]),
     "fsil_set_boundary_conditions": (_i, [_i]),
     "fsil_macro_download": (_i, [_i, _pd, _pd]),

@@ -84,6 +84,11 @@ class LBMBlock:
         assert f.shape == (19,) + self.dims
         _lib.check(self.lib.fsil_f_upload(self.handle, _lib.dptr(f)))
 
+    def fill(self, f19) -> None:
+        """Every cell set to the same 19 populations (a uniform initial state)."""
+        v = np.ascontiguousarray(np.asarray(f19, dtype=np.float64).reshape(19))
+        _lib.check(self.lib.fsil_f_fill(self.handle, _lib.dptr(v)))
+
     def download(self) -> np.ndarray:
         f = np.empty((19,) + self.dims, dtype=np.float64)
         _lib.check(self.lib.fsil_f_download(self.handle, _lib.dptr(f)))

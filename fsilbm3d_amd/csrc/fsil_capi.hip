@@ -823,6 +823,25 @@ int fsil_f_upload(int h, const double *fIn) {
   return 0;
 }
 
+int fsil_f_fill(int h, const double *f19) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!f19) return fail("null populations");
+  const dim3 grid((unsigned)((b->N + 255) / 256));
+  if (b->precision == 0) {
+    Fill19 a;
+    for (int q = 0; q < Q; ++q) a.v[q] = f19[q];
+    hipLaunchKernelGGL(k_fill19, grid, dim3(256), 0, b->stream, b->fA.p, b->N, a);
+  } else {
+    Fill19f a;  // FP32 deviation storage: g = f - w, rounded once (as fsil_f_upload)
+    for (int q = 0; q < Q; ++q) a.v[q] = (float)(f19[q] - wt(q));
+    hipLaunchKernelGGL(k_fill19f, grid, dim3(256), 0, b->stream, b->gA.p, b->N, a);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
 int fsil_f_download(int h, double *fIn) {
   Block *b = get_block(h);
   if (!b) return 1;

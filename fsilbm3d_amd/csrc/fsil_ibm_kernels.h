@@ -335,6 +335,26 @@ __global__ void k_zero(double *__restrict__ p, long long n) {
   if (i < n) p[i] = 0.0;
 }
 
+// every cell set to the same 19 populations (a uniform initial state)
+struct Fill19 {
+  double v[Q];
+};
+struct Fill19f {
+  float v[Q];
+};
+__global__ __launch_bounds__(256) void k_fill19(double *__restrict__ f, long long N, Fill19 a) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q * N + n] = a.v[q];
+}
+__global__ __launch_bounds__(256) void k_fill19f(float *__restrict__ g, long long N, Fill19f a) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+#pragma unroll
+  for (int q = 0; q < Q; ++q) g[q * N + n] = a.v[q];
+}
+
 // plain copy (bandwidth calibration of the HBM counters and the roofline)
 __global__ __launch_bounds__(256) void k_copy(const double *__restrict__ a, double *__restrict__ b,
                                               long long n) {

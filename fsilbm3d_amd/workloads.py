@@ -9,6 +9,11 @@ flexible_plate:   D3Q19 384x192x192, uniform inflow U = 0.05 (101) at x-min, out
                   co-rotational beam), mass ratio 1, KB = 0.1, KS = 1000, leading-edge node clamped
                   and heaving with amplitude 0.3 L at 0.2 U/L; numsubstep 4, multi-direct forcing to
                   1e-6 (at most 10 iterations).
+school:           D3Q19 1024x384x384 (151 M cells), the flexible-plate flow and solver settings, 16 such
+                  plates on a 4x4 diamond lattice (2 L streamwise, 1.5 L spanwise, odd rows shifted by 1 L),
+                  leading edges from x = 256, rows centred in z at y = 192, each plate its own group with
+                  heave phase U(0, 360) degrees from numpy's default generator seeded 20261202
+                  (36,864 markers, 16 beam bodies).
 """
 from __future__ import annotations
 
@@ -67,11 +72,16 @@ class FlexiblePlate:
     def flow_kwargs(self) -> dict:
         return dict(denIn=1.0, uvwIn=(self.U, 0.0, 0.0))
 
+    def equilibrium(self) -> np.ndarray:
+        """The 19 populations of the uniform inflow state (initialise_flow,
+        FluidDomain.f90:524-544); every cell holds them at the start."""
+        return np.asarray(P.equilibrium_point(1.0, (self.U, 0.0, 0.0)), np.float64)
+
     def initial_state(self) -> np.ndarray:
         """Uniform-flow equilibrium, as initialise_flow leaves the block
         (FluidDomain.f90:524-544); 19 constants broadcast over the block."""
         f = np.empty((19,) + tuple(self.dims), np.float64)
-        feq = P.equilibrium_point(1.0, (self.U, 0.0, 0.0))
+        feq = self.equilibrium()
         for q in range(19):
             f[q].fill(feq[q])
         return f
@@ -79,3 +89,38 @@ class FlexiblePlate:
 
 def flexible_plate(dims=(384, 192, 192)) -> FlexiblePlate:
     return FlexiblePlate(dims=tuple(dims))
+
+
+@dataclasses.dataclass
+class School(FlexiblePlate):
+    """Sixteen flexible plates of the flexible-plate case in one block."""
+    dims: Tuple[int, int, int] = (1024, 384, 384)
+    rows: int = 4
+    cols: int = 4
+    x0: float = 256.0
+    seed: int = 20261202
+
+    def leading_edges(self):
+        dx, dz = 2.0 * self.L, 1.5 * self.L
+        z0 = 0.5 * (self.dims[2] - (self.rows - 1) * dz)
+        out = []
+        for j in range(self.rows):
+            for i in range(self.cols):
+                out.append((self.x0 + i * dx + (0.5 * dx if j % 2 else 0.0), 0.5 * self.dims[1], z0 + j * dz))
+        return out
+
+    def phases_deg(self) -> np.ndarray:
+        return np.random.default_rng(self.seed).uniform(0.0, 360.0, self.rows * self.cols)
+
+    def plates(self):
+        out = []
+        for le, ph in zip(self.leading_edges(), self.phases_deg()):
+            one = FlexiblePlate(dims=self.dims, U=self.U, L=self.L, Re=self.Re, le=le, amp=self.amp, freq=self.freq,
+                                numsubstep=self.numsubstep, ntolLBM=self.ntolLBM, dtolLBM=self.dtolLBM, KB=self.KB,
+                                KS=self.KS, denR=self.denR, body_model=self.body_model, bc=self.bc).plate()
+            out.append(dataclasses.replace(one, XYZPhi=(0.0, float(ph), 0.0)))
+        return out
+
+
+def school(dims=(1024, 384, 384)) -> School:
+    return School(dims=tuple(dims))

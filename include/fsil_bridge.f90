@@ -11,7 +11,7 @@ module fsil_bridge
     implicit none
     private
     public :: fsil_init, fsil_finalize, fsil_synchronize, fsil_block_create, fsil_block_destroy, &
-              fsil_set_flow, fsil_set_time, fsil_get_volume_force, fsil_f_upload, fsil_f_download, &
+              fsil_set_flow, fsil_set_time, fsil_get_volume_force, fsil_f_upload, fsil_f_fill, fsil_f_download, &
               fsil_set_boundary_conditions, fsil_macro_download, fsil_collide_stream_bc, fsil_steps, &
               fsil_ibm_define_bodies, fsil_ibm_set_params, fsil_ibm_interaction_force, fsil_check, &
               fsil_version, fsil_time_collide, fsil_time_steps, fsil_ibm_band_info, fsil_copy_bandwidth, &
@@ -73,6 +73,11 @@ module fsil_bridge
             import :: c_int, c_double
             integer(c_int), value :: handle
             real(c_double), intent(in) :: fIn(*)
+        end function
+        integer(c_int) function fsil_f_fill(handle, f19) bind(C, name='fsil_f_fill')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), intent(in) :: f19(19)
         end function
         integer(c_int) function fsil_f_download(handle, fIn) bind(C, name='fsil_f_download')
             import :: c_int, c_double

@@ -71,6 +71,9 @@ int fsil_get_volume_force(int handle, double *vf);
 /* Copy fIn(z,y,x,0:18) (pre-collision, post-boundary state) to / from the
  * device.  Replaces nothing: the reference's fIn lives in host memory. */
 int fsil_f_upload(int handle, const double *fIn);
+/* Every cell set to the same 19 populations f19[q]: the uniform state of
+ * initialise_flow (FluidDomain.f90:524-544) without a host copy of the field. */
+int fsil_f_fill(int handle, const double *f19);
 int fsil_f_download(int handle, double *fIn);
 
 /* set_boundary_conditions_ on the stored state as called once at start-up
