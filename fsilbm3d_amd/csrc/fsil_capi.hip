@@ -1145,7 +1145,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   const int warm = (net && !done0) ? 1 : 0;
   hipLaunchKernelGGL(k_ibm_reset, dim3(std::min(1024, (E + 255) / 256)), dim3(256), 0, s, I.ctl.p, I.dcell.p,
-                     I.cellmap.p, I.dstate.p, done0, warm, E, I.scan.p, I.dptr.p);
+                     I.cellmap.p, I.dstate.p, done0, warm, E, I.scan.p, I.dptr.p, I.Eforce.p, 3 * M);
   HIPCHK(hipGetLastError());
   StencilArgs sa{};
   sa.M = M;
@@ -1168,8 +1168,14 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   hipLaunchKernelGGL(k_ibm_claim, dim3(nclaim), dim3(kClaimThreads), 0, s, E, I.keys.p, I.cellmap.p, I.dcell.p,
                      b->yd, b->zd, I.blkbox.p, I.ctl.p);
   HIPCHK(hipGetLastError());
+  // calculate_interaction_force (Solidbody.f90:869-918): band velocity of the stored state with the count
+  const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
+  BandMacroArgs ma{};
+  ma.f = b->precision == 0 ? b->fA.p : nullptr;
+  ma.g = b->precision == 0 ? nullptr : b->gA.p;
+  ma.N = b->N; ma.cell = I.dcell.p; ma.cap = I.cap; ma.hv0 = hv0; ma.hv1 = hv1; ma.hv2 = hv2; ma.band_u = I.band_u.p;
   hipLaunchKernelGGL(k_ibm_count, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys.p, I.cellmap.p, I.dbidx.p,
-                     I.scan.p, I.flags.p, I.ctl.p);
+                     I.scan.p, I.flags.p, I.ctl.p, ma);
   HIPCHK(hipGetLastError());
   size_t tb = I.cub_bytes;
   HIPCHK(hipcub::DeviceScan::InclusiveSum(I.cub_tmp.p, tb, I.scan.p, I.dptr.p + 1, E, s));
@@ -1177,17 +1183,6 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                      I.tmp.p);
   hipLaunchKernelGGL(k_ibm_rank, dim3((E + 255) / 256), dim3(256), 0, s, E, I.ctl.p, I.dbidx.p, I.dptr.p, I.tmp.p,
                      I.dentry.p, I.blkbox.p, nclaim);
-  HIPCHK(hipGetLastError());
-  // calculate_interaction_force (Solidbody.f90:869-918)
-  const double hv0 = 0.5 * b->VF[0] * b->dh, hv1 = 0.5 * b->VF[1] * b->dh, hv2 = 0.5 * b->VF[2] * b->dh;
-  if (b->precision == 0)
-    hipLaunchKernelGGL(k_band_macro, dim3((E + 255) / 256), dim3(256), 0, s, b->fA.p, b->N, I.dcell.p, I.ctl.p,
-                       I.cap, hv0, hv1, hv2, I.band_u.p);
-  else
-    hipLaunchKernelGGL(k_band_macro32, dim3((E + 255) / 256), dim3(256), 0, s, b->gA.p, b->N, I.dcell.p,
-                       I.ctl.p, I.cap, hv0, hv1, hv2, I.band_u.p);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_zero, dim3((3 * M + 255) / 256), dim3(256), 0, s, I.Eforce.p, 3LL * M);
   HIPCHK(hipGetLastError());
   const double invh3p = 0.5 * dt * ((1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh)) / I.denIn;
   double dsum = 0.0;

@@ -75,16 +75,19 @@ __host__ __device__ inline int trim4(int i, int nd, int *out, int bclo, int bchi
   return err;
 }
 
-// reset of the control words; clears the previous band from the cell map and
-// the per-cell entry counts (E of them) of this call
+// reset of the control words; clears the previous band from the cell map,
+// the per-cell entry counts (E of them) of this call and the marker forces
 __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const long long *__restrict__ prev_cell,
                                                    int *__restrict__ cellmap, double *__restrict__ dstate,
                                                    int done0, int warm, int E, int *__restrict__ count,
-                                                   int *__restrict__ band_ptr) {
+                                                   int *__restrict__ band_ptr, double *__restrict__ Eforce,
+                                                   int n3M) {
   const int nprev = ctl[kNband];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nprev; i += gridDim.x * blockDim.x)
     cellmap[prev_cell[i]] = -1;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < E; i += gridDim.x * blockDim.x) count[i] = 0;
+  // v_Eforce = 0 (Solidbody.f90:889)
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n3M; i += gridDim.x * blockDim.x) Eforce[i] = 0.0;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     band_ptr[0] = 0;
     ctl[kNew] = 0;
@@ -217,16 +220,39 @@ __global__ __launch_bounds__(kClaimThreads) void k_ibm_claim(int E, const unsign
   }
 }
 
+__device__ __forceinline__ void band_macro_cell(const double *__restrict__ f, long long N, long long n, int b,
+                                                int cap, double hv0, double hv1, double hv2,
+                                                double *__restrict__ band_u);
+__device__ __forceinline__ void band_macro32_cell(const float *__restrict__ g, long long N, long long n, int b,
+                                                  int cap, double hv0, double hv1, double hv2,
+                                                  double *__restrict__ band_u);
+
+// the band velocity of the stored state (FP64 field f, or FP32 deviations g)
+struct BandMacroArgs {
+  const double *f;
+  const float *g;
+  long long N;
+  const long long *cell;
+  int cap;
+  double hv0, hv1, hv2;
+  double *band_u;
+};
+
 // every entry: its band cell and a slot among the cell's entries (the slots
 // are in arrival order; k_ibm_rank restores entry order).  A cell without a
 // band number would be an inconsistent build: reported in ctl[kErr] (code 4),
-// the entry skipped.
+// the entry skipped.  Thread b < nband also forms band cell b's velocity
+// from the stored state (k_band_macro's value).
 __global__ __launch_bounds__(256) void k_ibm_count(int E, const unsigned *__restrict__ keys,
                                                    const int *__restrict__ cellmap, int *__restrict__ bidx,
                                                    int *__restrict__ count, int *__restrict__ slot,
-                                                   int *__restrict__ ctl) {
+                                                   int *__restrict__ ctl, BandMacroArgs ma) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
+  if (e < ctl[kNew]) {
+    if (ma.f) band_macro_cell(ma.f, ma.N, ma.cell[e], e, ma.cap, ma.hv0, ma.hv1, ma.hv2, ma.band_u);
+    else band_macro32_cell(ma.g, ma.N, ma.cell[e], e, ma.cap, ma.hv0, ma.hv1, ma.hv2, ma.band_u);
+  }
   const int b = cellmap_load(&cellmap[keys[e]]);
   if (b < 0 || b >= E) {
     atomicMin(&ctl[kErr], 4);
@@ -284,15 +310,9 @@ __global__ __launch_bounds__(256) void k_ibm_rank(int E, int *__restrict__ ctl, 
 
 // band macros: den/u of every band cell from the stored state, same
 // expressions as the collision kernel (FluidDomain.f90:1136-1139)
-__global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f, long long N,
-                                                    const long long *__restrict__ cell,
-                                                    const int *__restrict__ ctl, int cap,
-                                                    double hv0, double hv1, double hv2,
-                                                    double *__restrict__ band_u, int only_on_fallback = 0) {
-  if (only_on_fallback && ctl[kFallback] != 1) return;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ctl[kNband]) return;
-  const long long n = cell[b];
+__device__ __forceinline__ void band_macro_cell(const double *__restrict__ f, long long N, long long n, int b,
+                                                int cap, double hv0, double hv1, double hv2,
+                                                double *__restrict__ band_u) {
   double g[Q];
 #pragma unroll
   for (int q = 0; q < Q; ++q) g[q] = f[q * N + n];
@@ -301,6 +321,17 @@ __global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f
   band_u[b] = (m1 + hv0) / s;
   band_u[cap + b] = (m2 + hv1) / s;
   band_u[2LL * cap + b] = (m3 + hv2) / s;
+}
+
+__global__ __launch_bounds__(256) void k_band_macro(const double *__restrict__ f, long long N,
+                                                    const long long *__restrict__ cell,
+                                                    const int *__restrict__ ctl, int cap,
+                                                    double hv0, double hv1, double hv2,
+                                                    double *__restrict__ band_u, int only_on_fallback = 0) {
+  if (only_on_fallback && ctl[kFallback] != 1) return;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ctl[kNband]) return;
+  band_macro_cell(f, N, cell[b], b, cap, hv0, hv1, hv2, band_u);
 }
 
 // band macros of the FP32-deviation state (the immersed-boundary coupling
@@ -314,7 +345,12 @@ __global__ __launch_bounds__(256) void k_band_macro32(const float *__restrict__ 
   if (only_on_fallback && ctl[kFallback] != 1) return;
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ctl[kNband]) return;
-  const long long n = cell[b];
+  band_macro32_cell(g, N, cell[b], b, cap, hv0, hv1, hv2, band_u);
+}
+
+__device__ __forceinline__ void band_macro32_cell(const float *__restrict__ g, long long N, long long n, int b,
+                                                  int cap, double hv0, double hv1, double hv2,
+                                                  double *__restrict__ band_u) {
   double dr = 0.0, j1 = 0.0, j2 = 0.0, j3 = 0.0;
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
