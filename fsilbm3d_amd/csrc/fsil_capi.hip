@@ -219,6 +219,8 @@ struct Block {
   int model = 1;
   double tau = 1, Omega = 1, Omega2 = 1;
   int precision = 0;  // 0 FP64 plain, 1 FP32 deviation storage
+  int nt = 0;         // population streams nontemporal (see stream_policy)
+  bool nt_forced = false;  // FSIL_STREAM_HINT set at creation
   DevBuf<double> fA, fB;
   DevBuf<float> gA, gB;
   DevBuf<double> Mc, Mf;
@@ -369,12 +371,43 @@ int launch_faces(Block &b) {
 
 dim3 cell_block(const Block &b) {
   // one block spans whole z-rows where possible so the z-shifted stores of a
-  // row stay inside one workgroup (and one XCD's L2)
-  int tz = b.zd <= 256 ? b.zd : 256;
+  // row stay inside one workgroup (and one XCD's L2); longer rows are cut into
+  // the width of 256, 192 or 128 that leaves the fewest idle lanes
+  // (384-cell rows: two blocks of 192 rather than 256 + a half-idle 256)
+  int tz = b.zd;
+  if (b.zd > 256) {
+    int best = 0;
+    for (int w : {256, 192, 128}) {
+      const int lanes = (b.zd + w - 1) / w * w;
+      if (best == 0 || lanes < (b.zd + best - 1) / best * best) best = w;
+    }
+    tz = best;
+  }
   int ty = std::max(1, 256 / tz);
   ty = std::min(ty, b.yd);
   return dim3(tz, ty, 1);
 }
+
+// Cache policy of the fused kernel's population streams.  Measured on MI355X
+// (both storage forms, periodic blocks): plain loads and stores are 8-12%
+// faster at 384x192x192 and 512x256x256 (1-5 GB per population copy), where
+// the caches merge the partial lines of the z-shifted stores; nontemporal ones
+// are 3-14% faster at 1024x384x384 (11.5-23 GB per copy).  The switch is at
+// 8 GiB per copy.  Blocks with coupled bodies always stream nontemporally
+// (stream_hint_for): there the beam solver runs beside the collide kernel and
+// the plain form's cache footprint slows the whole step (flexible plate,
+// 384x192x192: 0.92 -> 1.19 ms FP32-deviation, 1.24 -> 1.43 ms FP64, though
+// the collide kernel itself is 11% faster).  FSIL_STREAM_HINT=nt|plain
+// overrides the choice (at block creation) for tests that run both.
+int stream_policy(long long N, int precision) {
+  const char *e = getenv("FSIL_STREAM_HINT");
+  if (e && !strcmp(e, "nt")) return 1;
+  if (e && !strcmp(e, "plain")) return 0;
+  const double copy_bytes = (double)Q * (double)N * (precision == 0 ? 8.0 : 4.0);
+  return copy_bytes > 8.0 * 1024 * 1024 * 1024 ? 1 : 0;
+}
+
+int stream_hint_for(const Block &b, bool band) { return band && !b.nt_forced ? 1 : b.nt; }
 
 template <int MODEL>
 void launch_collide_model(const CollideArgs &a, bool force, bool band, bool aux, dim3 grid,
@@ -403,6 +436,7 @@ int collide_stream(Block &b) {
   a.Mf = b.Mf.p;
   const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
   const bool band = b.ibm.active && b.ibm.M > 0;
+  a.nt = stream_hint_for(b, band);
   if (band) {
     a.ctl = b.ibm.ctl.p;
     a.cellmap = b.ibm.cellmap.p;
@@ -449,6 +483,7 @@ int collide_stream32(Block &b) {
   }
   const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
   const bool band = b.ibm.active && b.ibm.M > 0;
+  a.nt = stream_hint_for(b, band);
   if (band) {
     a.ctl = b.ibm.ctl.p;
     a.cellmap = b.ibm.cellmap.p;
@@ -687,6 +722,8 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   if (iCollidModel == 3 && (!m_collid || !m_force)) return fail("MRT needs M_COLLID and M_FORCE");
   if (!(precision == 0 || precision == 1)) return fail("precision must be 0 (FP64) or 1 (FP32 deviation)");
   if (precision == 1 && iCollidModel != 1) return fail("FP32-deviation form supports SRT only");
+  if (precision == 1 && (long long)dims[0] * dims[1] * dims[2] > (1LL << 30))
+    return fail("FP32-deviation blocks hold at most 2^30 cells (32-bit byte offsets per population plane)");
   if (precision == 1)
     for (int k = 0; k < 6; ++k)
       if (bc[k] == BC_NEQ_DIRICHLET || bc[k] == BC_WALL_HALFWAY || bc[k] == BC_MOVING_WALL_HALFWAY)
@@ -712,6 +749,8 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   b.model = iCollidModel;
   b.tau = relax[0]; b.Omega = relax[1]; b.Omega2 = relax[2];
   b.precision = precision;
+  b.nt = stream_policy(b.N, precision);
+  b.nt_forced = getenv("FSIL_STREAM_HINT") != nullptr;
   b.sh = std::make_shared<StreamHolder>();
   HIPCHK(hipStreamCreateWithFlags(&b.sh->s, hipStreamNonBlocking));
   b.stream = b.sh->s;

@@ -30,6 +30,78 @@ __host__ __device__ constexpr double wt(int q) {
   return q == 0 ? 1.0 / 3.0 : (q <= 6 ? 1.0 / 18.0 : 1.0 / 36.0);
 }
 
+// Population planes in global memory.  A uniform plane base is kept in scalar
+// registers (the empty asm stops the compiler from folding it back into
+// per-lane 64-bit address arithmetic), so every access is one scalar base plus
+// a 32-bit per-lane byte offset, and the base advances by one plane per
+// direction with two scalar adds.
+template <typename T> using gptr = __attribute__((address_space(1))) T *;
+template <typename T>
+__device__ __forceinline__ gptr<T> sbase(gptr<T> g) {
+  asm volatile("" : "+s"(g));
+  return g;
+}
+
+// a 32-bit per-lane offset defined afresh where it is used, so the address
+// selection there sees its zero extension (scalar base + 32-bit offset form)
+__device__ __forceinline__ unsigned voff(unsigned o) {
+  asm volatile("" : "+v"(o));
+  return o;
+}
+
+// The 19 values of one cell, v[q] = plane q at byte offset off.  nt streams
+// them past the caches (nontemporal); the choice is uniform over a launch.
+template <typename T>
+__device__ __forceinline__ void load_cell(T (&v)[Q], const T *planes, size_t plane_bytes, unsigned off, bool nt) {
+  gptr<const char> base = sbase((gptr<const char>)planes);
+  if (nt) {
+    const unsigned o = voff(off);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      v[q] = __builtin_nontemporal_load((gptr<const T>)(base + o));
+      base = sbase(base + plane_bytes);
+    }
+  } else {
+    const unsigned o = voff(off);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      v[q] = *((gptr<const T>)(base + o));
+      base = sbase(base + plane_bytes);
+    }
+  }
+}
+
+// Push of one cell's post-collision values: v[q] to plane q at cell
+// n + step_x + step_y + step_z, where the steps are st[0..5] = (+x, -x, +y,
+// -y, +z, -z) for the axes e_q moves along (periodic wrap folded into them).
+// The destinations are formed inside each branch, next to their stores.
+template <typename T>
+__device__ __forceinline__ unsigned push_offset(int q, unsigned n, const unsigned (&st)[6]) {
+  const unsigned sx = ex(q) > 0 ? st[0] : (ex(q) < 0 ? st[1] : 0u);
+  const unsigned sy = ey(q) > 0 ? st[2] : (ey(q) < 0 ? st[3] : 0u);
+  const unsigned sz = ez(q) > 0 ? st[4] : (ez(q) < 0 ? st[5] : 0u);
+  return (n + sx + sy + sz) * (unsigned)sizeof(T);
+}
+
+template <typename T>
+__device__ __forceinline__ void store_cell(const T (&v)[Q], T *planes, size_t plane_bytes, unsigned n,
+                                           const unsigned (&st)[6], bool nt) {
+  gptr<char> base = sbase((gptr<char>)planes);
+  if (nt) {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      __builtin_nontemporal_store(v[q], (gptr<T>)(base + push_offset<T>(q, n, st)));
+      base = sbase(base + plane_bytes);
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      *((gptr<T>)(base + push_offset<T>(q, n, st))) = v[q];
+      base = sbase(base + plane_bytes);
+    }
+  }
+}
+
 // Boundary codes (ConstParams.f90:31-34)
 enum BcCode {
   BC_FLUID = 0,

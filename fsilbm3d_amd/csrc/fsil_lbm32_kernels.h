@@ -29,6 +29,7 @@ struct Collide32Args {
   const double *__restrict__ band_u;  // [3][cap]
   const double *__restrict__ band_F;  // [3][cap]
   int cap;
+  int nt;  // nontemporal population loads and stores (large blocks)
 };
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -43,6 +44,50 @@ __device__ __forceinline__ float dotu32(float u1, float u2, float u3) {
   return (float)ex(q) * u1 + (float)ey(q) * u2 + (float)ez(q) * u3;
 }
 
+// the nine opposite pairs (q, oppo(q)) by their lower member
+constexpr int kPairQ[9] = {1, 3, 5, 7, 8, 11, 12, 15, 16};
+
+// e_q . v for a D3Q19 direction, written out so that zero components cost nothing
+template <typename T>
+__device__ __forceinline__ T edot(int q, T v1, T v2, T v3) {
+  T s = T(-0.0);  // -0 + x folds to x
+  if (ex(q) > 0) s = s + v1;
+  if (ex(q) < 0) s = s - v1;
+  if (ey(q) > 0) s = s + v2;
+  if (ey(q) < 0) s = s - v2;
+  if (ez(q) > 0) s = s + v3;
+  if (ez(q) < 0) s = s - v3;
+  return s;
+}
+
+// Density deviation and momentum of the stored deviations, summed in FP64 (the
+// FP32 values convert exactly; the pair sums g_q + g_qbar and differences
+// g_q - g_qbar are formed first, so the momentum uses nine terms).
+__device__ __forceinline__ void moments32(const float *g, double &dr, double &j1, double &j2, double &j3) {
+  dr = (double)g[0];
+  j1 = -0.0; j2 = -0.0; j3 = -0.0;
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+    const int q = kPairQ[p];
+    const double a = (double)g[q], b = (double)g[oppo(q)];
+    dr = dr + (a + b);
+    const double d = a - b;
+    if (ex(q) > 0) j1 = j1 + d;
+    if (ex(q) < 0) j1 = j1 - d;
+    if (ey(q) > 0) j2 = j2 + d;
+    if (ey(q) < 0) j2 = j2 - d;
+    if (ez(q) > 0) j3 = j3 + d;
+    if (ez(q) < 0) j3 = j3 - d;
+  }
+}
+
+// keeps a uniform base address in scalar registers and out of the compiler's
+// address arithmetic, so a load takes it with a 32-bit per-lane offset
+// Collision of one cell in the deviation form, per opposite pair (q, qbar):
+//   A = drho + rho*(4.5 (u.e_q)^2 - 1.5 u.u),  B = 3 rho (u.e_q)
+//   g'_q = (1-Omega) g_q + Omega w (A + B),  g'_qbar = (1-Omega) g_qbar + Omega w (A - B)
+// and, in forced cells, the Guo term c1*3dh*w*(e.F (1 + 3 u.e) - u.F) added
+// (FluidDomain.f90:1217-1227 rearranged around w_q).
 template <bool FORCE, bool BAND>
 __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,24 +95,16 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const int x = blockIdx.z;
   if (z >= a.zd || y >= a.yd) return;
   const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
+  const unsigned nb = n << 2;  // blocks of this form hold at most 2^30 cells
   const size_t plane = (size_t)a.N * sizeof(float);
-  const char *gin = reinterpret_cast<const char *>(a.gin);
-  const unsigned nb = n << 2;
   float g[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q)
-    g[q] = __builtin_nontemporal_load(reinterpret_cast<const float *>(gin + q * plane + nb));
-  float dr = 0.f, cr = 0.f, j1 = 0.f, c1 = 0.f, j2 = 0.f, c2 = 0.f, j3 = 0.f, c3 = 0.f;
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    kahan_add(dr, cr, g[q]);
-    if (ex(q) != 0) kahan_add(j1, c1, ex(q) > 0 ? g[q] : -g[q]);
-    if (ey(q) != 0) kahan_add(j2, c2, ey(q) > 0 ? g[q] : -g[q]);
-    if (ez(q) != 0) kahan_add(j3, c3, ez(q) > 0 ? g[q] : -g[q]);
-  }
-  const float rho = 1.0f + dr;
-  const float inv = 1.0f / rho;
-  float u1 = (j1 + a.hv[0]) * inv, u2 = (j2 + a.hv[1]) * inv, u3 = (j3 + a.hv[2]) * inv;
+  load_cell(g, a.gin, plane, nb, a.nt != 0);
+  double dr64, j1, j2, j3;
+  moments32(g, dr64, j1, j2, j3);
+  const float dr = (float)dr64;
+  const float rho = (float)(1.0 + dr64);
+  const float inv = __builtin_amdgcn_rcpf(rho);
+  float u1 = ((float)j1 + a.hv[0]) * inv, u2 = ((float)j2 + a.hv[1]) * inv, u3 = ((float)j3 + a.hv[2]) * inv;
   bool forced = FORCE;
   float F1 = a.VF[0], F2 = a.VF[1], F3 = a.VF[2];
   if constexpr (BAND) {
@@ -84,29 +121,38 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
       }
     }
   }
-  const float uSqr = u1 * u1 + u2 * u2 + u3 * u3;
+  const float omc = 1.0f - a.Omega;
+  const float m15 = -1.5f * (u1 * u1 + u2 * u2 + u3 * u3);
+  const float rho3 = 3.0f * rho;
   float post[Q];
+  post[0] = __builtin_fmaf(omc, g[0], (a.Omega * (float)wt(0)) * __builtin_fmaf(rho, m15, dr));
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    float ux;
-    switch (q) {
-#define C(k) case k: ux = dotu32<k>(u1, u2, u3); break;
-      C(0) C(1) C(2) C(3) C(4) C(5) C(6) C(7) C(8) C(9) C(10) C(11) C(12) C(13) C(14) C(15) C(16) C(17) C(18)
-#undef C
-    }
-    const float w = (float)wt(q);
-    const float feqd = w * (dr + rho * (ux * (3.0f + 4.5f * ux) - 1.5f * uSqr));
-    float t = g[q] + a.Omega * (feqd - g[q]);
-    if (FORCE || BAND) {
-      if (forced) {
-        const float e1 = (float)ex(q), e2 = (float)ey(q), e3 = (float)ez(q);
-        const float fl = a.dt3 * w *
-                         ((e1 - u1 + 3.0f * ux * e1) * F1 + (e2 - u2 + 3.0f * ux * e2) * F2 +
-                          (e3 - u3 + 3.0f * ux * e3) * F3);
-        t = t + a.c1 * fl;
+  for (int p = 0; p < 9; ++p) {
+    const int q = kPairQ[p], o = oppo(q);
+    const float Ow = a.Omega * (float)wt(q);
+    const float ux = edot(q, u1, u2, u3);
+    const float A = __builtin_fmaf(rho, __builtin_fmaf(4.5f * ux, ux, m15), dr);
+    const float B = rho3 * ux;
+    const float OwA = Ow * A, OwB = Ow * B;
+    post[q] = __builtin_fmaf(omc, g[q], OwA + OwB);
+    post[o] = __builtin_fmaf(omc, g[o], OwA - OwB);
+  }
+  if (FORCE || BAND) {
+    if (forced) {
+      const float cdt = a.c1 * a.dt3;
+      const float uF = u1 * F1 + u2 * F2 + u3 * F3;
+      post[0] = __builtin_fmaf(cdt * (float)wt(0), -uF, post[0]);
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        const int q = kPairQ[p], o = oppo(q);
+        const float k = cdt * (float)wt(q);
+        const float ux = edot(q, u1, u2, u3);
+        const float eF = edot(q, F1, F2, F3);
+        const float c = __builtin_fmaf(3.0f * ux, eF, -uF);
+        post[q] = __builtin_fmaf(k, c + eF, post[q]);
+        post[o] = __builtin_fmaf(k, c - eF, post[o]);
       }
     }
-    post[q] = t;
   }
   const unsigned yz = (unsigned)a.yd * (unsigned)a.zd, zd = (unsigned)a.zd;
   const unsigned sxp = x + 1 == a.xd ? 0u - (unsigned)(a.xd - 1) * yz : yz;
@@ -115,15 +161,8 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
-  char *gout = reinterpret_cast<char *>(a.gout);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const unsigned sx = ex(q) > 0 ? sxp : (ex(q) < 0 ? sxm : 0u);
-    const unsigned sy = ey(q) > 0 ? syp : (ey(q) < 0 ? sym : 0u);
-    const unsigned sz = ez(q) > 0 ? szp : (ez(q) < 0 ? szm : 0u);
-    const unsigned d = n + sx + sy + sz;
-    __builtin_nontemporal_store(post[q], reinterpret_cast<float *>(gout + q * plane + (d << 2)));
-  }
+  const unsigned st[6] = {sxp, sxm, syp, sym, szp, szm};
+  store_cell(post, a.gout, plane, n, st, a.nt != 0);
 }
 
 // den/u of the FP32-deviation state, in FP64 output arrays

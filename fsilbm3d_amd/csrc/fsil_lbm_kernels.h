@@ -95,6 +95,7 @@ struct CollideArgs {
   double *nb_den[6];
   double *nb_u[6];   // [3][plane]
   double *hw[6];     // [q][plane]
+  int nt;            // nontemporal population loads and stores (large blocks)
 };
 
 // plane index of a cell on face `face` in the reference's (outer, inner) loop order
@@ -113,13 +114,10 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
   // accepts), so each access is a scalar plane base plus one 32-bit offset
   const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
   const size_t plane = (size_t)a.N * sizeof(double);
-  const char *fin = reinterpret_cast<const char *>(a.fin);
   const unsigned nb = n << 3;
 
   double f[Q];
-#pragma unroll
-  for (int q = 0; q < Q; ++q)
-    f[q] = __builtin_nontemporal_load(reinterpret_cast<const double *>(fin + q * plane + nb));
+  load_cell(f, a.fin, plane, nb, a.nt != 0);
 
   double rho, m1, m2, m3;
   moments(f, rho, m1, m2, m3);
@@ -247,15 +245,8 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
   const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
-  char *fout = reinterpret_cast<char *>(a.fout);
-#pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const unsigned sx = ex(q) > 0 ? sxp : (ex(q) < 0 ? sxm : 0u);
-    const unsigned sy = ey(q) > 0 ? syp : (ey(q) < 0 ? sym : 0u);
-    const unsigned sz = ez(q) > 0 ? szp : (ez(q) < 0 ? szm : 0u);
-    const unsigned d = n + sx + sy + sz;
-    __builtin_nontemporal_store(post[q], reinterpret_cast<double *>(fout + q * plane + (d << 3)));
-  }
+  const unsigned st[6] = {sxp, sxm, syp, sym, szp, szm};
+  store_cell(post, a.fout, plane, n, st, a.nt != 0);
 }
 
 template <int MODEL, bool FORCE, bool BAND, bool AUX>

@@ -1,0 +1,44 @@
+"""The fused collide/push kernel streams the populations with plain or with
+nontemporal cache hints, chosen per block by its size (FSIL_STREAM_HINT
+overrides the choice at block creation).  The hint changes only how memory is
+cached, so both give bit-identical states; the plain-hinted runs are the ones
+the other parity tests compare with the oracle at these sizes."""
+import numpy as np
+import pytest
+
+from tests.golden.cases import BY_NAME
+from tests.helpers import derived, flow_kwargs
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_block(pc, precision, nsteps):
+    from fsilbm3d_amd.block import LBMBlock
+    case = pc.case
+    d = derived(case)
+    blk = LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params, precision=precision)
+    blk.set_flow(**flow_kwargs(case))
+    blk.upload(pc.initial_state())
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    t = 0.0
+    for _ in range(nsteps):
+        t = t + case.dh
+        blk.update_volume_force(t)
+        blk.collision_streaming_bc()
+    f = blk.download()
+    blk.close()
+    return f
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("name", ["periodic_srt_force", "bc_inflow_outflow_slip"])
+def test_stream_hints_bit_identical(name, precision, monkeypatch):
+    pc = BY_NAME[name]
+    out = {}
+    for hint in ("plain", "nt"):
+        print(f"[gpu] stream hint {hint}: {name} precision {precision}", flush=True)
+        monkeypatch.setenv("FSIL_STREAM_HINT", hint)
+        out[hint] = _run_block(pc, precision, 20)
+    monkeypatch.delenv("FSIL_STREAM_HINT")
+    assert np.array_equal(out["plain"], out["nt"])
