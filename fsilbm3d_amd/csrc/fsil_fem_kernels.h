@@ -43,42 +43,48 @@ struct FemArrays {
 
 __device__ inline void fem_sync() { __syncthreads(); }
 
-// s + t[0] + t[1] + ... + t[n-1], in that order.  The terms are read sixteen
-// at a time into registers, the next sixteen while the current ones are
-// added, so only the chain of additions is serial.
+// s + t[0] + t[1] + ... + t[n-1], in that order.  The terms are read in
+// batches of sixteen into registers, the next batch while the current one is
+// added, so only the chain of additions is serial.  Every batch read is
+// unconditional (the last full batch is read once more, unused) so that the
+// wait before a batch's additions covers that batch alone and not the
+// prefetch behind it.
 __device__ inline double ordered_sum(double s, const double *t, int n) {
   constexpr int B = 16;
-  int i = 0;
-  if (n >= B) {
+  const int nb = n / B;
+  // drain every outstanding memory access first (generic accesses elsewhere
+  // in the kernel count on the LDS counter too), so that inside the loop only
+  // the in-order LDS reads are pending and a batch waits for itself alone
+  __builtin_amdgcn_s_waitcnt(0x0070);  // vmcnt(0) expcnt(7) lgkmcnt(0)
+  if (nb > 0) {
     double a[B], b[B];
 #pragma unroll
     for (int j = 0; j < B; ++j) a[j] = t[j];
-    i = B;  // first term not yet read
+    int k = 0;
+    // the scheduling barriers keep each prefetch ahead of the additions it
+    // is meant to overlap
     for (;;) {
-      const bool more = i + B <= n;
-      if (more) {
+      const double *nx = t + B * min(k + 1, nb - 1);
 #pragma unroll
-        for (int j = 0; j < B; ++j) b[j] = t[i + j];
-      }
+      for (int j = 0; j < B; ++j) b[j] = nx[j];
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int j = 0; j < B; ++j) s = s + a[j];
-      if (!more) break;
-      i += B;
-      const bool more2 = i + B <= n;
-      if (more2) {
+      __builtin_amdgcn_sched_barrier(0);
+      if (++k == nb) break;
+      const double *nx2 = t + B * min(k + 1, nb - 1);
 #pragma unroll
-        for (int j = 0; j < B; ++j) a[j] = t[i + j];
-      }
+      for (int j = 0; j < B; ++j) a[j] = nx2[j];
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int j = 0; j < B; ++j) s = s + b[j];
-      if (!more2) break;
-      i += B;
+      __builtin_amdgcn_sched_barrier(0);
+      if (++k == nb) break;
     }
   }
-  for (; i < n; ++i) s = s + t[i];
+  for (int i = nb * B; i < n; ++i) s = s + t[i];
   return s;
 }
-
 
 // The conjugate-gradient vectors of one body: in LDS when they fit (the usual
 // case), else in the global work arrays; indexed by the body's DOF number.
