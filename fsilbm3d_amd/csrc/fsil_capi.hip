@@ -724,6 +724,8 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   if (precision == 1 && iCollidModel != 1) return fail("FP32-deviation form supports SRT only");
   if (precision == 1 && (long long)dims[0] * dims[1] * dims[2] > (1LL << 30))
     return fail("FP32-deviation blocks hold at most 2^30 cells (32-bit byte offsets per population plane)");
+  if (precision == 0 && (long long)dims[0] * dims[1] * dims[2] > (1LL << 29))
+    return fail("FP64 blocks hold at most 2^29 cells (32-bit byte offsets per population plane)");
   if (precision == 1)
     for (int k = 0; k < 6; ++k)
       if (bc[k] == BC_NEQ_DIRICHLET || bc[k] == BC_WALL_HALFWAY || bc[k] == BC_MOVING_WALL_HALFWAY)

@@ -42,3 +42,16 @@ def test_stream_hints_bit_identical(name, precision, monkeypatch):
         out[hint] = _run_block(pc, precision, 20)
     monkeypatch.delenv("FSIL_STREAM_HINT")
     assert np.array_equal(out["plain"], out["nt"])
+
+
+def test_block_size_caps():
+    """The fused kernels address a population plane with 32-bit byte offsets:
+    blocks beyond 2^29 cells (FP64) or 2^30 cells (FP32 deviations) are refused
+    at creation, before any allocation."""
+    from fsilbm3d_amd._lib import FsilError
+    from fsilbm3d_amd.block import LBMBlock
+    print("[gpu] block size caps", flush=True)
+    with pytest.raises(FsilError, match=r"2\^29"):
+        LBMBlock((1024, 1024, 513), precision=0)
+    with pytest.raises(FsilError, match=r"2\^30"):
+        LBMBlock((1024, 1024, 1025), precision=1)
