@@ -109,9 +109,10 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_
     t0 = time.perf_counter()
     for _ in range(steps):
         t = t + 1.0
-        # the coupling call waits for its own end (so the beam sub-steps are
-        # dispatched before the collide kernel fills the CUs); the timed run
-        # reads no forces back
+        # the beam sub-steps are queued behind the coupling call, the host then
+        # waits for the call's report before it launches the fluid update (so
+        # the beam workgroup is resident before the collide kernel fills the
+        # CUs); the timed run reads no forces back
         fsi.step(t)
         its.append(fsi.iters)
         dm.append(fsi.dmax)

@@ -43,6 +43,7 @@ SIGNATURES = {
     "fsil_ibm_interaction_force": (_i, [_i, _d, _pd, _pd, _pd, _pd, _pd, _pd, _pi, _pd]),
     "fsil_ibm_band_info": (_i, [_i, _pi, _pi]),
     "fsil_ibm_forces": (_i, [_i, _pd, _pd]),
+    "fsil_ibm_report": (_i, [_i, _pi, _pd]),
     "fsil_copy_bandwidth": (_i, [C.c_longlong, _i, _pd]),
     "fsil_tree_add_son": (_i, [_i, _i, _i, _pi]),
     "fsil_tree_pair_info": (_i, [_i, _i, _pi]),

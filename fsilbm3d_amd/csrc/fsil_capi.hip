@@ -1324,6 +1324,15 @@ int fsil_ibm_forces(int h, double *Eforce, double *lodFlow) {
   return ibm_check_report(*b);
 }
 
+int fsil_ibm_report(int h, int *iters, double *dmax) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (ibm_check_report(*b)) return 1;
+  if (iters) *iters = b->ibm.last_iters;
+  if (dmax) *dmax = b->ibm.last_dmax;
+  return 0;
+}
+
 int fsil_ibm_set_warm_start(int h, int mlp, int record) {
   Block *b = get_block(h);
   if (!b) return 1;
@@ -1642,10 +1651,12 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
       d.ntolFEM = F.ntolFEM; d.dtolFEM = F.dtolFEM; d.dampK = F.dampK; d.dampM = F.dampM; d.GeoGamma = F.GeoGamma;
       d.newton_iters = 0; d.cg_iters = 0; d.status = 0; d.dnorm = 0.0;
     }
-  // after the coupling call that produced lodFlow (block's stream)
+  // the descriptors go up first (the previous sub-steps are done with the
+  // buffer), then the kernel waits for the coupling call that produced
+  // lodFlow (block's stream)
+  HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
   HIPCHK(hipEventRecord(S.ev_in, b->stream));
   HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
-  HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
   hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, F.ddesc.p, F.nbodies, numsubstep, A,
                      lds_nodes, vec_lds, cg_lds, coef_off);
   HIPCHK(hipGetLastError());

@@ -143,11 +143,24 @@ class DeviceBodyBlock:
         _lib.check(self.lib.fsil_fem_info(self.block.handle, i, _lib.iptr(a), _lib.iptr(b), _lib.dptr(c)))
         return int(a[0]), int(b[0]), float(c[0])
 
+    def report(self):
+        """Wait for the last coupling call and take its iteration count and
+        residual (its errors are raised here)."""
+        it = np.zeros(1, dtype=np.int32)
+        dm = np.zeros(1)
+        _lib.check(self.lib.fsil_ibm_report(self.block.handle, _lib.iptr(it), _lib.dptr(dm)))
+        self.iters, self.dmax = int(it[0]), float(dm[0])
+
     def ibm_fem(self, time: float, wait: bool = True):
-        """IBM_FEM (LBMBlockComm.f90:320-338) at block time ``time``."""
+        """IBM_FEM (LBMBlockComm.f90:320-338) at block time ``time``.  The beam
+        sub-steps are queued behind the coupling call on the device before the
+        host waits for the call (wait=True), so they start as soon as the call
+        ends and ahead of the fluid update that the caller launches next."""
         blk = self.block
-        self.interaction_force(blk.dh, wait)
+        self.interaction_force(blk.dh, False)
         _lib.check(self.lib.fsil_fem_substeps(blk.handle, float(time), self.numsubstep, blk.dh))
+        if wait:
+            self.report()
 
     def step(self, time: float, wait: bool = True):
         blk = self.block

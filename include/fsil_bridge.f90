@@ -20,7 +20,8 @@ module fsil_bridge
               fsil_extract_interpolate_layer, fsil_interpolation_father_to_son, fsil_deliver_son_to_father, &
               fsil_tree_steps, fsil_time_tree_steps, fsil_mlp_create, fsil_mlp_destroy, fsil_mlp_get_params, &
               fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train, fsil_ibm_set_warm_start, &
-              fsil_ibm_warm_start_data, fsil_ibm_forces
+              fsil_ibm_warm_start_data, fsil_ibm_forces, &
+              fsil_ibm_report
 
     interface
         integer(c_int) function fsil_version() bind(C, name='fsil_version')
@@ -139,6 +140,12 @@ module fsil_bridge
             import :: c_int, c_double
             integer(c_int), value :: handle
             real(c_double), intent(out) :: Eforce(*), lodFlow(*)
+        end function
+        integer(c_int) function fsil_ibm_report(handle, iters, dmax) bind(C, name='fsil_ibm_report')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: dmax
         end function
         integer(c_int) function fsil_ibm_band_info(handle, nband, bbox) bind(C, name='fsil_ibm_band_info')
             import :: c_int

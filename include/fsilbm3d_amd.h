@@ -133,6 +133,13 @@ int fsil_ibm_interaction_force(int handle, double dt, const double *Exyz, const 
  * records); the coupling call itself need not copy them out every step. */
 int fsil_ibm_forces(int handle, double *Eforce, double *lodFlow);
 
+/* Iteration count and final residual of the last fsil_ibm_interaction_force
+ * (either pointer may be NULL), waiting for that call if it was made without
+ * read-backs, with its errors reported here: the values calculate_interaction_force
+ * (Solidbody.f90:869-918) ends its loop with.  Lets a caller queue the beam
+ * sub-steps behind a coupling call on the device before it waits for the call. */
+int fsil_ibm_report(int handle, int *iters, double *dmax);
+
 /* Plate bodies on the device beam solver, carried by the block (read_solid_files,
  * Beam_ReadBuild, Beam_Initialise, calculate_angle_material with isKB = 1:
  * Solidbody.f90:73-216, SolidSolver.f90:1270-1446, 1533-1616).  Per body b:

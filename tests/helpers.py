@@ -162,11 +162,18 @@ def plate_specs(pc):
     return out
 
 
-def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None, wait=True):
+def _lib_check(rc):
+    from fsilbm3d_amd import _lib
+    _lib.check(rc)
+
+
+def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None, wait=True, wait_first=False):
     """Run a ParityCase through the C-ABI on the GPU; returns the final state.
     Bodies run on the device beam solver when device_fem (default: when any
     plate is elastic), else with the host rigid kinematics.  wait=False: the
-    device-body coupling calls read nothing back (no host wait per step)."""
+    device-body coupling calls read nothing back (no host wait per step).
+    wait_first: each step waits for its coupling call before queuing the beam
+    sub-steps (instead of queuing them first and then collecting the report)."""
     from fsilbm3d_amd.block import LBMBlock
     from fsilbm3d_amd.fsi import DeviceBodyBlock, FSIBlock
     case = pc.case
@@ -194,12 +201,17 @@ def run_device(pc, f0=None, nsteps=None, record=None, device_fem=None, wait=True
     time = 0.0
     for _ in range(n):
         time = time + case.dh
-        if isinstance(fsi, DeviceBodyBlock):
+        if isinstance(fsi, DeviceBodyBlock) and wait_first:
+            blk.update_volume_force(time)
+            fsi.interaction_force(blk.dh, True)
+            _lib_check(fsi.lib.fsil_fem_substeps(blk.handle, float(time), fsi.numsubstep, blk.dh))
+            blk.collision_streaming_bc()
+        elif isinstance(fsi, DeviceBodyBlock):
             fsi.step(time, wait=wait)
         else:
             fsi.step(time)
         if record is not None and bodies:
-            rec = dict(iters=fsi.iters, Eforce=fsi.Eforce[:fsi.M].copy(),
+            rec = dict(iters=fsi.iters, dmax=getattr(fsi, "dmax", None), Eforce=fsi.Eforce[:fsi.M].copy(),
                        lod=[fsi.body_loads(i).copy() for i in range(len(bodies))])
             if isinstance(fsi, DeviceBodyBlock):
                 rec["beam"] = [fsi.state(i) for i in range(len(bodies))]

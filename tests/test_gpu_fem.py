@@ -55,3 +55,24 @@ def test_coupling_without_read_backs_matches(name):
     fb = run_device(pc, device_fem=True, wait=False)
     n, m = mismatch(fb, fa)
     assert n == 0, f"{n} differ, max {m:.3e}"
+
+
+def test_queued_substeps_with_report_match_waiting_call():
+    """The step queues the beam sub-steps behind its coupling call on the device
+    and then collects the call's report (fsil_ibm_report); against the waiting
+    coupling call followed by the sub-steps, every step's forcing-loop
+    iteration count and residual, the beam loads and the final state agree bit
+    for bit."""
+    pc = BY_NAME["fem_flexible_plate"]
+    print("[gpu] queued sub-steps + report vs waiting coupling call", flush=True)
+    ra, rb = [], []
+    fa = run_device(pc, device_fem=True, record=ra, wait_first=True)
+    fb = run_device(pc, device_fem=True, record=rb)
+    n, m = mismatch(fb, fa)
+    assert n == 0, f"{n} differ, max {m:.3e}"
+    assert len(ra) == len(rb) > 0
+    for a, b in zip(ra, rb):
+        assert a["iters"] == b["iters"] and a["dmax"] == b["dmax"]
+        for la, lb in zip(a["lod"], b["lod"]):
+            assert np.array_equal(la, lb)
+
