@@ -388,21 +388,34 @@ dim3 cell_block(const Block &b) {
   return dim3(tz, ty, 1);
 }
 
+// The FP32-deviation kernel exchanges the z-moving values of a row through LDS when a
+// workgroup is whole z-rows and the grid has no idle threads (every thread
+// reaches the exchange's barrier); FSIL_ZROW=0 turns it off.
+int zrow_exchange(const Block &b, dim3 blk) {
+  const char *e = getenv("FSIL_ZROW");
+  if (e && !strcmp(e, "0")) return 0;
+  return ((int)blk.x == b.zd && b.yd % (int)blk.y == 0 && blk.x * blk.y <= 256) ? 1 : 0;
+}
+
 // Cache policy of the fused kernel's population streams.  Measured on MI355X
 // (both storage forms, periodic blocks): plain loads and stores are 8-12%
 // faster at 384x192x192 and 512x256x256 (1-5 GB per population copy), where
 // the caches merge the partial lines of the z-shifted stores; nontemporal ones
 // are 3-14% faster at 1024x384x384 (11.5-23 GB per copy).  The switch is at
-// 8 GiB per copy.  Blocks with coupled bodies always stream nontemporally
+// 8 GiB per copy.  FP32-deviation blocks whose workgroups are whole z-rows
+// store whole lines (zrow_exchange) and stream nontemporally at every size
+// (384x192x192: 5.49 TB/s against 5.22 plain; 512x256x256: 5.76 against
+// 5.43).  Blocks with coupled bodies always stream nontemporally
 // (stream_hint_for): there the beam solver runs beside the collide kernel and
 // the plain form's cache footprint slows the whole step (flexible plate,
 // 384x192x192: 0.92 -> 1.19 ms FP32-deviation, 1.24 -> 1.43 ms FP64, though
 // the collide kernel itself is 11% faster).  FSIL_STREAM_HINT=nt|plain
 // overrides the choice (at block creation) for tests that run both.
-int stream_policy(long long N, int precision) {
+int stream_policy(long long N, int precision, bool zrow) {
   const char *e = getenv("FSIL_STREAM_HINT");
   if (e && !strcmp(e, "nt")) return 1;
   if (e && !strcmp(e, "plain")) return 0;
+  if (precision == 1 && zrow) return 1;  // whole-line stores: nontemporal is faster at every size measured
   const double copy_bytes = (double)Q * (double)N * (precision == 0 ? 8.0 : 4.0);
   return copy_bytes > 8.0 * 1024 * 1024 * 1024 ? 1 : 0;
 }
@@ -493,6 +506,7 @@ int collide_stream32(Block &b) {
   }
   const dim3 blk = cell_block(b);
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
+  a.zrow = zrow_exchange(b, blk);
   if (force && band) hipLaunchKernelGGL((k_collide_push32<true, true>), grid, blk, 0, b.stream, a);
   else if (force) hipLaunchKernelGGL((k_collide_push32<true, false>), grid, blk, 0, b.stream, a);
   else if (band) hipLaunchKernelGGL((k_collide_push32<false, true>), grid, blk, 0, b.stream, a);
@@ -751,7 +765,7 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   b.model = iCollidModel;
   b.tau = relax[0]; b.Omega = relax[1]; b.Omega2 = relax[2];
   b.precision = precision;
-  b.nt = stream_policy(b.N, precision);
+  b.nt = stream_policy(b.N, precision, precision == 1 && zrow_exchange(b, cell_block(b)));
   b.nt_forced = getenv("FSIL_STREAM_HINT") != nullptr;
   b.sh = std::make_shared<StreamHolder>();
   HIPCHK(hipStreamCreateWithFlags(&b.sh->s, hipStreamNonBlocking));

@@ -102,6 +102,30 @@ __device__ __forceinline__ void store_cell(const T (&v)[Q], T *planes, size_t pl
   }
 }
 
+// Whole z-rows per workgroup (blockDim.x == zDim and every thread a cell of
+// the block): the ten values that move along z (e_z = +-1) are exchanged
+// through LDS so that each thread stores, at its own z, the value arriving
+// there from z - e_z (periodic within the row).  A wave's stores then cover
+// whole cache lines instead of lines shifted by one value.  lds holds
+// 10 * blockDim.x * blockDim.y values.
+template <typename T>
+__device__ __forceinline__ void shift_rows_z(T (&post)[Q], T *lds) {
+  const int nt = (int)(blockDim.x * blockDim.y);
+  const int zr = (int)threadIdx.x, zd = (int)blockDim.x, row = (int)threadIdx.y * zd;
+  const int t = row + zr;
+  int k = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    if (ez(q) != 0) lds[(k++) * nt + t] = post[q];
+  __syncthreads();
+  const int from_m = row + (zr == 0 ? zd - 1 : zr - 1);  // sender of e_z = +1
+  const int from_p = row + (zr == zd - 1 ? 0 : zr + 1);  // sender of e_z = -1
+  k = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    if (ez(q) != 0) post[q] = lds[(k++) * nt + (ez(q) > 0 ? from_m : from_p)];
+}
+
 // Boundary codes (ConstParams.f90:31-34)
 enum BcCode {
   BC_FLUID = 0,

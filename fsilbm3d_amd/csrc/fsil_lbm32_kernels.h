@@ -29,7 +29,8 @@ struct Collide32Args {
   const double *__restrict__ band_u;  // [3][cap]
   const double *__restrict__ band_F;  // [3][cap]
   int cap;
-  int nt;  // nontemporal population loads and stores (large blocks)
+  int nt;    // nontemporal population loads and stores (large blocks)
+  int zrow;  // workgroups are whole z-rows covering the block exactly (shift_rows_z)
 };
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -161,7 +162,9 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
-  const unsigned st[6] = {sxp, sxm, syp, sym, szp, szm};
+  __shared__ float zx[10 * 256];
+  if (a.zrow) shift_rows_z(post, zx);
+  const unsigned st[6] = {sxp, sxm, syp, sym, a.zrow ? 0u : szp, a.zrow ? 0u : szm};
   store_cell(post, a.gout, plane, n, st, a.nt != 0);
 }
 

@@ -1,8 +1,10 @@
 """The fused collide/push kernel streams the populations with plain or with
-nontemporal cache hints, chosen per block by its size (FSIL_STREAM_HINT
-overrides the choice at block creation).  The hint changes only how memory is
-cached, so both give bit-identical states; the plain-hinted runs are the ones
-the other parity tests compare with the oracle at these sizes."""
+nontemporal cache hints, chosen per block (FSIL_STREAM_HINT overrides the
+choice at block creation), and the FP32-deviation kernel exchanges the values
+that move along z through LDS when its workgroups are whole z-rows
+(FSIL_ZROW=0 turns that off).  Both change only how memory is written, so
+every combination gives bit-identical states; the default ones are those the
+other parity tests compare with the oracle."""
 import numpy as np
 import pytest
 
@@ -37,11 +39,16 @@ def test_stream_hints_bit_identical(name, precision, monkeypatch):
     pc = BY_NAME[name]
     out = {}
     for hint in ("plain", "nt"):
-        print(f"[gpu] stream hint {hint}: {name} precision {precision}", flush=True)
-        monkeypatch.setenv("FSIL_STREAM_HINT", hint)
-        out[hint] = _run_block(pc, precision, 20)
+        for zrow in ("1", "0"):
+            print(f"[gpu] stream hint {hint}, z-row exchange {zrow}: {name} precision {precision}", flush=True)
+            monkeypatch.setenv("FSIL_STREAM_HINT", hint)
+            monkeypatch.setenv("FSIL_ZROW", zrow)
+            out[hint, zrow] = _run_block(pc, precision, 20)
     monkeypatch.delenv("FSIL_STREAM_HINT")
-    assert np.array_equal(out["plain"], out["nt"])
+    monkeypatch.delenv("FSIL_ZROW")
+    ref = out["plain", "1"]
+    for k, v in out.items():
+        assert np.array_equal(v, ref), k
 
 
 def test_block_size_caps():
