@@ -394,7 +394,10 @@ dim3 cell_block(const Block &b) {
 int zrow_exchange(const Block &b, dim3 blk) {
   const char *e = getenv("FSIL_ZROW");
   if (e && !strcmp(e, "0")) return 0;
-  return ((int)blk.x == b.zd && b.yd % (int)blk.y == 0 && blk.x * blk.y <= 256) ? 1 : 0;
+  if (!((int)blk.x == b.zd && b.yd % (int)blk.y == 0 && blk.x * blk.y <= 256)) return 0;
+  // rows of whole waves: lane permutes, 320 bytes of LDS (FSIL_ZROW=1 forces the LDS form)
+  if (b.zd % 64 == 0 && !(e && !strcmp(e, "1"))) return 2;
+  return 1;
 }
 
 // Cache policy of the fused kernel's population streams.  Measured on MI355X
@@ -507,10 +510,11 @@ int collide_stream32(Block &b) {
   const dim3 blk = cell_block(b);
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   a.zrow = zrow_exchange(b, blk);
-  if (force && band) hipLaunchKernelGGL((k_collide_push32<true, true>), grid, blk, 0, b.stream, a);
-  else if (force) hipLaunchKernelGGL((k_collide_push32<true, false>), grid, blk, 0, b.stream, a);
-  else if (band) hipLaunchKernelGGL((k_collide_push32<false, true>), grid, blk, 0, b.stream, a);
-  else hipLaunchKernelGGL((k_collide_push32<false, false>), grid, blk, 0, b.stream, a);
+  const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
+  if (force && band) hipLaunchKernelGGL((k_collide_push32<true, true>), grid, blk, lds, b.stream, a);
+  else if (force) hipLaunchKernelGGL((k_collide_push32<true, false>), grid, blk, lds, b.stream, a);
+  else if (band) hipLaunchKernelGGL((k_collide_push32<false, true>), grid, blk, lds, b.stream, a);
+  else hipLaunchKernelGGL((k_collide_push32<false, false>), grid, blk, lds, b.stream, a);
   HIPCHK(hipGetLastError());
   std::swap(b.gA.p, b.gB.p);
   std::swap(b.gA.n, b.gB.n);

@@ -126,6 +126,37 @@ __device__ __forceinline__ void shift_rows_z(T (&post)[Q], T *lds) {
     if (ez(q) != 0) post[q] = lds[(k++) * nt + (ez(q) > 0 ? from_m : from_p)];
 }
 
+// The same exchange for rows of whole waves (blockDim.x a multiple of 64):
+// inside a wave the values move by lane permutes (no LDS allocation); only
+// each wave's first and last lanes go through edge[], 20 values per wave
+// (at most 4 waves per workgroup: 320 bytes for FP32).  Leaves the LDS free
+// for a large workgroup that shares the CU.
+__device__ __forceinline__ void shift_rows_z_waves(float (&post)[Q], float *edge) {
+  const int zd = (int)blockDim.x, wpr = zd >> 6;
+  const int lane = (int)threadIdx.x & 63, wr = (int)threadIdx.x >> 6;
+  const int w = (int)threadIdx.y * wpr + wr, rw0 = w - wr;
+  if (lane == 0 || lane == 63) {
+    int k = 0;
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+      if (ez(q) != 0) edge[(w * 2 + (lane == 63 ? 1 : 0)) * 10 + (k++)] = post[q];
+  }
+  __syncthreads();
+  const int prev_w = rw0 + (wr == 0 ? wpr - 1 : wr - 1), next_w = rw0 + (wr == wpr - 1 ? 0 : wr + 1);
+  const int src_m = (lane == 0 ? 0 : lane - 1) << 2, src_p = (lane == 63 ? 63 : lane + 1) << 2;
+  int k = 0;
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+    if (ez(q) != 0) {
+      const int v = __builtin_amdgcn_ds_bpermute(ez(q) > 0 ? src_m : src_p, __float_as_int(post[q]));
+      float f = __int_as_float(v);
+      if (ez(q) > 0 && lane == 0) f = edge[(prev_w * 2 + 1) * 10 + k];
+      if (ez(q) < 0 && lane == 63) f = edge[(next_w * 2 + 0) * 10 + k];
+      post[q] = f;
+      ++k;
+    }
+}
+
 // Boundary codes (ConstParams.f90:31-34)
 enum BcCode {
   BC_FLUID = 0,

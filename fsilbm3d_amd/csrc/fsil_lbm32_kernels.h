@@ -30,7 +30,8 @@ struct Collide32Args {
   const double *__restrict__ band_F;  // [3][cap]
   int cap;
   int nt;    // nontemporal population loads and stores (large blocks)
-  int zrow;  // workgroups are whole z-rows covering the block exactly (shift_rows_z)
+  int zrow;  // workgroups are whole z-rows covering the block exactly: 1 shift_rows_z,
+             // 2 shift_rows_z_waves (rows of whole waves)
 };
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -162,10 +163,17 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
   const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
-  __shared__ float zx[10 * 256];
-  if (a.zrow) shift_rows_z(post, zx);
+  // LDS of the exchange, sized at launch (zrow32_lds_bytes)
+  extern __shared__ float zx[];
+  if (a.zrow == 2) shift_rows_z_waves(post, zx);
+  else if (a.zrow) shift_rows_z(post, zx);
   const unsigned st[6] = {sxp, sxm, syp, sym, a.zrow ? 0u : szp, a.zrow ? 0u : szm};
   store_cell(post, a.gout, plane, n, st, a.nt != 0);
+}
+
+// dynamic LDS of k_collide_push32 for its exchange mode and workgroup size
+inline size_t zrow32_lds_bytes(int zrow, unsigned threads) {
+  return zrow == 2 ? 4 * 2 * 10 * sizeof(float) : (zrow == 1 ? 10 * (size_t)threads * sizeof(float) : 0);
 }
 
 // den/u of the FP32-deviation state, in FP64 output arrays
