@@ -388,13 +388,21 @@ dim3 cell_block(const Block &b) {
   return dim3(tz, ty, 1);
 }
 
+// Workgroup of the FP32-deviation kernel (at most 512 threads, ~70 VGPRs):
+// rows of whole waves up to 512 cells are one workgroup (384-cell rows: six
+// waves, so the row exchange applies), else the shape of cell_block.
+dim3 cell_block32(const Block &b) {
+  if (b.zd > 256 && b.zd <= 512 && b.zd % 64 == 0) return dim3(b.zd, 1, 1);
+  return cell_block(b);
+}
+
 // The FP32-deviation kernel exchanges the z-moving values of a row through LDS when a
 // workgroup is whole z-rows and the grid has no idle threads (every thread
 // reaches the exchange's barrier); FSIL_ZROW=0 turns it off.
 int zrow_exchange(const Block &b, dim3 blk) {
   const char *e = getenv("FSIL_ZROW");
   if (e && !strcmp(e, "0")) return 0;
-  if (!((int)blk.x == b.zd && b.yd % (int)blk.y == 0 && blk.x * blk.y <= 256)) return 0;
+  if (!((int)blk.x == b.zd && b.yd % (int)blk.y == 0 && blk.x * blk.y <= 512)) return 0;
   // rows of whole waves: lane permutes, 320 bytes of LDS (FSIL_ZROW=1 forces the LDS form)
   if (b.zd % 64 == 0 && !(e && !strcmp(e, "1"))) return 2;
   return 1;
@@ -507,7 +515,7 @@ int collide_stream32(Block &b) {
     a.band_F = b.ibm.band_F.p;
     a.cap = b.ibm.cap;
   }
-  const dim3 blk = cell_block(b);
+  const dim3 blk = cell_block32(b);
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   a.zrow = zrow_exchange(b, blk);
   const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
@@ -769,7 +777,7 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   b.model = iCollidModel;
   b.tau = relax[0]; b.Omega = relax[1]; b.Omega2 = relax[2];
   b.precision = precision;
-  b.nt = stream_policy(b.N, precision, precision == 1 && zrow_exchange(b, cell_block(b)));
+  b.nt = stream_policy(b.N, precision, precision == 1 && zrow_exchange(b, cell_block32(b)));
   b.nt_forced = getenv("FSIL_STREAM_HINT") != nullptr;
   b.sh = std::make_shared<StreamHolder>();
   HIPCHK(hipStreamCreateWithFlags(&b.sh->s, hipStreamNonBlocking));

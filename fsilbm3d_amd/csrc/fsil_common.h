@@ -129,8 +129,8 @@ __device__ __forceinline__ void shift_rows_z(T (&post)[Q], T *lds) {
 // The same exchange for rows of whole waves (blockDim.x a multiple of 64):
 // inside a wave the values move by lane permutes (no LDS allocation); only
 // each wave's first and last lanes go through edge[], 20 values per wave
-// (at most 4 waves per workgroup: 320 bytes for FP32).  Leaves the LDS free
-// for a large workgroup that shares the CU.
+// (80 bytes per wave for FP32).  Leaves the LDS free for a large workgroup
+// that shares the CU.
 __device__ __forceinline__ void shift_rows_z_waves(float (&post)[Q], float *edge) {
   const int zd = (int)blockDim.x, wpr = zd >> 6;
   const int lane = (int)threadIdx.x & 63, wr = (int)threadIdx.x >> 6;

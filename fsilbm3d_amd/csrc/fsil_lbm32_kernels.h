@@ -91,7 +91,7 @@ __device__ __forceinline__ void moments32(const float *g, double &dr, double &j1
 // and, in forced cells, the Guo term c1*3dh*w*(e.F (1 + 3 u.e) - u.F) added
 // (FluidDomain.f90:1217-1227 rearranged around w_q).
 template <bool FORCE, bool BAND>
-__global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
+__global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int x = blockIdx.z;
@@ -173,7 +173,7 @@ __global__ __launch_bounds__(256) void k_collide_push32(Collide32Args a) {
 
 // dynamic LDS of k_collide_push32 for its exchange mode and workgroup size
 inline size_t zrow32_lds_bytes(int zrow, unsigned threads) {
-  return zrow == 2 ? 4 * 2 * 10 * sizeof(float) : (zrow == 1 ? 10 * (size_t)threads * sizeof(float) : 0);
+  return zrow == 2 ? (threads / 64) * 2 * 10 * sizeof(float) : (zrow == 1 ? 10 * (size_t)threads * sizeof(float) : 0);
 }
 
 // den/u of the FP32-deviation state, in FP64 output arrays

@@ -64,7 +64,7 @@ def test_block_size_caps():
         LBMBlock((1024, 1024, 1025), precision=1)
 
 
-@pytest.mark.parametrize("dims", [(6, 4, 64), (4, 4, 192), (4, 4, 256)])
+@pytest.mark.parametrize("dims", [(6, 4, 64), (4, 4, 192), (4, 4, 256), (4, 4, 320), (4, 4, 384)])
 def test_whole_wave_row_exchange_bit_identical(dims, monkeypatch):
     """Rows of whole waves (z a multiple of 64) exchange the z-moving values by
     lane permutes; against the LDS form (FSIL_ZROW=1) and plain shifted
