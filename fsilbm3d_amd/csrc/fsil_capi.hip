@@ -6,6 +6,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -78,6 +79,26 @@ struct DevBuf {
     return e;
   }
 };
+
+// Host wait for an event of the step loop (the coupling call's report, the
+// beam sub-steps): the event is polled for up to 5 ms before a blocking wait,
+// whose wake-up latency would otherwise add to every step.  FSIL_SPIN=0 waits
+// blocking from the start.
+hipError_t step_wait(hipEvent_t ev) {
+  static const bool spin = [] {
+    const char *e = getenv("FSIL_SPIN");
+    return !(e && !strcmp(e, "0"));
+  }();
+  if (spin) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (;;) {
+      const hipError_t e = hipEventQuery(ev);
+      if (e != hipErrorNotReady) return e;
+      if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(5)) break;
+    }
+  }
+  return hipEventSynchronize(ev);
+}
 
 // Pinned host copy of a coupling call's control words, written by k_ibm_report.
 struct PinnedReport {
@@ -1394,7 +1415,7 @@ int ibm_check_report(Block &b) {
   Ibm &I = b.ibm;
   if (!I.rep || !I.rep->pending) return 0;
   I.rep->pending = false;
-  HIPCHK(hipEventSynchronize(I.rep->ev));
+  HIPCHK(step_wait(I.rep->ev));
   const IbmReport &R = *I.rep->h;
   const int err = R.ctl[kErr];
   if (err != 0x7fffffff) {
@@ -1419,7 +1440,7 @@ int ibm_check_report(Block &b) {
 int fem_collect(Block &b) {
   Fem &F = b.fem;
   if (!F.fs || !F.fs->pending) return 0;
-  HIPCHK(hipEventSynchronize(F.fs->ev_done));
+  HIPCHK(step_wait(F.fs->ev_done));
   F.fs->pending = false;
   const int nb = F.nbodies;
   const int nsub = nb ? F.fs->n_staged / nb : 0;
