@@ -457,8 +457,12 @@ int stream_hint_for(const Block &b, bool band) { return band && !b.nt_forced ? 1
 template <int MODEL>
 void launch_collide_model(const CollideArgs &a, bool force, bool band, bool aux, dim3 grid,
                           dim3 blk, hipStream_t s) {
-#define L(F, B, X) \
-  if (force == F && band == B && aux == X) { hipLaunchKernelGGL((k_collide_push<MODEL, F, B, X>), grid, blk, 0, s, a); return; }
+#define L(F, B, X)                                                                                  \
+  if (force == F && band == B && aux == X) {                                                        \
+    if (a.nt) hipLaunchKernelGGL((k_collide_push<MODEL, F, B, X, true>), grid, blk, 0, s, a);        \
+    else hipLaunchKernelGGL((k_collide_push<MODEL, F, B, X, false>), grid, blk, 0, s, a);            \
+    return;                                                                                         \
+  }
   L(false, false, false) L(true, false, false) L(false, true, false) L(true, true, false)
   L(false, false, true) L(true, false, true) L(false, true, true) L(true, true, true)
 #undef L
@@ -540,10 +544,14 @@ int collide_stream32(Block &b) {
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   a.zrow = zrow_exchange(b, blk);
   const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
-  if (force && band) hipLaunchKernelGGL((k_collide_push32<true, true>), grid, blk, lds, b.stream, a);
-  else if (force) hipLaunchKernelGGL((k_collide_push32<true, false>), grid, blk, lds, b.stream, a);
-  else if (band) hipLaunchKernelGGL((k_collide_push32<false, true>), grid, blk, lds, b.stream, a);
-  else hipLaunchKernelGGL((k_collide_push32<false, false>), grid, blk, lds, b.stream, a);
+#define L32(F, B)                                                                                      \
+  if (a.nt) hipLaunchKernelGGL((k_collide_push32<F, B, true>), grid, blk, lds, b.stream, a);             \
+  else hipLaunchKernelGGL((k_collide_push32<F, B, false>), grid, blk, lds, b.stream, a);
+  if (force && band) { L32(true, true) }
+  else if (force) { L32(true, false) }
+  else if (band) { L32(false, true) }
+  else { L32(false, false) }
+#undef L32
   HIPCHK(hipGetLastError());
   std::swap(b.gA.p, b.gB.p);
   std::swap(b.gA.n, b.gB.n);

@@ -29,7 +29,7 @@ struct Collide32Args {
   const double *__restrict__ band_u;  // [3][cap]
   const double *__restrict__ band_F;  // [3][cap]
   int cap;
-  int nt;    // nontemporal population loads and stores (large blocks)
+  int nt;    // nontemporal population loads and stores (selects the kernel instantiation)
   int zrow;  // workgroups are whole z-rows covering the block exactly: 1 shift_rows_z,
              // 2 shift_rows_z_waves (rows of whole waves)
 };
@@ -90,7 +90,7 @@ __device__ __forceinline__ void moments32(const float *g, double &dr, double &j1
 //   g'_q = (1-Omega) g_q + Omega w (A + B),  g'_qbar = (1-Omega) g_qbar + Omega w (A - B)
 // and, in forced cells, the Guo term c1*3dh*w*(e.F (1 + 3 u.e) - u.F) added
 // (FluidDomain.f90:1217-1227 rearranged around w_q).
-template <bool FORCE, bool BAND>
+template <bool FORCE, bool BAND, bool NT>
 __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   const unsigned nb = n << 2;  // blocks of this form hold at most 2^30 cells
   const size_t plane = (size_t)a.N * sizeof(float);
   float g[Q];
-  load_cell(g, a.gin, plane, nb, a.nt != 0);
+  load_cell(g, a.gin, plane, nb, NT);
   double dr64, j1, j2, j3;
   moments32(g, dr64, j1, j2, j3);
   const float dr = (float)dr64;
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   if (a.zrow == 2) shift_rows_z_waves(post, zx);
   else if (a.zrow) shift_rows_z(post, zx);
   const unsigned st[6] = {sxp, sxm, syp, sym, a.zrow ? 0u : szp, a.zrow ? 0u : szm};
-  store_cell(post, a.gout, plane, n, st, a.nt != 0);
+  store_cell(post, a.gout, plane, n, st, NT);
 }
 
 // dynamic LDS of k_collide_push32 for its exchange mode and workgroup size

@@ -95,7 +95,7 @@ struct CollideArgs {
   double *nb_den[6];
   double *nb_u[6];   // [3][plane]
   double *hw[6];     // [q][plane]
-  int nt;            // nontemporal population loads and stores (large blocks)
+  int nt;            // nontemporal population loads and stores (selects the kernel instantiation)
 };
 
 // plane index of a cell on face `face` in the reference's (outer, inner) loop order
@@ -108,7 +108,7 @@ __device__ __forceinline__ int face_plane_index(int face, int x, int y, int z, i
 
 // One cell: load its 19 pre-collision populations, collide, push the 19
 // post-collision values to the neighbours (x+e_q, periodic wrap).
-template <int MODEL, bool FORCE, bool BAND, bool AUX>
+template <int MODEL, bool FORCE, bool BAND, bool AUX, bool NT>
 __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, int y, int z) {
   // 32-bit cell index and byte offsets (N < 2^29 for every block this library
   // accepts), so each access is a scalar plane base plus one 32-bit offset
@@ -117,7 +117,7 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
   const unsigned nb = n << 3;
 
   double f[Q];
-  load_cell(f, a.fin, plane, nb, a.nt != 0);
+  load_cell(f, a.fin, plane, nb, NT);
 
   double rho, m1, m2, m3;
   moments(f, rho, m1, m2, m3);
@@ -246,16 +246,16 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
   const unsigned st[6] = {sxp, sxm, syp, sym, szp, szm};
-  store_cell(post, a.fout, plane, n, st, a.nt != 0);
+  store_cell(post, a.fout, plane, n, st, NT);
 }
 
-template <int MODEL, bool FORCE, bool BAND, bool AUX>
+template <int MODEL, bool FORCE, bool BAND, bool AUX, bool NT>
 __global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int x = blockIdx.z;
   if (z >= a.zd || y >= a.yd) return;
-  collide_push_cell<MODEL, FORCE, BAND, AUX>(a, x, y, z);
+  collide_push_cell<MODEL, FORCE, BAND, AUX, NT>(a, x, y, z);
 }
 
 // ---------------------------------------------------------------------------
