@@ -2,8 +2,8 @@
 infrastructure only): a 12 -> 64 -> 64 -> 64 -> 3 perceptron with tanh hidden
 layers, mean-squared-error loss, backpropagation and Adam, in float64 numpy.
 
-The reference (gaoak/FSILBM3D) has no such network; the method is the
-north-star extension of calculate_interaction_force (Solidbody.f90:869-918):
+The reference (gaoak/FSILBM3D) has no such network; the method is a learned
+warm start added to calculate_interaction_force (Solidbody.f90:869-918):
 the network guesses the converged marker force from the interpolated fluid
 velocity, the marker velocity, the marker's previous force and its normal.
 This file pins the device kernels (forward, gradients, Adam step) against
