@@ -111,3 +111,17 @@ def test_compare_blocks_containment():
     over = blk((14, 3, 3), (20, 8, 8), bc=(0, 302, 302, 302, 302, 302), dh=0.5)
     assert compare_blocks(fat, over) == 1
     assert compare_blocks(big, over) == 0
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; asked for more it exits with status 2 and says
+    why, before touching the device."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    root = Path(__file__).resolve().parents[1]
+    p = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=str(root))
+    assert p.returncode == 2
+    assert "one GPU" in p.stderr
+    assert p.stdout.strip() == ""
