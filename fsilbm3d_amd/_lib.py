@@ -59,7 +59,9 @@ SIGNATURES = {
     "fsil_mlp_forward": (_i, [_i, _pf, _i, _pf, _i]),
     "fsil_mlp_loss_grad": (_i, [_i, _pf, _pf, _i, _pd, _pf]),
     "fsil_mlp_train": (_i, [_i, _pf, _pf, _i, _i, _i, _d, _i, _pd, _pd]),
-    "fsil_ibm_set_warm_start": (_i, [_i, _i, _i]),
+    "fsil_ibm_set_warm_start": (_i, [_i, _i, _i, _i]),
+    "fsil_ibm_rounds": (_i, [_i, _pi]),
+    "fsil_solid_info": (_i, [_i, _i, _pd, _pd]),
     "fsil_ibm_warm_start_data": (_i, [_i, _pf, _pf, _pi]),
     "fsil_fem_define_bodies": (_i, [_i, _i, _pi, _pi, _pd, _pi, _pi, _pi, _pi, _pd, _pd, _d, _d, _d, _d]),
     "fsil_fem_set_solver": (_i, [_i, _d, _d, _d, _d, _d, _d, _i]),

@@ -376,6 +376,39 @@ __host__ __device__ inline void el_body_stress(BeamElem &e) {
     }
 }
 
+// Segment_StrainEnergy_D after FormStiffMatrix (SolidSolver.f90:841-907,
+// 2340-2348): stretching (stiffness entries (1,1), (1,7), (7,1), (7,7)) and
+// bending/torsion (all others) energy 0.5 ub^T K ub of the local deformation
+__host__ __device__ inline void el_strain_energy(const BeamElem &e, double *es, double *eb) {
+  const double du = e.dx1 - e.dx0, dv = e.dy1 - e.dy0, dw = e.dz1 - e.dz0;
+  const double dl = ((e.dx0 + e.dx1) * du + (e.dy0 + e.dy1) * dv + (e.dz0 + e.dz1) * dw) / (e.len0 + e.len1);
+  double t[3], a[3], b[3];
+  const double *T = e.tee;
+  angle_triad(e.tee, e.tn1, t);
+  a[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
+  a[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
+  a[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
+  angle_triad(e.tee, e.tn2, t);
+  b[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
+  b[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
+  b[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
+  const double ub[NED] = {0.0, 0.0, 0.0, a[0], a[1], a[2], dl, 0.0, 0.0, b[0], b[1], b[2]};
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = 0; i < NED; ++i) {
+    double x = 0.0, y = 0.0;
+    for (int k = 0; k < NED; ++k) {
+      const double K = e.stl[i * NED + k];
+      const bool stretch = (i == 0 || i == 6) && (k == 0 || k == 6);
+      x = x + (stretch ? K : 0.0) * ub[k];
+      y = y + (stretch ? 0.0 : K) * ub[k];
+    }
+    s1 = s1 + x * ub[i];
+    s2 = s2 + y * ub[i];
+  }
+  *es = 0.5 * s1;
+  *eb = 0.5 * s2;
+}
+
 // Segment_UpdateMatrix (SolidSolver.f90:131-157)
 __host__ __device__ inline void el_update_matrix(BeamElem &e, const double *c, double gamma, double dampM,
                                                  double dampK) {

@@ -155,9 +155,11 @@ struct Ibm {
   bool active = false;  // band valid for the next collision
   int warm_mlp = 0;     // network handle of the learned warm start (0: off)
   int record = 0;       // compute the network's features and targets every call
+  int skip = 0;         // forcing rounds the learned guess stands for
+  bool have_prev = false;  // a coupling call has run since the bodies were defined (Eprev valid)
   DevBuf<float> wsX, wsY, wsP;               // features (12 per marker), targets, guesses (3 per marker)
-  DevBuf<double> Eprev, Enrm, tl0;           // previous force, normals, zero-guess residual terms
-  int last_warm = 0, last_fallback = 0;
+  DevBuf<double> Eprev, Enrm, Esnap;         // previous force, normals, force after `skip` rounds
+  int last_warm = 0, last_fallback = 0, last_rounds = 0;
   int last_iters = 0;
   double last_dmax = 0.0;
 };
@@ -203,6 +205,7 @@ struct Fem {
   DevBuf<double> pos, dsp, vel, acc, dspO, velO, accO, lodInte, lodEffe, lodExte, vBC, x, r, p, Ap, z, xF, lbuf, blk,
       beta;
   DevBuf<int> fixed, dof_ptr, dof_e, dof_i, node_ptr, node_e, node_h, body_of_elem, elem_first_marker, body_goff;
+  DevBuf<double> info;  // fsil_solid_info: per-element energies and the 78 row values
   bool markers_on_device = false;
   size_t lds_limit = 0;  // the device's LDS bytes per workgroup
   size_t lds_set = 64 * 1024;  // dynamic LDS bytes k_fem_substep may take
@@ -267,6 +270,7 @@ struct Block {
 std::map<int, std::unique_ptr<Block>> g_blocks;
 int g_next_handle = 1;
 int g_device = -1;
+
 
 Block *get_block(int h) {
   auto it = g_blocks.find(h);
@@ -727,6 +731,7 @@ FemArrays fem_arrays(Block &b) {
 
 int fem_collect(Block &b);
 int fem_order_after(Block &b);
+int fill_solid_info_args(Block &b, int body, const double *refs, double *buf, SolidInfoArgs &a);
 int ibm_check_report(Block &b);
 
 }  // namespace
@@ -1176,9 +1181,12 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
     HIPCHK(I.wsP.ensure(3 * m1));
     HIPCHK(I.Eprev.ensure(3 * m1));
     HIPCHK(I.Enrm.ensure(3 * m1));
-    HIPCHK(I.tl0.ensure(m1));
+    HIPCHK(I.Esnap.ensure(3 * m1));
     HIPCHK(hipMemsetAsync(I.Eprev.p, 0, 3 * m1 * sizeof(double), b->stream));
     HIPCHK(hipMemsetAsync(I.Enrm.p, 0, 3 * m1 * sizeof(double), b->stream));
+    // dstate: residual, (unused), the last call's residual at iteration skip+1 (-1: none), (unused)
+    const double d0[4] = {1e10, 0.0, -1.0, 0.0};
+    if (upload(I.dstate, d0, 4, b->stream)) return 1;
   }
   HIPCHK(hipMemsetAsync(I.ctl.p, 0, kCtlWords * sizeof(int), b->stream));
   HIPCHK(hipMemsetAsync(I.cellmap.p, 0xff, (size_t)b->N * sizeof(int), b->stream));  // -1
@@ -1239,9 +1247,10 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     // the network's parameters are written on its own stream (training)
     HIPCHK(hipStreamSynchronize(net->s));
   }
-  const int warm = (net && !done0) ? 1 : 0;
+  // the guess needs the previous call's force (Eprev) and the skip within the loop's cap
+  const int warm = (net && !done0 && I.have_prev && I.skip < I.ntol) ? 1 : 0;
   hipLaunchKernelGGL(k_ibm_reset, dim3(std::min(1024, (E + 255) / 256)), dim3(256), 0, s, I.ctl.p, I.dcell.p,
-                     I.cellmap.p, I.dstate.p, done0, warm, E, I.scan.p, I.dptr.p, I.Eforce.p, 3 * M);
+                     I.cellmap.p, I.dstate.p, done0, warm, I.skip, E, I.scan.p, I.dptr.p, I.Eforce.p, 3 * M);
   HIPCHK(hipGetLastError());
   StencilArgs sa{};
   sa.M = M;
@@ -1285,30 +1294,32 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   for (int body = 0; body < I.nbodies; ++body) dsum = dsum + (double)I.nmark[body];
   if (warm || I.record) {
     hipLaunchKernelGGL(k_ws_features, dim3((M + 3) / 4), dim3(256), 0, s, M, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
-                       I.Evel.p, I.Ea.p, I.Eprev.p, I.Enrm.p, I.Uref, I.wsX.p, I.tl0.p);
+                       I.Evel.p, I.Ea.p, I.Eprev.p, I.Enrm.p, I.Uref, I.wsX.p);
     HIPCHK(hipGetLastError());
   }
   if (warm) {
-    // the learned starting force, applied to the band as one forcing round
+    // the learned guess of the force after `skip` rounds, applied to the band
+    // as one correction of the uncorrected velocity
     hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((M + 63) / 64), dim3(256), 0, s, I.wsX.p, M, net->th.p, I.wsP.p);
-    hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Uref, invh3p,
-                       I.Eforce.p, I.fT.p);
+    hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Eprev.p, I.Uref,
+                       invh3p, I.Eforce.p, I.fT.p);
     hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, -1, MdfTest{}, I.cap, I.dptr.p,
                        I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
-    hipLaunchKernelGGL(k_ws_dm0, dim3(1), dim3(256), 0, s, I.tl0.p, M, dsum, I.Uref, I.dstate.p);
     HIPCHK(hipGetLastError());
   }
-  // MDF loop (Solidbody.f90:893-906): ntol rounds are queued (one more with
-  // the warm start, in case it falls back); a round is k_interp and
-  // k_correct, whose workgroup 0 makes the stop test, and the kernels of the
-  // rounds after the stop return at once
+  // MDF loop (Solidbody.f90:893-906): the rounds are queued (with the warm
+  // start one more than the cap, in case it falls back and restarts); a round
+  // is k_interp and k_correct, whose last workgroup makes the stop test, and
+  // the kernels of the rounds after the stop return at once.  A plain call
+  // that records training data keeps the force after `skip` rounds (Esnap).
   const int rounds = I.ntol + warm;
+  const int snap = (!warm && I.record && I.skip > 0) ? I.skip - 1 : -1;
   MdfTest mt{};
   mt.tl = I.tolpart.p; mt.dstate = I.dstate.p; mt.M = M; mt.ntol = I.ntol;
-  mt.dsum = dsum; mt.Uref = I.Uref; mt.dtol = I.dtol;
+  mt.dsum = dsum; mt.Uref = I.Uref; mt.dtol = I.dtol; mt.skip = I.skip;
   for (int it = 0; it < rounds; ++it) {
     hipLaunchKernelGGL(k_interp, dim3((M + 3) / 4), dim3(256), 0, s, M, it, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
-                       I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p);
+                       I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p, snap, I.Esnap.p);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, it, mt, I.cap, I.dptr.p,
                        I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
@@ -1326,11 +1337,13 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
       HIPCHK(hipGetLastError());
     }
   }
-  if (warm || I.record) {
-    hipLaunchKernelGGL(k_ws_targets, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.Eforce.p, I.Ea.p, I.Uref,
-                       I.wsY.p, I.Eprev.p);
+  if (I.warm_mlp || I.record) {
+    // training targets of a plain call; the previous force of the next call
+    hipLaunchKernelGGL(k_ws_targets, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.ctl.p, I.skip, I.Eforce.p,
+                       I.Esnap.p, I.Ea.p, I.Uref, (I.record && !warm) ? 1 : 0, I.wsY.p, I.Eprev.p);
     HIPCHK(hipGetLastError());
   }
+  I.have_prev = true;
   const double invh3s = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
   hipLaunchKernelGGL(k_spread, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p, I.dentry.p,
                      I.dEw.p, I.Eforce.p, invh3s, b->VF[0], b->VF[1], b->VF[2], I.band_F.p);
@@ -1388,12 +1401,28 @@ int fsil_ibm_report(int h, int *iters, double *dmax) {
   return 0;
 }
 
-int fsil_ibm_set_warm_start(int h, int mlp, int record) {
+int fsil_ibm_set_warm_start(int h, int mlp, int record, int skip) {
   Block *b = get_block(h);
   if (!b) return 1;
   if (mlp && !get_mlp(mlp)) return 1;
-  b->ibm.warm_mlp = mlp;
-  b->ibm.record = record ? 1 : 0;
+  Ibm &I = b->ibm;
+  if (skip < 0 || (I.ntol > 0 && skip >= I.ntol)) return fail("warm-start skip %d outside [0, ntolLBM)", skip);
+  if (mlp && !I.warm_mlp && !I.record && I.M > 0) {
+    // the previous force of the next call: the last call's final marker force
+    HIPCHK(hipMemcpyAsync(I.Eprev.p, I.Eforce.p, 3 * (size_t)I.M * sizeof(double), hipMemcpyDeviceToDevice,
+                          b->stream));
+  }
+  I.warm_mlp = mlp;
+  I.record = record ? 1 : 0;
+  I.skip = skip;
+  return 0;
+}
+
+int fsil_ibm_rounds(int h, int *rounds) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (ibm_check_report(*b)) return 1;
+  if (rounds) *rounds = b->ibm.last_rounds;
   return 0;
 }
 
@@ -1436,8 +1465,10 @@ int ibm_check_report(Block &b) {
     return fail("immersed-boundary stencil of marker %d (body %d): index out of %s bound", m - I.moff[body] + 1,
                 body + 1, why == 1 ? "min" : "max");
   }
-  I.last_warm = R.ctl[kWarm];
+  I.last_warm = R.ctl[kGuess] && !R.ctl[kFallback];  // the guess was kept
   I.last_fallback = R.ctl[kFallback] != 0;
+  // forcing rounds run: interpolation rounds plus the guess's correction
+  I.last_rounds = R.ctl[kRan] + 1 + R.ctl[kGuess];
   I.last_iters = R.ctl[kIters];
   I.last_dmax = R.dmax;
   return 0;
@@ -1463,6 +1494,29 @@ int fem_collect(Block &b) {
       F.desc[bi].status = r.status;
       F.desc[bi].dnorm = r.dnorm;
     }
+  return 0;
+}
+
+// Kernel arguments of k_solid_info for body `body`, with `buf` (3 nEL + 78
+// doubles of device memory) for the per-element energies and the rows.  The
+// node vectors and loads are those of the last beam sub-steps and coupling
+// call (ordered before on the block's stream by the caller's waits).
+int fill_solid_info_args(Block &b, int body, const double *refs, double *buf, SolidInfoArgs &a) {
+  Fem &F = b.fem;
+  Ibm &I = b.ibm;
+  const FemBodyDesc &d = F.desc[body];
+  if (I.ndof != 6 * F.nND) return fail("immersed-boundary bodies do not match the beam bodies");
+  a.el = F.el.p + d.eoff;
+  a.pos = F.pos.p + d.goff;
+  a.vel = F.vel.p + d.goff;
+  a.acc = F.acc.p + d.goff;
+  a.lod = I.lod.p + d.goff;
+  a.nND = d.nND;
+  a.nEL = d.nEL;
+  for (int k = 0; k < 3; ++k) a.XYZo[k] = F.bh[body].XYZo[k];
+  for (int k = 0; k < 6; ++k) a.refs[k] = refs[k];
+  a.parts = buf;
+  a.rows = buf + 3 * (size_t)d.nEL;
   return 0;
 }
 
@@ -1736,6 +1790,25 @@ int fsil_fem_state(int h, int body, double *pos, double *vel, double *acc, doubl
   if (acc) HIPCHK(hipMemcpyAsync(acc, F.acc.p + d.goff, n * sizeof(double), hipMemcpyDeviceToHost, s));
   if (dsp) HIPCHK(hipMemcpyAsync(dsp, F.dsp.p + d.goff, n * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
+  return 0;
+}
+
+int fsil_solid_info(int h, int body, const double *refs, double *rows) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (body < 0 || body >= F.nbodies) return fail("body %d out of range", body);
+  if (!refs || !rows) return fail("null refs or rows");
+  if (fem_collect(*b)) return 1;
+  if (ibm_check_report(*b)) return 1;
+  const FemBodyDesc &d = F.desc[body];
+  HIPCHK(F.info.ensure(3 * (size_t)d.nEL + 78));
+  SolidInfoArgs a{};
+  if (fill_solid_info_args(*b, body, refs, F.info.p, a)) return 1;
+  hipLaunchKernelGGL(k_solid_info, dim3(1), dim3(256), 0, b->stream, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rows, a.rows, 78 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 

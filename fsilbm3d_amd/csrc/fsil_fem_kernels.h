@@ -721,4 +721,115 @@ __global__ __launch_bounds__(256) void k_fem_markers(FemMarkerArgs a) {
     for (int k = 0; k < 3; ++k) a.elem_xc[3LL * e + k] = 0.5 * (E.x1[k] + E.x1[6 + k]);
 }
 
+// ---------------------------------------------------------------------------
+// Beam_write_solid_info (SolidSolver.f90:1729-1799): the values of one row of
+// each DatInfo file of one body, every sum in the reference's order.
+// refs = Lref, Uref, Aref, Fref, Pref, Eref (Solidbody.f90:276-280).
+// rows (78): firstNode[15] lastNode[15] centerNode[15] nodeAverage[12]
+// forces[6] power[7] energy[8].  One workgroup: the element energies in
+// parallel (parts[2*nEL..]), then thread 0 adds everything in order.
+struct SolidInfoArgs {
+  const BeamElem *el;   // the body's elements
+  const double *pos, *vel, *acc, *lod;  // the body's node vectors (6 per node) and lodFlow
+  int nND, nEL;
+  double XYZo[3], refs[6];
+  double *parts;        // 3 * nEL scratch: strain (2) and kinetic energy per element
+  double *rows;         // 78 outputs
+};
+
+__device__ inline void solid_node_row(const SolidInfoArgs &a, int n, double *r) {
+  const double Lref = a.refs[0], Uref = a.refs[1], Aref = a.refs[2];
+  for (int k = 0; k < 3; ++k) {
+    r[k] = a.XYZo[k] / Lref;
+    r[3 + k] = (a.pos[6 * n + k] - a.XYZo[k]) / Lref;
+    r[6 + k] = a.pos[6 * n + 3 + k];
+    r[9 + k] = a.vel[6 * n + k] / Uref;
+    r[12 + k] = a.acc[6 * n + k] / Aref;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_solid_info(SolidInfoArgs a) {
+  for (int i = threadIdx.x; i < a.nEL; i += blockDim.x) {
+    const BeamElem &e = a.el[i];
+    double es, eb;
+    el_strain_energy(e, &es, &eb);
+    double ve[NED];
+    for (int k = 0; k < 6; ++k) {
+      ve[k] = a.vel[6 * e.node0 + k];
+      ve[6 + k] = a.vel[6 * e.node1 + k];
+    }
+    double d = 0.0;
+    for (int j = 0; j < NED; ++j) {
+      double mv = 0.0;
+      for (int k = 0; k < NED; ++k) mv = mv + e.mas[j * NED + k] * ve[k];
+      d = d + ve[j] * mv;
+    }
+    a.parts[3 * i] = es;
+    a.parts[3 * i + 1] = eb;
+    a.parts[3 * i + 2] = 0.5 * d;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double Lref = a.refs[0], Uref = a.refs[1], Aref = a.refs[2], Fref = a.refs[3], Pref = a.refs[4],
+               Eref = a.refs[5];
+  const int nND = a.nND;
+  solid_node_row(a, 0, a.rows);
+  solid_node_row(a, nND - 1, a.rows + 15);
+  solid_node_row(a, (nND + 1) / 2 - 1, a.rows + 30);
+  double *r = a.rows + 45;
+  for (int k = 0; k < 3; ++k) {
+    // lumped node masses mss(k, n): diagonal of the element mass matrices, element order
+    double ms = 0.0, xs = 0.0, vs = 0.0, as = 0.0;
+    for (int n = 0; n < nND; ++n) {
+      double m = 0.0;
+      for (int i = 0; i < a.nEL; ++i) {
+        const BeamElem &e = a.el[i];
+        if (e.node0 == n) m = m + e.mas[k * NED + k];
+        if (e.node1 == n) m = m + e.mas[(k + 6) * NED + k + 6];
+      }
+      ms = ms + m;
+      xs = xs + a.pos[6 * n + k] * m;
+      vs = vs + a.vel[6 * n + k] * m;
+      as = as + a.acc[6 * n + k] * m;
+    }
+    r[k] = a.XYZo[k] / Lref;
+    r[3 + k] = (xs / ms - a.XYZo[k]) / Lref;
+    r[6 + k] = (vs / ms) / Uref;
+    r[9 + k] = (as / ms) / Aref;
+  }
+  r = a.rows + 57;
+  double P[3];
+  for (int k = 0; k < 3; ++k) {
+    double f = 0.0, p = 0.0;
+    for (int n = 0; n < nND; ++n) f = f + a.lod[6 * n + k];
+    for (int n = 0; n < nND; ++n) p = p + a.lod[6 * n + k] * a.vel[6 * n + k];
+    r[k] = a.XYZo[k] / Lref;
+    r[3 + k] = f / Fref;
+    P[k] = p / Pref;
+  }
+  double *pw = a.rows + 63;
+  for (int k = 0; k < 3; ++k) pw[k] = a.XYZo[k] / Lref;
+  pw[3] = (P[0] + P[1]) + P[2];
+  pw[4] = P[0];
+  pw[5] = P[1];
+  pw[6] = P[2];
+  double Es = 0.0, Eb = 0.0, Ev = 0.0;
+  for (int i = 0; i < a.nEL; ++i) {
+    Es = Es + a.parts[3 * i];
+    Eb = Eb + a.parts[3 * i + 1];
+  }
+  Es = Es / Eref;
+  Eb = Eb / Eref;
+  const double Ep = Es + Eb;
+  for (int i = 0; i < a.nEL; ++i) Ev = Ev + a.parts[3 * i + 2];
+  Ev = Ev / Eref;
+  double *en = a.rows + 70;
+  for (int k = 0; k < 3; ++k) en[k] = a.XYZo[k] / Lref;
+  en[3] = Ev + Ep;
+  en[4] = Ev;
+  en[5] = Ep;
+  en[6] = Es;
+  en[7] = Eb;
+}
+
 }  // namespace fsil

@@ -39,7 +39,9 @@ enum IbmCtl {
   kDone0 = 12,     // the done word at the start of a loop (no iteration at all)
   kNew = 13,       // band cells claimed so far in this call (becomes kNband)
   kRan = 14,       // the last forcing round whose interpolation ran (-1: none)
-  kCtlWords = 16
+  kGuess = 15,     // 1: the learned guess was applied in this call (kept on fallback)
+  kSkip = 16,      // forcing rounds the applied guess stands for (0 without a guess)
+  kCtlWords = 20
 };
 
 // 4-point Peskin kernel (Solidbody.f90:822-833), FP64; sqrt is correctly rounded
@@ -79,9 +81,9 @@ __host__ __device__ inline int trim4(int i, int nd, int *out, int bclo, int bchi
 // the per-cell entry counts (E of them) of this call and the marker forces
 __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const long long *__restrict__ prev_cell,
                                                    int *__restrict__ cellmap, double *__restrict__ dstate,
-                                                   int done0, int warm, int E, int *__restrict__ count,
-                                                   int *__restrict__ band_ptr, double *__restrict__ Eforce,
-                                                   int n3M) {
+                                                   int done0, int warm, int skip, int E,
+                                                   int *__restrict__ count, int *__restrict__ band_ptr,
+                                                   double *__restrict__ Eforce, int n3M) {
   const int nprev = ctl[kNband];
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nprev; i += gridDim.x * blockDim.x)
     cellmap[prev_cell[i]] = -1;
@@ -94,8 +96,12 @@ __global__ __launch_bounds__(256) void k_ibm_reset(int *__restrict__ ctl, const 
     ctl[kRan] = -1;
     ctl[kErr] = 0x7fffffff;
     ctl[kDone] = done0;
-    ctl[kIters] = 0;
+    // a learned guess stands for the first `skip` forcing rounds: the loop
+    // count continues from there
+    ctl[kIters] = warm ? skip : 0;
     ctl[kWarm] = warm;
+    ctl[kGuess] = warm;
+    ctl[kSkip] = warm ? skip : 0;
     ctl[kFallback] = 0;
     ctl[kDone0] = done0;
     dstate[0] = 1e10;  // dmax before the first iteration (Solidbody.f90:893)
@@ -421,9 +427,15 @@ __device__ inline double mdf_tl_sum(const double *tl, int M) {
 
 // stop test of the multi-direct-forcing loop (Solidbody.f90:895-906) from the
 // summed |dV|_1 of the markers: dmax = tol / (sum M_b * Uref), one iteration
-// counted; sets the done word when the loop condition fails
+// counted; sets the done word when the loop condition fails.
+// dstate[2] keeps the residual of loop iteration skip+1 (the first one after
+// a learned guess standing for `skip` rounds) for the next call: a guessed
+// call whose first residual exceeds twice that of the previous call restarts
+// from the zero guess (k_ws_fallback, k_band_macro on fallback), which then
+// runs the plain loop.  A negative dstate[2] (no previous call) always
+// falls back.
 __device__ inline void mdf_decide(int *ctl, double *dstate, double tol, double dsum, double Uref, int ntol,
-                                  double dtol) {
+                                  double dtol, int skip) {
   const int it = ctl[kIters] + 1;
   ctl[kIters] = it;
   if (!isfinite(tol)) {
@@ -433,15 +445,17 @@ __device__ inline void mdf_decide(int *ctl, double *dstate, double tol, double d
   }
   const double dm = tol / (dsum * Uref);
   dstate[0] = dm;
-  if (it == 1 && ctl[kWarm] == 1 && !(dm < dstate[1])) {
-    // the guessed force did not lower the residual below the zero guess's:
-    // restart the loop from zero (k_ws_fallback, k_band_macro on fallback)
-    ctl[kWarm] = 0;
-    ctl[kFallback] = 1;
-    ctl[kIters] = 0;
-    ctl[kDone] = ctl[kDone0];
-    dstate[0] = 1e10;
-    return;
+  if (it == skip + 1) {
+    const double ref = dstate[2];
+    dstate[2] = dm;
+    if (ctl[kWarm] == 1 && !(ref >= 0.0 && dm <= 2.0 * ref)) {
+      ctl[kWarm] = 0;
+      ctl[kFallback] = 1;
+      ctl[kIters] = 0;
+      ctl[kDone] = ctl[kDone0];
+      dstate[0] = 1e10;
+      return;
+    }
   }
   if (!(it < ntol && dm > dtol)) ctl[kDone] = 1;
 }
@@ -460,7 +474,8 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
                                                 const double *__restrict__ Ea, double invh3,
                                                 double *__restrict__ Eforce,
                                                 double *__restrict__ fT,
-                                                double *__restrict__ tl) {
+                                                double *__restrict__ tl, int snap = -1,
+                                                double *__restrict__ Esnap = nullptr) {
   if (ctl[kDone]) return;  // uniform over the grid
   if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kRan] = round;
   __shared__ double t[4][3][64];
@@ -496,6 +511,11 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
     Eforce[3LL * m] = ef1 + F1;
     Eforce[3LL * m + 1] = ef2 + F2;
     Eforce[3LL * m + 2] = ef3 + F3;
+    if (round == snap) {  // the force after round snap+1 (a training target of the warm start)
+      Esnap[3LL * m] = ef1 + F1;
+      Esnap[3LL * m + 1] = ef2 + F2;
+      Esnap[3LL * m + 2] = ef3 + F3;
+    }
     fT[3LL * m] = F1 * invh3;
     fT[3LL * m + 1] = F2 * invh3;
     fT[3LL * m + 2] = F3 * invh3;
@@ -538,6 +558,7 @@ struct MdfTest {
   double *dstate;
   int M, ntol;
   double dsum, Uref, dtol;
+  int skip;  // rounds a learned guess stands for (the fallback test is at iteration skip+1)
 };
 
 // velocity correction of every band cell: u -= fT*rx*ry*rz over its entries
@@ -556,7 +577,7 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
   if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
   if (round >= 0 && blockIdx.x == gridDim.x - 1) {
     const double tol = mdf_tl_sum(mt.tl, mt.M);
-    if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol);
+    if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol, mt.skip);
   }
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ctl[kNband]) return;
@@ -671,12 +692,21 @@ __global__ __launch_bounds__(64) void k_ibm_report(const int *__restrict__ ctl, 
 
 // ---------------------------------------------------------------------------
 // Learned warm start (no counterpart in the reference)
+//
+// The forcing loop is linear in the accumulated marker force: after j rounds
+// from the zero guess, Eforce = E_j and the band velocity is
+// u0 - S(E_j h^-3 / (2 rho)) with S the spreading of the stencil weights.  A
+// guess of E_k applied as one correction of the uncorrected band velocity u0
+// therefore puts the loop in the state of round k, and the loop goes on from
+// iteration k + 1 to the same cap and tolerance.  The network predicts E_k
+// as a correction to the marker's force of the previous call:
+//   E_k ~ Eprev + Uref |Ea| net(X)
+// and is trained on E_k recorded from plain calls.
 
 // Features of every marker for the network, one wave per marker: the
 // interpolated velocity of the uncorrected band (as k_interp forms it), the
 // marker velocity, the marker's force of the previous call and its normal:
 //   X = [U/Uref, V/Uref, Fprev/(Uref |Ea|), n]
-// and the zero guess's |V - U|_1 (the residual a plain loop starts from).
 __global__ __launch_bounds__(256) void k_ws_features(int M, const int *__restrict__ bidx,
                                                      const float *__restrict__ Ew,
                                                      const double *__restrict__ band_u, int cap,
@@ -684,7 +714,7 @@ __global__ __launch_bounds__(256) void k_ws_features(int M, const int *__restric
                                                      const double *__restrict__ Ea,
                                                      const double *__restrict__ Eprev,
                                                      const double *__restrict__ Enrm, double Uref,
-                                                     float *__restrict__ X, double *__restrict__ tl0) {
+                                                     float *__restrict__ X) {
   __shared__ double t[4][3][64];
   const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + w;
@@ -706,38 +736,29 @@ __global__ __launch_bounds__(256) void k_ws_features(int M, const int *__restric
     }
     const double fs = Uref * fabs(Ea[m]);
     float *x = X + 12LL * m;
-    double r = 0.0;
     for (int c = 0; c < 3; ++c) {
       x[c] = (float)(v[c] / Uref);
       x[3 + c] = (float)(Evel[3LL * m + c] / Uref);
       x[6 + c] = (float)(fs > 0.0 ? Eprev[3LL * m + c] / fs : 0.0);
       x[9 + c] = Enrm ? (float)Enrm[3LL * m + c] : 0.f;
-      r = r + fabs(Evel[3LL * m + c] - v[c]);
     }
-    tl0[m] = r;
   }
 }
 
-// dstate[1] = the zero guess's residual, reduced as the stop test reduces
-__global__ __launch_bounds__(256) void k_ws_dm0(const double *__restrict__ tl0, int M, double dsum, double Uref,
-                                                double *__restrict__ dstate) {
-  const double tol = mdf_tl_sum(tl0, M);
-  if (threadIdx.x == 0) dstate[1] = tol / (dsum * Uref);
-}
-
-// the network's guess as the starting force: Eforce = F0, fT = F0 * invh3
-// (applied to the band by k_correct as one forcing round)
+// the network's guess of E_k as the starting force: Eforce = Eprev + Y Uref
+// |Ea|, fT = Eforce * invh3 (applied to the band by k_correct as one
+// correction of the uncorrected velocity)
 __global__ __launch_bounds__(256) void k_ws_apply(int M, const float *__restrict__ Y, const double *__restrict__ Ea,
-                                                  double Uref, double invh3, double *__restrict__ Eforce,
-                                                  double *__restrict__ fT) {
+                                                  const double *__restrict__ Eprev, double Uref, double invh3,
+                                                  double *__restrict__ Eforce, double *__restrict__ fT) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * M) return;
-  const double F0 = (double)Y[i] * (Uref * fabs(Ea[i / 3]));
+  const double F0 = Eprev[i] + (double)Y[i] * (Uref * fabs(Ea[i / 3]));
   Eforce[i] = F0;
   fT[i] = F0 * invh3;
 }
 
-// restart from the zero guess when k_mdf_check asked for it
+// restart from the zero guess when the stop test asked for it
 __global__ __launch_bounds__(256) void k_ws_fallback(int M, int *__restrict__ ctl, double *__restrict__ Eforce) {
   if (ctl[kFallback] != 1) return;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -747,15 +768,21 @@ __global__ void k_ws_fallback_end(int *__restrict__ ctl) {
   if (threadIdx.x == 0 && ctl[kFallback] == 1) ctl[kFallback] = 2;
 }
 
-// training targets (the converged force over Uref |Ea|) and the previous
-// force of the next call
-__global__ __launch_bounds__(256) void k_ws_targets(int M, const double *__restrict__ Eforce,
-                                                    const double *__restrict__ Ea, double Uref,
+// training target of a plain call, (E_k - Eprev) / (Uref |Ea|) with E_k the
+// force after `skip` rounds (the final force when the loop stopped earlier),
+// then Eprev = the call's final force, the previous force of the next call
+__global__ __launch_bounds__(256) void k_ws_targets(int M, const int *__restrict__ ctl, int skip,
+                                                    const double *__restrict__ Eforce,
+                                                    const double *__restrict__ Esnap,
+                                                    const double *__restrict__ Ea, double Uref, int record,
                                                     float *__restrict__ Y, double *__restrict__ Eprev) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= 3 * M) return;
-  const double fs = Uref * fabs(Ea[i / 3]);
-  Y[i] = (float)(fs > 0.0 ? Eforce[i] / fs : 0.0);
+  if (record) {
+    const double fs = Uref * fabs(Ea[i / 3]);
+    const double Ek = (skip > 0 && ctl[kIters] >= skip) ? Esnap[i] : Eforce[i];
+    Y[i] = (float)(fs > 0.0 ? (Ek - Eprev[i]) / fs : 0.0);
+  }
   Eprev[i] = Eforce[i];
 }
 

@@ -15,6 +15,10 @@ from . import _lib
 from .block import LBMBlock
 from .solid import RigidPlate
 
+# DatInfo files of Beam_write_solid_info (SolidSolver.f90:1729-1799) and their row widths
+INFO_WIDTHS = (("firstNode", 15), ("lastNode", 15), ("centerNode", 15), ("nodeAverage", 12), ("forces", 6),
+               ("power", 7), ("energy", 8))
+
 
 @dataclasses.dataclass
 class PlateSpec:
@@ -78,6 +82,7 @@ class DeviceBodyBlock:
         self._lodFlow = np.zeros(6 * int(nND.sum()))
         self._stale = False
         self.iters = 0
+        self.rounds = 0
         self.dmax = 0.0
 
     def interaction_force(self, dt: float, wait: bool = True):
@@ -90,6 +95,8 @@ class DeviceBodyBlock:
             _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
                                                            None, None, _lib.iptr(it), _lib.dptr(dm)))
             self.iters, self.dmax = int(it[0]), float(dm[0])
+            _lib.check(self.lib.fsil_ibm_rounds(self.block.handle, _lib.iptr(it)))
+            self.rounds = int(it[0])
         else:
             _lib.check(self.lib.fsil_ibm_interaction_force(self.block.handle, float(dt), None, None, None, None,
                                                            None, None, None, None))
@@ -113,10 +120,11 @@ class DeviceBodyBlock:
         self._fetch()
         return self._lodFlow
 
-    def set_warm_start(self, net=None, record: bool = False):
-        """Learned warm start of the direct-forcing loop (fsilbm3d_amd.mlp.WarmStartNet)."""
+    def set_warm_start(self, net=None, record: bool = False, skip: int = 0):
+        """Learned warm start of the direct-forcing loop (fsilbm3d_amd.mlp.WarmStartNet):
+        the network's guess stands for the first ``skip`` forcing rounds."""
         _lib.check(self.lib.fsil_ibm_set_warm_start(self.block.handle, net.handle if net is not None else 0,
-                                                    int(bool(record))))
+                                                    int(bool(record)), int(skip)))
 
     def warm_start_data(self):
         """(features [M,12], targets [M,3], guess used, fell back) of the last step."""
@@ -136,6 +144,19 @@ class DeviceBodyBlock:
         _lib.check(self.lib.fsil_fem_state(self.block.handle, i, *[_lib.dptr(a) for a in out]))
         return [a.reshape(-1, 6) for a in out]
 
+    def solid_info(self, i: int, refs) -> dict:
+        """One DatInfo row per file of body i (Beam_write_solid_info,
+        SolidSolver.f90:1729-1799), computed on the device; refs = (Lref,
+        Uref, Aref, Fref, Pref, Eref)."""
+        rows = np.zeros(78)
+        _lib.check(self.lib.fsil_solid_info(self.block.handle, int(i),
+                                            _lib.dptr(np.ascontiguousarray(refs, dtype=np.float64)), _lib.dptr(rows)))
+        out, o = {}, 0
+        for name, w in INFO_WIDTHS:
+            out[name] = rows[o:o + w].copy()
+            o += w
+        return out
+
     def info(self, i: int):
         a = np.zeros(1, np.int32)
         b = np.zeros(1, np.int32)
@@ -150,6 +171,8 @@ class DeviceBodyBlock:
         dm = np.zeros(1)
         _lib.check(self.lib.fsil_ibm_report(self.block.handle, _lib.iptr(it), _lib.dptr(dm)))
         self.iters, self.dmax = int(it[0]), float(dm[0])
+        _lib.check(self.lib.fsil_ibm_rounds(self.block.handle, _lib.iptr(it)))
+        self.rounds = int(it[0])
 
     def ibm_fem(self, time: float, wait: bool = True):
         """IBM_FEM (LBMBlockComm.f90:320-338) at block time ``time``.  The beam

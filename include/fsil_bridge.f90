@@ -20,6 +20,7 @@ module fsil_bridge
               fsil_extract_interpolate_layer, fsil_interpolation_father_to_son, fsil_deliver_son_to_father, &
               fsil_tree_steps, fsil_time_tree_steps, fsil_mlp_create, fsil_mlp_destroy, fsil_mlp_get_params, &
               fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train, fsil_ibm_set_warm_start, &
+              fsil_ibm_rounds, fsil_solid_info, &
               fsil_ibm_warm_start_data, fsil_ibm_forces, &
               fsil_ibm_report
 
@@ -273,9 +274,20 @@ module fsil_bridge
             real(c_float), intent(in) :: X(*), Y(*)
             real(c_double), intent(out) :: loss_before, loss_after
         end function
-        integer(c_int) function fsil_ibm_set_warm_start(handle, mlp, record) bind(C, name='fsil_ibm_set_warm_start')
+        integer(c_int) function fsil_ibm_set_warm_start(handle, mlp, record, skip) bind(C, name='fsil_ibm_set_warm_start')
             import :: c_int
-            integer(c_int), value :: handle, mlp, record
+            integer(c_int), value :: handle, mlp, record, skip
+        end function
+        integer(c_int) function fsil_solid_info(handle, body, refs, rows) bind(C, name='fsil_solid_info')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, body
+            real(c_double), intent(in) :: refs(6)
+            real(c_double), intent(out) :: rows(78)
+        end function
+        integer(c_int) function fsil_ibm_rounds(handle, rounds) bind(C, name='fsil_ibm_rounds')
+            import :: c_int
+            integer(c_int), value :: handle
+            integer(c_int), intent(out) :: rounds
         end function
         integer(c_int) function fsil_ibm_warm_start_data(handle, X, Y, info) bind(C, name='fsil_ibm_warm_start_data')
             import :: c_int, c_float

@@ -167,6 +167,16 @@ int fsil_fem_update_markers(int handle);
  * the last fsil_ibm_interaction_force. */
 int fsil_fem_substeps(int handle, double time, int numsubstep, double deltat);
 
+/* Beam_write_solid_info for body b (SolidSolver.f90:1729-1799): the values of
+ * one row of each DatInfo/Group<g>_<file>.dat the reference appends at info
+ * cadence, computed on the device from the last beam sub-steps and coupling
+ * call, every sum in the reference's order.
+ *   refs[6]   Lref, Uref, Aref, Fref, Pref, Eref (Solidbody.f90:276-280)
+ *   rows[78]  firstNode[15] lastNode[15] centerNode[15] nodeAverage[12]
+ *             forces[6] power[7] energy[8], columns as the reference writes them
+ * The host keeps the file writing (formats, ZONE lines). */
+int fsil_solid_info(int handle, int body, const double *refs, double *rows);
+
 /* Node state of body b (6*nND each, node-major), and the last sub-step's
  * Newton iterations, conjugate-gradient iterations and displacement norm. */
 int fsil_fem_state(int handle, int body, double *pos, double *vel, double *acc, double *dsp);
@@ -219,10 +229,14 @@ int fsil_time_tree_steps(int root, int nsteps, double t0, double dt, double *tot
 /* ---------------------------------------------------------------------------
  * Learned warm start of the multi-direct-forcing loop.  No counterpart in the
  * reference: it extends calculate_interaction_force (Solidbody.f90:869-918)
- * with a per-marker network 12 -> 64 -> 64 -> 64 -> 3 (tanh hidden layers)
- * that guesses the converged marker force; the loop still runs to the same
- * tolerance.  Parameters: 9347 floats, W1[64][12] b1 W2[64][64] b2 W3[64][64]
- * b3 W4[3][64] b4. */
+ * with a per-marker network 12 -> 64 -> 64 -> 64 -> 3 (tanh hidden layers).
+ * The loop is linear in the accumulated marker force, so a guess of the force
+ * E_k that `skip` plain rounds reach, applied as one correction of the band
+ * velocity, puts the loop in the state of round `skip`; the loop then goes on
+ * from iteration skip+1 to the same cap ntolLBM and tolerance dtolLBM.  The
+ * network predicts E_k - Eprev (Eprev: the marker's final force of the
+ * previous call) over Uref |Ea|.  Parameters: 9347 floats, W1[64][12] b1
+ * W2[64][64] b2 W3[64][64] b3 W4[3][64] b4. */
 
 /* A network with Glorot-uniform weights from the splitmix64 stream of seed. */
 int fsil_mlp_create(int seed, int *handle);
@@ -240,17 +254,23 @@ int fsil_mlp_loss_grad(int handle, const float *X, const float *Y, int n, double
 int fsil_mlp_train(int handle, const float *X, const float *Y, int n, int epochs, int batch, double lr, int seed,
                    double *loss_before, double *loss_after);
 
-/* Use network `mlp` (0: none) to guess the starting marker force of every
- * coupling call of block `handle`; record != 0 computes the network's features
- * and training targets on every call even without a network.  When the guess
- * does not lower the first residual below the zero guess's, the call restarts
- * from zero (the plain loop's result). */
-int fsil_ibm_set_warm_start(int handle, int mlp, int record);
+/* Use network `mlp` (0: none) to guess the force after `skip` rounds
+ * (0 <= skip < ntolLBM) in every coupling call of block `handle` that has a
+ * previous call; record != 0 computes the network's features and, in plain
+ * calls, its training targets (E_skip - Eprev) / (Uref |Ea|).  A guessed call
+ * whose first residual exceeds twice the residual the previous call had at
+ * iteration skip+1 restarts from the zero guess and runs the plain loop. */
+int fsil_ibm_set_warm_start(int handle, int mlp, int record, int skip);
+
+/* Forcing rounds the last fsil_ibm_interaction_force ran: interpolation and
+ * correction rounds, plus one for the correction that applies a learned
+ * guess (a restarted call counts every round it ran). */
+int fsil_ibm_rounds(int handle, int *rounds);
 
 /* Features X[M][12] (interpolated velocity / Uref, marker velocity / Uref,
- * previous force / (Uref |Ea|), normal) and targets Y[M][3] (converged force /
- * (Uref |Ea|)) of the last coupling call; info[0] = guess used, info[1] =
- * fell back to zero. */
+ * previous force / (Uref |Ea|), normal) and targets Y[M][3] ((E_skip - Eprev)
+ * / (Uref |Ea|), plain calls) of the last coupling call; info[0] = guess used,
+ * info[1] = fell back to zero. */
 int fsil_ibm_warm_start_data(int handle, float *X, float *Y, int *info);
 
 #ifdef __cplusplus

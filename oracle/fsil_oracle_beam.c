@@ -955,3 +955,109 @@ void orc_beam_marker_elem(const Beam *b, int *me, int *en) {
     en[2 * i + 1] = b->el[i].node1;
   }
 }
+
+/* ------------------------------------------------------------------ */
+/* Beam_write_solid_info (SolidSolver.f90:1729-1799): the values of one DatInfo
+ * row per file, in the reference's arithmetic order.  refs = Lref, Uref, Aref,
+ * Fref, Pref, Eref (Solidbody.f90:276-280); XYZo the body's reference point.
+ * rows (78 values): firstNode[15] lastNode[15] centerNode[15] nodeAverage[12]
+ * forces[6] power[7] energy[8]. */
+
+/* Segment_StrainEnergy_D after FormStiffMatrix (SolidSolver.f90:841-907,
+ * 2340-2348): stretching and bending/torsion energy of the local deformation */
+static void strain_energy(Seg *e, double *es, double *eb) {
+  form_stiff(e);
+  const double du = e->dx1 - e->dx0, dv = e->dy1 - e->dy0, dw = e->dz1 - e->dz0;
+  const double dl = ((e->dx0 + e->dx1) * du + (e->dy0 + e->dy1) * dv + (e->dz0 + e->dz1) * dw) / (e->len0 + e->len1);
+  double tx, ty, tz, tx1, ty1, tz1, tx2, ty2, tz2;
+  get_angle_triad(e->tee, e->tn1, &tx, &ty, &tz);
+  global_to_local(e->tee, tx, ty, tz, &tx1, &ty1, &tz1);
+  get_angle_triad(e->tee, e->tn2, &tx, &ty, &tz);
+  global_to_local(e->tee, tx, ty, tz, &tx2, &ty2, &tz2);
+  const double ub[NE] = {0.0, 0.0, 0.0, tx1, ty1, tz1, dl, 0.0, 0.0, tx2, ty2, tz2};
+  double S[NE][NE], B[NE][NE];
+  for (int i = 0; i < NE; ++i)
+    for (int j = 0; j < NE; ++j) { S[i][j] = 0.0; B[i][j] = e->stf[i][j]; }
+  S[0][0] = e->stf[0][0]; S[0][6] = e->stf[0][6]; S[6][6] = e->stf[6][6]; S[6][0] = e->stf[6][0];
+  B[0][0] = 0.0; B[0][6] = 0.0; B[6][6] = 0.0; B[6][0] = 0.0;
+  double s1 = 0.0, s2 = 0.0;
+  for (int i = 0; i < NE; ++i) {
+    double a = 0.0, c = 0.0;
+    for (int k = 0; k < NE; ++k) { a = a + S[i][k] * ub[k]; c = c + B[i][k] * ub[k]; }
+    s1 = s1 + a * ub[i];
+    s2 = s2 + c * ub[i];
+  }
+  *es = 0.5 * s1;
+  *eb = 0.5 * s2;
+}
+
+static void node_row(const Beam *b, int n, const double *XYZo, const double *refs, double *r) {
+  const double Lref = refs[0], Uref = refs[1], Aref = refs[2];
+  for (int k = 0; k < 3; ++k) {
+    r[k] = XYZo[k] / Lref;
+    r[3 + k] = (b->pos[6 * n + k] - XYZo[k]) / Lref;
+    r[6 + k] = b->pos[6 * n + 3 + k];
+    r[9 + k] = b->vel[6 * n + k] / Uref;
+    r[12 + k] = b->acc[6 * n + k] / Aref;
+  }
+}
+
+void orc_beam_solid_info(Beam *b, const double *XYZo, const double *refs, double *rows) {
+  const double Lref = refs[0], Uref = refs[1], Aref = refs[2], Fref = refs[3], Pref = refs[4], Eref = refs[5];
+  const int nND = b->nND;
+  node_row(b, 0, XYZo, refs, rows);
+  node_row(b, nND - 1, XYZo, refs, rows + 15);
+  node_row(b, (nND + 1) / 2 - 1, XYZo, refs, rows + 30);
+  double *r = rows + 45;
+  for (int k = 0; k < 3; ++k) {
+    double ms = 0.0, xs = 0.0, vs = 0.0, as = 0.0;
+    for (int n = 0; n < nND; ++n) ms = ms + b->mss[3 * n + k];
+    for (int n = 0; n < nND; ++n) xs = xs + b->pos[6 * n + k] * b->mss[3 * n + k];
+    for (int n = 0; n < nND; ++n) vs = vs + b->vel[6 * n + k] * b->mss[3 * n + k];
+    for (int n = 0; n < nND; ++n) as = as + b->acc[6 * n + k] * b->mss[3 * n + k];
+    r[k] = XYZo[k] / Lref;
+    r[3 + k] = (xs / ms - XYZo[k]) / Lref;
+    r[6 + k] = (vs / ms) / Uref;
+    r[9 + k] = (as / ms) / Aref;
+  }
+  r = rows + 57;
+  double P[3];
+  for (int k = 0; k < 3; ++k) {
+    double f = 0.0, p = 0.0;
+    for (int n = 0; n < nND; ++n) f = f + b->lodFlow[6 * n + k];
+    for (int n = 0; n < nND; ++n) p = p + b->lodFlow[6 * n + k] * b->vel[6 * n + k];
+    r[k] = XYZo[k] / Lref;
+    r[3 + k] = f / Fref;
+    P[k] = p / Pref;
+  }
+  double *pw = rows + 63;
+  for (int k = 0; k < 3; ++k) pw[k] = XYZo[k] / Lref;
+  pw[3] = (P[0] + P[1]) + P[2];
+  pw[4] = P[0]; pw[5] = P[1]; pw[6] = P[2];
+  double Es = 0.0, Eb = 0.0, Ev = 0.0;
+  for (int i = 0; i < b->nEL; ++i) {
+    double es, eb;
+    strain_energy(&b->el[i], &es, &eb);
+    Es = Es + es;
+    Eb = Eb + eb;
+  }
+  Es = Es / Eref;
+  Eb = Eb / Eref;
+  const double Ep = Es + Eb;
+  for (int i = 0; i < b->nEL; ++i) {
+    const Seg *e = &b->el[i];
+    double ve[NE];
+    for (int k = 0; k < 6; ++k) { ve[k] = b->vel[6 * e->node0 + k]; ve[6 + k] = b->vel[6 * e->node1 + k]; }
+    double d = 0.0;
+    for (int j = 0; j < NE; ++j) {
+      double mv = 0.0;
+      for (int k = 0; k < NE; ++k) mv = mv + e->mas[j][k] * ve[k];
+      d = d + ve[j] * mv;
+    }
+    Ev = Ev + 0.5 * d;
+  }
+  Ev = Ev / Eref;
+  double *en = rows + 70;
+  for (int k = 0; k < 3; ++k) en[k] = XYZo[k] / Lref;
+  en[3] = Ev + Ep; en[4] = Ev; en[5] = Ep; en[6] = Es; en[7] = Eb;
+}

@@ -287,6 +287,7 @@ class OracleBeam:
         L.orc_beam_asfac.argtypes = [C.c_void_p]
         L.orc_beam_asfac.restype = C.c_double
         L.orc_beam_marker_elem.argtypes = [C.c_void_p, _pi, _pi]
+        L.orc_beam_solid_info.argtypes = [C.c_void_p, _pd, _pd, _pd]
         cls._sig_done = True
 
     def __init__(self, nodes8, elements, constraints, *, iBodyModel=2, isMotionGiven=(1, 1, 1, 1, 1, 1), denR=1.0,
@@ -347,9 +348,44 @@ class OracleBeam:
         lib().orc_beam_state(self.h, *[_d(a) for a in out])
         return [a.reshape(self.nND, 6) for a in out]
 
+    def solid_info(self, XYZo, refs):
+        """One DatInfo row per file of Beam_write_solid_info (SolidSolver.f90:1729-1799)
+        as a dict name -> values; refs = (Lref, Uref, Aref, Fref, Pref, Eref)."""
+        rows = np.zeros(78)
+        lib().orc_beam_solid_info(self.h, _d(np.ascontiguousarray(XYZo, dtype=np.float64)),
+                                  _d(np.ascontiguousarray(refs, dtype=np.float64)), _d(rows))
+        return split_info_rows(rows)
+
     def info(self):
         a = np.zeros(1, np.int32)
         b = np.zeros(1, np.int32)
         c = np.zeros(1)
         lib().orc_beam_info(self.h, _i(a), _i(b), _d(c))
         return int(a[0]), int(b[0]), float(c[0])
+
+
+INFO_WIDTHS = (("firstNode", 15), ("lastNode", 15), ("centerNode", 15), ("nodeAverage", 12), ("forces", 6),
+               ("power", 7), ("energy", 8))
+
+
+def split_info_rows(rows):
+    """The 78 values of one info step as DatInfo file name -> row."""
+    out, o = {}, 0
+    for name, w in INFO_WIDTHS:
+        out[name] = np.asarray(rows[o:o + w])
+        o += w
+    return out
+
+
+def reference_scales(UrefType, uvwIn, Uref, Lref, TrefType, Tref, denIn, Asfac):
+    """Lref, Uref, Aref, Fref, Pref, Eref as calculate_reference_params derives
+    them (Solidbody.f90:219-284), for the reference-type choices the tests use
+    (LrefType 1; UrefType 0 or >= 8; TrefType 0 or >= 2)."""
+    U = abs(uvwIn[0]) if UrefType == 0 else Uref
+    T = Lref / U if TrefType == 0 else Tref
+    A = U / T
+    q = (0.5 * denIn) * (U * U)
+    F = q * Asfac
+    E = (q * Asfac) * Lref
+    P = (q * Asfac) * U
+    return (Lref, U, A, F, P, E)

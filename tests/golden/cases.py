@@ -216,6 +216,79 @@ BY_NAME: Dict[str, ParityCase] = {c.name: c for c in CASES}
 SINGLE = [c for c in CASES if not c.case.sons]      # one fluid block
 TREE = [c for c in CASES if c.case.sons]            # a root block with refined sons
 
+# DatInfo files the reference writes per body group at every info step
+# (Beam_write_solid_info, SolidSolver.f90:1729-1799): columns as written there.
+INFO_FILES = ("forces", "firstNode", "lastNode", "centerNode", "nodeAverage", "power", "energy")
+
+
+def heaving_plate_128(nsteps: int = 2000) -> ParityCase:
+    """The rigid heaving-plate workload at its full size: 128x64x64, uniform
+    inflow 0.05 (101), outflow (103), slip faces (302), Re 100 on a 16-cell
+    chord; one rigid plate of 17 chord nodes x 16 span samples (256 markers)
+    centred at x = 40, heave 0.5 L at 0.25 U/L; 3 direct-forcing rounds.  The
+    state starts as the uniform inflow equilibrium."""
+    case = Case(dims=(128, 64, 64), bc=(101, 103, 302, 302, 302, 302), Re=100.0, uvwIn=(0.05, 0.0, 0.0),
+                UrefType=0, Uref=0.05, LrefType=1, Lref=16.0, TrefType=0, ntolLBM=3, dtolLBM=1e-14)
+    plate = Plate(17, -8.0, 8.0, 8.0, 16, XYZo=(40.0, 32.0, 32.0), XYZAmpl=(0.0, 8.0, 0.0), Freq=7.8125e-4)
+    return ParityCase("c1_heaving_plate", case, nsteps=nsteps, seed=41, ubase=(0.05, 0, 0), neq=0.0,
+                      drho=0.0, du=0.0, plates=[plate])
+
+
+def flexible_plate_384(nsteps: int = 20) -> ParityCase:
+    """The flexible-plate workload at its full size: 384x192x192, inflow 0.05,
+    Re 500 on L = 48; one elastic plate of 49 chord nodes x 48 span samples
+    (2304 markers, mass ratio 1, KB 0.1, KS 1000), leading edge clamped at
+    x = 96 and heaving 0.3 L at 0.2 U/L; numsubstep 4, direct forcing to 1e-6
+    with at most 10 rounds.  The state starts as the uniform inflow equilibrium."""
+    case = Case(dims=(384, 192, 192), bc=(101, 103, 302, 302, 302, 302), Re=500.0, uvwIn=(0.05, 0.0, 0.0),
+                UrefType=0, Uref=0.05, LrefType=1, Lref=48.0, TrefType=0, ntolLBM=10, dtolLBM=1e-6,
+                numsubstep=4, dtolFEM=1e-10, ntolFEM=20)
+    plate = Plate(49, -24.0, 24.0, 24.0, 48, XYZo=(120.0, 96.0, 96.0), XYZAmpl=(0.0, 14.4, 0.0),
+                  Freq=0.2 * 0.05 / 48.0, iBodyModel=2, denR=1.0, KB=0.1, KS=1000.0)
+    return ParityCase("c3_flexible_plate", case, nsteps=nsteps, seed=43, ubase=(0.05, 0, 0), neq=0.0,
+                      drho=0.0, du=0.0, plates=[plate])
+
+
+@dataclasses.dataclass
+class LongCase:
+    """A full-size workload run by the reference for many steps.  Its fixture
+    keeps every DatInfo row of every step and a few small windows of the
+    final state (the whole state is 80 MB - 2 GB)."""
+    pc: ParityCase
+    windows: List[tuple]        # (x0, x1, y0, y1, z0, z1), half-open, 0-based
+    npsize: int = 8
+
+    @property
+    def name(self) -> str:
+        return self.pc.name
+
+    def uniform_state(self) -> np.ndarray:
+        from fsilbm3d_amd.params import equilibrium_point
+        f19 = equilibrium_point(1.0, self.pc.ubase)
+        f = np.empty((19,) + tuple(self.pc.case.dims))
+        for q in range(19):
+            f[q].fill(f19[q])
+        return f
+
+    def window(self, f: np.ndarray, k: int) -> np.ndarray:
+        x0, x1, y0, y1, z0, z1 = self.windows[k]
+        return np.ascontiguousarray(f[:, x0:x1, y0:y1, z0:z1])
+
+
+LONG_CASES: List[LongCase] = [
+    LongCase(heaving_plate_128(2000), windows=[(36, 44, 28, 36, 28, 36), (30, 34, 20, 44, 30, 34),
+                                                (0, 2, 0, 8, 0, 8), (124, 128, 28, 36, 28, 36)]),
+    LongCase(flexible_plate_384(20), windows=[(116, 124, 92, 100, 92, 100), (94, 98, 90, 102, 94, 98),
+                                               (142, 146, 90, 102, 94, 98), (0, 2, 0, 8, 0, 8),
+                                               (380, 384, 92, 100, 92, 100)]),
+]
+LONG_BY_NAME: Dict[str, LongCase] = {c.name: c for c in LONG_CASES}
+
+
+def info_rows(fx: dict, name: str, group: int = 1) -> np.ndarray:
+    """Rows of DatInfo/Group<g>_<name>.dat kept in a fixture."""
+    return fx[f"{name}_g{group}"]
+
 
 def fixture_path(name: str) -> str:
     return os.path.join(HERE, f"{name}.npz")

@@ -4,12 +4,20 @@ reference sources by oracle/build_ref.sh), from the seeded initial state
 injected through its own restart file, and the state it writes after N steps
 is stored as tests/golden/<case>.npz:
 
-  f_final  (19, xDim, yDim, zDim) float64  post-stream + BC state after N steps
-  forces   per body group, the DatInfo Group<g>_forces.dat rows (10 digits)
+  f_final        (19, xDim, yDim, zDim) float64  post-stream + BC state after N steps
+  <file>_g<g>    per body group, the rows of DatInfo/Group<g>_<file>.dat
+                 (file: forces, firstNode, lastNode, centerNode, nodeAverage,
+                 power, energy; 10 significant digits, one row per step)
+
+The full-size cases (LONG_CASES: the 128x64x64 heaving plate for 2000 steps,
+the 384x192x192 flexible plate for 20 steps) keep every DatInfo row and, in
+place of f_final, small windows of the final state (window<k>, with the
+bounds in windows).
 
 Run:  python tests/golden/make_golden.py [case ...]
 """
 import os
+import shutil
 import sys
 
 import numpy as np
@@ -18,7 +26,29 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
 from oracle.refcase import run_reference  # noqa: E402
-from tests.golden.cases import CASES, fixture_path  # noqa: E402
+from tests.golden.cases import CASES, INFO_FILES, LONG_CASES, fixture_path  # noqa: E402
+
+
+def _isnum(s):
+    try:
+        float(s)
+        return True
+    except ValueError:
+        return False
+
+
+def read_info(workdir, ngroups):
+    """Numeric rows of every DatInfo group file (ZONE/title lines skipped)."""
+    out = {}
+    for g in range(ngroups):
+        for name in INFO_FILES:
+            p = os.path.join(workdir, "DatInfo", f"Group{g + 1:03d}_{name}.dat")
+            if not os.path.exists(p):
+                continue
+            rows = [l.split() for l in open(p) if l.strip()]
+            rows = [[float(v) for v in row] for row in rows if all(_isnum(v) for v in row)]
+            out[f"{name}_g{g + 1}"] = np.array(rows)
+    return out
 
 
 def make(pc):
@@ -33,23 +63,26 @@ def make(pc):
             out[f"f_final_son{k + 1}"] = fs
     else:
         out = {"f_final": r.f, "step": np.array(r.step)}
-    for g in range(len(pc.plates)):
-        p = os.path.join(r.workdir, "DatInfo", f"Group{g + 1:03d}_forces.dat")
-        rows = [l.split() for l in open(p) if l.strip() and not l.lstrip().startswith(("ZONE", "VARIABLES", "TITLE", "variables"))]
-        rows = [[float(v) for v in row] for row in rows if all(_isnum(v) for v in row)]
-        out[f"forces_g{g + 1}"] = np.array(rows)
-    import shutil
+    out.update(read_info(r.workdir, len(pc.plates)))
     shutil.rmtree(r.workdir, ignore_errors=True)
     np.savez_compressed(fixture_path(pc.name), **out)
     return r.step
 
 
-def _isnum(s):
-    try:
-        float(s)
-        return True
-    except ValueError:
-        return False
+def make_long(lc):
+    pc = lc.pc
+    case = pc.full_case()
+    case.npsize = lc.npsize
+    case.info_every = 1
+    r = run_reference(case, lc.uniform_state(), pc.nsteps, keep=True, timeout=3600.0)
+    out = {"step": np.array(r.step), "windows": np.array(lc.windows, dtype=np.int32),
+           "step_seconds": np.array(r.step_seconds)}
+    for k in range(len(lc.windows)):
+        out[f"window{k}"] = lc.window(r.f, k)
+    out.update(read_info(r.workdir, len(pc.plates)))
+    shutil.rmtree(r.workdir, ignore_errors=True)
+    np.savez_compressed(fixture_path(pc.name), **out)
+    return r.step
 
 
 if __name__ == "__main__":
@@ -59,3 +92,8 @@ if __name__ == "__main__":
             continue
         step = make(pc)
         print(f"{pc.name}: {step} steps -> {os.path.basename(fixture_path(pc.name))}", flush=True)
+    for lc in LONG_CASES:
+        if names and lc.name not in names:
+            continue
+        step = make_long(lc)
+        print(f"{lc.name}: {step} steps -> {os.path.basename(fixture_path(lc.name))}", flush=True)

@@ -32,6 +32,25 @@ def derived(case):
     return dict(nu=nu, tau=tau, Omega=om, Omega2=om2, Mc=Mc, Mf=Mf, Uref=uref, Tref=tref)
 
 
+def info_scales(case, d, asfacs):
+    """Lref, Uref, Aref, Fref, Pref, Eref of a case whose reference area is
+    the largest body's (allocate_solid_memory, Solidbody.f90:302-305)."""
+    from oracle.oracle import reference_scales
+    asfac = asfacs[int(np.argmax(asfacs))] if len(asfacs) else 0.0
+    return reference_scales(case.UrefType, case.uvwIn, case.Uref, case.Lref, case.TrefType, case.Tref, case.denIn,
+                            asfac)
+
+
+def plate_asfac(p):
+    """Sum of the element lengths areaElem00 in element order (SolidSolver.f90:83-85, 1375)."""
+    x = p.nodes()[:, :3]
+    s = 0.0
+    for e in range(len(x) - 1):
+        d = x[e + 1] - x[e]
+        s = s + float(np.sqrt((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]))
+    return s
+
+
 def make_bodies(pc, denIn):
     out = []
     for p in pc.plates:
@@ -105,6 +124,8 @@ def run_oracle(pc, f0=None, nsteps=None, record=None):
             if record is not None:
                 record[-1]["beam"] = [b.state() for b in bodies]
                 record[-1]["fem_info"] = [b.info() for b in bodies]
+                refs = info_scales(case, d, [b.Asfac for b in bodies])
+                record[-1]["solid_info"] = [b.solid_info(p.XYZo, refs) for b, p in zip(bodies, pc.plates)]
     return f
 
 
@@ -303,4 +324,74 @@ def run_plate_fsi(w, precision, nsteps, record_every=1, net=None, warm_from=None
     if collect:
         out["X"] = np.concatenate(XS)
         out["Y"] = np.concatenate(YS)
+    return out
+
+
+def print_equal(dev, ref, rtol=0.0, atol=0.0):
+    """True where a value equals the reference's DatInfo entry (written with
+    10 significant digits, E20.10) after rounding to 10 digits, or, with rtol
+    or atol, lies within half a unit of the 10th digit plus that tolerance."""
+    dev = np.asarray(dev, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    fmt = np.vectorize(lambda v: float("%.9e" % v))(dev) if dev.size else dev
+    same = fmt == ref
+    if rtol or atol:
+        mag = np.where(ref != 0.0, np.floor(np.log10(np.abs(np.where(ref != 0.0, ref, 1.0)))), 0.0)
+        half = 0.5 * 10.0 ** (mag - 9.0)
+        same = same | (np.abs(dev - ref) <= half * (1.0 + 1e-9) + rtol * np.abs(ref) + atol)
+    return same
+
+
+def run_device_long(lc, nsteps=None, precision=0, info=True, net=None, warm_from=None, skip=0, collect=0):
+    """A full-size workload (tests/golden/cases.py LongCase) on the device
+    from the uniform inflow state (filled on the device): per-step DatInfo rows
+    of every body (dict file -> array [steps, width]), iteration counts, the
+    forcing rounds run and the final state's windows (FP64 form only)."""
+    from fsilbm3d_amd.block import LBMBlock
+    from fsilbm3d_amd.params import equilibrium_point
+    from fsilbm3d_amd.fsi import DeviceBodyBlock
+    pc = lc.pc
+    case = pc.case
+    d = derived(case)
+    blk = LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params, precision=precision)
+    blk.set_flow(**flow_kwargs(case))
+    blk.fill(equilibrium_point(1.0, pc.ubase))
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    fsi = DeviceBodyBlock(blk, plate_specs(pc), ntolLBM=case.ntolLBM, dtolLBM=case.dtolLBM, Uref=d["Uref"],
+                          denIn=case.denIn, Lref=case.Lref, numsubstep=case.numsubstep,
+                          IBPenaltyAlpha=case.IBPenaltyAlpha, dampK=case.dampK, dampM=case.dampM,
+                          GeoGamma=case.GeoGamma, NewmarkGamma=case.NewmarkGamma, NewmarkBeta=case.NewmarkBeta,
+                          dtolFEM=case.dtolFEM, ntolFEM=case.ntolFEM)
+    refs = info_scales(case, d, [plate_asfac(p) for p in pc.plates])
+    n = pc.nsteps if nsteps is None else nsteps
+    rows = [dict() for _ in pc.plates]
+    iters, rounds, XS, YS = [], [], [], []
+    time = 0.0
+    for k in range(n):
+        use = net is not None and warm_from is not None and k >= warm_from
+        if net is not None or collect or skip:
+            fsi.set_warm_start(net if use else None, record=k < collect, skip=skip)
+        time = time + case.dh
+        fsi.step(time)
+        iters.append(fsi.iters)
+        rounds.append(fsi.rounds)
+        if k < collect:
+            X, Y, _, _ = fsi.warm_start_data()
+            XS.append(X)
+            YS.append(Y)
+        if info:
+            for i in range(len(pc.plates)):
+                for name, v in fsi.solid_info(i, refs).items():
+                    rows[i].setdefault(name, []).append(v)
+    out = dict(rows=[{k: np.array(v) for k, v in r.items()} for r in rows], iters=iters, rounds=rounds,
+               pos=fsi.state(0)[0].copy())
+    if precision == 0:
+        f = blk.download()
+        out["windows"] = [lc.window(f, k) for k in range(len(lc.windows))]
+        del f
+    if collect:
+        out["X"] = np.concatenate(XS)
+        out["Y"] = np.concatenate(YS)
+    blk.close()
     return out

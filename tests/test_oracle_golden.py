@@ -86,3 +86,79 @@ def test_tree_fixtures_pin_the_coupling():
         assert all(n > 0 for n in differs())
     finally:
         T.grid_transform, T.OracleTree.extract, T.interp_plane = gt, ex, ip
+
+
+BODY = [c for c in CASES if c.plates]
+
+
+@pytest.mark.parametrize("pc", BODY, ids=[c.name for c in BODY])
+def test_oracle_solid_info_matches_reference_datinfo(pc):
+    """Every DatInfo row the reference wrote for every body at every step
+    (forces = Cd/Cl/Cz, first/last/centre node, node average, power, energy;
+    Beam_write_solid_info, SolidSolver.f90:1729-1799) equals the oracle's
+    value rounded to the 10 digits the reference prints."""
+    from tests.golden.cases import INFO_FILES
+    from tests.helpers import print_equal
+    fx = load_fixture(pc.name)
+    rec = []
+    run_oracle(pc, record=rec)
+    for g in range(len(pc.plates)):
+        for name in INFO_FILES:
+            ref = fx[f"{name}_g{g + 1}"]
+            got = np.array([r["solid_info"][g][name] for r in rec])
+            assert got.shape == ref.shape, (name, got.shape, ref.shape)
+            ok = print_equal(got, ref)
+            assert ok.all(), f"{pc.name} group {g + 1} {name}: {int((~ok).sum())} entries differ"
+
+
+@pytest.mark.parametrize("name,nsteps", [("c1_heaving_plate", 20), ("c3_flexible_plate", 2)])
+def test_oracle_matches_full_size_reference_runs(name, nsteps):
+    """The full-size workloads (128x64x64 rigid heaving plate, 2000 reference
+    steps kept; 384x192x192 flexible plate, 20 kept): the oracle's first steps
+    reproduce every DatInfo row of the reference at its print precision."""
+    import dataclasses
+    from tests.golden.cases import INFO_FILES, LONG_BY_NAME
+    from tests.helpers import print_equal
+    lc = LONG_BY_NAME[name]
+    fx = load_fixture(name)
+    rec = []
+    run_oracle(dataclasses.replace(lc.pc, nsteps=nsteps), f0=lc.uniform_state(), record=rec)
+    for fname in INFO_FILES:
+        ref = fx[f"{fname}_g1"][:nsteps]
+        got = np.array([r["solid_info"][0][fname] for r in rec])
+        ok = print_equal(got, ref)
+        assert ok.all(), f"{name} {fname}: {int((~ok).sum())} entries differ"
+
+
+def test_long_fixtures_cover_their_steps():
+    """The full-size fixtures hold one row per step and the final windows."""
+    from tests.golden.cases import INFO_FILES, LONG_CASES
+    for lc in LONG_CASES:
+        fx = load_fixture(lc.name)
+        assert int(fx["step"]) == lc.pc.nsteps
+        for fname in INFO_FILES:
+            assert fx[f"{fname}_g1"].shape[0] == lc.pc.nsteps
+        for k, (x0, x1, y0, y1, z0, z1) in enumerate(lc.windows):
+            assert fx[f"window{k}"].shape == (19, x1 - x0, y1 - y0, z1 - z0)
+
+
+def test_planted_datinfo_mistakes_are_caught():
+    """The reference's DatInfo rows pin the reductions: force coefficients
+    over the span-wise area instead of the chord sum (Fref, Solidbody.f90:278),
+    or the kinetic energy without its factor 1/2, each break them."""
+    from tests.golden.cases import BY_NAME
+    from tests.helpers import print_equal
+    pc = BY_NAME["fem_flexible_plate"]
+    fx = load_fixture(pc.name)
+    rec = []
+    run_oracle(pc, record=rec)
+    cd = np.array([r["solid_info"][0]["forces"] for r in rec])
+    assert print_equal(cd, fx["forces_g1"]).all()
+    wrong = cd.copy()
+    wrong[:, 3:] = wrong[:, 3:] / (pc.plates[0].lspan + pc.plates[0].rspan)
+    assert not print_equal(wrong[:, 3:], fx["forces_g1"][:, 3:]).any()
+    en = np.array([r["solid_info"][0]["energy"] for r in rec])
+    assert print_equal(en, fx["energy_g1"]).all()
+    wrong = en.copy()
+    wrong[:, 4] = 2.0 * wrong[:, 4]
+    assert not print_equal(wrong[:, 4], fx["energy_g1"][:, 4]).any()
