@@ -62,6 +62,7 @@ SIGNATURES = {
     "fsil_ibm_set_warm_start": (_i, [_i, _i, _i, _i]),
     "fsil_ibm_rounds": (_i, [_i, _pi]),
     "fsil_solid_info": (_i, [_i, _i, _pd, _pd]),
+    "fsil_solid_info_static": (_i, [_i, _i, _pd, _pd]),
     "fsil_ibm_warm_start_data": (_i, [_i, _pf, _pf, _pi]),
     "fsil_fem_define_bodies": (_i, [_i, _i, _pi, _pi, _pd, _pi, _pi, _pi, _pi, _pd, _pd, _d, _d, _d, _d]),
     "fsil_fem_set_solver": (_i, [_i, _d, _d, _d, _d, _d, _d, _i]),

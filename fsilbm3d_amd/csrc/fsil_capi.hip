@@ -271,6 +271,15 @@ std::map<int, std::unique_ptr<Block>> g_blocks;
 int g_next_handle = 1;
 int g_device = -1;
 
+// Device memory of fsil_solid_info_static, for callers that cannot allocate:
+// the per-element energies (3 per element) and the 78 row values of one body
+// of at most kStaticInfoElems elements.  Every device the library runs on
+// holds one copy until the process ends; g_scratch keeps its device address
+// per device (taken in fsil_init).  Calls on one device share the array, so
+// calls that overlap on one device race.
+constexpr int kStaticInfoElems = 256;
+static __device__ __align__(256) double scratch[3 * kStaticInfoElems + 78];
+std::map<int, double *> g_scratch;
 
 Block *get_block(int h) {
   auto it = g_blocks.find(h);
@@ -750,6 +759,12 @@ int fsil_init(int device) {
   if (device < 0 || device >= n) return fail("device %d out of range (%d devices)", device, n);
   HIPCHK(hipSetDevice(device));
   g_device = device;
+  if (!g_scratch.count(device)) {
+    double *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail("no device address for the static scratch on device %d", device);
+    g_scratch[device] = ptr;
+  }
   return 0;
 }
 
@@ -1805,6 +1820,29 @@ int fsil_solid_info(int h, int body, const double *refs, double *rows) {
   HIPCHK(F.info.ensure(3 * (size_t)d.nEL + 78));
   SolidInfoArgs a{};
   if (fill_solid_info_args(*b, body, refs, F.info.p, a)) return 1;
+  hipLaunchKernelGGL(k_solid_info, dim3(1), dim3(256), 0, b->stream, a);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(rows, a.rows, 78 * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int fsil_solid_info_static(int h, int body, const double *refs, double *rows) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (body < 0 || body >= F.nbodies) return fail("body %d out of range", body);
+  if (!refs || !rows) return fail("null refs or rows");
+  const auto it = g_scratch.find(g_device);
+  if (it == g_scratch.end()) return fail("fsil_init has not set up the static scratch of device %d", g_device);
+  const FemBodyDesc &d = F.desc[body];
+  if (d.nEL > kStaticInfoElems)
+    return fail("fsil_solid_info_static: body %d has %d elements, at most %d fit the static scratch (use "
+                "fsil_solid_info)", body + 1, d.nEL, kStaticInfoElems);
+  if (fem_collect(*b)) return 1;
+  if (ibm_check_report(*b)) return 1;
+  SolidInfoArgs a{};
+  if (fill_solid_info_args(*b, body, refs, it->second, a)) return 1;
   hipLaunchKernelGGL(k_solid_info, dim3(1), dim3(256), 0, b->stream, a);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(rows, a.rows, 78 * sizeof(double), hipMemcpyDeviceToHost, b->stream));

@@ -177,6 +177,14 @@ int fsil_fem_substeps(int handle, double time, int numsubstep, double deltat);
  * The host keeps the file writing (formats, ZONE lines). */
 int fsil_solid_info(int handle, int body, const double *refs, double *rows);
 
+/* fsil_solid_info without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Its
+ * kernel works in a static device array of the library (one per device, set
+ * up by fsil_init) that holds bodies of at most 256 elements; a larger body
+ * is an error.  Calls that overlap on one device share that array and race. */
+int fsil_solid_info_static(int handle, int body, const double *refs, double *rows);
+
 /* Node state of body b (6*nND each, node-major), and the last sub-step's
  * Newton iterations, conjugate-gradient iterations and displacement norm. */
 int fsil_fem_state(int handle, int body, double *pos, double *vel, double *acc, double *dsp);
