@@ -61,6 +61,8 @@ SIGNATURES = {
     "fsil_mlp_train": (_i, [_i, _pf, _pf, _i, _i, _i, _d, _i, _pd, _pd]),
     "fsil_ibm_set_warm_start": (_i, [_i, _i, _i, _i]),
     "fsil_ibm_rounds": (_i, [_i, _pi]),
+    "fsil_set_viscosity": (_i, [_i, _d]),
+    "fsil_tau_download": (_i, [_i, _pd]),
     "fsil_solid_info": (_i, [_i, _i, _pd, _pd]),
     "fsil_solid_info_static": (_i, [_i, _i, _pd, _pd]),
     "fsil_ibm_warm_start_data": (_i, [_i, _pf, _pf, _pi]),

@@ -50,6 +50,9 @@ class LBMBlock:
                                          self.precision, _lib.iptr(h)))
         self.handle = int(h[0])
         self.blktime = 0.0
+        self.nu = float(nu)
+        if self.iCollidModel in (11, 14, 15):
+            _lib.check(lib.fsil_set_viscosity(self.handle, self.nu))
         self.set_flow()
 
     @property
@@ -83,6 +86,12 @@ class LBMBlock:
         f = np.ascontiguousarray(fIn, dtype=np.float64)
         assert f.shape == (19,) + self.dims
         _lib.check(self.lib.fsil_f_upload(self.handle, _lib.dptr(f)))
+
+    def tau_all(self) -> np.ndarray:
+        """Per-cell relaxation time tau_all(z,y,x) as (xDim, yDim, zDim)."""
+        out = np.empty(self.dims)
+        _lib.check(self.lib.fsil_tau_download(self.handle, _lib.dptr(out)))
+        return out
 
     def fill(self, f19) -> None:
         """Every cell set to the same 19 populations (a uniform initial state)."""

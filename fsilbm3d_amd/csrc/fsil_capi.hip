@@ -248,6 +248,8 @@ struct Block {
   DevBuf<double> fA, fB;
   DevBuf<float> gA, gB;
   DevBuf<double> Mc, Mf;
+  DevBuf<double> tau_all, uf;  // LES models: per-cell tau; the velocity field of models 14, 15
+  double nu = std::nan("");    // flow%nu (models 14, 15)
   // flow conditions (FlowCondition.f90:11-26)
   double denIn = 1, uvwIn[3] = {0, 0, 0}, shear[3] = {0, 0, 0};
   int vkind = 0;
@@ -316,6 +318,16 @@ Mlp *get_mlp(int h) {
     return nullptr;
   }
   return it->second.get();
+}
+
+bool is_les(int model) { return model == 11 || model == 14 || model == 15; }
+
+// ConstParams.f90:40-45 (Csmag 0.17, CWALE 0.5), in the reference's order
+double les_constant(int model) {
+  const double Csmag = 0.17, CWALE = 0.50;
+  if (model == 11) return 2.0 * Csmag * Csmag * std::sqrt(2.0) * 9.0;
+  if (model == 14) return CWALE * CWALE;
+  return 2.5 * Csmag * Csmag;
 }
 
 int plane_size(const Block &b, int face) {
@@ -519,9 +531,26 @@ int collide_stream(Block &b) {
   const bool aux = (m102 | mhw) != 0;
   const dim3 blk = cell_block(b);
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
+  if (is_les(b.model)) {
+    a.tau_all = b.tau_all.p;
+    a.uf = b.uf.p;
+    a.tau = b.tau;
+    a.nu = b.nu;
+    a.les_c = les_constant(b.model);
+    if (b.model != 11) {
+      if (std::isnan(b.nu)) return fail("collision models 14 and 15 need flow%%nu (fsil_set_viscosity)");
+      // the velocity field the gradients read, before any cell is collided
+      if (band) hipLaunchKernelGGL(k_les_velocity<true>, grid, blk, 0, b.stream, a);
+      else hipLaunchKernelGGL(k_les_velocity<false>, grid, blk, 0, b.stream, a);
+      HIPCHK(hipGetLastError());
+    }
+  }
   if (b.model == 1) launch_collide_model<1>(a, force, band, aux, grid, blk, b.stream);
   else if (b.model == 2) launch_collide_model<2>(a, force, band, aux, grid, blk, b.stream);
-  else launch_collide_model<3>(a, force, band, aux, grid, blk, b.stream);
+  else if (b.model == 3) launch_collide_model<3>(a, force, band, aux, grid, blk, b.stream);
+  else if (b.model == 11) launch_collide_model<11>(a, force, band, aux, grid, blk, b.stream);
+  else if (b.model == 14) launch_collide_model<14>(a, force, band, aux, grid, blk, b.stream);
+  else launch_collide_model<15>(a, force, band, aux, grid, blk, b.stream);
   HIPCHK(hipGetLastError());
   for (int face = 0; face < 6; ++face)
     if ((mhw >> face) & 1) b.hw_valid[face] = true;
@@ -792,8 +821,9 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   if (dims[0] < 4 || dims[1] < 4 || dims[2] < 4) return fail("block dimensions must be >= 4");
   if (dims[0] > 32767 || dims[1] > 32767 || dims[2] > 32767)
     return fail("Grid number exceeds 32767");  // FluidDomain.f90:88-91
-  if (!(iCollidModel == 1 || iCollidModel == 2 || iCollidModel == 3))
-    return fail("collision model %d not provided (1 SRT, 2 TRT, 3 MRT)", iCollidModel);
+  if (!(iCollidModel == 1 || iCollidModel == 2 || iCollidModel == 3 || is_les(iCollidModel)))
+    return fail("collision model %d not provided (1 SRT, 2 TRT, 3 MRT, 11 Smagorinsky, 14 WALE, 15 Vreman)",
+                iCollidModel);
   if (iCollidModel == 3 && (!m_collid || !m_force)) return fail("MRT needs M_COLLID and M_FORCE");
   if (!(precision == 0 || precision == 1)) return fail("precision must be 0 (FP64) or 1 (FP32 deviation)");
   if (precision == 1 && iCollidModel != 1) return fail("FP32-deviation form supports SRT only");
@@ -840,6 +870,15 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   } else {
     HIPCHK(b.gA.ensure(nf));
     HIPCHK(b.gB.ensure(nf));
+  }
+  if (is_les(iCollidModel)) {
+    // tau_all = tau (calculate_SRT_params, FluidDomain.f90:450-456)
+    HIPCHK(b.tau_all.ensure(b.N));
+    hipLaunchKernelGGL(k_fill_value, dim3((unsigned)((b.N + 255) / 256)), dim3(256), 0, b.stream, b.tau_all.p, b.N,
+                       b.tau);
+    HIPCHK(hipGetLastError());
+    if (iCollidModel != 11) HIPCHK(b.uf.ensure(3 * (size_t)b.N));
+    HIPCHK(hipStreamSynchronize(b.stream));
   }
   if (iCollidModel == 3) {
     HIPCHK(b.Mc.ensure(Q * Q));
@@ -910,6 +949,26 @@ int fsil_set_time(int h, double blktime) {
   if (!b) return 1;
   b->blktime = blktime;
   update_vf(*b);
+  return 0;
+}
+
+int fsil_set_viscosity(int h, double nu) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  b->nu = nu;
+  return 0;
+}
+
+int fsil_tau_download(int h, double *tau_all) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!tau_all) return fail("null tau_all");
+  if (!is_les(b->model)) {
+    for (long long n = 0; n < b->N; ++n) tau_all[n] = b->tau;
+    return 0;
+  }
+  HIPCHK(hipMemcpyAsync(tau_all, b->tau_all.p, b->N * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 
@@ -2064,6 +2123,8 @@ int fsil_tree_add_son(int father, int son, int interpolateScheme, int *index) {
     a = it == g_blocks.end() ? 0 : it->second->father;
   }
   if (F->precision != 0 || S->precision != 0) return fail("block trees use the FP64 plain form");
+  if (is_les(F->model) || is_les(S->model))
+    return fail("block trees take collision models 1-3 (the LES models' per-cell tau_all is not exchanged)");
   // check_blocks_params (LBMBlockComm.f90:508-544)
   int r[3], per[3];
   for (int a = 0; a < 3; ++a) {

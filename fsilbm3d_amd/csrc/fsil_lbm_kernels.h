@@ -96,7 +96,53 @@ struct CollideArgs {
   double *nb_u[6];   // [3][plane]
   double *hw[6];     // [q][plane]
   int nt;            // nontemporal population loads and stores (selects the kernel instantiation)
+  // LES collision models 11 (Smagorinsky), 14 (WALE), 15 (Vreman)
+  const double *__restrict__ uf;  // [3][N] velocity the collision reads (models 14, 15: neighbour gradients)
+  double *__restrict__ tau_all;   // [N] per-cell relaxation time (WALE reads the previous step's)
+  double tau, nu;                 // block tau, flow%nu
+  double les_c;                   // CsmagConst, CWALEConst or CvremConst
 };
+
+// velocity of one cell as the collision reads it: the moments of the stored
+// state with the uniform volume force, or the immersed-boundary band's
+// corrected value (FluidDomain.f90:1128-1145, Solidbody.f90:1034-1048)
+template <bool BAND>
+__device__ __forceinline__ void cell_velocity(const CollideArgs &a, const double (&f)[Q], unsigned n, int x, int y,
+                                              int z, double &rho, double &u1, double &u2, double &u3, bool &in_band,
+                                              int &b) {
+  double m1, m2, m3;
+  moments(f, rho, m1, m2, m3);
+  u1 = (m1 + a.hv[0]) / rho;
+  u2 = (m2 + a.hv[1]) / rho;
+  u3 = (m3 + a.hv[2]) / rho;
+  in_band = false;
+  b = -1;
+  if constexpr (BAND) {
+    if (x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] && z <= a.ctl[5]) {
+      b = a.cellmap[n];
+      if (b >= 0) {
+        u1 = a.band_u[b];
+        u2 = a.band_u[a.cap + b];
+        u3 = a.band_u[2 * a.cap + b];
+        in_band = true;
+      }
+    }
+  }
+}
+
+// derivative of velocity component c along axis ax at (x,y,z) from the full
+// field uf: central inside, one-sided second order at the block's first and
+// last cell, times s (center_diff / onesid_diff, FluidDomain.f90:1425-1436)
+__device__ __forceinline__ double les_diff(const CollideArgs &a, int x, int y, int z, int ax, int c, double s) {
+  const double *u = a.uf + (size_t)c * (size_t)a.N;
+  const int i = ax == 0 ? x : (ax == 1 ? y : z);
+  const int nd = ax == 0 ? a.xd : (ax == 1 ? a.yd : a.zd);
+  const long long st = ax == 0 ? (long long)a.yd * a.zd : (ax == 1 ? (long long)a.zd : 1LL);
+  const long long n = ((long long)x * a.yd + y) * a.zd + z;
+  if (i > 0 && i < nd - 1) return (u[n + st] - u[n - st]) * s;
+  const long long d = i == 0 ? st : -st;
+  return (-3.0 * u[n] + 4.0 * u[n + d] - u[n + 2 * d]) * s;
+}
 
 // plane index of a cell on face `face` in the reference's (outer, inner) loop order
 __device__ __forceinline__ int face_plane_index(int face, int x, int y, int z, int yd, int zd) {
@@ -205,6 +251,86 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
       post[p] = f[p] + (tp + tm);
       post[m] = f[m] + (tp - tm);
     }
+  } else if constexpr (MODEL == 11 || MODEL == 14 || MODEL == 15) {
+    // LES: SRT with a per-cell omega (FluidDomain.f90:1239-1258); the
+    // non-equilibrium moments from fneq = -(fEq - f) (smag/WALE, :1271-1277)
+    double fn[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) fn[q] = -fe[q];
+    const double Q11 = fn[1] + fn[2] + fn[7] + fn[8] + fn[9] + fn[10] + fn[11] + fn[12] + fn[13] + fn[14];
+    const double Q22 = fn[3] + fn[4] + fn[7] + fn[8] + fn[9] + fn[10] + fn[15] + fn[16] + fn[17] + fn[18];
+    const double Q33 = fn[5] + fn[6] + fn[11] + fn[12] + fn[13] + fn[14] + fn[15] + fn[16] + fn[17] + fn[18];
+    const double Q12 = fn[7] - fn[8] - fn[9] + fn[10];
+    const double Q13 = fn[11] - fn[12] - fn[13] + fn[14];
+    const double Q23 = fn[15] - fn[16] - fn[17] + fn[18];
+    double omega;
+    if constexpr (MODEL == 11) {
+      // smag (:1265-1280)
+      const double Qs = Q11 * Q11 + Q22 * Q22 + Q33 * Q33 + 2.0 * (Q12 * Q12 + Q13 * Q13 + Q23 * Q23);
+      const double tau_t = sqrt(a.tau * a.tau + a.les_c * sqrt(Qs) / rho);
+      a.tau_all[n] = 0.5 * (a.tau + tau_t);
+      omega = 2.0 / (a.tau + tau_t);
+    } else if constexpr (MODEL == 14) {
+      // WALE (:1308-1424); the reference's `invdh` holds dh
+      const double s = a.dh, tp = a.tau_all[n];
+      const double S11 = -1.5 * s * Q11 / (rho * tp), S22 = -1.5 * s * Q22 / (rho * tp);
+      const double S33 = -1.5 * s * Q33 / (rho * tp), S12 = -1.5 * s * Q12 / (rho * tp);
+      const double S13 = -1.5 * s * Q13 / (rho * tp), S23 = -1.5 * s * Q23 / (rho * tp);
+      const double S = S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23);
+      double ox = les_diff(a, x, y, z, 1, 2, s);
+      ox = 0.5 * (ox - les_diff(a, x, y, z, 2, 1, s));
+      double oy = les_diff(a, x, y, z, 2, 0, s);
+      oy = 0.5 * (oy - les_diff(a, x, y, z, 0, 2, s));
+      double oz = les_diff(a, x, y, z, 0, 1, s);
+      oz = 0.5 * (oz - les_diff(a, x, y, z, 1, 0, s));
+      const double O12 = -0.5 * oz, O13 = 0.5 * oy, O23 = -0.5 * ox;
+      const double O = 2.0 * (O12 * O12 + O23 * O23 + O13 * O13);
+      const double SO11 = -(0.0 + S11 * S11 * O12 * O12 + S11 * S11 * O13 * O13 + 0.0 + S12 * S12 * O12 * O12 +
+                            S12 * S12 * O13 * O13 + 0.0 + S13 * S13 * O12 * O12 + S13 * S13 * O13 * O13);
+      const double SO22 = -(S12 * S12 * O12 * O12 + 0.0 + S12 * S12 * O23 * O23 + S22 * S22 * O12 * O12 + 0.0 +
+                            S22 * S22 * O23 * O23 + S23 * S23 * O12 * O12 + 0.0 + S23 * S23 * O23 * O23);
+      const double SO33 = -(S13 * S13 * O13 * O13 + S13 * S13 * O23 * O23 + 0.0 + S23 * S23 * O13 * O13 +
+                            S23 * S23 * O23 * O23 + 0.0 + S33 * S33 * O13 * O13 + S33 * S33 * O23 * O23 + 0.0);
+      const double SO12 = -(0.0 + 0.0 + S11 * S12 * O13 * O23 + 0.0 + 0.0 + S12 * S22 * O13 * O23 + 0.0 + 0.0 +
+                            S13 * S23 * O13 * O23);
+      const double SO13 = (0.0 + S11 * S13 * O12 * O23 + 0.0 + 0.0 + S12 * S23 * O12 * O23 + 0.0 + 0.0 +
+                           S13 * S33 * O12 * O23 + 0.0);
+      const double SO23 = -(S12 * S13 * O12 * O13 + 0.0 + 0.0 + S22 * S23 * O12 * O13 + 0.0 + 0.0 +
+                            S23 * S33 * O12 * O13 + 0.0 + 0.0);
+      const double SO = SO11 + SO22 + SO33 + 2.0 * (SO12 + SO13 + SO23);
+      const double SdSd = (S * S + O * O) / 6.0 + 2.0 * S * O / 3.0 + 2.0 * SO;
+      double OP = pow(SdSd, 1.5) / (pow(S, 2.5) + pow(SdSd, 1.25));
+      if (!isfinite(OP) || OP < 0.0) OP = 0.0;
+      const double tn = (a.nu + a.les_c * OP * a.dh * a.dh) / (a.dh * (1.0 / 3.0)) + 0.5;
+      a.tau_all[n] = tn;
+      omega = 1.0 / tn;
+    } else {
+      // vrem (:1437-1507)
+      const double s = a.dh;
+      const double a11 = 0.5 * les_diff(a, x, y, z, 0, 0, s), a21 = 0.5 * les_diff(a, x, y, z, 0, 1, s);
+      const double a31 = 0.5 * les_diff(a, x, y, z, 0, 2, s), a12 = 0.5 * les_diff(a, x, y, z, 1, 0, s);
+      const double a22 = 0.5 * les_diff(a, x, y, z, 1, 1, s), a32 = 0.5 * les_diff(a, x, y, z, 1, 2, s);
+      const double a13 = 0.5 * les_diff(a, x, y, z, 2, 0, s), a23 = 0.5 * les_diff(a, x, y, z, 2, 1, s);
+      const double a33 = 0.5 * les_diff(a, x, y, z, 2, 2, s);
+      const double b11 = a11 * a11, b12 = a12 * a12, b13 = a13 * a13, b21 = a21 * a21, b22 = a22 * a22;
+      const double b23 = a23 * a23, b31 = a31 * a31, b32 = a32 * a32, b33 = a33 * a33;
+      const double aa = b11 + b12 + b13 + b21 + b22 + b23 + b31 + b32 + b33;
+      const double c12 = a11 * a21 + a12 * a22 + a13 * a23, c13 = a11 * a31 + a12 * a32 + a13 * a33;
+      const double c23 = a21 * a31 + a22 * a32 + a23 * a33;
+      const double bb = (b11 + b12 + b13) * (b21 + b22 + b23) - c12 * c12 + (b11 + b12 + b13) * (b31 + b32 + b33) -
+                        c13 * c13 + (b21 + b22 + b23) * (b31 + b32 + b33) - c23 * c23;
+      double OP = sqrt(bb / aa);
+      if (!isfinite(OP)) OP = 0.0;
+      const double tn = (a.nu + a.les_c * OP * a.dh * a.dh) / (a.dh * (1.0 / 3.0)) + 0.5;
+      a.tau_all[n] = tn;
+      omega = 1.0 / tn;
+    }
+    const double cf = 1.0 - 0.5 * omega;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const double t = f[q] + omega * fe[q];
+      post[q] = forced ? t + cf * fl[q] : t;
+    }
   } else {
     // MRT  (FluidDomain.f90:1238): f + MATMUL(M_COLLID,fEq) + MATMUL(M_FORCE,Flb)
 #pragma unroll
@@ -256,6 +382,35 @@ __global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
   const int x = blockIdx.z;
   if (z >= a.zd || y >= a.yd) return;
   collide_push_cell<MODEL, FORCE, BAND, AUX, NT>(a, x, y, z);
+}
+
+// Velocity field the LES models 14 and 15 differentiate: every cell's u as
+// the collision reads it (uuu after calculate_macro_quantities_ and the
+// immersed-boundary correction), into a.uf [3][N] before the fused kernel runs.
+template <bool BAND>
+__global__ __launch_bounds__(256) void k_les_velocity(CollideArgs a) {
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int x = blockIdx.z;
+  if (z >= a.zd || y >= a.yd) return;
+  const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
+  double f[Q];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) f[q] = a.fin[(size_t)q * a.N + n];
+  double rho, u1, u2, u3;
+  bool in_band;
+  int b;
+  cell_velocity<BAND>(a, f, n, x, y, z, rho, u1, u2, u3, in_band, b);
+  double *uf = const_cast<double *>(a.uf);
+  uf[n] = u1;
+  uf[(size_t)a.N + n] = u2;
+  uf[2 * (size_t)a.N + n] = u3;
+}
+
+// every cell set to v
+__global__ __launch_bounds__(256) void k_fill_value(double *__restrict__ p, long long n, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // ---------------------------------------------------------------------------

@@ -20,7 +20,7 @@ module fsil_bridge
               fsil_extract_interpolate_layer, fsil_interpolation_father_to_son, fsil_deliver_son_to_father, &
               fsil_tree_steps, fsil_time_tree_steps, fsil_mlp_create, fsil_mlp_destroy, fsil_mlp_get_params, &
               fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train, fsil_ibm_set_warm_start, &
-              fsil_ibm_rounds, fsil_solid_info, fsil_solid_info_static, &
+              fsil_ibm_rounds, fsil_solid_info, fsil_solid_info_static, fsil_set_viscosity, fsil_tau_download, &
               fsil_ibm_warm_start_data, fsil_ibm_forces, &
               fsil_ibm_report
 
@@ -290,6 +290,16 @@ module fsil_bridge
             integer(c_int), value :: handle, body
             real(c_double), intent(in) :: refs(6)
             real(c_double), intent(out) :: rows(78)
+        end function
+        integer(c_int) function fsil_set_viscosity(handle, nu) bind(C, name='fsil_set_viscosity')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), value :: nu
+        end function
+        integer(c_int) function fsil_tau_download(handle, tau_all) bind(C, name='fsil_tau_download')
+            import :: c_int, c_double
+            integer(c_int), value :: handle
+            real(c_double), intent(out) :: tau_all(*)
         end function
         integer(c_int) function fsil_ibm_rounds(handle, rounds) bind(C, name='fsil_ibm_rounds')
             import :: c_int

@@ -42,10 +42,12 @@ int fsil_synchronize(void);
  *   dims[3]   xDim, yDim, zDim                     (FluidDomain.f90:73)
  *   geom[7]   dh, xmin, ymin, zmin, xmax, ymax, zmax (FluidDomain.f90:75,93-104)
  *   bc[6]     BndConds x-,x+,y-,y+,z-,z+: 0,1,101-104,201-204,301,302
- *   iCollidModel 1 SRT, 2 TRT, 3 MRT              (FluidDomain.f90:1225-1238)
+ *   iCollidModel 1 SRT, 2 TRT, 3 MRT              (FluidDomain.f90:1225-1238),
+ *             11 Smagorinsky, 14 WALE, 15 Vreman LES (FluidDomain.f90:1239-1258, 1265-1507;
+ *             FP64 form, single blocks; 14 and 15 need fsil_set_viscosity)
  *   relax[3]  tau, Omega, Omega2                    (FluidDomain.f90:449-464)
  *   m_collid, m_force  19x19 Fortran-order M_COLLID, M_FORCE for MRT, else NULL
- *   precision 0: FP64 plain; 1: FP32 storage of f - w (periodic SRT blocks)
+ *   precision 0: FP64 plain; 1: FP32 storage of f - w (SRT; every boundary code but 102/203/204)
  *   handle    out: block handle */
 int fsil_block_create(int id, const int *dims, const double *geom, const int *bc, int iCollidModel,
                       const double *relax, const double *m_collid, const double *m_force,
@@ -64,6 +66,14 @@ int fsil_set_flow(int handle, double denIn, const double *uvwIn, const double *s
 /* Set LBMblks%blktime and evaluate update_volume_force_ at it
  * (main.f90:97, FluidDomain.f90:1174-1180). */
 int fsil_set_time(int handle, double blktime);
+
+/* flow%nu (Solidbody.f90:282), the molecular viscosity the WALE and Vreman
+ * models add their eddy viscosity to (FluidDomain.f90:1421, 1504). */
+int fsil_set_viscosity(int handle, double nu);
+
+/* The per-cell relaxation time tau_all(z,y,x) (FluidDomain.f90:35, 454-455):
+ * written by the LES models each step; tau everywhere for models 1-3. */
+int fsil_tau_download(int handle, double *tau_all);
 
 /* The volume force of the current block time (vf[3]). */
 int fsil_get_volume_force(int handle, double *vf);

@@ -468,3 +468,140 @@ void orc_spread(int nmark, const int *Ei, const float *Ew, const double *Eforce,
         }
   }
 }
+
+/* ---------------- LES collision models ---------------- */
+/* collision_ with iCollidModel 11 (Smagorinsky), 14 (WALE) and 15 (Vreman):
+ * FluidDomain.f90:1239-1258 (dispatch), 1265-1280 (smag), 1308-1424 (WALE),
+ * 1425-1436 (center/one-sided differences), 1437-1507 (vrem); constants
+ * ConstParams.f90:39-45.  u is the full velocity field the collision reads
+ * (macro quantities with the immersed-boundary correction), tau_all the
+ * per-cell relaxation time (read by WALE from the previous step, written by
+ * all three). */
+static double cdiff(double g1, double g2, double s) { return (g1 - g2) * s; }
+static double odiff(double g1, double g2, double g3, double s) { return (-3.0 * g1 + 4.0 * g2 - g3) * s; }
+
+/* derivative of velocity component c along axis a at cell (x,y,z), the
+ * reference's interior/edge choice (no wrap) */
+static double dudx(const double *u, const dims_t *d, int x, int y, int z, int a, int c, double s) {
+  const size_t N = (size_t)d->xd * d->yd * d->zd;
+  const double *uc = u + (size_t)c * N;
+  int p[3] = {x, y, z};
+  const int nd = a == 0 ? d->xd : (a == 1 ? d->yd : d->zd);
+  const int i = p[a];
+#define AT(k) (p[a] = (k), uc[IX(d, p[0], p[1], p[2])])
+  double r;
+  if (i > 0 && i < nd - 1) {
+    const double g1 = AT(i + 1), g2 = AT(i - 1);
+    r = cdiff(g1, g2, s);
+  } else if (i == 0) {
+    const double g1 = AT(i), g2 = AT(i + 1), g3 = AT(i + 2);
+    r = odiff(g1, g2, g3, s);
+  } else {
+    const double g1 = AT(i), g2 = AT(i - 1), g3 = AT(i - 2);
+    r = odiff(g1, g2, g3, s);
+  }
+#undef AT
+  return r;
+}
+
+void orc_collision_les(double *f, const double *den, const double *u, const double *force, int xd, int yd, int zd,
+                       int model, double tau, double nu, double dh, double *tau_all) {
+  init_wt();
+  const dims_t d = {xd, yd, zd};
+  const size_t N = (size_t)xd * yd * zd;
+  const double dt3 = 3.0 * dh;
+  const double Cs2 = 1.0 / 3.0, Csmag = 0.17, CWALE = 0.50;
+  const double CsmagConst = 2.0 * Csmag * Csmag * sqrt(2.0) * 9.0;
+  const double CWALEConst = CWALE * CWALE;
+  const double CvremConst = 2.5 * Csmag * Csmag;
+  for (int x = 0; x < xd; ++x)
+    for (int y = 0; y < yd; ++y)
+      for (int z = 0; z < zd; ++z) {
+        const size_t n = IX(&d, x, y, z);
+        const double u1 = u[n], u2 = u[N + n], u3 = u[2 * N + n];
+        const double F1 = force[n], F2 = force[N + n], F3 = force[2 * N + n];
+        const double rho = den[n];
+        double uSqr = 0.0;
+        uSqr = uSqr + u1 * u1;
+        uSqr = uSqr + u2 * u2;
+        uSqr = uSqr + u3 * u3;
+        double uxyz[Q], fEq[Q], Flb[Q], fin[Q], fn[Q];
+        for (int q = 0; q < Q; ++q) {
+          const double e1 = EE[q][0], e2 = EE[q][1], e3 = EE[q][2];
+          fin[q] = f[q * N + n];
+          uxyz[q] = u1 * e1 + u2 * e2 + u3 * e3;
+          fEq[q] = WT[q] * rho * ((1.0 - 1.5 * uSqr) + uxyz[q] * (3.0 + 4.5 * uxyz[q])) - fin[q];
+          Flb[q] = dt3 * WT[q] *
+                   ((e1 - u1 + 3.0 * uxyz[q] * e1) * F1 + (e2 - u2 + 3.0 * uxyz[q] * e2) * F2 +
+                    (e3 - u3 + 3.0 * uxyz[q] * e3) * F3);
+          fn[q] = -fEq[q];
+        }
+        const double Q11 = fn[1] + fn[2] + fn[7] + fn[8] + fn[9] + fn[10] + fn[11] + fn[12] + fn[13] + fn[14];
+        const double Q22 = fn[3] + fn[4] + fn[7] + fn[8] + fn[9] + fn[10] + fn[15] + fn[16] + fn[17] + fn[18];
+        const double Q33 = fn[5] + fn[6] + fn[11] + fn[12] + fn[13] + fn[14] + fn[15] + fn[16] + fn[17] + fn[18];
+        const double Q12 = fn[7] - fn[8] - fn[9] + fn[10];
+        const double Q13 = fn[11] - fn[12] - fn[13] + fn[14];
+        const double Q23 = fn[15] - fn[16] - fn[17] + fn[18];
+        double omega;
+        if (model == 11) {
+          const double Qs = Q11 * Q11 + Q22 * Q22 + Q33 * Q33 + 2.0 * (Q12 * Q12 + Q13 * Q13 + Q23 * Q23);
+          const double tau_t = sqrt(tau * tau + CsmagConst * sqrt(Qs) / rho);
+          tau_all[n] = 0.5 * (tau + tau_t);
+          omega = 2.0 / (tau + tau_t);
+        } else if (model == 14) {
+          const double s = dh; /* the reference's `invdh` holds dh */
+          const double tp = tau_all[n];
+          const double S11 = -1.5 * s * Q11 / (rho * tp), S22 = -1.5 * s * Q22 / (rho * tp);
+          const double S33 = -1.5 * s * Q33 / (rho * tp), S12 = -1.5 * s * Q12 / (rho * tp);
+          const double S13 = -1.5 * s * Q13 / (rho * tp), S23 = -1.5 * s * Q23 / (rho * tp);
+          const double S = S11 * S11 + S22 * S22 + S33 * S33 + 2.0 * (S12 * S12 + S13 * S13 + S23 * S23);
+          double ox = dudx(u, &d, x, y, z, 1, 2, s);
+          ox = 0.5 * (ox - dudx(u, &d, x, y, z, 2, 1, s));
+          double oy = dudx(u, &d, x, y, z, 2, 0, s);
+          oy = 0.5 * (oy - dudx(u, &d, x, y, z, 0, 2, s));
+          double oz = dudx(u, &d, x, y, z, 0, 1, s);
+          oz = 0.5 * (oz - dudx(u, &d, x, y, z, 1, 0, s));
+          const double O12 = -0.5 * oz, O13 = 0.5 * oy, O23 = -0.5 * ox;
+          const double O = 2.0 * (O12 * O12 + O23 * O23 + O13 * O13);
+          const double SO11 = -(0.0 + S11 * S11 * O12 * O12 + S11 * S11 * O13 * O13 + 0.0 + S12 * S12 * O12 * O12 +
+                                S12 * S12 * O13 * O13 + 0.0 + S13 * S13 * O12 * O12 + S13 * S13 * O13 * O13);
+          const double SO22 = -(S12 * S12 * O12 * O12 + 0.0 + S12 * S12 * O23 * O23 + S22 * S22 * O12 * O12 + 0.0 +
+                                S22 * S22 * O23 * O23 + S23 * S23 * O12 * O12 + 0.0 + S23 * S23 * O23 * O23);
+          const double SO33 = -(S13 * S13 * O13 * O13 + S13 * S13 * O23 * O23 + 0.0 + S23 * S23 * O13 * O13 +
+                                S23 * S23 * O23 * O23 + 0.0 + S33 * S33 * O13 * O13 + S33 * S33 * O23 * O23 + 0.0);
+          const double SO12 = -(0.0 + 0.0 + S11 * S12 * O13 * O23 + 0.0 + 0.0 + S12 * S22 * O13 * O23 + 0.0 + 0.0 +
+                                S13 * S23 * O13 * O23);
+          const double SO13 = (0.0 + S11 * S13 * O12 * O23 + 0.0 + 0.0 + S12 * S23 * O12 * O23 + 0.0 + 0.0 +
+                               S13 * S33 * O12 * O23 + 0.0);
+          const double SO23 = -(S12 * S13 * O12 * O13 + 0.0 + 0.0 + S22 * S23 * O12 * O13 + 0.0 + 0.0 +
+                                S23 * S33 * O12 * O13 + 0.0 + 0.0);
+          const double SO = SO11 + SO22 + SO33 + 2.0 * (SO12 + SO13 + SO23);
+          const double SdSd = (S * S + O * O) / 6.0 + 2.0 * S * O / 3.0 + 2.0 * SO;
+          double OP = pow(SdSd, 1.5) / (pow(S, 2.5) + pow(SdSd, 1.25));
+          if (!isfinite(OP) || OP < 0.0) OP = 0.0;
+          const double tn = (nu + CWALEConst * OP * dh * dh) / (dh * Cs2) + 0.5;
+          tau_all[n] = tn;
+          omega = 1.0 / tn;
+        } else {
+          const double s = dh;
+          const double a11 = 0.5 * dudx(u, &d, x, y, z, 0, 0, s), a21 = 0.5 * dudx(u, &d, x, y, z, 0, 1, s);
+          const double a31 = 0.5 * dudx(u, &d, x, y, z, 0, 2, s), a12 = 0.5 * dudx(u, &d, x, y, z, 1, 0, s);
+          const double a22 = 0.5 * dudx(u, &d, x, y, z, 1, 1, s), a32 = 0.5 * dudx(u, &d, x, y, z, 1, 2, s);
+          const double a13 = 0.5 * dudx(u, &d, x, y, z, 2, 0, s), a23 = 0.5 * dudx(u, &d, x, y, z, 2, 1, s);
+          const double a33 = 0.5 * dudx(u, &d, x, y, z, 2, 2, s);
+          const double b11 = a11 * a11, b12 = a12 * a12, b13 = a13 * a13, b21 = a21 * a21, b22 = a22 * a22;
+          const double b23 = a23 * a23, b31 = a31 * a31, b32 = a32 * a32, b33 = a33 * a33;
+          const double aa = b11 + b12 + b13 + b21 + b22 + b23 + b31 + b32 + b33;
+          const double c12 = a11 * a21 + a12 * a22 + a13 * a23, c13 = a11 * a31 + a12 * a32 + a13 * a33;
+          const double c23 = a21 * a31 + a22 * a32 + a23 * a33;
+          const double bb = (b11 + b12 + b13) * (b21 + b22 + b23) - c12 * c12 + (b11 + b12 + b13) * (b31 + b32 + b33) -
+                            c13 * c13 + (b21 + b22 + b23) * (b31 + b32 + b33) - c23 * c23;
+          double OP = sqrt(bb / aa);
+          if (!isfinite(OP)) OP = 0.0;
+          const double tn = (nu + CvremConst * OP * dh * dh) / (dh * Cs2) + 0.5;
+          tau_all[n] = tn;
+          omega = 1.0 / tn;
+        }
+        for (int q = 0; q < Q; ++q) f[q * N + n] = fin[q] + omega * fEq[q] + (1.0 - 0.5 * omega) * Flb[q];
+      }
+}

@@ -42,6 +42,8 @@ def lib():
         L.orc_tables.argtypes = [_pi, _pi, _pd]
         L.orc_macro.argtypes = [_pd, C.c_int, C.c_int, C.c_int, _pd, C.c_double, _pd, _pd]
         L.orc_collision.argtypes = [_pd, _pd, _pd, _pd, C.c_size_t, C.c_int, C.c_double, C.c_double, _pd, _pd, C.c_double]
+        L.orc_collision_les.argtypes = [_pd, _pd, _pd, _pd, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, _pd]
         L.orc_streaming.argtypes = [_pd, _pd, C.c_int, C.c_int, C.c_int]
         L.orc_bc.argtypes = [_pd, _pd, _pd, C.c_int, C.c_int, C.c_int, _pd, _pi, _pd, C.c_int, C.c_double, C.POINTER(_pd)]
         L.orc_halfway_save.argtypes = [_pd, C.c_int, C.c_int, C.c_int, _pi, C.POINTER(_pd)]
@@ -106,7 +108,8 @@ class OracleBlock:
 
     def __init__(self, dims, dh, origin, bc, model, Omega, Omega2=0.0, Mc=None, Mf=None, *,
                  denIn=1.0, uvwIn=(0, 0, 0), shearRateIn=(0, 0, 0), velocityKind=0,
-                 volumeForceIn=(0, 0, 0), volumeForceAmp=0.0, volumeForceFreq=0.0, volumeForcePhi=0.0):
+                 volumeForceIn=(0, 0, 0), volumeForceAmp=0.0, volumeForceFreq=0.0, volumeForcePhi=0.0,
+                 tau=None, nu=None):
         self.dims = tuple(int(d) for d in dims)
         self.dh = float(dh)
         self.origin = tuple(float(v) for v in origin)
@@ -129,6 +132,14 @@ class OracleBlock:
         self.VFamp, self.VFfreq, self.VFphi = float(volumeForceAmp), float(volumeForceFreq), float(volumeForcePhi)
         xd, yd, zd = self.dims
         self.N = xd * yd * zd
+        # LES models 11/14/15: per-cell relaxation time tau_all, initialised to
+        # tau (calculate_SRT_params, FluidDomain.f90:450-456), and flow%nu
+        self.tau, self.nu = tau, nu
+        self.tau_all = None
+        if self.model in (11, 14, 15):
+            if tau is None or nu is None:
+                raise ValueError("the LES collision models need tau and nu")
+            self.tau_all = np.full(self.N, float(tau))
         # halfway planes, allocated on the first boundary call (FluidDomain.f90:651-654)
         self._hw_alloc = [False] * 6
         self._hw = [None] * 6
@@ -227,8 +238,12 @@ class OracleBlock:
         force[0] += VF[0]
         force[1] += VF[1]
         force[2] += VF[2]
-        lib().orc_collision(_d(f), _d(den), _d(u), _d(force), self.N, self.model, self.Omega, self.Omega2,
-                            _d(self.Mc), _d(self.Mf), self.dh)
+        if self.tau_all is not None:
+            lib().orc_collision_les(_d(f), _d(den), _d(u), _d(force), *self.dims, self.model, float(self.tau),
+                                    float(self.nu), self.dh, _d(self.tau_all))
+        else:
+            lib().orc_collision(_d(f), _d(den), _d(u), _d(force), self.N, self.model, self.Omega, self.Omega2,
+                                _d(self.Mc), _d(self.Mf), self.dh)
         if any(self._hw_alloc):
             lib().orc_halfway_save(_d(f), *self.dims, _i(self.bc), self._hw_ptrs())
         tmp = np.empty(self.N)

@@ -203,6 +203,17 @@ CASES: List[ParityCase] = [
                nsteps=5, seed=35, ubase=(0.05, 0, 0), neq=0.002,
                plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.0, 0.0), Freq=0.01,
                              iBodyModel=2, denR=1.0, KB=0.1, KS=1000.0)]),
+    ParityCase("les_smagorinsky_periodic", _periodic(collide=11, volumeForceIn=(1e-5, -2e-6, 3e-6)), nsteps=3, seed=51,
+               neq=0.05),
+    ParityCase("les_wale_open", _open(bc=(101, 103, 302, 201, 301, 301), collide=14, volumeForceIn=(2e-6, 0.0, 1e-6)),
+               nsteps=3, seed=52, ubase=(0.05, 0, 0), neq=0.05, du=0.03),
+    ParityCase("les_vreman_walls", _open(bc=(101, 104, 201, 202, 302, 203), collide=15, shearRateIn=(1e-3, 2e-3, -1e-3)),
+               nsteps=3, seed=53, ubase=(0.05, 0, 0), neq=0.05, du=0.03),
+    ParityCase("les_wale_heaving_plate", _open(dims=(20, 12, 12), bc=(101, 103, 302, 302, 302, 302), collide=14,
+                                               ntolLBM=3, dtolLBM=1e-14),
+               nsteps=3, seed=54, ubase=(0.05, 0, 0), neq=0.02,
+               plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.5, 0.0), XYZPhi=(0.0, 30.0, 0.0),
+                             Freq=0.02)]),
     ParityCase("twoblock_linear", _open(dims=(16, 12, 12), bc=(101, 103, 302, 302, 302, 302),
                                         sons=[SubBlock(dims=(13, 11, 11), dh=0.5, origin=(4.0, 3.0, 3.0))]),
                nsteps=3, seed=41, ubase=(0.05, 0, 0), neq=0.01),

@@ -92,7 +92,7 @@ def run_oracle(pc, f0=None, nsteps=None, record=None):
     case = pc.case
     d = derived(case)
     ob = OracleBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["Omega"], d["Omega2"], d["Mc"], d["Mf"],
-                     **flow_kwargs(case))
+                     tau=d["tau"], nu=d["nu"], **flow_kwargs(case))
     f = np.array(pc.initial_state() if f0 is None else f0, dtype=np.float64, copy=True)
     ob.init_boundary(f, 0.0)
     bodies = oracle_bodies(pc, d)
@@ -127,6 +127,43 @@ def run_oracle(pc, f0=None, nsteps=None, record=None):
                 refs = info_scales(case, d, [b.Asfac for b in bodies])
                 record[-1]["solid_info"] = [b.solid_info(p.XYZo, refs) for b, p in zip(bodies, pc.plates)]
     return f
+
+
+def run_oracle_block(pc):
+    """A body-free ParityCase on the oracle; returns (state, tau_all as (x, y, z))."""
+    from oracle.oracle import OracleBlock
+    case = pc.case
+    d = derived(case)
+    ob = OracleBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["Omega"], d["Omega2"], d["Mc"], d["Mf"],
+                     tau=d["tau"], nu=d["nu"], **flow_kwargs(case))
+    f = np.array(pc.initial_state(), dtype=np.float64, copy=True)
+    ob.init_boundary(f, 0.0)
+    time = 0.0
+    for _ in range(pc.nsteps):
+        time = time + case.dh
+        ob.step(f, time)
+    tau = ob.tau_all.reshape(case.dims) if ob.tau_all is not None else np.full(case.dims, d["tau"])
+    return f, tau
+
+
+def run_device_block(pc):
+    """A body-free ParityCase through the C-ABI; returns (state, tau_all)."""
+    from fsilbm3d_amd.block import LBMBlock
+    case = pc.case
+    d = derived(case)
+    blk = LBMBlock(case.dims, case.dh, case.origin, case.bc, case.collide, d["nu"], case.params)
+    blk.set_flow(**flow_kwargs(case))
+    blk.upload(np.array(pc.initial_state(), dtype=np.float64))
+    blk.update_volume_force(0.0)
+    blk.set_boundary_conditions()
+    time = 0.0
+    for _ in range(pc.nsteps):
+        time = time + case.dh
+        blk.update_volume_force(time)
+        blk.collision_streaming_bc()
+    out = blk.download(), blk.tau_all()
+    blk.close()
+    return out
 
 
 def son_derived(case, sb):

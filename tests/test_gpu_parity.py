@@ -22,12 +22,19 @@ RIGID = [c for c in CASES if not c.case.sons and not any(p.iBodyModel == 2 for p
 
 @pytest.mark.parametrize("pc", RIGID, ids=[c.name for c in RIGID])
 def test_device_matches_reference_fixture(pc):
-    """Every fluid / rigid-body fixture bit for bit (elastic bodies: test_gpu_fem.py)."""
+    """Every fluid / rigid-body fixture bit for bit (elastic bodies:
+    test_gpu_fem.py).  The WALE model (14) raises to the powers 1.5, 2.5 and
+    1.25 (FluidDomain.f90:1419) with the device's pow, which may differ from
+    the host C library's in the last place: within 1e-12 relative there."""
     _say(f"[gpu] fixture case {pc.name} dims={pc.case.dims} steps={pc.nsteps}")
     fx = load_fixture(pc.name)
     f = run_device(pc)
     n, m = mismatch(f, fx["f_final"])
-    assert n == 0, f"{pc.name}: {n} values differ from the reference, max |diff| {m:.3e}"
+    if pc.case.collide == 14:
+        _say(f"[gpu] {pc.name}: {n} values differ, max {m:.3e}")
+        assert m <= 1e-12 * np.abs(fx["f_final"]).max()
+    else:
+        assert n == 0, f"{pc.name}: {n} values differ from the reference, max |diff| {m:.3e}"
 
 
 @pytest.mark.parametrize("name", ["ibm_static_plate", "ibm_heaving_plate", "ibm_pitching_two_plates"])
@@ -82,6 +89,30 @@ def test_random_periodic_models_match_oracle(model):
     _say(f"[gpu] random periodic model {model}")
     n, m = mismatch(run_oracle(pc), run_device(pc))
     assert n == 0, f"{n} differ, max {m:.3e}"
+
+
+@pytest.mark.parametrize("model", [11, 14, 15])
+def test_les_models_match_oracle(model):
+    """The LES models on a larger block with open faces and a volume force,
+    state and per-cell tau_all against the oracle (WALE within 1e-12, device
+    pow)."""
+    from oracle.refcase import Case
+    from tests.helpers import run_device_block, run_oracle_block
+    case = Case(dims=(40, 24, 36), bc=(101, 104, 201, 302, 301, 301), collide=model, Re=8.0, uvwIn=(0.05, 0.0, 0.0),
+                shearRateIn=(2e-3, -1e-3, 5e-4), UrefType=9, Uref=0.05, LrefType=1, Lref=1.0, TrefType=0,
+                volumeForceIn=(1e-5, 0.0, -2e-6))
+    pc = ParityCase(f"les_{model}", case, nsteps=4, seed=300 + model, ubase=(0.05, 0, 0), neq=0.05, du=0.04)
+    _say(f"[gpu] LES model {model} 40x24x36 vs oracle")
+    fo, to = run_oracle_block(pc)
+    fd, td = run_device_block(pc)
+    n, m = mismatch(fo, fd)
+    nt, mt = mismatch(to, td)
+    _say(f"[gpu] LES {model}: state {n} differ (max {m:.3e}), tau_all {nt} differ (max {mt:.3e})")
+    if model == 14:
+        assert m <= 1e-12 * np.abs(fo).max() and mt <= 1e-12 * np.abs(to).max()
+    else:
+        assert n == 0 and nt == 0
+    assert np.ptp(to) > 0.0
 
 
 def test_odd_sizes_and_all_open_faces():
