@@ -62,6 +62,12 @@ SIGNATURES = {
     "fsil_ibm_set_warm_start": (_i, [_i, _i, _i, _i]),
     "fsil_ibm_rounds": (_i, [_i, _pi]),
     "fsil_set_viscosity": (_i, [_i, _d]),
+    "fsil_flow_pack": (_i, [_i, _i, _d, _d, _pf]),
+    "fsil_write_flow": (_i, [_i, _i, _d, _d, C.c_char_p]),
+    "fsil_write_wait": (_i, []),
+    "fsil_write_continue": (_i, [_i, _pi, _i, _d, C.c_char_p]),
+    "fsil_fem_checkpoint": (_i, [_i, C.c_void_p, C.POINTER(C.c_longlong)]),
+    "fsil_fem_restore": (_i, [_i, C.c_void_p, C.c_longlong]),
     "fsil_tau_download": (_i, [_i, _pd]),
     "fsil_solid_info": (_i, [_i, _i, _pd, _pd]),
     "fsil_solid_info_static": (_i, [_i, _i, _pd, _pd]),

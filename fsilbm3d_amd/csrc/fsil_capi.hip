@@ -13,7 +13,9 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fsilbm3d_amd.h"
@@ -249,6 +251,7 @@ struct Block {
   DevBuf<float> gA, gB;
   DevBuf<double> Mc, Mf;
   DevBuf<double> tau_all, uf;  // LES models: per-cell tau; the velocity field of models 14, 15
+  DevBuf<float> outbuf;        // DatFlow fields of the last flow write (device staging)
   double nu = std::nan("");    // flow%nu (models 14, 15)
   // flow conditions (FlowCondition.f90:11-26)
   double denIn = 1, uvwIn[3] = {0, 0, 0}, shear[3] = {0, 0, 0};
@@ -328,6 +331,43 @@ double les_constant(int model) {
   if (model == 11) return 2.0 * Csmag * Csmag * std::sqrt(2.0) * 9.0;
   if (model == 14) return CWALE * CWALE;
   return 2.5 * Csmag * Csmag;
+}
+
+// Asynchronous flow-file writers (replace the reference's fork per write,
+// FluidDomain.f90:1702-1736): each write copies its packed fields into pinned
+// host memory on the block's stream and a host thread writes the file once the
+// copy's event has completed; fsil_write_wait joins them (the reference's
+// mywait loop, FluidDomain.f90:355-357).
+struct FlowWrite {
+  std::thread th;
+  explicit FlowWrite(std::thread &&t) : th(std::move(t)) {}
+  FlowWrite(FlowWrite &&) = default;
+  FlowWrite &operator=(FlowWrite &&) = default;
+  ~FlowWrite() {
+    if (th.joinable()) th.join();
+  }
+};
+std::mutex g_write_mu;
+std::vector<FlowWrite> g_writes;
+std::string g_write_err;
+
+void flow_write_job(hipEvent_t ev, char *host, size_t bytes, std::string path) {
+  std::string err;
+  if (hipEventSynchronize(ev) != hipSuccess) err = "flow write: device copy failed";
+  if (err.empty()) {
+    FILE *fp = fopen(path.c_str(), "wb");
+    if (!fp) err = "flow write: cannot open " + path;
+    else {
+      if (fwrite(host, 1, bytes, fp) != bytes) err = "flow write: short write to " + path;
+      if (fclose(fp) != 0 && err.empty()) err = "flow write: cannot close " + path;
+    }
+  }
+  (void)hipEventDestroy(ev);
+  (void)hipHostFree(host);
+  if (!err.empty()) {
+    std::lock_guard<std::mutex> lk(g_write_mu);
+    if (g_write_err.empty()) g_write_err = err;
+  }
 }
 
 int plane_size(const Block &b, int face) {
@@ -798,6 +838,7 @@ int fsil_init(int device) {
 }
 
 int fsil_finalize(void) {
+  (void)fsil_write_wait();
   for (auto &kv : g_blocks) {
     Block &b = *kv.second;
     if (b.stream) (void)hipStreamSynchronize(b.stream);
@@ -1093,6 +1134,107 @@ int fsil_macro_download(int h, double *den, double *uuu) {
   HIPCHK(hipMemcpyAsync(uuu, du.p, 3 * b->N * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
+}
+
+namespace {
+// DatFlow fields of block b over its output window into the device staging
+// buffer (on the block's stream); nw = cells in the window
+int flow_pack(Block &b, int off, double denIn, double Uref, long long &nw) {
+  if (off < 0 || 2 * off >= b.xd || 2 * off >= b.yd || 2 * off >= b.zd)
+    return fail("offsetOutput %d leaves no cells of a %dx%dx%d block", off, b.xd, b.yd, b.zd);
+  nw = (long long)(b.xd - 2 * off) * (b.yd - 2 * off) * (b.zd - 2 * off);
+  HIPCHK(b.outbuf.ensure(4 * (size_t)nw));
+  const dim3 grid((unsigned)((nw + 255) / 256)), blk(256);
+  const double hv0 = 0.5 * b.VF[0] * b.dh, hv1 = 0.5 * b.VF[1] * b.dh, hv2 = 0.5 * b.VF[2] * b.dh;
+  const double inv = 1.0 / Uref;
+  if (b.precision == 0)
+    hipLaunchKernelGGL(k_flow_pack<false>, grid, blk, 0, b.stream, b.fA.p, nullptr, b.N, b.xd, b.yd, b.zd, off, hv0,
+                       hv1, hv2, denIn, inv, b.outbuf.p);
+  else
+    hipLaunchKernelGGL(k_flow_pack<true>, grid, blk, 0, b.stream, nullptr, b.gA.p, b.N, b.xd, b.yd, b.zd, off, hv0,
+                       hv1, hv2, denIn, inv, b.outbuf.p);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
+int fsil_flow_pack(int h, int offsetOutput, double denIn, double Uref, float *out) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!out) return fail("null output");
+  long long nw = 0;
+  if (flow_pack(*b, offsetOutput, denIn, Uref, nw)) return 1;
+  HIPCHK(hipMemcpyAsync(out, b->outbuf.p, 4 * (size_t)nw * sizeof(float), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int fsil_write_flow(int h, int offsetOutput, double denIn, double Uref, const char *path) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (!path) return fail("null path");
+  long long nw = 0;
+  if (flow_pack(*b, offsetOutput, denIn, Uref, nw)) return 1;
+  const size_t head = 4 * sizeof(int) + 4 * sizeof(double), data = 4 * (size_t)nw * sizeof(float);
+  char *host = nullptr;
+  HIPCHK(hipHostMalloc((void **)&host, head + data, hipHostMallocDefault));
+  const int ih[4] = {b->xd - 2 * offsetOutput, b->yd - 2 * offsetOutput, b->zd - 2 * offsetOutput, b->id};
+  const double dh[4] = {b->xmin + offsetOutput * b->dh, b->ymin + offsetOutput * b->dh,
+                        b->zmin + offsetOutput * b->dh, b->dh};
+  memcpy(host, ih, sizeof(ih));
+  memcpy(host + sizeof(ih), dh, sizeof(dh));
+  hipEvent_t ev = nullptr;
+  hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMemcpyAsync(host + head, b->outbuf.p, data, hipMemcpyDeviceToHost, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(ev, b->stream);
+  if (e != hipSuccess) {
+    if (ev) (void)hipEventDestroy(ev);
+    (void)hipHostFree(host);
+    return fail("fsil_write_flow: %s", hipGetErrorString(e));
+  }
+  std::lock_guard<std::mutex> lk(g_write_mu);
+  g_writes.emplace_back(std::thread(flow_write_job, ev, host, head + data, std::string(path)));
+  return 0;
+}
+
+int fsil_write_wait(void) {
+  std::vector<FlowWrite> w;
+  {
+    std::lock_guard<std::mutex> lk(g_write_mu);
+    w.swap(g_writes);
+  }
+  for (auto &x : w)
+    if (x.th.joinable()) x.th.join();
+  std::lock_guard<std::mutex> lk(g_write_mu);
+  if (!g_write_err.empty()) {
+    const std::string e = g_write_err;
+    g_write_err.clear();
+    return fail("%s", e.c_str());
+  }
+  return 0;
+}
+
+int fsil_write_continue(int nblocks, const int *handles, int step, double time, const char *path) {
+  if (nblocks <= 0 || !handles || !path) return fail("bad arguments to fsil_write_continue");
+  FILE *fp = fopen(path, "wb");
+  if (!fp) return fail("cannot open %s", path);
+  int rc = 0;
+  const int hdr[2] = {nblocks, step};
+  if (fwrite(hdr, sizeof(int), 2, fp) != 2 || fwrite(&time, sizeof(double), 1, fp) != 1) rc = fail("write error on %s", path);
+  std::vector<double> f;
+  for (int k = 0; k < nblocks && rc == 0; ++k) {
+    Block *b = get_block(handles[k]);
+    if (!b) { rc = 1; break; }
+    const double g[4] = {b->xmin, b->ymin, b->zmin, b->dh};
+    const int d[3] = {b->xd, b->yd, b->zd};
+    f.resize((size_t)Q * b->N);
+    if (fsil_f_download(handles[k], f.data())) { rc = 1; break; }
+    if (fwrite(g, sizeof(double), 4, fp) != 4 || fwrite(d, sizeof(int), 3, fp) != 3 ||
+        fwrite(f.data(), sizeof(double), f.size(), fp) != f.size())
+      rc = fail("write error on %s", path);
+  }
+  if (fclose(fp) != 0 && rc == 0) rc = fail("cannot close %s", path);
+  return rc;
 }
 
 int fsil_collide_stream_bc(int h) {
@@ -1918,6 +2060,85 @@ int fsil_fem_info(int h, int body, int *newton_iters, int *cg_iters, double *dno
   if (newton_iters) *newton_iters = F.desc[body].newton_iters;
   if (cg_iters) *cg_iters = F.desc[body].cg_iters;
   if (dnorm) *dnorm = F.desc[body].dnorm;
+  return 0;
+}
+
+namespace {
+// Beam-state checkpoint of a block's bodies: a header, the element records,
+// then the node vectors in the order of Beam_write_solid_temp
+// (SolidSolver.f90:1669-1677): pos, dsp, vel, acc; lodInte, lodExte,
+// lodEffe; lodFlow, vBC.
+constexpr char kCkpMagic[8] = {'F', 'S', 'I', 'L', 'C', 'K', 'P', '1'};
+struct CkpHeader {
+  char magic[8];
+  int nbodies, nEL, nND, gEQ;
+};
+std::vector<std::pair<double *, size_t>> ckp_vectors(Block &b) {
+  Fem &F = b.fem;
+  const size_t g = 6 * (size_t)F.nND;
+  return {{F.pos.p, g}, {F.dsp.p, g}, {F.vel.p, g}, {F.acc.p, g}, {F.lodInte.p, g}, {F.lodExte.p, g},
+          {F.lodEffe.p, g}, {b.ibm.lod.p, (size_t)b.ibm.ndof}, {F.vBC.p, g}};
+}
+size_t ckp_bytes(Block &b) {
+  size_t n = sizeof(CkpHeader) + (size_t)b.fem.nEL * sizeof(BeamElem);
+  for (auto &v : ckp_vectors(b)) n += v.second * sizeof(double);
+  return n;
+}
+}  // namespace
+
+int fsil_fem_checkpoint(int h, void *buf, long long *bytes) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (F.nbodies <= 0) return fail("no beam bodies on block %d", h);
+  if (!bytes) return fail("null size");
+  const size_t need = ckp_bytes(*b);
+  if (!buf) {
+    *bytes = (long long)need;
+    return 0;
+  }
+  if (*bytes < (long long)need) return fail("checkpoint buffer of %lld bytes, %zu needed", *bytes, need);
+  if (fem_collect(*b)) return 1;
+  if (ibm_check_report(*b)) return 1;
+  char *o = (char *)buf;
+  CkpHeader hd{};
+  memcpy(hd.magic, kCkpMagic, 8);
+  hd.nbodies = F.nbodies; hd.nEL = F.nEL; hd.nND = F.nND; hd.gEQ = 6 * F.nND;
+  memcpy(o, &hd, sizeof(hd));
+  o += sizeof(hd);
+  HIPCHK(hipMemcpyAsync(o, F.el.p, (size_t)F.nEL * sizeof(BeamElem), hipMemcpyDeviceToHost, b->stream));
+  o += (size_t)F.nEL * sizeof(BeamElem);
+  for (auto &v : ckp_vectors(*b)) {
+    HIPCHK(hipMemcpyAsync(o, v.first, v.second * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    o += v.second * sizeof(double);
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  *bytes = (long long)need;
+  return 0;
+}
+
+int fsil_fem_restore(int h, const void *buf, long long bytes) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  Fem &F = b->fem;
+  if (F.nbodies <= 0) return fail("no beam bodies on block %d (define them first)", h);
+  if (!buf || bytes < (long long)sizeof(CkpHeader)) return fail("checkpoint too short");
+  CkpHeader hd;
+  memcpy(&hd, buf, sizeof(hd));
+  if (memcmp(hd.magic, kCkpMagic, 8) != 0) return fail("not a beam checkpoint");
+  if (hd.nbodies != F.nbodies || hd.nEL != F.nEL || hd.nND != F.nND)
+    return fail("checkpoint of %d bodies / %d elements / %d nodes, block has %d / %d / %d", hd.nbodies, hd.nEL,
+                hd.nND, F.nbodies, F.nEL, F.nND);
+  if (bytes < (long long)ckp_bytes(*b)) return fail("checkpoint truncated");
+  if (fem_collect(*b)) return 1;
+  const char *o = (const char *)buf + sizeof(hd);
+  HIPCHK(hipMemcpyAsync(F.el.p, o, (size_t)F.nEL * sizeof(BeamElem), hipMemcpyHostToDevice, b->stream));
+  o += (size_t)F.nEL * sizeof(BeamElem);
+  for (auto &v : ckp_vectors(*b)) {
+    HIPCHK(hipMemcpyAsync(v.first, o, v.second * sizeof(double), hipMemcpyHostToDevice, b->stream));
+    o += v.second * sizeof(double);
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
 }
 

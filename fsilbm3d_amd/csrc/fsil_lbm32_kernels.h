@@ -12,6 +12,7 @@
 // coupling's corrected velocity and force (BAND), as in the FP64 form.
 #pragma once
 #include "fsil_common.h"
+#include "fsil_lbm_kernels.h"
 
 namespace fsil {
 
@@ -196,6 +197,54 @@ __global__ __launch_bounds__(256) void k_macro32(const float *__restrict__ g, lo
   u[n] = ((double)j1 + (double)hv0) / rho;
   u[N + n] = ((double)j2 + (double)hv1) / rho;
   u[2 * N + n] = ((double)j3 + (double)hv2) / rho;
+}
+
+// ---------------------------------------------------------------------------
+// The instantaneous DatFlow fields of write_flow_ (FluidDomain.f90:1652-1664,
+// outputtype 1): over the output window (offsetOutput cells left out on every
+// face), real(4) p = Cs2 (den - denIn), u, v, w = uuu / Uref, with den/uuu as
+// calculate_macro_quantities_ leaves them (FP64 form: k_macro's arithmetic;
+// FP32-deviation form: k_macro32's).  out: [4][nx*ny*nz], z fastest.
+template <bool F32>
+__global__ __launch_bounds__(256) void k_flow_pack(const double *__restrict__ f, const float *__restrict__ g,
+                                                   long long N, int xd, int yd, int zd, int off, double hv0,
+                                                   double hv1, double hv2, double denIn, double invUref,
+                                                   float *__restrict__ out) {
+  const int nx = xd - 2 * off, ny = yd - 2 * off, nz = zd - 2 * off;
+  const long long nw = (long long)nx * ny * nz;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nw) return;
+  const int z = (int)(i % nz) + off, y = (int)((i / nz) % ny) + off, x = (int)(i / ((long long)nz * ny)) + off;
+  const long long n = ((long long)x * yd + y) * zd + z;
+  double rho, u1, u2, u3;
+  if constexpr (F32) {
+    float dr = 0.f, cr = 0.f, j1 = 0.f, c1 = 0.f, j2 = 0.f, c2 = 0.f, j3 = 0.f, c3 = 0.f;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const float v = g[q * N + n];
+      kahan_add(dr, cr, v);
+      if (ex(q) != 0) kahan_add(j1, c1, ex(q) > 0 ? v : -v);
+      if (ey(q) != 0) kahan_add(j2, c2, ey(q) > 0 ? v : -v);
+      if (ez(q) != 0) kahan_add(j3, c3, ez(q) > 0 ? v : -v);
+    }
+    rho = 1.0 + (double)dr;
+    u1 = ((double)j1 + (double)(float)hv0) / rho;
+    u2 = ((double)j2 + (double)(float)hv1) / rho;
+    u3 = ((double)j3 + (double)(float)hv2) / rho;
+  } else {
+    double h[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) h[q] = f[q * N + n];
+    double m1, m2, m3;
+    moments(h, rho, m1, m2, m3);
+    u1 = (m1 + hv0) / rho;
+    u2 = (m2 + hv1) / rho;
+    u3 = (m3 + hv2) / rho;
+  }
+  out[i] = (float)((1.0 / 3.0) * (rho - denIn));
+  out[nw + i] = (float)(u1 * invUref);
+  out[2 * nw + i] = (float)(u2 * invUref);
+  out[3 * nw + i] = (float)(u3 * invUref);
 }
 
 }  // namespace fsil

@@ -21,6 +21,7 @@ module fsil_bridge
               fsil_tree_steps, fsil_time_tree_steps, fsil_mlp_create, fsil_mlp_destroy, fsil_mlp_get_params, &
               fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train, fsil_ibm_set_warm_start, &
               fsil_ibm_rounds, fsil_solid_info, fsil_solid_info_static, fsil_set_viscosity, fsil_tau_download, &
+              fsil_flow_pack, fsil_write_flow, fsil_write_wait, fsil_write_continue, fsil_fem_checkpoint, fsil_fem_restore, &
               fsil_ibm_warm_start_data, fsil_ibm_forces, &
               fsil_ibm_report
 
@@ -300,6 +301,41 @@ module fsil_bridge
             import :: c_int, c_double
             integer(c_int), value :: handle
             real(c_double), intent(out) :: tau_all(*)
+        end function
+        integer(c_int) function fsil_flow_pack(handle, offsetOutput, denIn, Uref, out) bind(C, name='fsil_flow_pack')
+            import :: c_int, c_double, c_float
+            integer(c_int), value :: handle, offsetOutput
+            real(c_double), value :: denIn, Uref
+            real(c_float), intent(out) :: out(*)
+        end function
+        integer(c_int) function fsil_write_flow(handle, offsetOutput, denIn, Uref, path) bind(C, name='fsil_write_flow')
+            import :: c_int, c_double, c_char
+            integer(c_int), value :: handle, offsetOutput
+            real(c_double), value :: denIn, Uref
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function fsil_write_wait() bind(C, name='fsil_write_wait')
+            import :: c_int
+        end function
+        integer(c_int) function fsil_write_continue(nblocks, handles, step, time, path) &
+                bind(C, name='fsil_write_continue')
+            import :: c_int, c_double, c_char
+            integer(c_int), value :: nblocks, step
+            integer(c_int), intent(in) :: handles(*)
+            real(c_double), value :: time
+            character(kind=c_char), intent(in) :: path(*)
+        end function
+        integer(c_int) function fsil_fem_checkpoint(handle, buf, nbytes) bind(C, name='fsil_fem_checkpoint')
+            import :: c_int, c_ptr, c_long_long
+            integer(c_int), value :: handle
+            type(c_ptr), value :: buf
+            integer(c_long_long), intent(inout) :: nbytes
+        end function
+        integer(c_int) function fsil_fem_restore(handle, buf, nbytes) bind(C, name='fsil_fem_restore')
+            import :: c_int, c_ptr, c_long_long
+            integer(c_int), value :: handle
+            type(c_ptr), value :: buf
+            integer(c_long_long), value :: nbytes
         end function
         integer(c_int) function fsil_ibm_rounds(handle, rounds) bind(C, name='fsil_ibm_rounds')
             import :: c_int

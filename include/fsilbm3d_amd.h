@@ -94,6 +94,38 @@ int fsil_set_boundary_conditions(int handle);
  * as main.f90:107 calls it): den(z,y,x), uuu(z,y,x,1:3). */
 int fsil_macro_download(int handle, double *den, double *uuu);
 
+/* ---------------------------------------------------------------------------
+ * Output and restart (FluidDomain.f90:268-285, 1628-1736, 1770-1789;
+ * SolidSolver.f90:1669-1692).  The host keeps file naming and cadence. */
+
+/* The instantaneous DatFlow fields of write_flow_ (outputtype 1) computed on
+ * the device from the stored state: out[4][nx*ny*nz] real(4) p = Cs2 (den -
+ * denIn), u, v, w = uuu / Uref over the output window (offsetOutput cells left
+ * out on every face), z fastest. */
+int fsil_flow_pack(int handle, int offsetOutput, double denIn, double Uref, float *out);
+
+/* write_flow_ without the fork: packs on the device, copies into pinned host
+ * memory on the block's stream and returns; a host thread writes `path` in the
+ * reference's stream layout (nx, ny, nz, ID int32; xmin, ymin, zmin, dh real8
+ * of the window; p, u, v, w real(4)) once the copy completes. */
+int fsil_write_flow(int handle, int offsetOutput, double denIn, double Uref, const char *path);
+
+/* Wait for every pending fsil_write_flow (the reference's mywait loop before
+ * the next write, FluidDomain.f90:355-357); reports the first write error. */
+int fsil_write_wait(void);
+
+/* write_continue_blocks (FluidDomain.f90:268-285, 1770-1776) from the device
+ * states of nblocks blocks: nblocks, step (int32), time / Tref (real8), then
+ * per block xmin, ymin, zmin, dh (real8), xDim, yDim, zDim (int32), fIn. */
+int fsil_write_continue(int nblocks, const int *handles, int step, double time, const char *path);
+
+/* Checkpoint of the block's beam bodies (Beam_write_solid_temp / read, which
+ * the reference declares at SolidSolver.f90:1669-1692): element records, then
+ * pos, dsp, vel, acc, lodInte, lodExte, lodEffe, lodFlow, vBC.  With buf NULL
+ * *bytes receives the size; restore needs the same bodies defined. */
+int fsil_fem_checkpoint(int handle, void *buf, long long *bytes);
+int fsil_fem_restore(int handle, const void *buf, long long bytes);
+
 /* One fluid update of a block: collision_ (with the immersed-boundary
  * velocity/force of the last fsil_ibm_interaction_force on its band cells,
  * the uniform volume force elsewhere), halfwayBCset_, streaming_ and

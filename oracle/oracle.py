@@ -404,3 +404,24 @@ def reference_scales(UrefType, uvwIn, Uref, Lref, TrefType, Tref, denIn, Asfac):
     E = (q * Asfac) * Lref
     P = (q * Asfac) * U
     return (Lref, U, A, F, P, E)
+
+
+def datflow_bytes(f, VF, dh, denIn, Uref, offset, block_id, origin):
+    """The reference's instantaneous DatFlow file of one block (write_flow_ with
+    outputtype 1, FluidDomain.f90:1628-1720): header nxe, nye, nze, ID (int32)
+    and xmin, ymin, zmin, dh (real8) of the output window (offsetOutput cells
+    left out on every face), then real(4) p = Cs2 (den - denIn) and u, v, w /
+    Uref over the window, z fastest; den/uuu as calculate_macro_quantities_
+    leaves them after the step (main.f90:107)."""
+    den, u = macro(f, VF, dh)
+    xd, yd, zd = f.shape[1:]
+    o = int(offset)
+    sl = (slice(o, xd - o), slice(o, yd - o), slice(o, zd - o))
+    cs2 = 1.0 / 3.0
+    inv = 1.0 / Uref
+    p = (cs2 * (den[sl] - denIn)).astype(np.float32)
+    uu = [(u[k][sl] * inv).astype(np.float32) for k in range(3)]
+    nx, ny, nz = p.shape
+    head = np.array([nx, ny, nz, block_id], "<i4").tobytes()
+    head += np.array([origin[0] + o * dh, origin[1] + o * dh, origin[2] + o * dh, dh], "<f8").tobytes()
+    return head + b"".join(np.ascontiguousarray(a).astype("<f4").tobytes() for a in [p] + uu)

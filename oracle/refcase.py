@@ -111,6 +111,9 @@ class Case:
     groups: List[Group] = dataclasses.field(default_factory=list)
     npsize: int = 1
     info_every: Optional[int] = None    # write DatInfo every this many steps
+    flow_every: Optional[int] = None    # write DatFlow every this many steps (and at start-up)
+    offsetOutput: int = 0               # cells left out of DatFlow on every face
+    outputtype: int = 0                 # 1: instantaneous DatFlow files
     sons: List[SubBlock] = dataclasses.field(default_factory=list)
     interpolateScheme: int = 1
 
@@ -163,6 +166,8 @@ def inflow_text(case: Case, nsteps: int, restart: bool = True) -> str:
     if not restart:
         conti = big
     info = big if case.info_every is None else case.info_every * case.dh / tref
+    flow = big if case.flow_every is None else case.flow_every * case.dh / tref
+    wbegin = big if case.flow_every is None else 0.0
     L = []
     a = L.append
     a("# generated case")
@@ -174,9 +179,9 @@ def inflow_text(case: Case, nsteps: int, restart: bool = True) -> str:
     a("# timeSimTotal timeContiDelta")
     a(f"{_f(total)} {_f(conti)}")
     a("# timeWriteBegin timeWriteEnd")
-    a(f"{_f(big)} {_f(big + 1.0)}")
+    a(f"{_f(wbegin)} {_f(big + 1.0)}")
     a("# timeFlowDelta timeBodyDelta timeInfoDelta")
-    a(f"{_f(big)} {_f(big)} {_f(info)}")
+    a(f"{_f(flow)} {_f(big)} {_f(info)}")
     a("# Re denIn")
     a(f"{_f(case.Re)} {_f(case.denIn)}")
     a("# uvwIn")
@@ -230,7 +235,7 @@ def inflow_text(case: Case, nsteps: int, restart: bool = True) -> str:
         blocks.append((2 + i, b.collide, b.dims, b.dh, b.origin, b.bc, b.params))
     for k, (bid, col, dims, dh, org, bc, prm) in enumerate(blocks):
         a("# ID iCollideModel offsetOutput outputtype")
-        a(f"{bid} {col} 0 0")
+        a(f"{bid} {col} {case.offsetOutput if k == 0 else 0} {case.outputtype if k == 0 else 0}")
         a("# xDim yDim zDim")
         a(" ".join(str(int(d)) for d in dims))
         a("# dh xmin ymin zmin")

@@ -214,6 +214,9 @@ CASES: List[ParityCase] = [
                nsteps=3, seed=54, ubase=(0.05, 0, 0), neq=0.02,
                plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.5, 0.0), XYZPhi=(0.0, 30.0, 0.0),
                              Freq=0.02)]),
+    ParityCase("output_flow_continue", _open(bc=(101, 103, 302, 201, 301, 301), volumeForceIn=(2e-6, 0.0, 1e-6),
+                                             flow_every=3, offsetOutput=1, outputtype=1), nsteps=3, seed=61,
+               ubase=(0.05, 0, 0), neq=0.01),
     ParityCase("twoblock_linear", _open(dims=(16, 12, 12), bc=(101, 103, 302, 302, 302, 302),
                                         sons=[SubBlock(dims=(13, 11, 11), dh=0.5, origin=(4.0, 3.0, 3.0))]),
                nsteps=3, seed=41, ubase=(0.05, 0, 0), neq=0.01),

@@ -5,6 +5,8 @@ injected through its own restart file, and the state it writes after N steps
 is stored as tests/golden/<case>.npz:
 
   f_final        (19, xDim, yDim, zDim) float64  post-stream + BC state after N steps
+  datflow, continue_bytes  (cases that write flow output) the reference's last
+                 DatFlow/Flow*_b001 file and its DatContinue/continue* file, raw
   <file>_g<g>    per body group, the rows of DatInfo/Group<g>_<file>.dat
                  (file: forces, firstNode, lastNode, centerNode, nodeAverage,
                  power, energy; 10 significant digits, one row per step)
@@ -64,6 +66,15 @@ def make(pc):
     else:
         out = {"f_final": r.f, "step": np.array(r.step)}
     out.update(read_info(r.workdir, len(pc.plates)))
+    if case.flow_every:
+        # the last DatFlow file and the continue file, byte for byte
+        flows = sorted(x for x in os.listdir(os.path.join(r.workdir, "DatFlow")) if x.startswith("Flow"))
+        out["datflow"] = np.frombuffer(open(os.path.join(r.workdir, "DatFlow", flows[-1]), "rb").read(), np.uint8)
+        out["datflow_name"] = np.array(flows[-1])
+        conts = sorted(x for x in os.listdir(os.path.join(r.workdir, "DatContinue")) if x != "continue")
+        out["continue_bytes"] = np.frombuffer(open(os.path.join(r.workdir, "DatContinue", conts[-1]), "rb").read(),
+                                              np.uint8)
+        out["continue_name"] = np.array(conts[-1])
     shutil.rmtree(r.workdir, ignore_errors=True)
     np.savez_compressed(fixture_path(pc.name), **out)
     return r.step

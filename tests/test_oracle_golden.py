@@ -162,3 +162,28 @@ def test_planted_datinfo_mistakes_are_caught():
     wrong = en.copy()
     wrong[:, 4] = 2.0 * wrong[:, 4]
     assert not print_equal(wrong[:, 4], fx["energy_g1"][:, 4]).any()
+
+
+def test_oracle_output_files_match_reference_bytes():
+    """The reference's own DatFlow file (real(4) p, u, v, w over the output
+    window, write_flow_, FluidDomain.f90:1628-1720) and continue file
+    (write_continue_blocks, FluidDomain.f90:268-285, 1770-1776) after the last
+    step, byte for byte from the oracle's state."""
+    import os
+    import tempfile
+    from oracle.oracle import datflow_bytes
+    from oracle.refcase import write_continue
+    from tests.golden.cases import BY_NAME
+    from tests.helpers import derived
+    pc = BY_NAME["output_flow_continue"]
+    fx = load_fixture(pc.name)
+    f = run_oracle(pc)
+    case = pc.case
+    d = derived(case)
+    got = datflow_bytes(f, case.volumeForceIn, case.dh, case.denIn, d["Uref"], case.offsetOutput, 1, case.origin)
+    assert got == fx["datflow"].tobytes()
+    with tempfile.TemporaryDirectory() as td:
+        p = os.path.join(td, "c")
+        write_continue(p, f, case, pc.nsteps, pc.nsteps * case.dh / d["Tref"])
+        assert open(p, "rb").read() == fx["continue_bytes"].tobytes()
+    assert str(fx["datflow_name"]) == "Flow0000015000_b001" and str(fx["continue_name"]) == "continue0000015000"
