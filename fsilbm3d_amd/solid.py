@@ -207,6 +207,7 @@ class RigidPlate:
               W1[0] * math.sin(AoA[1]) * math.sin(AoA[2]) + W1[1] * math.cos(AoA[2]),
               W1[0] * math.sin(AoA[1]) * math.cos(AoA[2]) - W1[1] * math.sin(AoA[2])]
         W3 = _matvec(TTT, W2)
+        self.W3, self.UVW = W3, UVW
         for n in range(self.nND):
             rel = [self.pos[n, k] - self.XYZ[k] for k in range(3)]
             w = [W3[1] * rel[2] - W3[2] * rel[1], W3[2] * rel[0] - W3[0] * rel[2], W3[0] * rel[1] - W3[1] * rel[0]]
@@ -231,6 +232,7 @@ class RigidPlate:
         self.TTT00 = [[1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]]
         self.XYZ, AoA = self._place(t)
         self.TTT0 = aoa_to_ttt(AoA)
+        self.TTTnxt = self.TTT0
         AoAd = angle_triad(self.TTT0, self.TTT0)
         self._map_elements(self.TTT0, self.XYZ, AoAd)
         self._velocity(t, AoA, self.TTT0)
@@ -240,6 +242,7 @@ class RigidPlate:
         t = (time - deltat) + float(isubstep) * subdeltat
         self.XYZ, AoA = self._place(t)
         TTTnxt = aoa_to_ttt(AoA)
+        self.TTTnxt = TTTnxt
         AoAd = angle_triad(self.TTT0, TTTnxt)
         self._map_elements(TTTnxt, self.XYZ, AoAd)
         self._velocity(t, AoA, TTTnxt)
@@ -276,3 +279,81 @@ class RigidPlate:
     def force_coefficients(self, lodFlow: np.ndarray, Fref: float):
         """sum(lodFlow(1:gEQ:6))/Fref etc. (SolidSolver.f90:1761-1765)."""
         return tuple(_sum(lodFlow[k::6]) / Fref for k in range(3))
+
+
+def read_gmsh(text: str):
+    """Read_gmsh (Solidbody.f90:1088-1128): the nodes and triangles of a gmsh
+    2 ASCII mesh.  Four header lines, the node count and "id x y z" rows, two
+    lines, the element count, then every element from the first triangle
+    (type 2 with two tags, first tag 0) on is a triangle "id 2 2 0 t a b c"."""
+    lines = text.splitlines()
+    n = int(lines[4].split()[0])
+    xyz = np.array([[float(v) for v in lines[5 + i].split()[1:4]] for i in range(n)])
+    i = 5 + n + 2
+    ne = int(lines[i].split()[0])
+    i += 1
+    for j in range(ne):
+        parts = lines[i + j].split()
+        p = [int(v) for v in parts[1:5]]
+        if p[0] == 2 and p[1] == 2 and p[2] == 0:
+            ntri = ne - int(parts[0]) + 1
+            tri = np.array([[int(v) for v in lines[i + j + k].split()[5:8]] for k in range(ntri)], np.int32)
+            return xyz, tri
+    raise ValueError("no triangles in the mesh")
+
+
+def surface_geometry(xyz, tri, alpha: float, denIn: float):
+    """SurfaceBuildPosVelArea_ (Solidbody.f90:648-705): per triangle its
+    incenter (body frame) and area times -2 alpha denIn."""
+    beta = -(alpha * 2.0 * denIn)
+    E0 = np.zeros((len(tri), 3))
+    Ea = np.zeros(len(tri))
+    for i, (a, b, c) in enumerate(tri):
+        x1, y1, z1 = (float(v) for v in xyz[a - 1])
+        x2, y2, z2 = (float(v) for v in xyz[b - 1])
+        x3, y3, z3 = (float(v) for v in xyz[c - 1])
+        la = math.sqrt((x2 - x3) * (x2 - x3) + (y2 - y3) * (y2 - y3) + (z2 - z3) * (z2 - z3))
+        lb = math.sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1) + (z3 - z1) * (z3 - z1))
+        lc = math.sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2) + (z1 - z2) * (z1 - z2))
+        inv = 1.0 / (la + lb + lc)
+        E0[i] = [(la * x1 + lb * x2 + lc * x3) * inv, (la * y1 + lb * y2 + lc * y3) * inv,
+                 (la * z1 + lb * z2 + lc * z3) * inv]
+        ax = (z1 - z2) * (y3 - y2) + (y2 - y1) * (z3 - z2)
+        ay = (x1 - x2) * (z3 - z2) + (z2 - z1) * (x3 - x2)
+        az = (y1 - y2) * (x3 - x2) + (x2 - x1) * (y3 - y2)
+        Ea[i] = math.sqrt(ax * ax + ay * ay + az * az) * 0.5 * beta
+    return E0, Ea
+
+
+def surface_beam_nodes(xyz) -> np.ndarray:
+    """The two-node beam SurfacetoBeam_write makes of a surface mesh
+    (Solidbody.f90:1130-1170): its first two nodes written with F20.10, Lspan
+    = Rspan = 1, direction z."""
+    return np.array([[float("%.10f" % float(v)) for v in xyz[i]] + [1.0, 1.0, 0.0, 0.0, 1.0] for i in range(2)])
+
+
+class SurfaceBody(RigidPlate):
+    """A v_type = -1 body (triangulated surface from a gmsh mesh) with
+    iBodyModel = 1: one marker per triangle at its incenter, moved rigidly with
+    the body's beam (SurfaceUpdatePosVel_, Solidbody.f90:707-720) and loading
+    that beam's single element (vtor = 1)."""
+
+    def __init__(self, msh_text: str, **kw):
+        xyz, tri = read_gmsh(msh_text)
+        self.E0, self.Ea0 = surface_geometry(xyz, tri, kw.get("IBPenaltyAlpha", 1.0), kw.get("denIn", 1.0))
+        super().__init__(surface_beam_nodes(xyz), [(1, 2, 1)], **kw)
+        self.M = len(self.E0)
+        self.marker_elem = np.zeros(self.M, np.int32)
+        self.rtov = [0, self.M]
+
+    def update_markers(self):
+        T, W, U = self.TTTnxt, self.W3, self.UVW
+        n = len(self.E0)
+        X, V = np.zeros((n, 3)), np.zeros((n, 3))
+        for i in range(n):
+            rel = _matvec(T, [float(v) for v in self.E0[i]])
+            w = [W[1] * rel[2] - W[2] * rel[1], W[2] * rel[0] - W[0] * rel[2], W[0] * rel[1] - W[1] * rel[0]]
+            for k in range(3):
+                X[i, k] = rel[k] + self.XYZ[k]
+                V[i, k] = w[k] + U[k]
+        self.Exyz, self.Evel, self.Ea = X, V, self.Ea0.copy()

@@ -1061,3 +1061,45 @@ void orc_beam_solid_info(Beam *b, const double *XYZo, const double *refs, double
   for (int k = 0; k < 3; ++k) en[k] = XYZo[k] / Lref;
   en[3] = Ev + Ep; en[4] = Ev; en[5] = Ep; en[6] = Es; en[7] = Eb;
 }
+
+/* SurfaceUpdatePosVel_ (Solidbody.f90:707-720): markers of a v_type = -1 body
+ * (one per surface triangle) from their body-frame positions E0 and the rigid
+ * kinematics of the body's beam: x = TTTnxt E0 + XYZ, v = WWW3 x rel + UVW. */
+void orc_beam_surface_markers(const Beam *b, int n, const double *E0, double *Exyz, double *Evel) {
+  for (int i = 0; i < n; ++i) {
+    double rel[3];
+    for (int r = 0; r < 3; ++r) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s = s + b->TTTnxt[r][k] * E0[3 * i + k];
+      rel[r] = s;
+    }
+    const double *w = b->WWW3;
+    const double v[3] = {w[1] * rel[2] - w[2] * rel[1], w[2] * rel[0] - w[0] * rel[2], w[0] * rel[1] - w[1] * rel[0]};
+    for (int k = 0; k < 3; ++k) {
+      Exyz[3 * i + k] = rel[k] + b->XYZ[k];
+      Evel[3 * i + k] = v[k] + b->UVW[k];
+    }
+  }
+}
+
+/* SurfaceBuildPosVelArea_ (Solidbody.f90:648-705): per triangle (1-based node
+ * ids tri[3i..3i+2]) its incenter E0 and area times -2 alpha denIn. */
+void orc_surface_geometry(int ntri, const int *tri, const double *xyz, double alpha, double denIn, double *E0,
+                          double *Ea) {
+  const double beta = -(alpha * 2.0 * denIn);
+  for (int i = 0; i < ntri; ++i) {
+    const double *A = xyz + 3 * (tri[3 * i] - 1), *B = xyz + 3 * (tri[3 * i + 1] - 1), *C = xyz + 3 * (tri[3 * i + 2] - 1);
+    const double x1 = A[0], x2 = B[0], x3 = C[0], y1 = A[1], y2 = B[1], y3 = C[1], z1 = A[2], z2 = B[2], z3 = C[2];
+    const double la = sqrt((x2 - x3) * (x2 - x3) + (y2 - y3) * (y2 - y3) + (z2 - z3) * (z2 - z3));
+    const double lb = sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1) + (z3 - z1) * (z3 - z1));
+    const double lc = sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2) + (z1 - z2) * (z1 - z2));
+    const double invC = 1.0 / (la + lb + lc);
+    E0[3 * i] = (la * x1 + lb * x2 + lc * x3) * invC;
+    E0[3 * i + 1] = (la * y1 + lb * y2 + lc * y3) * invC;
+    E0[3 * i + 2] = (la * z1 + lb * z2 + lc * z3) * invC;
+    const double ax = ((z1 - z2) * (y3 - y2) + (y2 - y1) * (z3 - z2));
+    const double ay = ((x1 - x2) * (z3 - z2) + (z2 - z1) * (x3 - x2));
+    const double az = ((y1 - y2) * (x3 - x2) + (x2 - x1) * (y3 - y2));
+    Ea[i] = sqrt(ax * ax + ay * ay + az * az) * 0.5 * beta;
+  }
+}

@@ -303,6 +303,8 @@ class OracleBeam:
         L.orc_beam_asfac.restype = C.c_double
         L.orc_beam_marker_elem.argtypes = [C.c_void_p, _pi, _pi]
         L.orc_beam_solid_info.argtypes = [C.c_void_p, _pd, _pd, _pd]
+        L.orc_beam_surface_markers.argtypes = [C.c_void_p, C.c_int, _pd, _pd, _pd]
+        L.orc_surface_geometry.argtypes = [C.c_int, _pi, _pd, C.c_double, C.c_double, _pd, _pd]
         cls._sig_done = True
 
     def __init__(self, nodes8, elements, constraints, *, iBodyModel=2, isMotionGiven=(1, 1, 1, 1, 1, 1), denR=1.0,
@@ -362,6 +364,14 @@ class OracleBeam:
         out = [np.zeros(g) for _ in range(4)]
         lib().orc_beam_state(self.h, *[_d(a) for a in out])
         return [a.reshape(self.nND, 6) for a in out]
+
+    def surface_markers(self, E0):
+        """Markers of a surface body on this (two-node) beam's rigid kinematics."""
+        E0 = np.ascontiguousarray(E0, dtype=np.float64)
+        n = len(E0)
+        X, V = np.zeros((n, 3)), np.zeros((n, 3))
+        lib().orc_beam_surface_markers(self.h, n, _d(E0), _d(X), _d(V))
+        return X, V
 
     def solid_info(self, XYZo, refs):
         """One DatInfo row per file of Beam_write_solid_info (SolidSolver.f90:1729-1799)
@@ -425,3 +435,54 @@ def datflow_bytes(f, VF, dh, denIn, Uref, offset, block_id, origin):
     head = np.array([nx, ny, nz, block_id], "<i4").tobytes()
     head += np.array([origin[0] + o * dh, origin[1] + o * dh, origin[2] + o * dh, dh], "<f8").tobytes()
     return head + b"".join(np.ascontiguousarray(a).astype("<f4").tobytes() for a in [p] + uu)
+
+
+def surface_geometry(tri, xyz, alpha, denIn):
+    """Incenters and area weights of a triangulated surface (1-based tri)."""
+    lib()
+    OracleBeam._sigs()
+    tri = np.ascontiguousarray(tri, dtype=np.int32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    n = len(tri)
+    E0, Ea = np.zeros((n, 3)), np.zeros(n)
+    lib().orc_surface_geometry(n, _i(tri), _d(xyz), float(alpha), float(denIn), _d(E0), _d(Ea))
+    return E0, Ea
+
+
+class OracleSurface:
+    """A v_type = -1 body on the oracle: a rigid two-node beam (the one
+    SurfacetoBeam_write makes of the first two mesh nodes, Solidbody.f90:1130-1170)
+    carrying one marker per mesh triangle (Solidbody.f90:648-720)."""
+
+    def __init__(self, beam, tri, xyz, alpha, denIn):
+        self.beam = beam
+        self.E0, self.Ea0 = surface_geometry(tri, xyz, alpha, denIn)
+        self.M = len(self.E0)
+        self.nND, self.nEL = beam.nND, beam.nEL
+        self.iBodyModel, self.v_move = beam.iBodyModel, beam.v_move
+        self.marker_elem = np.zeros(self.M, np.int32)
+        self.elem_nodes = beam.elem_nodes
+        self.Asfac = beam.Asfac
+        self.Ea = self.Ea0.copy()
+        self.update_markers()
+
+    def update_markers(self):
+        self.Exyz, self.Evel = self.beam.surface_markers(self.E0)
+
+    def element_centres(self):
+        return self.beam.element_centres()
+
+    def set_lodflow(self, lod):
+        self.beam.set_lodflow(lod)
+
+    def structure(self, *a):
+        self.beam.structure(*a)
+
+    def state(self):
+        return self.beam.state()
+
+    def info(self):
+        return self.beam.info()
+
+    def solid_info(self, XYZo, refs):
+        return self.beam.solid_info(XYZo, refs)

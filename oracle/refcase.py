@@ -39,9 +39,9 @@ REF_BIN = os.path.join(HERE, "_ref", "FSILBM3D")
 @dataclasses.dataclass
 class Group:
     """One body group of the SolidBody section (Solidbody.f90:121-155)."""
-    plate: str                      # plate.dat text (Frame3D)
+    plate: str                      # plate.dat text (Frame3D), or the gmsh mesh text of a surface body
     iBodyModel: int = 1             # 1 rigid prescribed, 2 elastic beam
-    iBodyType: int = 1              # 1 plate given by a line mesh
+    iBodyType: int = 1              # 1 plate given by a line mesh, -1 surface given by a gmsh mesh
     isMotionGiven: Sequence[int] = (1, 1, 1, 1, 1, 1)
     denR: float = 1.0
     psR: float = 0.3
@@ -336,7 +336,9 @@ def run_reference(case: Case, f0: np.ndarray, nsteps: int, keep: bool = False,
         with open(os.path.join(wd, "inFlow.dat"), "w") as fh:
             fh.write(inflow_text(case, nsteps))
         for gi, g in enumerate(case.groups):
-            with open(os.path.join(wd, f"plate{gi + 1}.dat"), "w") as fh:
+            # a surface body's mesh is plate<k>.msh; the reference writes the .dat itself
+            ext = "msh" if g.iBodyType == -1 else "dat"
+            with open(os.path.join(wd, f"plate{gi + 1}.{ext}"), "w") as fh:
                 fh.write(g.plate)
         write_continue(os.path.join(wd, "DatContinue", "continue"), f0, case, 0, start_time_over_tref)
         env = dict(os.environ)
@@ -372,7 +374,8 @@ def time_reference_steps(case: Case, nsteps: int, threads: int, timeout: float =
         with open(os.path.join(wd, "inFlow.dat"), "w") as fh:
             fh.write(inflow_text(c, nsteps, restart=False))
         for gi, g in enumerate(c.groups):
-            with open(os.path.join(wd, f"plate{gi + 1}.dat"), "w") as fh:
+            ext = "msh" if g.iBodyType == -1 else "dat"
+            with open(os.path.join(wd, f"plate{gi + 1}.{ext}"), "w") as fh:
                 fh.write(g.plate)
         env = dict(os.environ)
         env["OMP_NUM_THREADS"] = str(threads)

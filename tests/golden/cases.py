@@ -109,6 +109,56 @@ class Plate:
                      initXYZVel=self.initXYZVel)
 
 
+def icosphere(radius: float, subdivisions: int = 1):
+    """Vertices and triangles (1-based) of a subdivided icosahedron."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    v = [np.array(p, np.float64) / math.sqrt(sum(c * c for c in p)) for p in v]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdivisions):
+        mid, nf = {}, []
+
+        def m(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                p = v[a] + v[b]
+                v.append(p / np.linalg.norm(p))
+                mid[k] = len(v) - 1
+            return mid[k]
+        for a, b, c in f:
+            ab, bc, ca = m(a, b), m(b, c), m(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return np.array(v) * radius, np.array(f, np.int32) + 1
+
+
+def gmsh_text(xyz, tri) -> str:
+    """A gmsh 2 ASCII mesh of the triangles, in the form Read_gmsh expects."""
+    L = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", str(len(xyz))]
+    L += [f"{i + 1} {float(p[0])!r} {float(p[1])!r} {float(p[2])!r}" for i, p in enumerate(xyz)]
+    L += ["$EndNodes", "$Elements", str(len(tri))]
+    L += [f"{i + 1} 2 2 0 1 {a} {b} {c}" for i, (a, b, c) in enumerate(tri)]
+    L += ["$EndElements"]
+    return "\n".join(L) + "\n"
+
+
+@dataclasses.dataclass
+class SurfacePlate(Plate):
+    """A rigid surface body (iBodyType -1): a sphere of `radius` triangulated
+    as a once-subdivided icosahedron (80 triangles), centred at XYZo."""
+    radius: float = 3.0
+
+    def mesh(self) -> str:
+        return gmsh_text(*icosphere(self.radius))
+
+    def group(self) -> Group:
+        g = super().group()
+        return dataclasses.replace(g, plate=self.mesh(), iBodyType=-1)
+
+
 @dataclasses.dataclass
 class ParityCase:
     name: str
@@ -214,6 +264,12 @@ CASES: List[ParityCase] = [
                nsteps=3, seed=54, ubase=(0.05, 0, 0), neq=0.02,
                plates=[Plate(5, -2.0, 2.0, 2.0, 4, XYZo=(8.0, 6.0, 6.0), XYZAmpl=(0.0, 1.5, 0.0), XYZPhi=(0.0, 30.0, 0.0),
                              Freq=0.02)]),
+    ParityCase("ibm_surface_sphere", _open(dims=(24, 16, 16), bc=(101, 103, 302, 302, 302, 302), ntolLBM=3,
+                                           dtolLBM=1e-14),
+               nsteps=3, seed=71, ubase=(0.05, 0, 0), neq=0.005,
+               plates=[SurfacePlate(2, 0.0, 1.0, 1.0, 1, XYZo=(10.0, 8.0, 8.0), XYZAmpl=(0.0, 1.0, 0.0),
+                                    XYZPhi=(0.0, 20.0, 0.0), AoAo=(0.0, 0.0, 10.0), AoAAmpl=(0.0, 0.0, 8.0),
+                                    Freq=0.02, radius=3.0)]),
     ParityCase("output_flow_continue", _open(bc=(101, 103, 302, 201, 301, 301), volumeForceIn=(2e-6, 0.0, 1e-6),
                                              flow_every=3, offsetOutput=1, outputtype=1), nsteps=3, seed=61,
                ubase=(0.05, 0, 0), neq=0.01),

@@ -52,8 +52,15 @@ def plate_asfac(p):
 
 
 def make_bodies(pc, denIn):
+    from fsilbm3d_amd.solid import SurfaceBody
+    from tests.golden.cases import SurfacePlate
     out = []
     for p in pc.plates:
+        if isinstance(p, SurfacePlate):
+            out.append(SurfaceBody(p.mesh(), XYZo=p.XYZo, XYZAmpl=p.XYZAmpl, XYZPhi=p.XYZPhi, AoAo=p.AoAo,
+                                   AoAAmpl=p.AoAAmpl, AoAPhi=p.AoAPhi, Freq=p.Freq, initXYZVel=p.initXYZVel,
+                                   denIn=denIn, IBPenaltyAlpha=pc.case.IBPenaltyAlpha))
+            continue
         out.append(RigidPlate(p.nodes(), p.elements(), XYZo=p.XYZo, XYZAmpl=p.XYZAmpl, XYZPhi=p.XYZPhi,
                               AoAo=p.AoAo, AoAAmpl=p.AoAAmpl, AoAPhi=p.AoAPhi, Freq=p.Freq,
                               initXYZVel=p.initXYZVel, denIn=denIn, IBPenaltyAlpha=pc.case.IBPenaltyAlpha))
@@ -68,20 +75,32 @@ def flow_kwargs(case):
 
 def oracle_bodies(pc, d):
     """Plate bodies on the oracle's beam restatement (rigid or elastic)."""
-    from oracle.oracle import OracleBeam
+    from oracle.oracle import OracleBeam, OracleSurface
+    from tests.golden.cases import SurfacePlate, icosphere
     case = pc.case
     out = []
     for p in pc.plates:
         cons = np.zeros((p.n_nodes, 6), np.int32)
         cons[0] = 1
         els = np.array(p.elements(), np.int32)
-        out.append(OracleBeam(p.nodes(), els, cons, iBodyModel=p.iBodyModel, isMotionGiven=p.isMotionGiven,
+        nodes = p.nodes()
+        if isinstance(p, SurfacePlate):
+            # the beam the reference writes of the mesh's first two nodes (F20.10)
+            xyz, tri = icosphere(p.radius)
+            nodes = np.array([[float("%.10f" % v) for v in xyz[i]] + [1.0, 1.0, 0.0, 0.0, 1.0] for i in range(2)])
+            cons = np.zeros((2, 6), np.int32)
+            cons[0] = 1
+            els = np.array([(1, 2, 1)], np.int32)
+        beam = OracleBeam(nodes, els, cons, iBodyModel=p.iBodyModel, isMotionGiven=p.isMotionGiven,
                               denR=p.denR, KB=p.KB, KS=p.KS, psR=p.psR, Freq=p.Freq, XYZo=p.XYZo, XYZAmpl=p.XYZAmpl,
                               XYZPhi=p.XYZPhi, AoAo=p.AoAo, AoAAmpl=p.AoAAmpl, AoAPhi=p.AoAPhi,
                               initXYZVel=p.initXYZVel, Lref=case.Lref, Uref=d["Uref"], denIn=case.denIn,
                               dampK=case.dampK, dampM=case.dampM, GeoGamma=case.GeoGamma,
                               NewmarkGamma=case.NewmarkGamma, NewmarkBeta=case.NewmarkBeta, dtolFEM=case.dtolFEM,
-                              ntolFEM=case.ntolFEM, IBPenaltyAlpha=case.IBPenaltyAlpha))
+                              ntolFEM=case.ntolFEM, IBPenaltyAlpha=case.IBPenaltyAlpha)
+        if isinstance(p, SurfacePlate):
+            beam = OracleSurface(beam, tri, xyz, case.IBPenaltyAlpha, case.denIn)
+        out.append(beam)
     return out
 
 

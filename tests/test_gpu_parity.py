@@ -37,7 +37,8 @@ def test_device_matches_reference_fixture(pc):
         assert n == 0, f"{pc.name}: {n} values differ from the reference, max |diff| {m:.3e}"
 
 
-@pytest.mark.parametrize("name", ["ibm_static_plate", "ibm_heaving_plate", "ibm_pitching_two_plates"])
+@pytest.mark.parametrize("name", ["ibm_static_plate", "ibm_heaving_plate", "ibm_pitching_two_plates",
+                                  "ibm_surface_sphere"])
 def test_device_ibm_forces_match_oracle(name):
     """Marker forces v_Eforce, beam loads lodFlow and the iteration count of
     every step equal the oracle's exactly."""
