@@ -18,7 +18,8 @@ CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libfsil.so")
 SOURCES = ["fsil_capi.hip"]
-HEADERS = ["fsil_common.h", "fsil_lbm_kernels.h", "fsil_lbm32_kernels.h", "fsil_ibm_kernels.h"]
+HEADERS = ["fsil_common.h", "fsil_lbm_kernels.h", "fsil_lbm32_kernels.h", "fsil_ibm_kernels.h", "fsil_beam.h",
+           "fsil_fem_kernels.h", "fsil_tree_kernels.h", "fsil_mlp_kernels.h"]
 
 
 def hipcc() -> str:

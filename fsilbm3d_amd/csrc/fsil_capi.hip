@@ -246,6 +246,7 @@ struct Block {
   double tau = 1, Omega = 1, Omega2 = 1;
   int precision = 0;  // 0 FP64 plain, 1 FP32 deviation storage
   int nt = 0;         // population streams nontemporal (see stream_policy)
+  int zrow = 0;       // FP32 row exchange of the z-moving values (zrow_exchange), fixed at creation
   bool nt_forced = false;  // FSIL_STREAM_HINT set at creation
   DevBuf<double> fA, fB;
   DevBuf<float> gA, gB;
@@ -484,7 +485,8 @@ dim3 cell_block32(const Block &b) {
 
 // The FP32-deviation kernel exchanges the z-moving values of a row through LDS when a
 // workgroup is whole z-rows and the grid has no idle threads (every thread
-// reaches the exchange's barrier); FSIL_ZROW=0 turns it off.
+// reaches the exchange's barrier); FSIL_ZROW=0 turns it off.  Resolved once at
+// block creation, so the launch and the cache policy agree.
 int zrow_exchange(const Block &b, dim3 blk) {
   const char *e = getenv("FSIL_ZROW");
   if (e && !strcmp(e, "0")) return 0;
@@ -624,7 +626,7 @@ int collide_stream32(Block &b) {
   }
   const dim3 blk = cell_block32(b);
   const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
-  a.zrow = zrow_exchange(b, blk);
+  a.zrow = b.zrow;
   const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
 #define L32(F, B)                                                                                      \
   if (a.nt) hipLaunchKernelGGL((k_collide_push32<F, B, true>), grid, blk, lds, b.stream, a);             \
@@ -897,7 +899,8 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   b.model = iCollidModel;
   b.tau = relax[0]; b.Omega = relax[1]; b.Omega2 = relax[2];
   b.precision = precision;
-  b.nt = stream_policy(b.N, precision, precision == 1 && zrow_exchange(b, cell_block32(b)));
+  b.zrow = precision == 1 ? zrow_exchange(b, cell_block32(b)) : 0;
+  b.nt = stream_policy(b.N, precision, b.zrow != 0);
   b.nt_forced = getenv("FSIL_STREAM_HINT") != nullptr;
   b.sh = std::make_shared<StreamHolder>();
   HIPCHK(hipStreamCreateWithFlags(&b.sh->s, hipStreamNonBlocking));

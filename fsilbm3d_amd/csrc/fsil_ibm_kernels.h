@@ -44,6 +44,8 @@ enum IbmCtl {
   kCtlWords = 20
 };
 
+static_assert(kErr == kBandErrWord, "the fused kernels read the coupling error word at kBandErrWord");
+
 // 4-point Peskin kernel (Solidbody.f90:822-833), FP64; sqrt is correctly rounded
 __host__ __device__ inline double peskin4(double x) {
   const double r = fabs(x);

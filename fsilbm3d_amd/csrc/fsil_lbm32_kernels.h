@@ -111,7 +111,7 @@ __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   bool forced = FORCE;
   float F1 = a.VF[0], F2 = a.VF[1], F3 = a.VF[2];
   if constexpr (BAND) {
-    if (x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] && z <= a.ctl[5]) {
+    if (a.ctl[kBandErrWord] == kBandNoErr && x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] && z <= a.ctl[5]) {
       const int b = a.cellmap[n];
       if (b >= 0) {
         u1 = (float)a.band_u[b];

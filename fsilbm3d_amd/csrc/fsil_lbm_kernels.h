@@ -68,6 +68,12 @@ __device__ __forceinline__ void moments(const double (&f)[Q], double &rho, doubl
   m3 = m3 - f[14]; m3 = m3 + f[15]; m3 = m3 + f[16]; m3 = m3 - f[17]; m3 = m3 - f[18];
 }
 
+// The coupling call's error word in its control words (IbmCtl::kErr): while it
+// holds an error (a call that did not wait, whose error the host reports at
+// its next call), the fused kernels leave the band overlay out.
+constexpr int kBandErrWord = 7;
+constexpr int kBandNoErr = 0x7fffffff;
+
 // ---------------------------------------------------------------------------
 // Arguments of the fused collide + push-stream kernel.
 struct CollideArgs {
@@ -173,7 +179,7 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
   bool forced = FORCE;
   double F1 = a.VF[0], F2 = a.VF[1], F3 = a.VF[2];
   if constexpr (BAND) {
-    if (x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] &&
+    if (a.ctl[kBandErrWord] == kBandNoErr && x >= a.ctl[0] && x <= a.ctl[3] && y >= a.ctl[1] && y <= a.ctl[4] && z >= a.ctl[2] &&
         z <= a.ctl[5]) {
       const int b = a.cellmap[n];
       if (b >= 0) {

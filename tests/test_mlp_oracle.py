@@ -58,3 +58,20 @@ def test_training_reduces_loss():
         l, g = MO.loss_grad(th, X, Y)
         th, m, v = MO.adam_step(th, g, m, v, t, 3e-3)
     assert MO.loss_grad(th, X, Y)[0] < 0.1 * l0
+
+
+def test_init_is_glorot_uniform():
+    """Each weight matrix lies within the Glorot-uniform bound sqrt(6/(fan_in+fan_out))
+    and fills it (max close to the bound), and every bias is zero."""
+    import numpy as np
+    from oracle.mlp_oracle import SIZES, init
+    th = init(7)
+    k = 0
+    for o, i in SIZES:
+        W = th[k:k + o * i]
+        lim = np.sqrt(6.0 / (o + i))
+        assert np.abs(W).max() <= lim and np.abs(W).max() > 0.9 * lim
+        assert abs(W.mean()) < 0.2 * lim
+        assert np.all(th[k + o * i:k + o * i + o] == 0.0)
+        k += o * i + o
+    assert k == th.size
