@@ -61,6 +61,7 @@ SIGNATURES = {
     "fsil_mlp_train": (_i, [_i, _pf, _pf, _i, _i, _i, _d, _i, _pd, _pd]),
     "fsil_ibm_set_warm_start": (_i, [_i, _i, _i, _i]),
     "fsil_ibm_rounds": (_i, [_i, _pi]),
+    "fsil_fsi_step": (_i, [_i, _d, _i, _pi, _pd]),
     "fsil_set_viscosity": (_i, [_i, _d]),
     "fsil_flow_pack": (_i, [_i, _i, _d, _d, _pf]),
     "fsil_write_flow": (_i, [_i, _i, _d, _d, C.c_char_p]),

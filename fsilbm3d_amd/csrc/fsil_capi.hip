@@ -3,7 +3,6 @@
 // Declarations and the reference routine each entry point replaces are in
 // include/fsilbm3d_amd.h.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -102,10 +101,10 @@ hipError_t step_wait(hipEvent_t ev) {
   return hipEventSynchronize(ev);
 }
 
-// Pinned host copy of a coupling call's control words, written by k_ibm_report.
+// Pinned host copy of a coupling call's control words, written by k_ibm_finish.
 struct PinnedReport {
   IbmReport *h = nullptr, *d = nullptr;  // host pointer and its device address
-  hipEvent_t ev = nullptr;               // recorded after k_ibm_report
+  hipEvent_t ev = nullptr;               // recorded after k_ibm_finish
   bool pending = false;                  // a report the host has not checked yet
   PinnedReport() = default;
   PinnedReport(const PinnedReport &) = delete;
@@ -138,9 +137,7 @@ struct Ibm {
   double dtol = 0.0, Uref = 1.0, denIn = 1.0;
   // device: markers and forces
   DevBuf<double> Exyz, Evel, Ea, Eforce, fT, tolpart, elem_xc, lod;
-  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_kind, dof_of_entry, first_marker;
-  DevBuf<double> load_term;                  // one term per dof_list entry
-  int nlist = 0;
+  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_kind, first_marker;
   // device: stencils and band (built each call, see fsil_ibm_kernels.h)
   int cap = 0;                               // 64*M, stride of the band arrays
   DevBuf<float> dEw;                         // 12*M
@@ -150,8 +147,6 @@ struct Ibm {
   DevBuf<int> flags, scan, dentry, dbidx, dptr, ctl, cellmap;
   DevBuf<long long> dcell;
   DevBuf<double> band_u, band_F, dstate;
-  DevBuf<char> cub_tmp;
-  size_t cub_bytes = 0;
   std::unique_ptr<PinnedReport> rep = std::make_unique<PinnedReport>();
   int ndof = 0;
   bool active = false;  // band valid for the next collision
@@ -1353,12 +1348,6 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   }
   if (upload(I.dof_ptr, ptr.data(), ptr.size(), b->stream)) return 1;
   if (upload(I.dof_list, lst.data(), lst.size(), b->stream)) return 1;
-  std::vector<int> owner(lst.size());
-  for (int d = 0; d < I.ndof; ++d)
-    for (int j = ptr[d]; j < ptr[d + 1]; ++j) owner[j] = d;
-  if (upload(I.dof_of_entry, owner.data(), owner.size(), b->stream)) return 1;
-  I.nlist = (int)lst.size();
-  HIPCHK(I.load_term.ensure(std::max<size_t>(1, lst.size())));
   if (upload(I.dof_kind, kind.data(), kind.size(), b->stream)) return 1;
   if (upload(I.dmarker_elem, I.marker_elem.data(), I.marker_elem.size(), b->stream)) return 1;
   HIPCHK(I.Exyz.ensure(3 * (size_t)I.M));
@@ -1409,12 +1398,6 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   }
   HIPCHK(hipMemsetAsync(I.ctl.p, 0, kCtlWords * sizeof(int), b->stream));
   HIPCHK(hipMemsetAsync(I.cellmap.p, 0xff, (size_t)b->N * sizeof(int), b->stream));  // -1
-  if (E > 0) {
-    size_t t2 = 0;
-    HIPCHK(hipcub::DeviceScan::InclusiveSum(nullptr, t2, I.scan.p, I.dptr.p + 1, E, b->stream));
-    I.cub_bytes = t2;
-    HIPCHK(I.cub_tmp.ensure(std::max<size_t>(1, I.cub_bytes)));
-  }
   HIPCHK(I.rep->ensure());
   HIPCHK(hipStreamSynchronize(b->stream));
   return 0;
@@ -1445,7 +1428,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   hipStream_t s = b->stream;
   const int M = I.M, E = 64 * M;
-  if (fem_order_after(*b)) return 1;  // k_loads rewrites the loads the beam sub-steps read
+  if (fem_order_after(*b)) return 1;  // k_ibm_finish rewrites the loads the beam sub-steps read
   if (!Exyz) {
     // markers computed on the device by fsil_fem_update_markers
     if (!b->fem.markers_on_device) return fail("no marker arrays given and no device markers computed");
@@ -1501,8 +1484,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   hipLaunchKernelGGL(k_ibm_count, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys.p, I.cellmap.p, I.dbidx.p,
                      I.scan.p, I.flags.p, I.ctl.p, ma);
   HIPCHK(hipGetLastError());
-  size_t tb = I.cub_bytes;
-  HIPCHK(hipcub::DeviceScan::InclusiveSum(I.cub_tmp.p, tb, I.scan.p, I.dptr.p + 1, E, s));
+  hipLaunchKernelGGL(k_band_scan, dim3(1), dim3(1024), 0, s, I.ctl.p, I.scan.p, I.dptr.p);
   hipLaunchKernelGGL(k_ibm_fill, dim3((E + 255) / 256), dim3(256), 0, s, E, I.dbidx.p, I.flags.p, I.dptr.p,
                      I.tmp.p);
   hipLaunchKernelGGL(k_ibm_rank, dim3((E + 255) / 256), dim3(256), 0, s, E, I.ctl.p, I.dbidx.p, I.dptr.p, I.tmp.p,
@@ -1563,22 +1545,18 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     HIPCHK(hipGetLastError());
   }
   I.have_prev = true;
-  const double invh3s = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
-  hipLaunchKernelGGL(k_spread, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, I.cap, I.dptr.p, I.dentry.p,
-                     I.dEw.p, I.Eforce.p, invh3s, b->VF[0], b->VF[1], b->VF[2], I.band_F.p);
-  HIPCHK(hipGetLastError());
-  if (I.ndof > 0) {
-    if (I.nlist > 0) {
-      hipLaunchKernelGGL(k_load_terms, dim3((I.nlist + 255) / 256), dim3(256), 0, s, I.nlist, I.dof_of_entry.p,
-                         I.dof_list.p, I.dof_kind.p, I.dmarker_elem.p, I.elem_xc.p, I.Exyz.p, I.Eforce.p,
-                         I.load_term.p);
-      HIPCHK(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_loads, dim3((I.ndof + 255) / 256), dim3(256), 0, s, I.ndof, I.dof_ptr.p, I.load_term.p,
-                       I.lod.p);
-    HIPCHK(hipGetLastError());
-  }
-  hipLaunchKernelGGL(k_ibm_report, dim3(1), dim3(64), 0, s, I.ctl.p, I.dstate.p, I.rep->d);
+  // spreading (Solidbody.f90:920-979), beam loads and the report in one launch
+  IbmFinishArgs fa{};
+  fa.ctl = I.ctl.p; fa.dstate = I.dstate.p; fa.rep = I.rep->d;
+  fa.cap = I.cap; fa.ptr = I.dptr.p; fa.entry = I.dentry.p; fa.Ew = I.dEw.p; fa.Eforce = I.Eforce.p;
+  fa.invh3 = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
+  fa.VF1 = b->VF[0]; fa.VF2 = b->VF[1]; fa.VF3 = b->VF[2];
+  fa.band_F = I.band_F.p;
+  fa.ndof = I.ndof; fa.dof_blocks = (I.ndof + 3) / 4;
+  fa.dof_ptr = I.dof_ptr.p; fa.dof_list = I.dof_list.p; fa.dof_kind = I.dof_kind.p;
+  fa.marker_elem = I.dmarker_elem.p; fa.elem_xc = I.elem_xc.p; fa.Exyz = I.Exyz.p;
+  fa.lod = I.lod.p;
+  hipLaunchKernelGGL(k_ibm_finish, dim3(fa.dof_blocks + (E + 255) / 256), dim3(256), 0, s, fa);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(I.rep->ev, s));
   I.rep->pending = true;
@@ -1977,6 +1955,8 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
       c[6] = subdt * (1.0 - F.NewmarkGamma);
       c[7] = subdt * F.NewmarkGamma;
       d.ntolFEM = F.ntolFEM; d.dtolFEM = F.dtolFEM; d.dampK = F.dampK; d.dampM = F.dampM; d.GeoGamma = F.GeoGamma;
+      // FP32-deviation blocks are held to a tolerance: their CG sums its dot products in a tree
+      d.tree_sums = b->precision == 1 ? 1 : 0;
       d.newton_iters = 0; d.cg_iters = 0; d.status = 0; d.dnorm = 0.0;
     }
   // the descriptors go up first (the previous sub-steps are done with the
@@ -1993,6 +1973,26 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   S.n_staged = (int)n;
   S.pending = true;
   return 0;
+}
+
+int fsil_fsi_step(int h, double time, int numsubstep, int *iters, double *dmax) {
+  Block *b = get_block(h);
+  if (!b) return 1;
+  if (b->fem.nbodies <= 0) return fail("fsil_fsi_step needs beam bodies on block %d", h);
+  // main.f90:95-97: block time and volume force
+  b->blktime = time;
+  update_vf(*b);
+  // IBM_FEM (LBMBlockComm.f90:320-338): markers, the coupling call queued
+  // without a host wait, the beam sub-steps queued behind it, then the wait
+  // for the call's report, so the beam workgroup is resident before the
+  // fluid update's kernels fill the device
+  if (fsil_fem_update_markers(h)) return 1;
+  if (fsil_ibm_interaction_force(h, b->dh, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr))
+    return 1;
+  if (fsil_fem_substeps(h, time, numsubstep, b->dh)) return 1;
+  if (fsil_ibm_report(h, iters, dmax)) return 1;
+  // collision, streaming and boundary faces (LBMBlockComm.f90:285-304)
+  return fsil_collide_stream_bc(h);
 }
 
 int fsil_fem_state(int h, int body, double *pos, double *vel, double *acc, double *dsp) {

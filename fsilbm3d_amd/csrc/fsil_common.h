@@ -8,6 +8,14 @@ namespace fsil {
 
 constexpr int Q = 19;
 
+// barrier of the lanes of one wave (they run in lockstep; the fences keep the
+// compiler from moving LDS accesses across it)
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // D3Q19 lattice, in the reference's direction order (ConstParams.f90:11-25).
 __host__ __device__ constexpr int ex(int q) {
   constexpr int e[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};

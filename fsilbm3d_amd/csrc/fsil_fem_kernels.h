@@ -21,7 +21,8 @@ namespace fsil {
 
 struct FemBodyDesc {
   int nND, nEL, eoff, goff, noff;   // element / dof / node offsets into the global arrays
-  int iBodyModel, ntolFEM, pad;
+  int iBodyModel, ntolFEM;
+  int tree_sums;                    // 1: CG dot products summed in a tree (FP32-deviation blocks), 0: in DOF order
   double dtolFEM, dampK, dampM, GeoGamma;
   double coeffs[8];
   double XYZ[3], AoAd[3], T[9];     // prescribed placement of this sub-step (host libm)
@@ -148,17 +149,48 @@ __device__ inline void fem_matvec(const FemArrays &A, const FemBodyDesc &D, cons
   fem_sync();
 }
 
-// barrier of the lanes of one wave (they run in lockstep; the fences keep the
-// compiler from moving LDS accesses across it)
-__device__ inline void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Sums of t1[0..n) and t2[0..n) (t2 may be null) by the whole workgroup in a
+// fixed tree order: per-thread strided partial sums, wave sums by lane
+// shuffles, then the waves' sums in wave order, which every thread adds
+// itself (a broadcast without a second barrier).  The forms whose results are
+// held to a tolerance rather than to the reference's rounding (FP32-deviation
+// blocks) use it for the conjugate-gradient dot products; the caller passes a
+// block barrier before the next call.
+__device__ inline void fem_tree_sums(const double *t1, const double *t2, int n, double &r1, double &r2) {
+  __shared__ double part[2][16];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a = a + t1[i];
+    if (t2) b = b + t2[i];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = a + __shfl_down(a, off, 64);
+    if (t2) b = b + __shfl_down(b, off, 64);
+  }
+  const int nw = ((int)blockDim.x + 63) >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    part[0][threadIdx.x >> 6] = a;
+    part[1][threadIdx.x >> 6] = b;
+  }
+  fem_sync();
+  double s1 = 0.0, s2 = 0.0;
+  for (int w = 0; w < nw; ++w) {
+    s1 = s1 + part[0][w];
+    s2 = s2 + part[1][w];
+  }
+  r1 = s1;
+  r2 = s2;
 }
 
-// sum of terms[0..n) in order by one thread, broadcast.  The caller passes
-// a block barrier before *shared is written again.
-__device__ inline double fem_serial_sum(const double *terms, int n, double *shared) {
+// sum of terms[0..n) in order by one thread (tree: fem_tree_sums), broadcast.
+// The caller passes a block barrier before *shared is written again.
+__device__ inline double fem_serial_sum(const double *terms, int n, double *shared, int tree = 0) {
+  if (tree) {
+    double r1, r2;
+    fem_tree_sums(terms, nullptr, n, r1, r2);
+    return r1;
+  }
   if (threadIdx.x == 0) *shared = ordered_sum(0.0, terms, n);
   fem_sync();
   return *shared;
@@ -188,7 +220,12 @@ __device__ inline void fem_precond(const FemArrays &A, const FemBodyDesc &D, con
 
 // the two ordered sums of prod[0..n) and prod[cap..cap+n) on two waves,
 // broadcast.  The caller passes a block barrier before shared is written again.
-__device__ inline void fem_pair_sums(const double *prod, int cap, int n, double *shared, double &r1, double &r2) {
+__device__ inline void fem_pair_sums(const double *prod, int cap, int n, double *shared, double &r1, double &r2,
+                                     int tree = 0) {
+  if (tree) {
+    fem_tree_sums(prod, prod + cap, n, r1, r2);
+    return;
+  }
   if (threadIdx.x == 0) shared[0] = ordered_sum(0.0, prod, n);
   else if (threadIdx.x == 64) shared[1] = ordered_sum(0.0, prod + cap, n);
   fem_sync();
@@ -202,7 +239,26 @@ __device__ inline void fem_pair_sums(const double *prod, int cap, int n, double 
 // is only the chain of additions.  fem_dot2 runs two such chains side by side
 // on two waves.
 __device__ inline void fem_dot_chunks(const double *a, const double *b, const double *c, const double *d, int n,
-                                      double *shared, double *stage, int cap, double &r1, double &r2) {
+                                      double *shared, double *stage, int cap, double &r1, double &r2, int tree = 0) {
+  if (tree) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int base = 0; base < n; base += cap) {
+      const int m = min(cap, n - base);
+      for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        stage[i] = a[base + i] * b[base + i];
+        if (c) stage[cap + i] = c[base + i] * d[base + i];
+      }
+      fem_sync();
+      double t1, t2;
+      fem_tree_sums(stage, c ? stage + cap : nullptr, m, t1, t2);
+      s1 = s1 + t1;
+      s2 = s2 + t2;
+      fem_sync();
+    }
+    r1 = s1;
+    r2 = s2;
+    return;
+  }
   double s1 = 0.0, s2 = 0.0;
   for (int base = 0; base < n; base += cap) {
     const int m = min(cap, n - base);
@@ -230,8 +286,8 @@ __device__ inline double fem_dot(const double *a, const double *b, int n, double
 }
 
 __device__ inline void fem_dot2(const double *a, const double *b, const double *c, const double *d, int n,
-                                double *shared, double *stage, int cap, double &r1, double &r2) {
-  fem_dot_chunks(a, b, c, d, n, shared, stage, cap, r1, r2);
+                                double *shared, double *stage, int cap, double &r1, double &r2, int tree = 0) {
+  fem_dot_chunks(a, b, c, d, n, shared, stage, cap, r1, r2, tree);
 }
 
 // LDS bytes of k_fem_substep: lds_nodes work areas of 6x12 doubles for the
@@ -522,7 +578,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     const double err = 1.0e-6;
     const int cap = nG;  // product staging: [0, nG) and [nG, 2 nG)
     double rr, rz;
-    fem_dot2(V.r, V.r, V.r, V.z, nG, sh_pair, lds, cap, rr, rz);
+    fem_dot2(V.r, V.r, V.r, V.z, nG, sh_pair, lds, cap, rr, rz, D.tree_sums);
     double rn = sqrt(rr);
     if (!(rn <= err)) {
       for (int g = threadIdx.x; g < nG; g += blockDim.x) V.p[g] = V.z[g];
@@ -533,7 +589,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
         ++cg_total;
         // A p with the fixed DOFs zeroed, and the products p_g (A p)_g
         fem_matvec(A, D, V, C, V.p, V.Ap, true, lds);
-        const double pAp = fem_serial_sum(lds, nG, sh_pair);
+        const double pAp = fem_serial_sum(lds, nG, sh_pair, D.tree_sums);
         if (fabs(pAp) <= 1.0e-30) { status = 2; break; }
         const double alpha = rsold / pAp;
         // x, r update, then z = M^-1 r (formed before the convergence test,
@@ -570,7 +626,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
         }
         fem_sync();
         double rsnew;
-        fem_pair_sums(lds, cap, nG, sh_pair, rr, rsnew);
+        fem_pair_sums(lds, cap, nG, sh_pair, rr, rsnew, D.tree_sums);
         rn = sqrt(rr);
         if (rn <= err) break;
         if (fabs(rsold) <= 1.0e-30) { status = 2; break; }

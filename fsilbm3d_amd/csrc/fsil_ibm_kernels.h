@@ -602,80 +602,33 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
   band_u[2LL * cap + b] = u3;
 }
 
-// Eulerian force of every band cell: (0 - sum F/h^3*rx*ry*rz) + VF
-// (ResetVolumeForce_ FluidDomain.f90:1195-1206, FluidVolumeForce_
-// Solidbody.f90:968-977, add_volume_force_ FluidDomain.f90:1182-1193)
-__global__ __launch_bounds__(256) void k_spread(const int *__restrict__ ctl, int cap,
-                                                const int *__restrict__ ptr,
-                                                const int *__restrict__ entry,
-                                                const float *__restrict__ Ew,
-                                                const double *__restrict__ Eforce, double invh3,
-                                                double VF1, double VF2, double VF3,
-                                                double *__restrict__ band_F) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ctl[kNband]) return;
-  double g1 = 0.0, g2 = 0.0, g3 = 0.0;
-  const int k1 = ptr[b + 1];
-  for (int k = ptr[b]; k < k1;) {
-    BandChunk c;
-    const int n = band_chunk(k, k1, entry, Ew, Eforce, c);
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j < n) {
-        const double h1 = c.v1[j] * invh3, h2 = c.v2[j] * invh3, h3 = c.v3[j] * invh3;
-        g1 = g1 + (-(((h1 * c.rx[j]) * c.ry[j]) * c.rz[j]));
-        g2 = g2 + (-(((h2 * c.rx[j]) * c.ry[j]) * c.rz[j]));
-        g3 = g3 + (-(((h3 * c.rx[j]) * c.ry[j]) * c.rz[j]));
-      }
-    k += n;
+// Inclusive scan of the per-cell entry counts of the band's cells (ctl[kNew],
+// final after k_ibm_claim) into the CSR pointers ptr[1..] (ptr[0] = 0), one
+// 1024-thread workgroup:
+// contiguous per-thread runs, a workgroup scan of the run totals, then the
+// runs written with their offsets.
+__global__ __launch_bounds__(1024) void k_band_scan(const int *__restrict__ ctl, const int *__restrict__ count,
+                                                     int *__restrict__ ptr) {
+  __shared__ int tot[1024];
+  // the cells claimed in this call (k_ibm_rank copies this count to kNband only later)
+  const int n = ctl[kNew];
+  const int per = (n + 1023) / 1024;
+  const int t = threadIdx.x, lo = t * per, hi = min(n, lo + per);
+  int s = 0;
+  for (int i = lo; i < hi; ++i) s += count[i];
+  tot[t] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = t >= off ? tot[t - off] : 0;
+    __syncthreads();
+    tot[t] += v;
+    __syncthreads();
   }
-  band_F[b] = g1 + VF1;
-  band_F[cap + b] = g2 + VF2;
-  band_F[2LL * cap + b] = g3 + VF3;
-}
-
-// Beam nodal loads lodFlow: per degree of freedom, the sum over its markers in
-// marker order of 0.5*F (translational) or 0.5*(r x F) (rotational) with r the
-// arm from the element axis midpoint (Solidbody.f90:944-967).
-// dof_list entry: marker id; dof_kind[d] = component (0..2 force, 3..5 moment).
-// k_load_terms forms every term (one thread per list entry), k_loads adds
-// each DOF's terms in list order.
-__global__ __launch_bounds__(256) void k_load_terms(int nlist, const int *__restrict__ dof_of_entry,
-                                                    const int *__restrict__ dof_list,
-                                                    const int *__restrict__ dof_kind,
-                                                    const int *__restrict__ marker_elem,
-                                                    const double *__restrict__ elem_xc,
-                                                    const double *__restrict__ Exyz,
-                                                    const double *__restrict__ Eforce,
-                                                    double *__restrict__ term) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nlist) return;
-  const int k = dof_kind[dof_of_entry[j]];
-  const int m = dof_list[j];
-  double val;
-  if (k < 3) {
-    val = Eforce[3LL * m + k];
-  } else {
-    const int el = marker_elem[m];
-    const double r1 = Exyz[3LL * m] - elem_xc[3LL * el];
-    const double r2 = Exyz[3LL * m + 1] - elem_xc[3LL * el + 1];
-    const double r3 = Exyz[3LL * m + 2] - elem_xc[3LL * el + 2];
-    const double F1 = Eforce[3LL * m], F2 = Eforce[3LL * m + 1], F3 = Eforce[3LL * m + 2];
-    if (k == 3) val = r2 * F3 - r3 * F2;
-    else if (k == 4) val = r3 * F1 - r1 * F3;
-    else val = r1 * F2 - r2 * F1;
+  int run = t ? tot[t - 1] : 0;
+  for (int i = lo; i < hi; ++i) {
+    run += count[i];
+    ptr[i + 1] = run;
   }
-  term[j] = 0.5 * val;
-}
-
-__global__ __launch_bounds__(256) void k_loads(int ndof, const int *__restrict__ dof_ptr,
-                                               const double *__restrict__ term, double *__restrict__ lod) {
-  const int d = blockIdx.x * blockDim.x + threadIdx.x;
-  if (d >= ndof) return;
-  double acc = 0.0;
-#pragma unroll 8
-  for (int j = dof_ptr[d]; j < dof_ptr[d + 1]; ++j) acc = acc + term[j];
-  lod[d] = acc;
 }
 
 // The control words and the last residual of a coupling call, written into
@@ -685,11 +638,95 @@ struct IbmReport {
   double dmax;
 };
 
-__global__ __launch_bounds__(64) void k_ibm_report(const int *__restrict__ ctl, const double *__restrict__ dstate,
-                                                   IbmReport *__restrict__ out) {
-  const int t = threadIdx.x;
-  if (t < kCtlWords) out->ctl[t] = ctl[t];
-  if (t == 0) out->dmax = dstate[0];
+// The end of a coupling call in one launch: the beam loads, the spreading and
+// the report.
+// Beam nodal loads lodFlow (Solidbody.f90:944-967): per degree of freedom, the
+// sum over its markers in marker order of 0.5*F (translational) or
+// 0.5*(r x F) (rotational) with r the arm from the element axis midpoint;
+// dof_list entry: marker id, dof_kind[d] = component (0..2 force, 3..5
+// moment).  The leading dof_blocks workgroups form them with one wave per
+// degree of freedom: the lanes form the DOF's terms into LDS, then lane 0 adds
+// them in list order.
+// Eulerian force of every band cell (ResetVolumeForce_ FluidDomain.f90:
+// 1195-1206, FluidVolumeForce_ Solidbody.f90:968-977, add_volume_force_
+// FluidDomain.f90:1182-1193): (0 - sum F/h^3*rx*ry*rz) + VF, by the other
+// workgroups, the first of which copies the control words into the report
+// (the last k_correct has made the stop test, so they are final).
+constexpr int kLoadChunk = 256;
+
+struct IbmFinishArgs {
+  const int *ctl;
+  const double *dstate;
+  IbmReport *rep;
+  int cap;
+  const int *ptr, *entry;
+  const float *Ew;
+  const double *Eforce;
+  double invh3, VF1, VF2, VF3;
+  double *band_F;
+  int ndof, dof_blocks;
+  const int *dof_ptr, *dof_list, *dof_kind, *marker_elem;
+  const double *elem_xc, *Exyz;
+  double *lod;
+};
+
+__device__ __forceinline__ double load_term(const IbmFinishArgs &a, int kind, int m) {
+  if (kind < 3) return 0.5 * a.Eforce[3LL * m + kind];
+  const int el = a.marker_elem[m];
+  const double r1 = a.Exyz[3LL * m] - a.elem_xc[3LL * el];
+  const double r2 = a.Exyz[3LL * m + 1] - a.elem_xc[3LL * el + 1];
+  const double r3 = a.Exyz[3LL * m + 2] - a.elem_xc[3LL * el + 2];
+  const double F1 = a.Eforce[3LL * m], F2 = a.Eforce[3LL * m + 1], F3 = a.Eforce[3LL * m + 2];
+  double val;
+  if (kind == 3) val = r2 * F3 - r3 * F2;
+  else if (kind == 4) val = r3 * F1 - r1 * F3;
+  else val = r1 * F2 - r2 * F1;
+  return 0.5 * val;
+}
+
+__global__ __launch_bounds__(256) void k_ibm_finish(IbmFinishArgs a) {
+  if ((int)blockIdx.x < a.dof_blocks) {
+    __shared__ double term[4][kLoadChunk];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int d = blockIdx.x * 4 + w;
+    if (d >= a.ndof) return;  // whole waves; no workgroup barrier follows
+    const int kind = a.dof_kind[d];
+    const int j0 = a.dof_ptr[d], j1 = a.dof_ptr[d + 1];
+    double acc = 0.0;
+    for (int base = j0; base < j1; base += kLoadChunk) {
+      const int n = min(kLoadChunk, j1 - base);
+      for (int q = lane; q < n; q += 64) term[w][q] = load_term(a, kind, a.dof_list[base + q]);
+      wave_sync();
+      if (lane == 0)
+        for (int q = 0; q < n; ++q) acc = acc + term[w][q];
+      wave_sync();
+    }
+    if (lane == 0) a.lod[d] = acc;
+    return;
+  }
+  const int blk = (int)blockIdx.x - a.dof_blocks;
+  if (blk == 0 && threadIdx.x < kCtlWords) a.rep->ctl[threadIdx.x] = a.ctl[threadIdx.x];
+  if (blk == 0 && threadIdx.x == 0) a.rep->dmax = a.dstate[0];
+  const int b = blk * blockDim.x + threadIdx.x;
+  if (b >= a.ctl[kNband]) return;
+  double g1 = 0.0, g2 = 0.0, g3 = 0.0;
+  const int k1 = a.ptr[b + 1];
+  for (int k = a.ptr[b]; k < k1;) {
+    BandChunk c;
+    const int n = band_chunk(k, k1, a.entry, a.Ew, a.Eforce, c);
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (j < n) {
+        const double h1 = c.v1[j] * a.invh3, h2 = c.v2[j] * a.invh3, h3 = c.v3[j] * a.invh3;
+        g1 = g1 + (-(((h1 * c.rx[j]) * c.ry[j]) * c.rz[j]));
+        g2 = g2 + (-(((h2 * c.rx[j]) * c.ry[j]) * c.rz[j]));
+        g3 = g3 + (-(((h3 * c.rx[j]) * c.ry[j]) * c.rz[j]));
+      }
+    k += n;
+  }
+  a.band_F[b] = g1 + a.VF1;
+  a.band_F[a.cap + b] = g2 + a.VF2;
+  a.band_F[2LL * a.cap + b] = g3 + a.VF3;
 }
 
 // ---------------------------------------------------------------------------

@@ -84,6 +84,8 @@ class DeviceBodyBlock:
         self.iters = 0
         self.rounds = 0
         self.dmax = 0.0
+        self._it = np.zeros(1, np.int32)
+        self._dm = np.zeros(1)
 
     def interaction_force(self, dt: float, wait: bool = True):
         """The coupling call; with wait=False nothing is read back (no host
@@ -186,7 +188,20 @@ class DeviceBodyBlock:
             self.report()
 
     def step(self, time: float, wait: bool = True):
+        """One time step (main.f90:95-107): with wait, the library's one-call
+        step (fsil_fsi_step); without, the calls one by one with a coupling
+        call that reads nothing back."""
         blk = self.block
+        if wait:
+            it = self._it
+            dm = self._dm
+            _lib.check(self.lib.fsil_fsi_step(blk.handle, float(time), self.numsubstep, _lib.iptr(it), _lib.dptr(dm)))
+            blk.blktime = float(time)
+            self.iters, self.dmax = int(it[0]), float(dm[0])
+            _lib.check(self.lib.fsil_ibm_rounds(blk.handle, _lib.iptr(it)))
+            self.rounds = int(it[0])
+            self._stale = True
+            return
         blk.update_volume_force(time)
         self.ibm_fem(time, wait)
         blk.collision_streaming_bc()

@@ -22,6 +22,7 @@ module fsil_bridge
               fsil_mlp_set_params, fsil_mlp_forward, fsil_mlp_loss_grad, fsil_mlp_train, fsil_ibm_set_warm_start, &
               fsil_ibm_rounds, fsil_solid_info, fsil_solid_info_static, fsil_set_viscosity, fsil_tau_download, &
               fsil_flow_pack, fsil_write_flow, fsil_write_wait, fsil_write_continue, fsil_fem_checkpoint, fsil_fem_restore, &
+              fsil_fsi_step, &
               fsil_ibm_warm_start_data, fsil_ibm_forces, &
               fsil_ibm_report
 
@@ -336,6 +337,13 @@ module fsil_bridge
             integer(c_int), value :: handle
             type(c_ptr), value :: buf
             integer(c_long_long), value :: nbytes
+        end function
+        integer(c_int) function fsil_fsi_step(handle, time, numsubstep, iters, dmax) bind(C, name='fsil_fsi_step')
+            import :: c_int, c_double
+            integer(c_int), value :: handle, numsubstep
+            real(c_double), value :: time
+            integer(c_int), intent(out) :: iters
+            real(c_double), intent(out) :: dmax
         end function
         integer(c_int) function fsil_ibm_rounds(handle, rounds) bind(C, name='fsil_ibm_rounds')
             import :: c_int

@@ -227,6 +227,14 @@ int fsil_solid_info(int handle, int body, const double *refs, double *rows);
  * is an error.  Calls that overlap on one device share that array and race. */
 int fsil_solid_info_static(int handle, int body, const double *refs, double *rows);
 
+/* One time step of a block whose bodies are on the device beam solver: the
+ * body of main.f90:95-107 for that block (tree_collision_streaming_IBM_FEM +
+ * IBM_FEM, LBMBlockComm.f90:279-338) in one call: block time and volume force,
+ * the device markers, the coupling call, numsubstep beam sub-steps of dh /
+ * numsubstep, and the fused collide/stream/boundary update.  iters and dmax
+ * (either may be NULL) receive the forcing loop's count and residual. */
+int fsil_fsi_step(int handle, double time, int numsubstep, int *iters, double *dmax);
+
 /* Node state of body b (6*nND each, node-major), and the last sub-step's
  * Newton iterations, conjugate-gradient iterations and displacement norm. */
 int fsil_fem_state(int handle, int body, double *pos, double *vel, double *acc, double *dsp);
