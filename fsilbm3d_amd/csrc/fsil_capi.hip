@@ -150,6 +150,8 @@ struct Ibm {
   std::unique_ptr<PinnedReport> rep = std::make_unique<PinnedReport>();
   int ndof = 0;
   bool active = false;  // band valid for the next collision
+  int box[6] = {0, 0, 0, 0, 0, 0};  // band box of the last checked call (kBoxLo..kBoxHi)
+  bool have_box = false;
   int warm_mlp = 0;     // network handle of the learned warm start (0: off)
   int record = 0;       // compute the network's features and targets every call
   int skip = 0;         // forcing rounds the learned guess stands for
@@ -266,6 +268,12 @@ struct Block {
   std::shared_ptr<StreamHolder> sh;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // split steps (fsil_fsi_step): the coupling runs on the beam stream while
+  // the bulk of the collision runs on the block's stream
+  hipEvent_t ev_fluid = nullptr;  // the block's stream before the bulk collision
+  bool split = false;             // the coupling's launches go to the beam stream
+  bool split_ok = true;           // FSIL_SPLIT=0 turns split steps off
+  int split_margin = 3;           // cells the predicted band box adds to the last one (FSIL_SPLIT_MARGIN)
 };
 
 std::map<int, std::unique_ptr<Block>> g_blocks;
@@ -281,6 +289,9 @@ int g_device = -1;
 constexpr int kStaticInfoElems = 256;
 static __device__ __align__(256) double scratch[3 * kStaticInfoElems + 78];
 std::map<int, double *> g_scratch;
+
+// stream of the coupling's launches: the beam stream during a split step
+hipStream_t coupling_stream(const Block &b) { return b.split ? b.fem.fs->s : b.stream; }
 
 Block *get_block(int h) {
   auto it = g_blocks.find(h);
@@ -530,7 +541,19 @@ void launch_collide_model(const CollideArgs &a, bool force, bool band, bool aux,
 #undef L
 }
 
-int collide_stream(Block &b) {
+// hull launch grid: the hole's row tiles (at least 256 workgroups); the
+// kernel strides over a larger hull
+unsigned hull_grid(const Block &b, const int *hole, dim3 blk) {
+  const long long nz = (b.zd + blk.x - 1) / blk.x;
+  const long long ny = (hole[3] - hole[2] + (int)blk.y) / (int)blk.y;
+  const long long t = (long long)(hole[1] - hole[0] + 1) * ny * nz;
+  return (unsigned)std::min<long long>(65535, std::max<long long>(256, t));
+}
+
+// the fused collide + push-stream launch of the whole block (region kRegionAll),
+// of the cells outside `hole` (kRegionBulk; no band overlay, no swap), or of
+// the hull of `hole` and the band box (kRegionHull)
+int collide_stream(Block &b, int region = kRegionAll, const int *hole = nullptr) {
   CollideArgs a{};
   a.fin = b.fA.p;
   a.fout = b.fB.p;
@@ -546,14 +569,20 @@ int collide_stream(Block &b) {
   a.Mc = b.Mc.p;
   a.Mf = b.Mf.p;
   const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
-  const bool band = b.ibm.active && b.ibm.M > 0;
-  a.nt = stream_hint_for(b, band);
+  const bool band = region != kRegionBulk && b.ibm.active && b.ibm.M > 0;
+  a.nt = stream_hint_for(b, band || region != kRegionAll);
   if (band) {
     a.ctl = b.ibm.ctl.p;
     a.cellmap = b.ibm.cellmap.p;
     a.band_u = b.ibm.band_u.p;
     a.band_F = b.ibm.band_F.p;
     a.cap = b.ibm.cap;
+  }
+  a.region = region;
+  if (region != kRegionAll) {
+    if (!hole || b.ibm.M == 0 || is_les(b.model)) return fail("collide region %d without a band", region);
+    for (int k = 0; k < 4; ++k) a.hole[k] = hole[k];
+    a.ctl = b.ibm.ctl.p;
   }
   int m102 = 0, mhw = 0;
   for (int face = 0; face < 6; ++face) {
@@ -567,7 +596,8 @@ int collide_stream(Block &b) {
   a.aux_face_mask_hw = mhw;
   const bool aux = (m102 | mhw) != 0;
   const dim3 blk = cell_block(b);
-  const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
+  const dim3 grid = region == kRegionHull ? dim3(hull_grid(b, hole, blk))
+                                          : dim3((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   if (is_les(b.model)) {
     a.tau_all = b.tau_all.p;
     a.uf = b.uf.p;
@@ -589,6 +619,7 @@ int collide_stream(Block &b) {
   else if (b.model == 14) launch_collide_model<14>(a, force, band, aux, grid, blk, b.stream);
   else launch_collide_model<15>(a, force, band, aux, grid, blk, b.stream);
   HIPCHK(hipGetLastError());
+  if (region == kRegionBulk) return 0;  // the hull launch completes the step
   for (int face = 0; face < 6; ++face)
     if ((mhw >> face) & 1) b.hw_valid[face] = true;
   std::swap(b.fA.p, b.fB.p);
@@ -596,8 +627,8 @@ int collide_stream(Block &b) {
   return 0;
 }
 
-// FP32-deviation form: all-periodic blocks (the stencil benchmark case)
-int collide_stream32(Block &b) {
+// FP32-deviation form; regions as collide_stream
+int collide_stream32(Block &b, int region = kRegionAll, const int *hole = nullptr) {
   Collide32Args a{};
   a.gin = b.gA.p;
   a.gout = b.gB.p;
@@ -610,8 +641,8 @@ int collide_stream32(Block &b) {
     a.VF[k] = (float)b.VF[k];
   }
   const bool force = (b.VF[0] != 0.0 || b.VF[1] != 0.0 || b.VF[2] != 0.0);
-  const bool band = b.ibm.active && b.ibm.M > 0;
-  a.nt = stream_hint_for(b, band);
+  const bool band = region != kRegionBulk && b.ibm.active && b.ibm.M > 0;
+  a.nt = stream_hint_for(b, band || region != kRegionAll);
   if (band) {
     a.ctl = b.ibm.ctl.p;
     a.cellmap = b.ibm.cellmap.p;
@@ -619,8 +650,15 @@ int collide_stream32(Block &b) {
     a.band_F = b.ibm.band_F.p;
     a.cap = b.ibm.cap;
   }
+  a.region = region;
+  if (region != kRegionAll) {
+    if (!hole || b.ibm.M == 0) return fail("collide region %d without a band", region);
+    for (int k = 0; k < 4; ++k) a.hole[k] = hole[k];
+    a.ctl = b.ibm.ctl.p;
+  }
   const dim3 blk = cell_block32(b);
-  const dim3 grid((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
+  const dim3 grid = region == kRegionHull ? dim3(hull_grid(b, hole, blk))
+                                          : dim3((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   a.zrow = b.zrow;
   const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
 #define L32(F, B)                                                                                      \
@@ -632,6 +670,7 @@ int collide_stream32(Block &b) {
   else { L32(false, false) }
 #undef L32
   HIPCHK(hipGetLastError());
+  if (region == kRegionBulk) return 0;  // the hull launch completes the step
   std::swap(b.gA.p, b.gB.p);
   std::swap(b.gA.n, b.gB.n);
   return 0;
@@ -841,6 +880,7 @@ int fsil_finalize(void) {
     if (b.stream) (void)hipStreamSynchronize(b.stream);
     if (b.ev0) (void)hipEventDestroy(b.ev0);
     if (b.ev1) (void)hipEventDestroy(b.ev1);
+    if (b.ev_fluid) (void)hipEventDestroy(b.ev_fluid);
   }
   g_blocks.clear();
   return 0;
@@ -902,6 +942,13 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
   b.stream = b.sh->s;
   HIPCHK(hipEventCreate(&b.ev0));
   HIPCHK(hipEventCreate(&b.ev1));
+  HIPCHK(hipEventCreateWithFlags(&b.ev_fluid, hipEventDisableTiming));
+  {
+    const char *e = getenv("FSIL_SPLIT");
+    b.split_ok = !(e && !strcmp(e, "0"));
+    const char *m = getenv("FSIL_SPLIT_MARGIN");
+    if (m) b.split_margin = atoi(m);
+  }
   const size_t nf = (size_t)Q * b.N;
   if (precision == 0) {
     HIPCHK(b.fA.ensure(nf));
@@ -946,6 +993,7 @@ int fsil_block_destroy(int h) {
   (void)hipStreamSynchronize(b->stream);
   (void)hipEventDestroy(b->ev0);
   (void)hipEventDestroy(b->ev1);
+  (void)hipEventDestroy(b->ev_fluid);
   // drop the pair that names this block as a son, and detach its own sons
   if (b->father) {
     auto it = g_blocks.find(b->father);
@@ -1426,7 +1474,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     if (dmax) *dmax = 0.0;
     return 0;
   }
-  hipStream_t s = b->stream;
+  hipStream_t s = coupling_stream(*b);
   const int M = I.M, E = 64 * M;
   if (fem_order_after(*b)) return 1;  // k_ibm_finish rewrites the loads the beam sub-steps read
   if (!Exyz) {
@@ -1662,6 +1710,8 @@ int ibm_check_report(Block &b) {
     return fail("immersed-boundary stencil of marker %d (body %d): index out of %s bound", m - I.moff[body] + 1,
                 body + 1, why == 1 ? "min" : "max");
   }
+  I.have_box = R.ctl[kNband] > 0;
+  for (int k = 0; k < 6; ++k) I.box[k] = R.ctl[kBoxLo + k];
   I.last_warm = R.ctl[kGuess] && !R.ctl[kFallback];  // the guess was kept
   I.last_fallback = R.ctl[kFallback] != 0;
   // forcing rounds run: interpolation rounds plus the guess's correction
@@ -1717,9 +1767,36 @@ int fill_solid_info_args(Block &b, int body, const double *refs, double *buf, So
   return 0;
 }
 
+int collide_any(Block &b, int region, const int *hole) {
+  return b.precision == 0 ? collide_stream(b, region, hole) : collide_stream32(b, region, hole);
+}
+
+// Rows (x0, x1, y0, y1) of the predicted band box of a step's coupling call
+// for a split step: the last checked call's band box widened by
+// split_margin cells (3: markers move well under a cell per step; a band
+// that leaves the prediction is still collided by the hull launch), y
+// aligned to the collide workgroup's rows so that a workgroup is skipped or
+// covered whole.  False when the step is not split: no earlier band box, no
+// beam stream, the LES models (WALE reads the relaxation time that a cell's
+// collision rewrites), or FSIL_SPLIT=0.
+bool split_hole(const Block &b, int *hole) {
+  const Ibm &I = b.ibm;
+  if (!b.split_ok || !I.have_box || I.M == 0 || is_les(b.model) || !b.fem.fs || !b.fem.fs->s) return false;
+  const int by = (int)(b.precision == 1 ? cell_block32(b) : cell_block(b)).y;
+  const int m = b.split_margin;
+  const int x0 = std::max(0, I.box[0] - m), x1 = std::min(b.xd - 1, I.box[3] + m);
+  int y0 = std::max(0, I.box[1] - m), y1 = std::min(b.yd - 1, I.box[4] + m);
+  if (x1 < x0 || y1 < y0) return false;
+  y0 = y0 / by * by;
+  y1 = std::min(b.yd - 1, (y1 / by + 1) * by - 1);
+  hole[0] = x0; hole[1] = x1; hole[2] = y0; hole[3] = y1;
+  return true;
+}
+
 // order work on the block's stream after the last beam sub-steps (device side)
 int fem_order_after(Block &b) {
   Fem &F = b.fem;
+  if (b.split) return 0;  // the coupling runs on the beam stream itself
   if (F.fs && F.fs->pending) HIPCHK(hipStreamWaitEvent(b.stream, F.fs->ev_done, 0));
   return 0;
 }
@@ -1874,7 +1951,7 @@ int fsil_fem_update_markers(int h) {
   a.body_of_elem = F.body_of_elem.p; a.elem_first_marker = F.elem_first_marker.p; a.body_goff = F.body_goff.p;
   a.beta = F.beta.p; a.nEL_total = F.nEL;
   a.Exyz = I.Exyz.p; a.Evel = I.Evel.p; a.Ea = I.Ea.p; a.elem_xc = I.elem_xc.p; a.Enrm = I.Enrm.p;
-  hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 3) / 4), dim3(256), 0, b->stream, a);
+  hipLaunchKernelGGL(k_fem_markers, dim3((F.nEL + 3) / 4), dim3(256), 0, coupling_stream(*b), a);
   HIPCHK(hipGetLastError());
   F.markers_on_device = true;
   return 0;
@@ -1961,10 +2038,12 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
     }
   // the descriptors go up first (the previous sub-steps are done with the
   // buffer), then the kernel waits for the coupling call that produced
-  // lodFlow (block's stream)
+  // lodFlow (block's stream; in a split step the call ran on this stream)
   HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
-  HIPCHK(hipEventRecord(S.ev_in, b->stream));
-  HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
+  if (!b->split) {
+    HIPCHK(hipEventRecord(S.ev_in, b->stream));
+    HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
+  }
   hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, F.ddesc.p, F.nbodies, numsubstep, A,
                      lds_nodes, vec_lds, cg_lds, coef_off);
   HIPCHK(hipGetLastError());
@@ -1982,6 +2061,29 @@ int fsil_fsi_step(int h, double time, int numsubstep, int *iters, double *dmax) 
   // main.f90:95-97: block time and volume force
   b->blktime = time;
   update_vf(*b);
+  int hole[4];
+  if (split_hole(*b, hole)) {
+    // The collision of the cells outside the predicted band box does not
+    // depend on the coupling: it runs on the block's stream while the
+    // markers, the coupling call and the beam sub-steps run on the beam
+    // stream; the rows of the band box follow the call's report event, then
+    // the boundary faces.
+    HIPCHK(hipEventRecord(b->ev_fluid, b->stream));
+    if (collide_any(*b, kRegionBulk, hole)) return 1;
+    HIPCHK(hipStreamWaitEvent(b->fem.fs->s, b->ev_fluid, 0));
+    b->split = true;
+    int rc = fsil_fem_update_markers(h);
+    if (!rc) rc = fsil_ibm_interaction_force(h, b->dh, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                            nullptr);
+    if (!rc) rc = fsil_fem_substeps(h, time, numsubstep, b->dh);
+    b->split = false;
+    if (rc) return 1;
+    HIPCHK(hipStreamWaitEvent(b->stream, b->ibm.rep->ev, 0));
+    if (collide_any(*b, kRegionHull, hole)) return 1;
+    if (launch_faces(*b)) return 1;
+    b->ibm.active = false;
+    return fsil_ibm_report(h, iters, dmax);
+  }
   // IBM_FEM (LBMBlockComm.f90:320-338): markers, the coupling call queued
   // without a host wait, the beam sub-steps queued behind it, then the wait
   // for the call's report, so the beam workgroup is resident before the

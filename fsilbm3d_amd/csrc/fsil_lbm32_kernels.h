@@ -33,6 +33,8 @@ struct Collide32Args {
   int nt;    // nontemporal population loads and stores (selects the kernel instantiation)
   int zrow;  // workgroups are whole z-rows covering the block exactly: 1 shift_rows_z,
              // 2 shift_rows_z_waves (rows of whole waves)
+  int region;   // CollideRegion
+  int hole[4];  // kRegionBulk / kRegionHull: x0, x1, y0, y1
 };
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -92,11 +94,7 @@ __device__ __forceinline__ void moments32(const float *g, double &dr, double &j1
 // and, in forced cells, the Guo term c1*3dh*w*(e.F (1 + 3 u.e) - u.F) added
 // (FluidDomain.f90:1217-1227 rearranged around w_q).
 template <bool FORCE, bool BAND, bool NT>
-__global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
-  const int z = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int x = blockIdx.z;
-  if (z >= a.zd || y >= a.yd) return;
+__device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int x, int y, int z, float *zx) {
   const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
   const unsigned nb = n << 2;  // blocks of this form hold at most 2^30 cells
   const size_t plane = (size_t)a.N * sizeof(float);
@@ -164,12 +162,31 @@ __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
-  // LDS of the exchange, sized at launch (zrow32_lds_bytes)
-  extern __shared__ float zx[];
   if (a.zrow == 2) shift_rows_z_waves(post, zx);
   else if (a.zrow) shift_rows_z(post, zx);
   const unsigned st[6] = {sxp, sxm, syp, sym, a.zrow ? 0u : szp, a.zrow ? 0u : szm};
   store_cell(post, a.gout, plane, n, st, NT);
+}
+
+// The z-row exchange has workgroup barriers: a workgroup of whole z-rows is
+// skipped or covered as a whole (the host aligns the hole's y-range to the
+// workgroup's rows, and a hull tile is a workgroup's shape inside the block).
+template <bool FORCE, bool BAND, bool NT>
+__global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
+  // LDS of the exchange, sized at launch (zrow32_lds_bytes)
+  extern __shared__ float zx[];
+  if (a.region == kRegionHull) {
+    for_hull_cells(a.ctl, a.hole, a.xd, a.yd, a.zd, [&](int x, int y, int z) {
+      if (z < a.zd && y < a.yd) collide_push32_cell<FORCE, BAND, NT>(a, x, y, z, zx);
+    });
+    return;
+  }
+  const int z = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int x = blockIdx.z;
+  if (z >= a.zd || y >= a.yd) return;
+  if (a.region == kRegionBulk && in_hole(a.hole, x, y)) return;
+  collide_push32_cell<FORCE, BAND, NT>(a, x, y, z, zx);
 }
 
 // dynamic LDS of k_collide_push32 for its exchange mode and workgroup size

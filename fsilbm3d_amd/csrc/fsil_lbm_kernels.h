@@ -74,6 +74,46 @@ __device__ __forceinline__ void moments(const double (&f)[Q], double &rho, doubl
 constexpr int kBandErrWord = 7;
 constexpr int kBandNoErr = 0x7fffffff;
 
+// Cells a collide launch covers.  kRegionAll: the whole grid.  kRegionBulk:
+// the cells outside the rows x in [hole[0], hole[1]], y in [hole[2], hole[3]]
+// (a predicted band box of whole z-rows).  kRegionHull: the rows of the
+// smallest box holding that hole and the band's box (ctl[0..5]), by a
+// grid-stride loop over tiles of the workgroup's shape with y aligned to the
+// workgroup's rows.  Both read only the pre-collision state, so a cell
+// collided by a bulk and then a hull launch of one step ends with the hull
+// launch's values, and the hull may cover more than the hole.
+enum CollideRegion { kRegionAll = 0, kRegionBulk = 1, kRegionHull = 2 };
+
+// the rows of a hull launch: x in [x0, x1], y in [y0, y1] with y0 a multiple of by
+__device__ inline void region_hull(const int *ctl, const int *hole, int xd, int yd, int by, int &x0, int &x1,
+                                   int &y0, int &y1) {
+  x0 = max(0, min(hole[0], ctl[0]));
+  x1 = min(xd - 1, max(hole[1], ctl[3]));
+  y0 = max(0, min(hole[2], ctl[1]));
+  y1 = min(yd - 1, max(hole[3], ctl[4]));
+  y0 = (y0 / by) * by;
+}
+
+// grid-stride tiles of a hull launch: calls cell(x, y, z) for every cell of
+// the hull rows (every thread runs the same number of iterations)
+template <typename F>
+__device__ __forceinline__ void for_hull_cells(const int *ctl, const int *hole, int xd, int yd, int zd, F cell) {
+  int x0, x1, y0, y1;
+  const int by = (int)blockDim.y, bz = (int)blockDim.x;
+  region_hull(ctl, hole, xd, yd, by, x0, x1, y0, y1);
+  if (x1 < x0 || y1 < y0) return;
+  const int ny = (y1 - y0 + by) / by, nz = (zd + bz - 1) / bz;
+  const int ntile = (x1 - x0 + 1) * ny * nz;
+  for (int t = blockIdx.x; t < ntile; t += gridDim.x) {
+    const int tz = t % nz, r = t / nz, ty = r % ny, x = x0 + r / ny;
+    cell(x, y0 + ty * by + (int)threadIdx.y, tz * bz + (int)threadIdx.x);
+  }
+}
+
+__device__ __forceinline__ bool in_hole(const int *hole, int x, int y) {
+  return x >= hole[0] && x <= hole[1] && y >= hole[2] && y <= hole[3];
+}
+
 // ---------------------------------------------------------------------------
 // Arguments of the fused collide + push-stream kernel.
 struct CollideArgs {
@@ -107,6 +147,8 @@ struct CollideArgs {
   double *__restrict__ tau_all;   // [N] per-cell relaxation time (WALE reads the previous step's)
   double tau, nu;                 // block tau, flow%nu
   double les_c;                   // CsmagConst, CWALEConst or CvremConst
+  int region;                     // CollideRegion
+  int hole[4];                    // kRegionBulk / kRegionHull: x0, x1, y0, y1
 };
 
 // velocity of one cell as the collision reads it: the moments of the stored
@@ -383,10 +425,17 @@ __device__ __forceinline__ void collide_push_cell(const CollideArgs &a, int x, i
 
 template <int MODEL, bool FORCE, bool BAND, bool AUX, bool NT>
 __global__ __launch_bounds__(256) void k_collide_push(CollideArgs a) {
+  if (a.region == kRegionHull) {
+    for_hull_cells(a.ctl, a.hole, a.xd, a.yd, a.zd, [&](int x, int y, int z) {
+      if (z < a.zd && y < a.yd) collide_push_cell<MODEL, FORCE, BAND, AUX, NT>(a, x, y, z);
+    });
+    return;
+  }
   const int z = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int x = blockIdx.z;
   if (z >= a.zd || y >= a.yd) return;
+  if (a.region == kRegionBulk && in_hole(a.hole, x, y)) return;
   collide_push_cell<MODEL, FORCE, BAND, AUX, NT>(a, x, y, z);
 }
 
