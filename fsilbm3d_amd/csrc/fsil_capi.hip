@@ -179,7 +179,9 @@ struct FemBodyHost {
 struct FemStream {
   hipStream_t s = nullptr;
   hipEvent_t ev_in = nullptr, ev_done = nullptr;
-  FemBodyDesc *h_stage = nullptr;  // pinned: numsubstep x nbodies descriptors
+  FemBodyDesc *h_stage = nullptr;  // pinned: numsubstep x nbodies descriptors, read and
+                                   // answered by the kernel in place
+  FemBodyDesc *d_stage = nullptr;  // device address of h_stage
   size_t cap = 0;
   int n_staged = 0;                // descriptors of the last call (results after ev_done)
   bool pending = false;            // a call whose results have not been checked
@@ -199,7 +201,6 @@ struct Fem {
   std::vector<FemBodyDesc> desc;
   double dampK = 0, dampM = 0, GeoGamma = 1, NewmarkGamma = 0.5, NewmarkBeta = 0.25, dtolFEM = 1e-10;
   int ntolFEM = 20;
-  DevBuf<FemBodyDesc> ddesc;
   DevBuf<BeamElem> el;
   DevBuf<double> pos, dsp, vel, acc, dspO, velO, accO, lodInte, lodEffe, lodExte, vBC, x, r, p, Ap, z, xF, lbuf, blk,
       beta;
@@ -2012,10 +2013,11 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   if (n > S.cap) {
     if (S.h_stage) HIPCHK(hipHostFree(S.h_stage));
     S.h_stage = nullptr;
+    S.d_stage = nullptr;
     S.cap = 0;
     HIPCHK(hipHostMalloc((void **)&S.h_stage, n * sizeof(FemBodyDesc), hipHostMallocDefault));
+    HIPCHK(hipHostGetDevicePointer((void **)&S.d_stage, S.h_stage, 0));
     S.cap = n;
-    HIPCHK(F.ddesc.ensure(n));
   }
   for (int isub = 1; isub <= numsubstep; ++isub)
     for (int bi = 0; bi < F.nbodies; ++bi) {
@@ -2036,18 +2038,17 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
       d.tree_sums = b->precision == 1 ? 1 : 0;
       d.newton_iters = 0; d.cg_iters = 0; d.status = 0; d.dnorm = 0.0;
     }
-  // the descriptors go up first (the previous sub-steps are done with the
-  // buffer), then the kernel waits for the coupling call that produced
-  // lodFlow (block's stream; in a split step the call ran on this stream)
-  HIPCHK(hipMemcpyAsync(F.ddesc.p, S.h_stage, n * sizeof(FemBodyDesc), hipMemcpyHostToDevice, S.s));
+  // the kernel reads the descriptors from the pinned staging area and writes
+  // its results there (the previous sub-steps are done with it: fem_collect);
+  // it waits for the coupling call that produced lodFlow (block's stream; in
+  // a split step the call ran on this stream)
   if (!b->split) {
     HIPCHK(hipEventRecord(S.ev_in, b->stream));
     HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
   }
-  hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, F.ddesc.p, F.nbodies, numsubstep, A,
+  hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, S.d_stage, F.nbodies, numsubstep, A,
                      lds_nodes, vec_lds, cg_lds, coef_off);
   HIPCHK(hipGetLastError());
-  HIPCHK(hipMemcpyAsync(S.h_stage, F.ddesc.p, n * sizeof(FemBodyDesc), hipMemcpyDeviceToHost, S.s));
   HIPCHK(hipEventRecord(S.ev_done, S.s));
   S.n_staged = (int)n;
   S.pending = true;

@@ -709,9 +709,23 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
 __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, int nbodies, int nsub, FemArrays A,
                                                    int lds_nodes, int vec_lds, int cg_lds, int coef_off) {
   __builtin_amdgcn_s_setprio(3);
+  // descs is pinned host memory: each sub-step's descriptor is read into LDS
+  // once, and its results are written back
+  __shared__ FemBodyDesc sD;
+  static_assert(sizeof(FemBodyDesc) % 8 == 0 && sizeof(FemBodyDesc) / 8 <= 512, "descriptor copy by 8-byte words");
   for (int isub = 0; isub < nsub; ++isub) {
-    fem_substep_body(descs[(size_t)isub * nbodies + blockIdx.x], A, lds_nodes, vec_lds, cg_lds, coef_off);
+    FemBodyDesc *g = descs + (size_t)isub * nbodies + blockIdx.x;
+    if (threadIdx.x < sizeof(FemBodyDesc) / 8)
+      reinterpret_cast<long long *>(&sD)[threadIdx.x] = reinterpret_cast<const long long *>(g)[threadIdx.x];
     fem_sync();
+    fem_substep_body(sD, A, lds_nodes, vec_lds, cg_lds, coef_off);
+    fem_sync();
+    if (threadIdx.x == 0) {
+      g->newton_iters = sD.newton_iters;
+      g->cg_iters = sD.cg_iters;
+      g->status = sD.status;
+      g->dnorm = sD.dnorm;
+    }
   }
 }
 
