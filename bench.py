@@ -4,11 +4,14 @@ Forms of the FSI step (all through the C-ABI, same steps, same inputs):
 fp64-plain (the reference's arithmetic), fp32-deviation (fluid stored and
 collided in FP32 as f - w; solid, coupling and reductions FP64), and
 fp32-deviation+learned (the same with the learned warm start of the
-direct-forcing loop, trained first on two seeded FP32 runs).  `value` is the
-FSI-step MLUPS of the first of fp32-deviation+learned, fp32-deviation whose
-drag and lift stay within 1e-3 (relative to the largest |drag|, |lift|) of
-the fp64-plain run over the timed steps, else of fp64-plain; every form is
-reported under "forms" with its force error and speed-up.
+direct-forcing loop: the network's guess stands for the first --learned-skip
+forcing rounds and the loop continues to the same cap and tolerance; trained
+first on two seeded FP32 runs).  `value` is the FSI-step MLUPS of the first of
+fp32-deviation+learned, fp32-deviation whose drag and lift stay within 1e-3
+(relative to the largest |drag|, |lift|) of the fp64-plain run over every
+timed step, else of fp64-plain; every form is reported under "forms" with its
+speed-up and its error against fp64-plain: drag, lift and tip position over
+every timed step, and the velocity field's L2 after the last.
 
 Headline workload (value): the flexible-plate FSI case -- D3Q19 384x192x192,
 uniform inflow U = 0.05 (101), outflow (103), slip lateral faces (302), Re 500
@@ -16,24 +19,27 @@ on L = 48 cells; one flexible plate (49 chord nodes x 48 span markers = 2304
 markers, mass ratio 1, KB 0.1, KS 1000, clamped leading edge heaving 0.3 L at
 0.2 U/L), numsubstep 4, multi-direct forcing to 1e-6 with at most 10
 iterations.  One step = the body of main.f90:93-147: volume force, markers,
-immersed-boundary coupling, 4 beam sub-steps, fused collide+stream+faces.
+immersed-boundary coupling, 4 beam sub-steps, fused collide+stream+faces
+(fsil_fsi_step: the collision outside the band box beside the coupling).
 MLUPS = cells x steps / wall seconds, host-side around exactly --steps steps
 (after --warmup steps) between device synchronisations, the initial state and
-the bodies already resident on the device.
+the bodies already resident on the device.  The case as configured diverges
+in the reference program itself at step 346, so the default window is steps
+11-260.
 
 Beside it: the stencil benchmark (D3Q19 512x256x256 fully periodic, SRT tau
 0.55, seeded equilibrium) in the FP64 plain and FP32-deviation forms, with the
 FP32 velocity L2 error against FP64; and the school (1024x384x384, 16 plates
-of the flexible-plate case on a 4x4 diamond lattice) in both forms, with the
-per-body force error of FP32 against FP64 after the same steps.
+of the flexible-plate case on a 4x4 diamond lattice) in the FP64, FP32 and
+FP32+learned forms, with the per-body force error against FP64.
 
 roofline: the fused collide+stream kernel of the headline form's FSI step
 (the band-overlay variant), algorithmic bytes per launch 19 reads + 19 writes
 per cell (304 B FP64, 152 B FP32-deviation), over its average duration from
 HIP events on the block's stream.
 cpu_baseline: the reference program itself (oracle/_ref, compiled from the
-reference sources) timed on this host's cores for a few steps of the same
-FSI case.
+reference sources) timed on this host's cores for --cpu-steps steps of the
+same FSI case, with the CPU model.
 """
 import argparse
 import json
@@ -75,13 +81,15 @@ def run_form(prec, f0, steps, warmup, nlaunch):
                 kernel_ms=kern_ms, achieved_gbs=achieved)
 
 
-def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_forces=False):
+def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_forces=False, skip=0):
     """Flexible-plate FSI steps through the C-ABI (device beam solver).
     precision 0: FP64 plain, 1: FP32-deviation fluid (FP64 solid and
-    coupling).  net: learned warm start during the timed steps.  collect:
-    record the network's features/targets of the first `collect` steps.
-    Returns timings; with record_forces the per-step force on the body over
-    the timed steps (a host read each step, so such a run is not timed)."""
+    coupling).  net: learned warm start during the timed steps, standing for
+    the first `skip` forcing rounds.  collect: record the network's
+    features/targets (after `skip` rounds) of the first `collect` steps.
+    Returns timings; with record_forces the per-step force on the body and
+    tip position over the timed steps and the final velocity field (host
+    reads each step, so such a run is not timed)."""
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock, synchronize
     from fsilbm3d_amd.fsi import DeviceBodyBlock
@@ -95,17 +103,17 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_
     XS, YS = [], []
     t = 0.0
     for k in range(warmup):
-        fsi.set_warm_start(None, record=k < collect)
+        fsi.set_warm_start(None, record=k < collect, skip=skip if k < collect else 0)
         t = t + 1.0
         fsi.step(t)
         if k < collect:
             X, Y, _, _ = fsi.warm_start_data()
             XS.append(X)
             YS.append(Y)
-    fsi.set_warm_start(net, record=False)
+    fsi.set_warm_start(net, record=False, skip=skip if net is not None else 0)
     synchronize()
-    its, dm = [], []
-    forces = []
+    its, dm, rounds = [], [], []
+    forces, tips = [], []
     t0 = time.perf_counter()
     for _ in range(steps):
         t = t + 1.0
@@ -116,12 +124,15 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_
         fsi.step(t)
         its.append(fsi.iters)
         dm.append(fsi.dmax)
+        rounds.append(fsi.rounds)
         if record_forces:
             forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
+            tips.append(fsi.state(0)[0][-1, :3].copy())
     synchronize()
     secs = time.perf_counter() - t0
     newton, cg, _ = fsi.info(0)
     pos = fsi.state(0)[0]
+    u_field = blk.calculate_macro_quantities()[1] if record_forces else None
     # the fused kernel with the band overlay, timed with HIP events on the block's stream
     fsi.set_warm_start(None)
     fsi.interaction_force(blk.dh)
@@ -131,18 +142,21 @@ def run_fsi(w, steps, warmup, nlaunch, precision=0, net=None, collect=0, record_
     out = dict(mlups=N * steps / secs / 1e6, ms_per_step=secs / steps * 1e3, kernel_ms=kern_ms,
                achieved_gbs=B_CELL[precision] * N / (kern_ms * 1e-3) / 1e9,
                mdf_iters_mean=float(np.mean(its)), mdf_residual_mean=float(np.mean(dm)),
+               mdf_rounds_mean=float(np.mean(rounds)),
                newton_iters_last=newton, cg_iters_last=cg,
-               tip_xyz=[float(v) for v in pos[-1, :3]], forces=np.array(forces), cells=N, time=t)
+               tip_xyz=[float(v) for v in pos[-1, :3]], forces=np.array(forces), tips=np.array(tips), u=u_field,
+               cells=N, time=t)
     if collect:
         out["X"] = np.concatenate(XS)
         out["Y"] = np.concatenate(YS)
     return out
 
 
-def run_school(steps, warmup, precision):
+def run_school(steps, warmup, precision, net=None, skip=0):
     """The school configuration (1024x384x384, 16 flexible plates): timed
-    steps, then each body's force (drag, lift, side) after the last step and
-    the band-overlay fused kernel's duration."""
+    steps (net: learned warm start standing for `skip` forcing rounds), then
+    each body's force (drag, lift, side) after the last step and the
+    band-overlay fused kernel's duration."""
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock, synchronize
     from fsilbm3d_amd.fsi import DeviceBodyBlock
@@ -159,6 +173,7 @@ def run_school(steps, warmup, precision):
     for _ in range(warmup):
         t = t + 1.0
         fsi.step(t)
+    fsi.set_warm_start(net, record=False, skip=skip if net is not None else 0)
     synchronize()
     its = []
     t0 = time.perf_counter()
@@ -178,19 +193,25 @@ def run_school(steps, warmup, precision):
                 mdf_iters_mean=float(np.mean(its)), bodies=len(loads))
 
 
-def force_error(a, b):
-    """max over steps of |drag_a - drag_b| and |lift_a - lift_b|, relative to
-    the largest |drag| / |lift| of b (the FP64 plain run)."""
+def force_error(a, b, L):
+    """Error of run a against run b (the FP64 plain run) over every timed
+    step: max |drag_a - drag_b| and |lift_a - lift_b| relative to the largest
+    |drag| / |lift| of b; max |tip_a - tip_b| (trailing-edge node position)
+    relative to the chord L; the velocity field's L2 after the last step,
+    relative to b's."""
     import numpy as np
     Fa, Fb = a["forces"][:, :2], b["forces"][:, :2]
     rel = np.abs(Fa - Fb).max(axis=0) / np.abs(Fb).max(axis=0)
-    return {"drag": float(rel[0]), "lift": float(rel[1])}
+    tip = float(np.linalg.norm(a["tips"] - b["tips"], axis=1).max() / L)
+    l2 = float(np.sqrt(((a["u"] - b["u"]) ** 2).sum() / (b["u"] ** 2).sum()))
+    return {"drag": float(rel[0]), "lift": float(rel[1]), "tip": tip, "velocity_l2": l2}
 
 
-def train_warm_start(w, steps, epochs, seed=20261202):
+def train_warm_start(w, steps, epochs, skip, seed=20261202):
     """The learned warm start trained on the solver's own trajectories: two
     seeded FP32 runs of the workload with heave amplitude in U(0.2, 0.4) L and
-    frequency in U(0.15, 0.3) U/L, every marker of every step a sample."""
+    frequency in U(0.15, 0.3) U/L, every marker of every step a sample; the
+    target is the force the plain loop reaches after `skip` rounds."""
     import dataclasses
     import numpy as np
     from fsilbm3d_amd.mlp import WarmStartNet
@@ -198,13 +219,14 @@ def train_warm_start(w, steps, epochs, seed=20261202):
     XS, YS = [], []
     for r in range(2):
         wr = dataclasses.replace(w, amp=0.2 + 0.2 * u[2 * r], freq=0.15 + 0.15 * u[2 * r + 1])
-        d = run_fsi(wr, 1, steps, 1, precision=1, collect=steps)
+        d = run_fsi(wr, 1, steps, 1, precision=1, collect=steps, skip=skip)
         XS.append(d["X"])
         YS.append(d["Y"])
     X, Y = np.concatenate(XS), np.concatenate(YS)
     net = WarmStartNet(seed & 0x7fffffff)
     before, after = net.train(X, Y, epochs=epochs, batch=1024, lr=2e-3, seed=seed & 0x7fffffff)
-    return net, {"samples": int(X.shape[0]), "epochs": epochs, "loss_before": before, "loss_after": after}
+    return net, {"samples": int(X.shape[0]), "epochs": epochs, "skip": skip, "loss_before": before,
+                 "loss_after": after}
 
 
 def l2_fp32_vs_fp64(f0, nsteps):
@@ -234,6 +256,16 @@ def reference_fsi_case(w):
                 dtolFEM=sk["dtolFEM"], ntolFEM=sk["ntolFEM"], groups=[g])
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, steps):
     from oracle.refcase import time_reference_steps
     threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
@@ -241,10 +273,11 @@ def cpu_baseline(w, steps):
     N = w.dims[0] * w.dims[1] * w.dims[2]
     tot = sum(secs)
     return {"value": N * len(secs) / tot / 1e6 if tot > 0 else None, "unit": "MLUPS", "cores": threads,
-            "kind": "reference",
+            "kind": "reference", "cpu": cpu_model(),
             "sample": f"{len(secs)} FSI steps of the flexible-plate case (384x192x192, one elastic plate, "
-                      f"reference program built with amdflang -O3 -fopenmp, {threads} OpenMP threads, from its own "
-                      f"uniform initial state); seconds per step {[round(s, 3) for s in secs]}"}
+                      f"reference program built from its sources by oracle/build_ref.sh, {threads} OpenMP threads, "
+                      f"from its own uniform initial state, {tot:.1f} s in all); seconds per step "
+                      f"{[round(s, 3) for s in secs]}"}
 
 
 def traffic_from_profiles(name="collide_push_fp64_512x256x256_pmc.json"):
@@ -277,6 +310,20 @@ def pmc_driver(kind="stencil"):
     import numpy as np
     from fsilbm3d_amd.block import LBMBlock, synchronize
     copy_gbs(2 << 30, 1)
+    if kind == "mlp":
+        # the network's training kernels on 2304 x 100 seeded samples, then
+        # 200 BF16 inference launches over the 2304 markers
+        from fsilbm3d_amd.mlp import WarmStartNet
+        rng = np.random.default_rng(5)
+        X = rng.standard_normal((2304 * 100, 12)).astype(np.float32)
+        Y = (0.1 * rng.standard_normal((2304 * 100, 3))).astype(np.float32)
+        net = WarmStartNet(5)
+        net.train(X, Y, epochs=2, batch=1024, lr=2e-3, seed=5)
+        for _ in range(200):
+            net.forward(X[:2304], bf16=True)
+        net.close()
+        log("pmc driver done")
+        return
     if kind == "fsi":
         from fsilbm3d_amd.workloads import flexible_plate
         w = flexible_plate()
@@ -298,22 +345,27 @@ def pmc_driver(kind="stencil"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    # 10 + 250 steps: the flexible-plate case as configured diverges in the
+    # reference program itself at step 346, so the window ends well before
+    ap.add_argument("--steps", type=int, default=250)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--stencil-steps", type=int, default=200)
     ap.add_argument("--learned-steps", type=int, default=50,
                     help="steps of each of the two training runs of the warm start (0: no learned form)")
     ap.add_argument("--learned-epochs", type=int, default=20)
+    ap.add_argument("--learned-skip", type=int, default=2,
+                    help="forcing rounds the learned guess stands for (the loop continues from round skip+1)")
     ap.add_argument("--nlaunch", type=int, default=50)
     ap.add_argument("--l2-steps", type=int, default=1000)
-    ap.add_argument("--cpu-steps", type=int, default=5)
-    ap.add_argument("--school-steps", type=int, default=20,
+    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--school-steps", type=int, default=200,
                     help="timed steps of the 16-plate 1024x384x384 school in each form (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--pmc-driver", nargs="?", const="stencil", default=None, choices=["stencil", "fsi"],
+    ap.add_argument("--pmc-driver", nargs="?", const="stencil", default=None, choices=["stencil", "fsi", "mlp"],
                     help="profiling driver: one 2 GiB copy kernel, then 3 launches of the FP64 fused kernel "
-                         "(stencil) or 3 FSI steps in each storage form (fsi)")
+                         "(stencil), 3 FSI steps in each storage form (fsi), or the warm-start network's "
+                         "training and BF16 inference kernels (mlp)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
@@ -331,10 +383,12 @@ def main():
     log("bench: device warm-up")
     copy_gbs(2 << 30, 2500)
 
-    def forces_of(r, precision, net=None):
-        # the same steps again, untimed, reading the body force after each one
-        r["forces"] = run_fsi(w, args.steps, args.warmup, 1, precision=precision, net=net,
-                              record_forces=True)["forces"]
+    def forces_of(r, precision, net=None, skip=0):
+        # the same steps again, untimed, reading the body force and tip after
+        # each one and the velocity field after the last
+        d = run_fsi(w, args.steps, args.warmup, 1, precision=precision, net=net, record_forces=True, skip=skip)
+        for k in ("forces", "tips", "u"):
+            r[k] = d[k]
 
     log("bench: flexible-plate FSI, FP64 plain form")
     fsi = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=0)
@@ -343,21 +397,20 @@ def main():
     log("bench: flexible-plate FSI, FP32-deviation fluid")
     fsi32 = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1)
     forces_of(fsi32, 1)
-    err32 = force_error(fsi32, fsi)
+    err32 = force_error(fsi32, fsi, w.L)
     log(f"bench: FSI FP32 {fsi32['mlups']:.1f} MLUPS, force error {err32}")
     learned = None
     if args.learned_steps > 0:
         log(f"bench: training the warm start on {2 * args.learned_steps} FP32 steps")
         t0 = time.perf_counter()
-        net, tinfo = train_warm_start(w, args.learned_steps, args.learned_epochs)
+        net, tinfo = train_warm_start(w, args.learned_steps, args.learned_epochs, args.learned_skip)
         tinfo["seconds"] = time.perf_counter() - t0
         log(f"bench: warm start trained {tinfo}")
-        fsiL = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1, net=net)
-        forces_of(fsiL, 1, net)
-        errL = force_error(fsiL, fsi)
+        fsiL = run_fsi(w, args.steps, args.warmup, args.nlaunch, precision=1, net=net, skip=args.learned_skip)
+        forces_of(fsiL, 1, net, args.learned_skip)
+        errL = force_error(fsiL, fsi, w.L)
         log(f"bench: FSI FP32+learned {fsiL['mlups']:.1f} MLUPS, force error {errL}")
-        learned = (fsiL, errL, tinfo)
-        net.close()
+        learned = (fsiL, errL, tinfo, net)
     school = None
     if args.school_steps > 0:
         log(f"bench: school 1024x384x384, 16 plates, {args.school_steps} steps per form")
@@ -365,6 +418,10 @@ def main():
         s32 = run_school(args.school_steps, 2, 1)
         scale = np.abs(s64["loads"][:, :2]).max(axis=0)
         serr = np.abs(s32["loads"][:, :2] - s64["loads"][:, :2]).max(axis=0) / scale
+        sL = None
+        if learned is not None:
+            sL = run_school(args.school_steps, 2, 1, net=learned[3], skip=args.learned_skip)
+            serrL = np.abs(sL["loads"][:, :2] - s64["loads"][:, :2]).max(axis=0) / scale
         school = {"workload": "school: D3Q19 1024x384x384, 16 flexible plates as the flexible-plate case on a 4x4 "
                               "diamond lattice (2 L x 1.5 L), per-plate heave phase",
                   "cells": s32["cells"], "bodies": s32["bodies"], "markers": 16 * 2304, "steps": args.school_steps,
@@ -377,7 +434,12 @@ def main():
                                  "kernel_ms": s64["kernel_ms"]},
                   "speedup_vs_fp64_plain": s32["mlups"] / s64["mlups"],
                   "per_body_force_error_vs_fp64_plain": {"drag": float(serr[0]), "lift": float(serr[1]),
-                                                         "after_steps": args.school_steps + 2}}
+                                                         "after_steps": args.school_steps + 2},
+                  "fp32_deviation_learned": None if sL is None else {
+                      "value": sL["mlups"], "ms_per_step": sL["ms_per_step"], "mdf_iters_mean": sL["mdf_iters_mean"],
+                      "skip": args.learned_skip,
+                      "per_body_force_error_vs_fp64_plain": {"drag": float(serrL[0]), "lift": float(serrL[1]),
+                                                             "after_steps": args.school_steps + 2}}}
         log(f"bench: school FP32 {s32['mlups']:.1f} MLUPS, FP64 {s64['mlups']:.1f} MLUPS, per-body error {serr}")
     log("bench: seeding 512x256x256 state")
     f0 = seeded_state(DIMS, 20261202)
@@ -400,17 +462,19 @@ def main():
             cpu = cpu_baseline(w, args.cpu_steps)
         except Exception as e:  # reported, not fatal: the baseline is not the target
             cpu = {"value": None, "unit": "MLUPS", "cores": None, "kind": "reference", "sample": f"failed: {e}"}
+    if learned is not None:
+        learned[3].close()
     N = DIMS[0] * DIMS[1] * DIMS[2]
     cp = copy_gbs()
     # headline form: FP32+learned when its drag/lift stay within 1e-3 of the
     # FP64 plain run, else FP32 alone under the same test, else FP64 plain
-    forms = {"fp64-plain": (fsi, {"drag": 0.0, "lift": 0.0})}
+    forms = {"fp64-plain": (fsi, {"drag": 0.0, "lift": 0.0, "tip": 0.0, "velocity_l2": 0.0})}
     forms["fp32-deviation"] = (fsi32, err32)
     if learned is not None:
         forms["fp32-deviation+learned"] = (learned[0], learned[1])
     head = "fp64-plain"
     for name in ("fp32-deviation+learned", "fp32-deviation"):
-        if name in forms and max(forms[name][1].values()) < 1e-3:
+        if name in forms and max(forms[name][1]["drag"], forms[name][1]["lift"]) < 1e-3:
             head = name
             break
     H = forms[head][0]
@@ -437,10 +501,14 @@ def main():
         "data": "synthetic (uniform-flow initial state, prescribed plate kinematics)",
         "config": {"workload": "flexible-plate FSI: D3Q19 384x192x192, inflow U 0.05, Re 500 on L 48, one elastic "
                                "plate 49 chord nodes x 48 span markers, numsubstep 4, MDF to 1e-6 (max 10)",
-                   "cells": fsi["cells"], "markers": 2304, "collision": "SRT", "form": head},
+                   "cells": fsi["cells"], "markers": 2304, "collision": "SRT", "form": head,
+                   "window": f"steps {args.warmup + 1}-{args.warmup + args.steps} timed; the case as configured "
+                             "diverges in the reference program itself at step 346 (its NaN abort), so no "
+                             "longer window exists"},
         "forms": {name: {"value": r["mlups"], "ms_per_step": r["ms_per_step"],
                          "speedup_vs_fp64_plain": r["mlups"] / fsi["mlups"], "force_error_vs_fp64_plain": e,
-                         "mdf_iters_mean": r["mdf_iters_mean"], "mdf_residual_mean": r["mdf_residual_mean"],
+                         "mdf_iters_mean": r["mdf_iters_mean"], "mdf_rounds_mean": r["mdf_rounds_mean"],
+                         "mdf_residual_mean": r["mdf_residual_mean"],
                          "newton_iters_last": r["newton_iters_last"], "cg_iters_last": r["cg_iters_last"],
                          "tip_xyz": r["tip_xyz"], "collide_kernel_ms": r["kernel_ms"]}
                   for name, (r, e) in forms.items()},
