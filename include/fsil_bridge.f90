@@ -118,15 +118,21 @@ module fsil_bridge
             integer(c_int), value :: handle, ntolLBM
             real(c_double), value :: dtolLBM, Uref, denIn
         end function
+        ! every array and output is optional: an absent one arrives as a null
+        ! pointer (Fortran 2018), so a host with its bodies on the device beam
+        ! solver omits the markers (fsil_fem_update_markers has formed them),
+        ! and a host omits the outputs it does not want back (no host wait;
+        ! fsil_ibm_report and fsil_ibm_forces read them later), e.g.
+        !   rc = fsil_ibm_interaction_force(h, dt, iters=it, dmax=dm)
         integer(c_int) function fsil_ibm_interaction_force(handle, dt, Exyz, Evel, Ea, elem_xc, Eforce, lodFlow, &
                                                            iters, dmax) bind(C, name='fsil_ibm_interaction_force')
             import :: c_int, c_double
             integer(c_int), value :: handle
             real(c_double), value :: dt
-            real(c_double), intent(in) :: Exyz(*), Evel(*), Ea(*), elem_xc(*)
-            real(c_double), intent(out) :: Eforce(*), lodFlow(*)
-            integer(c_int), intent(out) :: iters
-            real(c_double), intent(out) :: dmax
+            real(c_double), intent(in), optional :: Exyz(*), Evel(*), Ea(*), elem_xc(*)
+            real(c_double), intent(out), optional :: Eforce(*), lodFlow(*)
+            integer(c_int), intent(out), optional :: iters
+            real(c_double), intent(out), optional :: dmax
         end function
         integer(c_int) function fsil_time_collide(handle, nlaunch, avg_ms) bind(C, name='fsil_time_collide')
             import :: c_int, c_double

@@ -325,10 +325,14 @@ class RefResult:
 
 def run_reference(case: Case, f0: np.ndarray, nsteps: int, keep: bool = False,
                   plates: Optional[List[str]] = None, timeout: float = 600.0,
-                  env_extra: Optional[dict] = None, start_time_over_tref: float = 0.0) -> RefResult:
-    """Run the reference binary for ``nsteps`` steps from state ``f0``."""
-    if not os.path.exists(REF_BIN):
-        raise FileNotFoundError(f"{REF_BIN} missing: run oracle/build_ref.sh")
+                  env_extra: Optional[dict] = None, start_time_over_tref: float = 0.0,
+                  binary: Optional[str] = None) -> RefResult:
+    """Run the reference binary (or another build of the reference program
+    that reads the same inputs, ``binary``) for ``nsteps`` steps from state
+    ``f0``."""
+    exe = binary or REF_BIN
+    if not os.path.exists(exe):
+        raise FileNotFoundError(f"{exe} missing: run oracle/build_ref.sh (or the program's own build)")
     wd = tempfile.mkdtemp(prefix="fsilref_")
     try:
         for d in ("DatContinue", "DatInfo", "DatBody", "DatBodySpan", "DatFlow"):
@@ -345,7 +349,7 @@ def run_reference(case: Case, f0: np.ndarray, nsteps: int, keep: bool = False,
         env["OMP_NUM_THREADS"] = str(case.npsize)
         if env_extra:
             env.update(env_extra)
-        p = subprocess.run([REF_BIN], cwd=wd, capture_output=True, text=True, timeout=timeout, env=env)
+        p = subprocess.run([exe], cwd=wd, capture_output=True, text=True, timeout=timeout, env=env)
         if p.returncode != 0:
             raise RuntimeError(f"reference failed rc={p.returncode}\n{p.stdout[-3000:]}\n{p.stderr[-3000:]}")
         outs = sorted(x for x in os.listdir(os.path.join(wd, "DatContinue")) if x != "continue")

@@ -59,6 +59,38 @@ def test_fortran_binding_compiles():
     assert out.returncode == 0, out.stderr
 
 
+def test_fortran_coupling_call_with_absent_arguments_compiles(tmp_path):
+    """A host with its bodies on the device beam solver calls the coupling
+    with the marker arrays and the unwanted outputs omitted (they arrive as
+    null pointers); the call compiles against the module."""
+    fc = "/opt/rocm/bin/amdflang"
+    if not os.path.exists(fc):
+        pytest.skip("amdflang not present")
+    src = os.path.join(ROOT, "include", "fsil_bridge.f90")
+    mod = str(tmp_path)
+    out = subprocess.run([fc, "-c", "-ffree-form", src, "-module-dir", mod, "-o", os.path.join(mod, "b.o")],
+                         capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+    prog = tmp_path / "host.f90"
+    prog.write_text(
+        "subroutine device_bodies_step(h, dt, t, nsub)\n"
+        "    use iso_c_binding\n"
+        "    use fsil_bridge\n"
+        "    integer(c_int), intent(in) :: h, nsub\n"
+        "    real(c_double), intent(in) :: dt, t\n"
+        "    integer(c_int) :: it\n"
+        "    real(c_double) :: dm\n"
+        "    call fsil_check(fsil_fem_update_markers(h), 'markers')\n"
+        "    call fsil_check(fsil_ibm_interaction_force(h, dt), 'ibm')\n"
+        "    call fsil_check(fsil_fem_substeps(h, t, nsub, dt), 'fem')\n"
+        "    call fsil_check(fsil_ibm_report(h, it, dm), 'report')\n"
+        "    call fsil_check(fsil_ibm_interaction_force(h, dt, iters=it, dmax=dm), 'ibm with report')\n"
+        "end subroutine\n")
+    out = subprocess.run([fc, "-c", "-ffree-form", str(prog), "-module-dir", mod, "-I", mod,
+                          "-o", os.path.join(mod, "host.o")], capture_output=True, text=True)
+    assert out.returncode == 0, out.stderr
+
+
 def test_fortran_binding_covers_every_entry_point():
     """Every function the header declares has a bind(C) interface in the
     Fortran module, under the same name."""
