@@ -338,34 +338,42 @@ def mismatch(a, b):
     return int((a != b).sum()), float(d.max()) if d.size else 0.0
 
 
-def run_plate_fsi(w, precision, nsteps, record_every=1, net=None, warm_from=None, collect=0, solver=None):
+def run_plate_fsi(w, precision, nsteps, record_every=1, net=None, warm_from=None, collect=0, solver=None, skip=0,
+                  fill=False):
     """A flexible-plate workload (fsilbm3d_amd.workloads.FlexiblePlate) on the
     device in the given storage form; returns per-step total fluid force on
-    the body (sum of lodFlow x, y, z), MDF iterations and residuals, and the
-    final node positions.  net/warm_from: learned warm start from that step
-    on; collect: record the network's features/targets of the first steps."""
+    the body (sum of lodFlow x, y, z), MDF iterations, forcing rounds and
+    residuals, and the final node positions.  net/warm_from: learned warm
+    start from that step on, standing for `skip` forcing rounds; collect:
+    record the network's features/targets (after `skip` rounds) of the first
+    steps.  fill: start from the uniform equilibrium written on the device
+    (large blocks) instead of an uploaded state."""
     from fsilbm3d_amd.block import LBMBlock
     from fsilbm3d_amd.fsi import DeviceBodyBlock
     blk = LBMBlock(w.dims, 1.0, (0.0, 0.0, 0.0), w.bc, 1, w.nu, precision=precision)
     blk.set_flow(**w.flow_kwargs())
-    blk.upload(w.initial_state())
+    if fill:
+        blk.fill(w.equilibrium())
+    else:
+        blk.upload(w.initial_state())
     blk.update_volume_force(0.0)
     blk.set_boundary_conditions()
     kw = w.solver_kwargs()
     kw.update(solver or {})
     fsi = DeviceBodyBlock(blk, [w.plate()], **kw)
-    forces, iters, dmax, warm, fell = [], [], [], [], []
+    forces, iters, dmax, warm, fell, rounds = [], [], [], [], [], []
     XS, YS = [], []
     t = 0.0
     for k in range(nsteps):
         use = net is not None and warm_from is not None and k >= warm_from
-        fsi.set_warm_start(net if use else None, record=(k < collect) or use)
+        fsi.set_warm_start(net if use else None, record=(k < collect) or use, skip=skip)
         t = t + 1.0
         fsi.step(t)
         if (k + 1) % record_every == 0:
             forces.append(fsi.body_loads(0).reshape(-1, 6).sum(0)[:3].copy())
         iters.append(fsi.iters)
         dmax.append(fsi.dmax)
+        rounds.append(fsi.rounds)
         if k < collect or use:
             X, Y, used, fb = fsi.warm_start_data()
             if k < collect:
@@ -376,7 +384,7 @@ def run_plate_fsi(w, precision, nsteps, record_every=1, net=None, warm_from=None
     pos = fsi.state(0)[0].copy()
     den, uuu = blk.calculate_macro_quantities()
     blk.close()
-    out = dict(forces=np.array(forces), iters=iters, dmax=dmax, pos=pos, u=uuu, warm=warm, fell=fell)
+    out = dict(forces=np.array(forces), iters=iters, dmax=dmax, pos=pos, u=uuu, warm=warm, fell=fell, rounds=rounds)
     if collect:
         out["X"] = np.concatenate(XS)
         out["Y"] = np.concatenate(YS)
