@@ -249,6 +249,7 @@ struct Block {
   DevBuf<double> fA, fB;
   DevBuf<float> gA, gB;
   DevBuf<double> Mc, Mf;
+  DevBuf<float> Mc32, Mf32;  // FP32-deviation MRT: the matrices rounded to FP32
   DevBuf<double> tau_all, uf;  // LES models: per-cell tau; the velocity field of models 14, 15
   DevBuf<float> outbuf;        // DatFlow fields of the last flow write (device staging)
   double nu = std::nan("");    // flow%nu (models 14, 15)
@@ -636,6 +637,12 @@ int collide_stream32(Block &b, int region = kRegionAll, const int *hole = nullpt
   a.xd = b.xd; a.yd = b.yd; a.zd = b.zd; a.N = b.N;
   a.Omega = (float)b.Omega;
   a.c1 = (float)(1.0 - 0.5 * b.Omega);
+  a.hO = (float)(0.5 * b.Omega);
+  a.hO2 = (float)(0.5 * b.Omega2);
+  a.cT1 = (float)(0.5 - 0.25 * b.Omega);
+  a.cT2 = (float)(0.5 - 0.25 * b.Omega2);
+  a.Mc = b.Mc32.p;
+  a.Mf = b.Mf32.p;
   a.dt3 = (float)(3.0 * b.dh);
   for (int k = 0; k < 3; ++k) {
     a.hv[k] = (float)(0.5 * b.VF[k] * b.dh);
@@ -657,21 +664,38 @@ int collide_stream32(Block &b, int region = kRegionAll, const int *hole = nullpt
     for (int k = 0; k < 4; ++k) a.hole[k] = hole[k];
     a.ctl = b.ibm.ctl.p;
   }
+  int m102 = 0, mhw = 0;
+  for (int face = 0; face < 6; ++face) {
+    if (b.bc[face] == BC_NEQ_DIRICHLET) m102 |= 1 << face;
+    if (b.bc[face] == BC_WALL_HALFWAY || b.bc[face] == BC_MOVING_WALL_HALFWAY) mhw |= 1 << face;
+    a.nb_den[face] = b.nb_den[face].p;
+    a.nb_u[face] = b.nb_u[face].p;
+    a.hw[face] = b.hw[face].p;
+  }
+  a.aux_face_mask102 = m102;
+  a.aux_face_mask_hw = mhw;
   const dim3 blk = cell_block32(b);
   const dim3 grid = region == kRegionHull ? dim3(hull_grid(b, hole, blk))
                                           : dim3((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   a.zrow = b.zrow;
   const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
+#define L32M(M, F, B)                                                                                  \
+  if (a.nt) hipLaunchKernelGGL((k_collide_push32<M, F, B, true>), grid, blk, lds, b.stream, a);          \
+  else hipLaunchKernelGGL((k_collide_push32<M, F, B, false>), grid, blk, lds, b.stream, a);
 #define L32(F, B)                                                                                      \
-  if (a.nt) hipLaunchKernelGGL((k_collide_push32<F, B, true>), grid, blk, lds, b.stream, a);             \
-  else hipLaunchKernelGGL((k_collide_push32<F, B, false>), grid, blk, lds, b.stream, a);
+  if (b.model == 2) { L32M(2, F, B) }                                                                   \
+  else if (b.model == 3) { L32M(3, F, B) }                                                              \
+  else { L32M(1, F, B) }
   if (force && band) { L32(true, true) }
   else if (force) { L32(true, false) }
   else if (band) { L32(false, true) }
   else { L32(false, false) }
 #undef L32
+#undef L32M
   HIPCHK(hipGetLastError());
   if (region == kRegionBulk) return 0;  // the hull launch completes the step
+  for (int face = 0; face < 6; ++face)
+    if ((mhw >> face) & 1) b.hw_valid[face] = true;
   std::swap(b.gA.p, b.gB.p);
   std::swap(b.gA.n, b.gB.n);
   return 0;
@@ -905,15 +929,12 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
                 iCollidModel);
   if (iCollidModel == 3 && (!m_collid || !m_force)) return fail("MRT needs M_COLLID and M_FORCE");
   if (!(precision == 0 || precision == 1)) return fail("precision must be 0 (FP64) or 1 (FP32 deviation)");
-  if (precision == 1 && iCollidModel != 1) return fail("FP32-deviation form supports SRT only");
+  if (precision == 1 && !(iCollidModel >= 1 && iCollidModel <= 3))
+    return fail("FP32-deviation form supports collision models 1-3 (SRT, TRT, MRT), not %d", iCollidModel);
   if (precision == 1 && (long long)dims[0] * dims[1] * dims[2] > (1LL << 30))
     return fail("FP32-deviation blocks hold at most 2^30 cells (32-bit byte offsets per population plane)");
   if (precision == 0 && (long long)dims[0] * dims[1] * dims[2] > (1LL << 29))
     return fail("FP64 blocks hold at most 2^29 cells (32-bit byte offsets per population plane)");
-  if (precision == 1)
-    for (int k = 0; k < 6; ++k)
-      if (bc[k] == BC_NEQ_DIRICHLET || bc[k] == BC_WALL_HALFWAY || bc[k] == BC_MOVING_WALL_HALFWAY)
-        return fail("FP32-deviation form does not support boundary code %d (face %d)", bc[k], k + 1);
   for (int k = 0; k < 6; ++k) {
     const int c = bc[k];
     const bool ok = c == 0 || c == 1 || c == 101 || c == 102 || c == 103 || c == 104 || c == 201 ||
@@ -972,6 +993,17 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
     HIPCHK(b.Mf.ensure(Q * Q));
     HIPCHK(hipMemcpy(b.Mc.p, m_collid, Q * Q * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(b.Mf.p, m_force, Q * Q * sizeof(double), hipMemcpyHostToDevice));
+    if (precision == 1) {
+      float mc[Q * Q], mf[Q * Q];
+      for (int k = 0; k < Q * Q; ++k) {
+        mc[k] = (float)m_collid[k];
+        mf[k] = (float)m_force[k];
+      }
+      HIPCHK(b.Mc32.ensure(Q * Q));
+      HIPCHK(b.Mf32.ensure(Q * Q));
+      HIPCHK(hipMemcpy(b.Mc32.p, mc, sizeof(mc), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(b.Mf32.p, mf, sizeof(mf), hipMemcpyHostToDevice));
+    }
   }
   for (int face = 0; face < 6; ++face) {
     const int np = plane_size(b, face);
@@ -1126,11 +1158,6 @@ int fsil_f_download(int h, double *fIn) {
 int fsil_set_boundary_conditions(int h) {
   Block *b = get_block(h);
   if (!b) return 1;
-  if (b->precision != 0) {  // FP32 form: no code-102 planes (refused at creation)
-    if (launch_faces(*b)) return 1;
-    HIPCHK(hipStreamSynchronize(b->stream));
-    return 0;
-  }
   // Code 102 reads den/uuu of the first interior plane.  At start-up the
   // reference calls this right after initialise_, where den = denIn and uuu is
   // the inflow profile at every node (FluidDomain.f90:533-545), so those fields

@@ -22,6 +22,8 @@ struct Collide32Args {
   int xd, yd, zd;
   long long N;
   float Omega, c1, dt3;
+  float hO, hO2, cT1, cT2;                  // TRT: 0.5 Omega, 0.5 Omega2, 0.5 - 0.25 Omega, 0.5 - 0.25 Omega2
+  const float *__restrict__ Mc, *__restrict__ Mf;  // MRT: M_COLLID, M_FORCE rounded to FP32, column-major
   float hv[3];
   float VF[3];
   // immersed-boundary band overlay (corrected velocity and force, FP64)
@@ -35,6 +37,12 @@ struct Collide32Args {
              // 2 shift_rows_z_waves (rows of whole waves)
   int region;   // CollideRegion
   int hole[4];  // kRegionBulk / kRegionHull: x0, x1, y0, y1
+  // auxiliary planes of boundary codes 102 (den/u of the first interior
+  // plane) and 203/204 (post-collision boundary plane as f = g + w), FP64
+  int aux_face_mask102, aux_face_mask_hw;
+  double *nb_den[6];
+  double *nb_u[6];  // [3][plane]
+  double *hw[6];    // [q][plane]
 };
 
 __device__ __forceinline__ void kahan_add(float &s, float &c, float v) {
@@ -93,8 +101,14 @@ __device__ __forceinline__ void moments32(const float *g, double &dr, double &j1
 //   g'_q = (1-Omega) g_q + Omega w (A + B),  g'_qbar = (1-Omega) g_qbar + Omega w (A - B)
 // and, in forced cells, the Guo term c1*3dh*w*(e.F (1 + 3 u.e) - u.F) added
 // (FluidDomain.f90:1217-1227 rearranged around w_q).
-template <bool FORCE, bool BAND, bool NT>
-__device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int x, int y, int z, float *zx) {
+// TRT and MRT in the deviation form: with fe_q = (feq_q - w_q) - g_q and the
+// Guo term fl_q = 3 dh w_q (e_q.F (1 + 3 u.e_q) - u.F), the reference's
+// updates (FluidDomain.f90:1228-1238) read g'_q = g_q + (its f_q update);
+// fe of an opposite pair is w (A +- B) - g with the pair's A and B of the SRT
+// form.  MRT's matrices (rounded to FP32) are staged in LDS (ms, 2 x 361).
+template <int MODEL, bool FORCE, bool BAND, bool NT>
+__device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int x, int y, int z, float *zx,
+                                                    const float *ms) {
   const unsigned n = ((unsigned)x * (unsigned)a.yd + (unsigned)y) * (unsigned)a.zd + (unsigned)z;
   const unsigned nb = n << 2;  // blocks of this form hold at most 2^30 cells
   const size_t plane = (size_t)a.N * sizeof(float);
@@ -122,11 +136,79 @@ __device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int 
       }
     }
   }
+  if (a.aux_face_mask102) {
+    // the first interior plane of a code-102 face (as the FP64 form: density
+    // of the stored state, velocity after the band overlay)
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      if (!((a.aux_face_mask102 >> face) & 1)) continue;
+      const int axis = face >> 1, hi = face & 1;
+      const int c = axis == 0 ? x : (axis == 1 ? y : z);
+      const int nd = axis == 0 ? a.xd : (axis == 1 ? a.yd : a.zd);
+      if (c == (hi ? nd - 2 : 1)) {
+        const int p = face_plane_index(face, x, y, z, a.yd, a.zd);
+        const int np = axis == 0 ? a.yd * a.zd : (axis == 1 ? a.xd * a.zd : a.xd * a.yd);
+        a.nb_den[face][p] = 1.0 + dr64;
+        a.nb_u[face][p] = (double)u1;
+        a.nb_u[face][np + p] = (double)u2;
+        a.nb_u[face][2 * np + p] = (double)u3;
+      }
+    }
+  }
   const float omc = 1.0f - a.Omega;
   const float m15 = -1.5f * (u1 * u1 + u2 * u2 + u3 * u3);
   const float rho3 = 3.0f * rho;
   float post[Q];
-  post[0] = __builtin_fmaf(omc, g[0], (a.Omega * (float)wt(0)) * __builtin_fmaf(rho, m15, dr));
+  if constexpr (MODEL == 2 || MODEL == 3) {
+    float fe[Q], fl[Q];
+    fe[0] = (float)wt(0) * __builtin_fmaf(rho, m15, dr) - g[0];
+    const float uF = u1 * F1 + u2 * F2 + u3 * F3;
+    fl[0] = a.dt3 * (float)wt(0) * -uF;
+#pragma unroll
+    for (int p = 0; p < 9; ++p) {
+      const int q = kPairQ[p], o = oppo(q);
+      const float w = (float)wt(q);
+      const float ux = edot(q, u1, u2, u3);
+      const float A = __builtin_fmaf(rho, __builtin_fmaf(4.5f * ux, ux, m15), dr);
+      const float B = rho3 * ux;
+      fe[q] = w * (A + B) - g[q];
+      fe[o] = w * (A - B) - g[o];
+      const float eF = edot(q, F1, F2, F3);
+      const float c = __builtin_fmaf(3.0f * ux, eF, -uF);
+      fl[q] = a.dt3 * w * (c + eF);
+      fl[o] = a.dt3 * w * (c - eF);
+    }
+    if constexpr (MODEL == 2) {
+      // TRT (FluidDomain.f90:1230-1235): rest direction as SRT, pairs split
+      // into their symmetric (Omega) and antisymmetric (Omega2) parts
+      post[0] = g[0] + a.Omega * fe[0] + (forced ? a.c1 * fl[0] : 0.0f);
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        const int q = kPairQ[p], o = oppo(q);
+        float tp = a.hO * (fe[q] + fe[o]);
+        float tm = a.hO2 * (fe[q] - fe[o]);
+        if (forced) {
+          tp = __builtin_fmaf(a.cT1, fl[q] + fl[o], tp);
+          tm = __builtin_fmaf(a.cT2, fl[q] - fl[o], tm);
+        }
+        post[q] = g[q] + (tp + tm);
+        post[o] = g[o] + (tp - tm);
+      }
+    } else {
+      // MRT (FluidDomain.f90:1238): g + M_COLLID^T fe + M_FORCE^T fl
+#pragma unroll
+      for (int i = 0; i < Q; ++i) {
+        float s = 0.0f;
+#pragma unroll
+        for (int j = 0; j < Q; ++j) s = __builtin_fmaf(ms[j * Q + i], fe[j], s);
+        if (forced) {
+#pragma unroll
+          for (int j = 0; j < Q; ++j) s = __builtin_fmaf(ms[Q * Q + j * Q + i], fl[j], s);
+        }
+        post[i] = g[i] + s;
+      }
+    }
+  } else {
 #pragma unroll
   for (int p = 0; p < 9; ++p) {
     const int q = kPairQ[p], o = oppo(q);
@@ -138,6 +220,7 @@ __device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int 
     post[q] = __builtin_fmaf(omc, g[q], OwA + OwB);
     post[o] = __builtin_fmaf(omc, g[o], OwA - OwB);
   }
+  post[0] = __builtin_fmaf(omc, g[0], (a.Omega * (float)wt(0)) * __builtin_fmaf(rho, m15, dr));
   if (FORCE || BAND) {
     if (forced) {
       const float cdt = a.c1 * a.dt3;
@@ -155,6 +238,7 @@ __device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int 
       }
     }
   }
+  }
   const unsigned yz = (unsigned)a.yd * (unsigned)a.zd, zd = (unsigned)a.zd;
   const unsigned sxp = x + 1 == a.xd ? 0u - (unsigned)(a.xd - 1) * yz : yz;
   const unsigned sxm = x == 0 ? (unsigned)(a.xd - 1) * yz : 0u - yz;
@@ -162,6 +246,22 @@ __device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int 
   const unsigned sym = y == 0 ? (unsigned)(a.yd - 1) * zd : 0u - zd;
   const unsigned szp = z + 1 == a.zd ? 0u - (unsigned)(a.zd - 1) : 1u;
   const unsigned szm = z == 0 ? (unsigned)(a.zd - 1) : 0u - 1u;
+  if (a.aux_face_mask_hw) {
+    // the post-collision boundary plane of a code-203/204 face, before the
+    // z-row exchange moves the z-going values between threads
+#pragma unroll
+    for (int face = 0; face < 6; ++face) {
+      if (!((a.aux_face_mask_hw >> face) & 1)) continue;
+      const int axis = face >> 1, hi = face & 1;
+      const int c = axis == 0 ? x : (axis == 1 ? y : z);
+      const int nd = axis == 0 ? a.xd : (axis == 1 ? a.yd : a.zd);
+      if (c == (hi ? nd - 1 : 0)) {
+        const int p = face_plane_index(face, x, y, z, a.yd, a.zd);
+        const int np = axis == 0 ? a.yd * a.zd : (axis == 1 ? a.xd * a.zd : a.xd * a.yd);
+        for (int q = 0; q < Q; ++q) a.hw[face][(long long)q * np + p] = (double)post[q] + wt(q);
+      }
+    }
+  }
   if (a.zrow == 2) shift_rows_z_waves(post, zx);
   else if (a.zrow) shift_rows_z(post, zx);
   const unsigned st[6] = {sxp, sxm, syp, sym, a.zrow ? 0u : szp, a.zrow ? 0u : szm};
@@ -171,13 +271,23 @@ __device__ __forceinline__ void collide_push32_cell(const Collide32Args &a, int 
 // The z-row exchange has workgroup barriers: a workgroup of whole z-rows is
 // skipped or covered as a whole (the host aligns the hole's y-range to the
 // workgroup's rows, and a hull tile is a workgroup's shape inside the block).
-template <bool FORCE, bool BAND, bool NT>
+template <int MODEL, bool FORCE, bool BAND, bool NT>
 __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   // LDS of the exchange, sized at launch (zrow32_lds_bytes)
   extern __shared__ float zx[];
+  const float *ms = nullptr;
+  if constexpr (MODEL == 3) {
+    // the MRT matrices into LDS (every thread reads them in the same order: broadcasts)
+    __shared__ float mlds[2 * Q * Q];
+    const int nt = (int)(blockDim.x * blockDim.y);
+    for (int i = (int)(threadIdx.y * blockDim.x + threadIdx.x); i < 2 * Q * Q; i += nt)
+      mlds[i] = i < Q * Q ? a.Mc[i] : a.Mf[i - Q * Q];
+    __syncthreads();
+    ms = mlds;
+  }
   if (a.region == kRegionHull) {
     for_hull_cells(a.ctl, a.hole, a.xd, a.yd, a.zd, [&](int x, int y, int z) {
-      if (z < a.zd && y < a.yd) collide_push32_cell<FORCE, BAND, NT>(a, x, y, z, zx);
+      if (z < a.zd && y < a.yd) collide_push32_cell<MODEL, FORCE, BAND, NT>(a, x, y, z, zx, ms);
     });
     return;
   }
@@ -186,7 +296,7 @@ __global__ __launch_bounds__(512) void k_collide_push32(Collide32Args a) {
   const int x = blockIdx.z;
   if (z >= a.zd || y >= a.yd) return;
   if (a.region == kRegionBulk && in_hole(a.hole, x, y)) return;
-  collide_push32_cell<FORCE, BAND, NT>(a, x, y, z, zx);
+  collide_push32_cell<MODEL, FORCE, BAND, NT>(a, x, y, z, zx, ms);
 }
 
 // dynamic LDS of k_collide_push32 for its exchange mode and workgroup size

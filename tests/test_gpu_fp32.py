@@ -31,11 +31,15 @@ def _run_block(pc, precision, nsteps):
     return f, u
 
 
-@pytest.mark.parametrize("name", ["bc_inflow_outflow_slip", "bc_moving_walls", "periodic_srt_force"])
+@pytest.mark.parametrize("name", ["bc_inflow_outflow_slip", "bc_moving_walls", "periodic_srt_force", "periodic_trt",
+                                  "periodic_trt_force", "periodic_mrt", "periodic_mrt_force", "bc_mixed_trt_force",
+                                  "bc_neq_extrap2_walls", "bc_halfway_walls"])
 def test_fp32_open_faces_track_fp64(name):
-    """Inflow/outflow/slip, moving walls and a forced periodic block: the FP32
-    form's velocity stays within 1e-5 (relative L2) of the FP64 form after 50
-    steps, and its populations within 1e-6 absolute."""
+    """Inflow/outflow/slip, moving walls, forced periodic blocks, the TRT and
+    MRT collisions with and without a volume force (also TRT with open
+    faces), non-equilibrium extrapolation inflow (102) and halfway walls
+    (203/204): the FP32 form's velocity stays within 1e-5 (relative L2) of the
+    FP64 form after 50 steps, and its populations within 1e-6 absolute."""
     pc = BY_NAME[name]
     print(f"[gpu] fp32 faces {name}", flush=True)
     f64, u64 = _run_block(pc, 0, 50)
