@@ -604,27 +604,39 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
 
 // Inclusive scan of the per-cell entry counts of the band's cells (ctl[kNew],
 // final after k_ibm_claim) into the CSR pointers ptr[1..] (ptr[0] = 0), one
-// 1024-thread workgroup:
-// contiguous per-thread runs, a workgroup scan of the run totals, then the
-// runs written with their offsets.
+// 1024-thread workgroup: contiguous per-thread runs, their totals scanned
+// within each wave by lane shuffles and across the sixteen waves by one wave
+// (two barriers), then the runs written with their offsets.  Integer sums:
+// the result does not depend on the order.
 __global__ __launch_bounds__(1024) void k_band_scan(const int *__restrict__ ctl, const int *__restrict__ count,
                                                      int *__restrict__ ptr) {
-  __shared__ int tot[1024];
+  __shared__ int wtot[16];
   // the cells claimed in this call (k_ibm_rank copies this count to kNband only later)
   const int n = ctl[kNew];
   const int per = (n + 1023) / 1024;
   const int t = threadIdx.x, lo = t * per, hi = min(n, lo + per);
+  const int lane = t & 63, w = t >> 6;
   int s = 0;
   for (int i = lo; i < hi; ++i) s += count[i];
-  tot[t] = s;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    const int v = t >= off ? tot[t - off] : 0;
-    __syncthreads();
-    tot[t] += v;
-    __syncthreads();
+  int v = s;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int o = __shfl_up(v, off, 64);
+    if (lane >= off) v += o;
   }
-  int run = t ? tot[t - 1] : 0;
+  if (lane == 63) wtot[w] = v;
+  __syncthreads();
+  if (w == 0) {
+    int x = lane < 16 ? wtot[lane] : 0;
+#pragma unroll
+    for (int off = 1; off < 16; off <<= 1) {
+      const int o = __shfl_up(x, off, 64);
+      if (lane >= off) x += o;
+    }
+    if (lane < 16) wtot[lane] = x;
+  }
+  __syncthreads();
+  int run = (v - s) + (w ? wtot[w - 1] : 0);
   for (int i = lo; i < hi; ++i) {
     run += count[i];
     ptr[i + 1] = run;
