@@ -428,7 +428,7 @@ def main():
                   "form": "fp32-deviation", "value": s32["mlups"], "unit": "MLUPS", "ms_per_step": s32["ms_per_step"],
                   "mdf_iters_mean": s32["mdf_iters_mean"],
                   "roofline": {"bound": "hbm", "achieved": s32["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                               "frac": s32["achieved_gbs"] / HBM_PEAK_GBS, "kernel": "k_collide_push32<false,true,true>",
+                               "frac": s32["achieved_gbs"] / HBM_PEAK_GBS, "kernel": "k_collide_push32<1,false,true,true>",
                                "kernel_ms": s32["kernel_ms"], "bytes_per_cell": B_CELL[1]},
                   "fp64_plain": {"value": s64["mlups"], "ms_per_step": s64["ms_per_step"],
                                  "kernel_ms": s64["kernel_ms"]},
@@ -483,7 +483,7 @@ def main():
                 "frac": H["achieved_gbs"] / HBM_PEAK_GBS,
                 "traffic": traffic_from_profiles("collide_push32_band_384x192x192_pmc.json" if f32 else
                                                  "collide_push_band_384x192x192_pmc.json"),
-                "kernel": ("k_collide_push32<false,true,true>" if f32 else "k_collide_push<1,false,true,false,true>") +
+                "kernel": ("k_collide_push32<1,false,true,true>" if f32 else "k_collide_push<1,false,true,false,true>") +
                           " (band overlay) of the FSI step",
                 "kernel_ms": H["kernel_ms"], "bytes_per_cell": B_CELL[1 if f32 else 0]}
     out = {
