@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 
 namespace fsil {
 
@@ -29,7 +30,76 @@ struct BeamElem {
   double rot[9];
   double fint[NED], fload[NED], fdamp[NED];  // per-element contributions (internal force, loads)
   double gsc[9];  // geometric-stiffness coefficients gy1..gy4, gz1..gz4, gt (el_geom_scalars)
+  double ksc[10];  // the distinct entries of stl (StiffPattern): EA/L, ky1..ky4, kz1..kz4, GJ/L
+  static constexpr bool kFull = true;
 };
+
+// The fields of an element that a flexible body's Newton iterations read and
+// write, kept in LDS by the device sub-step kernel for the whole launch (the
+// element routines take either struct).  Without stiffness damping (which
+// reads the full rotated stiffness) and without the rotated mass matrix's
+// zero blocks: masb holds its diagonal 3x3 blocks 0 and 1, which blocks 2 and
+// 3 repeat (the same diagonal rotated by the same triad).  An odd number of
+// 8-byte words, so that one element per lane reads distinct LDS banks.
+struct BeamHot {
+  int node0, node1;
+  double dx0, dy0, dz0, len0, dx1, dy1, dz1, len1;
+  double xll1, xmm1, xnn1, geoFRM;
+  double prop[8];
+  double tee[9], tn1[9], tn2[9];
+  double gsc[9], ksc[10];
+  double masb[18];
+  double x1[NED], fint[NED], fload[NED];
+  static constexpr bool kFull = false;
+};
+constexpr int kHotWords = (int)(sizeof(BeamHot) / sizeof(double));
+static_assert(sizeof(BeamHot) % sizeof(double) == 0 && (kHotWords & 1) == 1, "odd word count");
+
+// entry (i, j), in a diagonal 3x3 block, of the rotated mass matrix
+__host__ __device__ inline double mas_entry(const BeamElem &e, int i, int j) { return e.mas[i * NED + j]; }
+__host__ __device__ inline double mas_entry(const BeamHot &e, int i, int j) {
+  return e.masb[((i / 3) & 1) * 9 + (i % 3) * 3 + j % 3];
+}
+
+// word w of BeamHot <- word src[w] of BeamElem (masb from the mas blocks 0, 1)
+struct HotMap {
+  short src[kHotWords];
+  constexpr HotMap() : src() {
+    put(offsetof(BeamHot, node0), offsetof(BeamElem, node0), 1);
+    put(offsetof(BeamHot, dx0), offsetof(BeamElem, dx0), 8);
+    put(offsetof(BeamHot, xll1), offsetof(BeamElem, xll1), 3);
+    put(offsetof(BeamHot, geoFRM), offsetof(BeamElem, geoFRM), 1);
+    put(offsetof(BeamHot, prop), offsetof(BeamElem, prop), 8);
+    put(offsetof(BeamHot, tee), offsetof(BeamElem, tee), 9);
+    put(offsetof(BeamHot, tn1), offsetof(BeamElem, tn1), 9);
+    put(offsetof(BeamHot, tn2), offsetof(BeamElem, tn2), 9);
+    put(offsetof(BeamHot, gsc), offsetof(BeamElem, gsc), 9);
+    put(offsetof(BeamHot, ksc), offsetof(BeamElem, ksc), 10);
+    put(offsetof(BeamHot, x1), offsetof(BeamElem, x1), NED);
+    put(offsetof(BeamHot, fint), offsetof(BeamElem, fint), NED);
+    put(offsetof(BeamHot, fload), offsetof(BeamElem, fload), NED);
+    for (int b = 0; b < 2; ++b)
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          src[offsetof(BeamHot, masb) / 8 + b * 9 + r * 3 + c] =
+              (short)(offsetof(BeamElem, mas) / 8 + (3 * b + r) * NED + 3 * b + c);
+  }
+  constexpr void put(size_t h, size_t e, int n) {
+    for (int k = 0; k < n; ++k) src[h / 8 + k] = (short)(e / 8 + k);
+  }
+};
+
+// 0 when the rotated mass matrix's diagonal blocks 2 and 3 equal blocks 0 and
+// 1 bit for bit, else 1 + the first entry that differs
+__host__ inline int el_mass_blocks_repeat(const BeamElem &e) {
+  for (int b = 0; b < 2; ++b)
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) {
+        const double *u = &e.mas[(3 * b + r) * NED + 3 * b + c], *v = &e.mas[(3 * b + 6 + r) * NED + 3 * b + 6 + c];
+        if (__builtin_memcmp(u, v, sizeof(double)) != 0) return 1 + (3 * b + 6 + r) * NED + 3 * b + 6 + c;
+      }
+  return 0;
+}
 
 __host__ __device__ inline double bpow(double x, int n) {
   double res = 0.0, cur = x;
@@ -172,7 +242,8 @@ __host__ __device__ inline void axis_dir_triad(double l, double m, double n, con
 }
 
 // Segment_cptdxyz1 (SolidSolver.f90:119-129)
-__host__ __device__ inline void el_cptdxyz1(BeamElem &e) {
+template <class ET>
+__host__ __device__ inline void el_cptdxyz1(ET &e) {
   e.dx1 = e.x1[6] - e.x1[0];
   e.dy1 = e.x1[7] - e.x1[1];
   e.dz1 = e.x1[8] - e.x1[2];
@@ -244,9 +315,56 @@ __host__ __device__ inline void el_form_stiff(BeamElem &e) {
 #undef KK
 }
 
+// Pattern of Segment_FormStiffMatrix (SolidSolver.f90:365-478): entry (i, j)
+// of the local stiffness is 0 (the formed matrix's exact +0), k (ksc[k-1]) or
+// -k (its negative, formed by negation); symmetric.  ksc order: EA/L, ky1,
+// ky2, ky3, ky4, kz1, kz2, kz3, kz4, GJ/L.
+struct StiffPattern {
+  signed char c[NED * NED];
+  constexpr StiffPattern() : c() {
+    const int e[26][3] = {{0, 0, 1},  {6, 6, 1},   {0, 6, -1},  {1, 1, 2},   {7, 7, 2},   {1, 7, -2},  {1, 5, 3},
+                          {1, 11, 3}, {5, 7, -3},  {7, 11, -3}, {5, 5, 4},   {11, 11, 4}, {5, 11, 5},  {2, 2, 6},
+                          {8, 8, 6},  {2, 8, -6},  {2, 4, -7},  {2, 10, -7}, {4, 8, 7},   {8, 10, 7},  {4, 4, 8},
+                          {10, 10, 8}, {4, 10, 9}, {3, 3, 10},  {9, 9, 10},  {3, 9, -10}};
+    for (int q = 0; q < 26; ++q) {
+      c[e[q][0] * NED + e[q][1]] = (signed char)e[q][2];
+      c[e[q][1] * NED + e[q][0]] = (signed char)e[q][2];
+    }
+  }
+};
+__device__ constexpr StiffPattern kStiffPatternD{};
+constexpr StiffPattern kStiffPatternH{};
+
+__host__ __device__ inline int stiff_code(int i, int j) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return kStiffPatternD.c[i * NED + j];
+#else
+  return kStiffPatternH.c[i * NED + j];
+#endif
+}
+
+// entry (i, j) of the local stiffness from its scalars
+__host__ __device__ inline double stiff_entry(const double *ksc, int i, int j) {
+  const int k = stiff_code(i, j);
+  return k == 0 ? 0.0 : (k > 0 ? ksc[k - 1] : -ksc[-k - 1]);
+}
+
+// ksc from the formed stl; 0 when the pattern reproduces every entry of stl
+// bit for bit, else 1 + the first entry that differs
+__host__ inline int el_stiff_scalars(BeamElem &e) {
+  const int at[10][2] = {{0, 0}, {1, 1}, {1, 5}, {5, 5}, {5, 11}, {2, 2}, {4, 8}, {4, 4}, {4, 10}, {3, 3}};
+  for (int k = 0; k < 10; ++k) e.ksc[k] = e.stl[at[k][0] * NED + at[k][1]];
+  for (int k = 0; k < NED * NED; ++k) {
+    const double v = stiff_entry(e.ksc, k / NED, k % NED);
+    if (__builtin_memcmp(&v, &e.stl[k], sizeof(double)) != 0) return 1 + k;
+  }
+  return 0;
+}
+
 // Segment_FormGeomMatrix (SolidSolver.f90:480-600): the nine distinct
 // coefficients of the geometric stiffness, into e.gsc
-__host__ __device__ inline void el_geom_scalars(BeamElem &e) {
+template <class ET>
+__host__ __device__ inline void el_geom_scalars(ET &e) {
   const double s = e.geoFRM;
   const double emod = e.prop[0], gmod = e.prop[1], area = e.prop[2], zix = e.prop[5], ziy = e.prop[6],
                ziz = e.prop[7];
@@ -345,7 +463,8 @@ __host__ __device__ inline double rkr_entry(const double *ek, const double *r, i
 
 // Segment_BodyStress_D (SolidSolver.f90:750-839): local internal force -> e.fint,
 // with the local stiffness of Segment_FormStiffMatrix (e.stl, formed once)
-__host__ __device__ inline void el_body_stress(BeamElem &e) {
+template <class ET>
+__host__ __device__ inline void el_body_stress(ET &e) {
   const double du = e.dx1 - e.dx0, dv = e.dy1 - e.dy0, dw = e.dz1 - e.dz0;
   const double dl = ((e.dx0 + e.dx1) * du + (e.dy0 + e.dy1) * dv + (e.dz0 + e.dz1) * dw) / (e.len0 + e.len1);
   double t[3], a[3], b[3];
@@ -360,10 +479,20 @@ __host__ __device__ inline void el_body_stress(BeamElem &e) {
   b[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
   const double ub[NED] = {0.0, 0.0, 0.0, a[0], a[1], a[2], dl, 0.0, 0.0, b[0], b[1], b[2]};
   e.geoFRM = dl * e.prop[0] * e.prop[2] / e.len0;
+  // fb = stl ub in ascending k.  The terms with a zero stiffness entry, and
+  // those with k in {0, 1, 2, 7, 8} where ub is zero, add a signed zero to a
+  // sum that starts at +0 and so change nothing; they are left out, and the
+  // entries come from the ten scalars rather than the stored matrix.
   double fb[NED];
+#pragma unroll
   for (int i = 0; i < NED; ++i) {
     double s = 0.0;
-    for (int k = 0; k < NED; ++k) s = s + e.stl[i * NED + k] * ub[k];
+#pragma unroll
+    for (int k = 0; k < NED; ++k) {
+      if (k == 0 || k == 1 || k == 2 || k == 7 || k == 8) continue;
+      if (stiff_code(i, k) == 0) continue;
+      s = s + stiff_entry(e.ksc, i, k) * ub[k];
+    }
     fb[i] = s;
   }
   for (int i = 0; i < NED; ++i) e.fint[i] = 0.0;
@@ -426,7 +555,8 @@ __host__ __device__ inline void el_update_matrix(BeamElem &e, const double *c, d
 
 // Segment_UpdateLoad (SolidSolver.f90:159-193) -> e.fload (mass load, then the
 // stiffness-damping load added in the same order as two LocToGlobal calls)
-__host__ __device__ inline void el_update_load(BeamElem &e, const double *c, double dampM, double dampK,
+template <class ET>
+__host__ __device__ inline void el_update_load(ET &e, const double *c, double dampM, double dampK,
                                                const double *dspO, const double *dsp, const double *vel,
                                                const double *acc) {
   double qM[NED], qC[NED], qMC[NED];
@@ -440,26 +570,29 @@ __host__ __device__ inline void el_update_load(BeamElem &e, const double *c, dou
   for (int i = 0; i < NED; ++i) qMC[i] = qM[i] + dampM * qC[i];
   for (int i = 0; i < NED; ++i) e.fload[i] = 0.0;
   for (int i = 0; i < 3; ++i) {
-    e.fload[i] = e.mas[i * NED + i] * qMC[i];
-    e.fload[i + 6] = e.mas[(i + 6) * NED + i + 6] * qMC[i + 6];
+    e.fload[i] = mas_entry(e, i, i) * qMC[i];
+    e.fload[i + 6] = mas_entry(e, i + 6, i + 6) * qMC[i + 6];
   }
   for (int i = 0; i < 3; ++i) {
     double s = 0.0, t = 0.0;
-    for (int k = 0; k < 3; ++k) s = s + e.mas[(3 + i) * NED + 3 + k] * qMC[3 + k];
-    for (int k = 0; k < 3; ++k) t = t + e.mas[(9 + i) * NED + 9 + k] * qMC[9 + k];
+    for (int k = 0; k < 3; ++k) s = s + mas_entry(e, 3 + i, 3 + k) * qMC[3 + k];
+    for (int k = 0; k < 3; ++k) t = t + mas_entry(e, 9 + i, 9 + k) * qMC[9 + k];
     e.fload[3 + i] = s;
     e.fload[9 + i] = t;
   }
-  if (dampK > 0.0)
-    for (int i = 0; i < NED; ++i) {
-      double s = 0.0;
-      for (int k = 0; k < NED; ++k) s = s + e.stf[i * NED + k] * qC[k];
-      e.fdamp[i] = dampK * s;
-    }
+  if constexpr (ET::kFull) {
+    if (dampK > 0.0)
+      for (int i = 0; i < NED; ++i) {
+        double s = 0.0;
+        for (int k = 0; k < NED; ++k) s = s + e.stf[i * NED + k] * qC[k];
+        e.fdamp[i] = dampK * s;
+      }
+  }
 }
 
 // Segment_UpdateTriad_D + Segment_MakeTriad_ee (SolidSolver.f90:966-1064)
-__host__ __device__ inline void el_update_triads(BeamElem &e, const double *dspnn) {
+template <class ET>
+__host__ __device__ inline void el_update_triads(ET &e, const double *dspnn) {
   double rr[9];
   finite_rot(dspnn[6 * e.node0 + 3], dspnn[6 * e.node0 + 4], dspnn[6 * e.node0 + 5], rr);
   bmm3(rr, e.tn1, e.tn1);

@@ -208,7 +208,10 @@ struct Fem {
   DevBuf<double> info;  // fsil_solid_info: per-element energies and the 78 row values
   bool markers_on_device = false;
   size_t lds_limit = 0;  // the device's LDS bytes per workgroup
-  size_t lds_set = 64 * 1024;  // dynamic LDS bytes k_fem_substep may take
+  size_t lds_set[2] = {64 * 1024, 64 * 1024};  // dynamic LDS bytes k_fem_substep<false>, <true> may take
+  // every element's rotated mass blocks repeat (the LDS element form holds
+  // them); FSIL_FEM_HOT=0 keeps the element fields in global memory
+  bool hot_ok = true;
 };
 
 // A stream shared by the blocks of one tree (father and sons run in order).
@@ -829,6 +832,7 @@ void fem_init_body(const double *xyz8, int nND, int nEL, const int *elems, const
     el_rkr(e, e.mas);
     el_form_stiff(e);
     for (int k = 0; k < NED * NED; ++k) e.stl[k] = e.stf[k];
+    (void)el_stiff_scalars(e);  // checked by the caller
   }
 }
 
@@ -1853,6 +1857,10 @@ int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, c
   int mcount = 0;
   F.desc.assign(nbodies, FemBodyDesc{});
   F.bh.assign(nbodies, FemBodyHost{});
+  {
+    const char *e = getenv("FSIL_FEM_HOT");
+    F.hot_ok = !(e && !strcmp(e, "0"));
+  }
   for (int bi = 0; bi < nbodies; ++bi) {
     if (nND[bi] < 2 || nEL[bi] < 1) return fail("body %d: needs nodes and elements", bi + 1);
     if (nND[bi] > 510)  // the solver keeps a body's CG products in one LDS staging area
@@ -1866,6 +1874,11 @@ int fsil_fem_define_bodies(int h, int nbodies, const int *nND, const int *nEL, c
     std::vector<double> pos, vel;
     fem_init_body(xyz8 + 8 * xo, nND[bi], nEL[bi], elems + 3 * eo, cons + 6 * xo, iBodyModel[bi],
                   isMotionGiven + 6 * bi, mat + 4 * bi, kin + 22 * bi, Lref, Uref, denIn, F.bh[bi], els, pos, vel);
+    for (int e = 0; e < nEL[bi]; ++e) {
+      if (const int k = el_stiff_scalars(els[e]))
+        return fail("body %d element %d: local stiffness entry %d off its pattern", bi + 1, e + 1, k);
+      if (el_mass_blocks_repeat(els[e])) F.hot_ok = false;
+    }
     FemBodyDesc &d = F.desc[bi];
     d.nND = nND[bi]; d.nEL = nEL[bi]; d.eoff = F.nEL; d.noff = F.nND; d.goff = 6 * F.nND;
     d.iBodyModel = iBodyModel[bi];
@@ -1998,43 +2011,50 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   FemArrays A = fem_arrays(*b);
   int max_nd = 1;
   for (const FemBodyDesc &d : F.desc) max_nd = std::max(max_nd, d.nND);
-  const int lds_nodes = std::min(max_nd, 512 / 6);  // 6 threads per node; 6x12 FP64 work areas (<= 48 KiB)
   int max_ne = 1;
   for (const FemBodyDesc &d : F.desc) max_ne = std::max(max_ne, d.nEL);
-  // LDS: the CG vectors and flags, then the element matrices, as far as the
-  // device's workgroup LDS holds them (beyond 64 KiB after raising the
-  // kernel's dynamic-LDS limit)
+  // LDS: the work area (dot-product staging, and the CG's assembly lists when
+  // they fit), the CG vectors and flags, the element matrices and the element
+  // fields, each as far as the device's workgroup LDS holds them (beyond
+  // 64 KiB after raising the kernel's dynamic-LDS limit)
   const size_t lds_lim = F.lds_limit > 1024 ? F.lds_limit - 1024 : 64 * 1024;
-  auto allow = [&](size_t bytes) {
-    if (bytes <= F.lds_set) return true;
-    if (hipFuncSetAttribute((const void *)k_fem_substep, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) ==
-        hipSuccess) {
-      F.lds_set = bytes;
+  auto allow = [&](int hot, size_t bytes) {
+    if (bytes <= F.lds_set[hot]) return true;
+    const void *k = hot ? (const void *)k_fem_substep<true> : (const void *)k_fem_substep<false>;
+    if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes) == hipSuccess) {
+      F.lds_set[hot] = bytes;
       return true;
     }
     (void)hipGetLastError();
     return false;
   };
-  int vec_lds = 0, coef_off = 0;
-  size_t lds_bytes = fem_lds_bytes(lds_nodes, 0, max_nd, max_ne);
+  size_t cg_need = 0;
+  for (const FemBodyDesc &d : F.desc) cg_need = std::max(cg_need, fem_cg_work(d.nND, d.nEL));
+  const int stage = std::max(12 * max_nd, 16);  // 2 nG of dot-product staging, >= one maximum per wave
+  // (the areas' offsets are even, from the kernel's 16-byte aligned base)
+  int cg_lds = 1, work = ((int)std::max(cg_need, (size_t)stage) + 1) & ~1;
+  if (fem_lds_bytes(work, 1, max_nd, max_ne) > lds_lim) {
+    cg_lds = 0;
+    work = (stage + 1) & ~1;
+  }
+  int vec_lds = 0, coef_off = 0, hot_off = 0;
+  size_t lds_bytes = fem_lds_bytes(work, 0, max_nd, max_ne);
   {
-    const size_t b1 = fem_lds_bytes(lds_nodes, 1, max_nd, max_ne);
-    if (b1 <= lds_lim && allow(b1)) {
+    const size_t b1 = fem_lds_bytes(work, 1, max_nd, max_ne);
+    if (b1 <= lds_lim && allow(0, b1)) {
       vec_lds = 1;
       lds_bytes = b1;
       const size_t b2 = b1 + (size_t)max_ne * kCoefStride * sizeof(double);
-      if (b2 <= lds_lim && allow(b2)) {
-        coef_off = (int)(b1 / sizeof(double));
+      if (b2 <= lds_lim && allow(0, b2)) {
+        coef_off = (int)(b1 / sizeof(double)) - 1;  // past the slack double
         lds_bytes = b2;
+        const size_t b3 = b2 + (size_t)max_ne * sizeof(BeamHot);
+        if (F.hot_ok && F.dampK <= 0.0 && b3 <= lds_lim && allow(1, b3)) {
+          hot_off = (int)(b2 / sizeof(double)) - 1;
+          lds_bytes = b3;
+        }
       }
     }
-  }
-  // CG assembly lists in the block-inverse area: 2 nG staging + 12 nEL products + ints
-  int cg_lds = 1;
-  for (const FemBodyDesc &d : F.desc) {
-    const int nG = 6 * d.nND, nent = NED * d.nEL;
-    const size_t need = 2 * (size_t)nG + NED * (size_t)d.nEL + ((size_t)nG + 1 + nent + 1) / 2;
-    if (need > 72 * (size_t)lds_nodes) cg_lds = 0;
   }
   const size_t n = (size_t)numsubstep * F.nbodies;
   if (n > S.cap) {
@@ -2073,8 +2093,12 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
     HIPCHK(hipEventRecord(S.ev_in, b->stream));
     HIPCHK(hipStreamWaitEvent(S.s, S.ev_in, 0));
   }
-  hipLaunchKernelGGL(k_fem_substep, dim3(F.nbodies), dim3(512), lds_bytes, S.s, S.d_stage, F.nbodies, numsubstep, A,
-                     lds_nodes, vec_lds, cg_lds, coef_off);
+  if (hot_off)
+    hipLaunchKernelGGL(k_fem_substep<true>, dim3(F.nbodies), dim3(512), lds_bytes, S.s, S.d_stage, F.nbodies,
+                       numsubstep, A, work, vec_lds, cg_lds, coef_off, hot_off);
+  else
+    hipLaunchKernelGGL(k_fem_substep<false>, dim3(F.nbodies), dim3(512), lds_bytes, S.s, S.d_stage, F.nbodies,
+                       numsubstep, A, work, vec_lds, cg_lds, coef_off, 0);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(S.ev_done, S.s));
   S.n_staged = (int)n;

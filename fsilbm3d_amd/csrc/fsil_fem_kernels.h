@@ -290,33 +290,68 @@ __device__ inline void fem_dot2(const double *a, const double *b, const double *
   fem_dot_chunks(a, b, c, d, n, shared, stage, cap, r1, r2, tree);
 }
 
-// LDS bytes of k_fem_substep: lds_nodes work areas of 6x12 doubles for the
-// block inverses (also the staging area of the dot products) and one int
-// flag per node; then, when vec_lds, the five CG vectors, the inverse blocks,
-// the lifted fixed values, the fixed flags and the element node pairs of the
-// largest body; then, optionally, kCoefStride doubles per element of the
-// largest body (the element matrices, at coef_off doubles).
-__host__ __device__ inline size_t fem_lds_bytes(int lds_nodes, int vec_lds, int max_nd, int max_ne) {
-  size_t d = 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
+// LDS of k_fem_substep, in doubles, behind one double of slack (fem_lds_base)
+// and each area an even number of doubles: a work area of `work` doubles (the
+// staging area of the dot products, 2 nG, and, when cg_lds, the CG's
+// assembly lists and element products behind it; also the waves' maxima);
+// then, when vec_lds, the five CG vectors, the inverse blocks, the lifted
+// fixed values, the fixed flags and the element node pairs of the largest
+// body; then, optionally, kCoefStride doubles per element of the largest body
+// (the element matrices, at coef_off doubles); then, optionally, one BeamHot
+// per element of the largest body (at hot_off doubles).
+__host__ __device__ inline size_t fem_lds_bytes(int work, int vec_lds, int max_nd, int max_ne) {
+  size_t d = 1 + (((size_t)work + 1) & ~(size_t)1);
   // five CG vectors, inverse blocks, xF; the fixed flags and element node pairs (ints)
-  if (vec_lds) d += 6 * 6 * (size_t)max_nd + 36 * (size_t)max_nd + (6 * (size_t)max_nd + 2 * (size_t)max_ne + 1) / 2;
+  if (vec_lds)
+    d += (6 * 6 * (size_t)max_nd + 36 * (size_t)max_nd + (6 * (size_t)max_nd + 2 * (size_t)max_ne + 1) / 2 + 1) &
+         ~(size_t)1;
   return d * sizeof(double);
 }
 
-// One sub-step of one body by the whole workgroup.
-__device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int lds_nodes, int vec_lds, int cg_lds,
-                                        int coef_off) {
+// work-area doubles the CG's assembly lists need for one body (cg_lds)
+__host__ __device__ inline size_t fem_cg_work(int nND, int nEL) {
+  const size_t nG = 6 * (size_t)nND, nent = NED * (size_t)nEL;
+  return 2 * nG + nent + (nG + 1 + nent + 1) / 2;
+}
+
+__device__ constexpr HotMap kHotMapD{};
+
+// The dynamic LDS of k_fem_substep from its first 16-byte boundary (the host
+// reserves one double of slack; its offsets are even): measured on MI355X,
+// the CG loop of the flexible plate takes 39 us per sub-step with its vectors
+// on 16-byte boundaries against 48 us 8 bytes past them.
+__device__ inline double *fem_lds_base() {
+  extern __shared__ double lds_raw[];
+  return lds_raw + (((size_t)lds_raw & 15) ? 1 : 0);
+}
+
+// the stiffness and geometric-stiffness patterns, staged in LDS by the kernel
+struct FemPatterns {
+  signed char stiff[NED * NED], geo[NED * NED];
+};
+
+// One sub-step of one body by the whole workgroup.  HOT: the body's element
+// fields of the Newton iterations are in LDS (H, BeamHot per element), else
+// every element access goes to the global BeamElem array.
+template <bool HOT>
+__device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int work, int vec_lds, int cg_lds,
+                                        int coef_off, BeamHot *H, const FemPatterns &P) {
   __shared__ double sh_val;
   __shared__ double sh_pair[2];
   __shared__ int sh_flag;
-  extern __shared__ double lds[];
+  double *lds = fem_lds_base();
   const int nE = D.nEL, nN = D.nND, nG = 6 * nN, g0 = D.goff;
+  // element e of this body (body-local index)
+  auto EL = [&](int e) -> auto & {
+    if constexpr (HOT) return H[e];
+    else return A.el[D.eoff + e];
+  };
   // the element matrices (coef) in LDS, transposed per element ([k][i]), when
   // they fit behind the other areas
   double *coefL = coef_off ? lds + coef_off : nullptr;
   FemVecs V;
   if (vec_lds) {
-    double *vb = lds + 72 * (size_t)lds_nodes + ((size_t)lds_nodes + 1) / 2;
+    double *vb = lds + work;
     V.x = vb; V.r = vb + nG; V.p = vb + 2 * nG; V.z = vb + 3 * nG; V.Ap = vb + 4 * nG; V.blk = vb + 5 * nG;
     V.xF = V.blk + 36 * (size_t)nN;
     int *fx = reinterpret_cast<int *>(V.xF + nG);
@@ -374,9 +409,8 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
   fem_sync();
   for (int g = threadIdx.x; g < nG; g += blockDim.x) {
     const int last = A.dof_ptr[g0 + g + 1] - 1;
-    const BeamElem &E = A.el[A.dof_e[last]];
     const int li = A.dof_i[last];
-    A.vBC[g0 + g] = E.xnxt[li] - E.x1[li];
+    A.vBC[g0 + g] = A.el[A.dof_e[last]].xnxt[li] - EL(A.dof_e[last] - D.eoff).x1[li];
     A.lodExte[g0 + g] = (A.lod[g0 + g] + 0.0) + 0.0;   // lodFlow + lodGrav + lodRepl (gravity, repulsion zero)
     A.dspO[g0 + g] = A.dsp[g0 + g];
     A.velO[g0 + g] = A.vel[g0 + g];
@@ -391,10 +425,9 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     // UpdateMatrixANDLoad (SolidSolver.f90:1932-1947).  Per element: internal
     // force from the local stiffness, local geometric stiffness, rotation.
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
-      BeamElem &E = A.el[D.eoff + e];
+      auto &E = EL(e);
       el_body_stress(E);
       el_geom_scalars(E);
-      el_rotate(E);
     }
     fem_sync();
     // Per 3x3 block of the element matrices: R^T K R, R^T G R and the Newmark
@@ -402,21 +435,20 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     // entry is summed as el_rkr sums it.
     for (int t = threadIdx.x; t < nE * 16; t += blockDim.x) {
       const int e = t >> 4, blkij = t & 15;
-      BeamElem &E = A.el[D.eoff + e];
+      auto &E = EL(e);
       const int j1 = (blkij >> 2) * 3, j2 = (blkij & 3) * 3;
       double r[9], ks[9], kg[9];
-      double gs[9];
+      // the rotation R (RotateMatrix: the transposed triad tee)
 #pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        r[q] = E.rot[q];
-        gs[q] = E.gsc[q];
-      }
+      for (int q = 0; q < 9; ++q) r[q] = E.tee[(q % 3) * 3 + q / 3];
 #pragma unroll
       for (int a2 = 0; a2 < 3; ++a2)
 #pragma unroll
         for (int b2 = 0; b2 < 3; ++b2) {
-          ks[a2 * 3 + b2] = E.stl[(j1 + a2) * NED + j2 + b2];
-          kg[a2 * 3 + b2] = geo_entry(gs, j1 + a2, j2 + b2);
+          const int at = (j1 + a2) * NED + j2 + b2;
+          const int cs = P.stiff[at], cg = P.geo[at];
+          ks[a2 * 3 + b2] = cs == 0 ? 0.0 : (cs > 0 ? E.ksc[cs - 1] : -E.ksc[-cs - 1]);
+          kg[a2 * 3 + b2] = cg == 0 ? 0.0 : (cg > 0 ? E.gsc[cg - 1] : -E.gsc[-cg - 1]);
         }
 #pragma unroll
       for (int k = 0; k < 3; ++k)
@@ -436,27 +468,35 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
           }
           const int idx = (j1 + k) * NED + j2 + ii;
           const double tanv = sR + D.GeoGamma * gR;
-          double cv = tanv + c[0] * E.mas[idx] + c[1] * D.dampM * E.mas[idx];
-          if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
-          if (D.dampK > 0.0) E.stf[idx] = sR;  // read only by the stiffness-damping load
-          if (coefL) coefL[e * kCoefStride + (j2 + ii) * NED + j1 + k] = cv;
-          else E.coef[idx] = cv;
+          // the rotated mass matrix is block diagonal: its off-diagonal 3x3
+          // blocks hold exact +0 (the rotation of a zero block)
+          const double m = j1 == j2 ? mas_entry(E, j1 + k, j2 + ii) : 0.0;
+          double cv = tanv + c[0] * m + c[1] * D.dampM * m;
+          if constexpr (!HOT) {  // the LDS form is taken without stiffness damping
+            if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
+            if (D.dampK > 0.0) E.stf[idx] = sR;  // read only by the stiffness-damping load
+          }
+          if (coefL) {
+            coefL[e * kCoefStride + (j2 + ii) * NED + j1 + k] = cv;
+          } else {
+            if constexpr (!HOT) E.coef[idx] = cv;  // the LDS form always has coefL
+          }
         }
     }
     fem_sync();
-    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
-      BeamElem &E = A.el[D.eoff + e];
-      el_update_load(E, c, D.dampM, D.dampK, A.dspO + g0, A.dsp + g0, A.vel + g0, A.acc + g0);
-    }
+    for (int e = threadIdx.x; e < nE; e += blockDim.x)
+      el_update_load(EL(e), c, D.dampM, D.dampK, A.dspO + g0, A.dsp + g0, A.vel + g0, A.acc + g0);
     fem_sync();
     for (int g = threadIdx.x; g < nG; g += blockDim.x) {
       double li = 0.0;
-      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k) li = li + A.el[A.dof_e[k]].fint[A.dof_i[k]];
+      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
+        li = li + EL(A.dof_e[k] - D.eoff).fint[A.dof_i[k]];
       A.lodInte[g0 + g] = li;
       double le = A.lodExte[g0 + g] - li;
       for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k) {
-        le = le + A.el[A.dof_e[k]].fload[A.dof_i[k]];
-        if (D.dampK > 0.0) le = le + A.el[A.dof_e[k]].fdamp[A.dof_i[k]];
+        le = le + EL(A.dof_e[k] - D.eoff).fload[A.dof_i[k]];
+        if constexpr (!HOT)
+          if (D.dampK > 0.0) le = le + A.el[A.dof_e[k]].fdamp[A.dof_i[k]];
       }
       A.lodEffe[g0 + g] = le;
     }
@@ -646,10 +686,11 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     if (threadIdx.x == 0) sh_val = 0.0;
     fem_sync();
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
-      BeamElem &E = A.el[D.eoff + e];
+      auto &E = EL(e);
+      const double *x0 = A.el[D.eoff + e].x0;
       for (int k = 0; k < 6; ++k) {
-        E.x1[k] = E.x0[k] + A.dsp[g0 + 6 * E.node0 + k];
-        E.x1[6 + k] = E.x0[6 + k] + A.dsp[g0 + 6 * E.node1 + k];
+        E.x1[k] = x0[k] + A.dsp[g0 + 6 * E.node0 + k];
+        E.x1[6 + k] = x0[6 + k] + A.dsp[g0 + 6 * E.node1 + k];
       }
       el_cptdxyz1(E);
       el_update_triads(E, V.x);
@@ -678,7 +719,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     fem_sync();
     for (int n = threadIdx.x; n < nN; n += blockDim.x) {
       const int last = A.node_ptr[D.noff + n + 1] - 1;
-      const BeamElem &E = A.el[A.node_e[last]];
+      const auto &E = EL(A.node_e[last] - D.eoff);
       const double *src = A.node_h[last] ? E.x1 + 6 : E.x1;
       for (int k = 0; k < 6; ++k) A.pos[g0 + 6 * n + k] = src[k];
     }
@@ -706,25 +747,60 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
 // at raised priority (s_setprio): the collide kernel's waves that share the CU
 // are HBM-bound and lose little, while this chain of dependent reductions
 // would otherwise wait behind them for every instruction.
+//
+// HOT: a flexible body's element fields of the Newton iterations (BeamHot)
+// are read from the element array into LDS at hot_off before the first
+// sub-step and written back after the last; the host takes this form when
+// the LDS holds it, no stiffness damping is set and every element's mass
+// blocks repeat (BeamHot).
+template <bool HOT>
 __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, int nbodies, int nsub, FemArrays A,
-                                                   int lds_nodes, int vec_lds, int cg_lds, int coef_off) {
+                                                   int work, int vec_lds, int cg_lds, int coef_off, int hot_off) {
   __builtin_amdgcn_s_setprio(3);
+  double *lds = fem_lds_base();
   // descs is pinned host memory: each sub-step's descriptor is read into LDS
   // once, and its results are written back
   __shared__ FemBodyDesc sD;
+  __shared__ FemPatterns sP;
   static_assert(sizeof(FemBodyDesc) % 8 == 0 && sizeof(FemBodyDesc) / 8 <= 512, "descriptor copy by 8-byte words");
+  for (int t = threadIdx.x; t < NED * NED; t += blockDim.x) {
+    sP.stiff[t] = kStiffPatternD.c[t];
+    sP.geo[t] = kGeoPatternD.c[t];
+  }
+  BeamHot *H = HOT ? reinterpret_cast<BeamHot *>(lds + hot_off) : nullptr;
+  long long *hw = reinterpret_cast<long long *>(H);
+  bool hot = false;
   for (int isub = 0; isub < nsub; ++isub) {
     FemBodyDesc *g = descs + (size_t)isub * nbodies + blockIdx.x;
     if (threadIdx.x < sizeof(FemBodyDesc) / 8)
       reinterpret_cast<long long *>(&sD)[threadIdx.x] = reinterpret_cast<const long long *>(g)[threadIdx.x];
     fem_sync();
-    fem_substep_body(sD, A, lds_nodes, vec_lds, cg_lds, coef_off);
+    if constexpr (HOT) {
+      if (isub == 0 && sD.iBodyModel != 1) {
+        hot = true;
+        const int nW = sD.nEL * kHotWords;
+        for (int t = threadIdx.x; t < nW; t += blockDim.x) {
+          const int e = t / kHotWords, w = t - e * kHotWords;
+          hw[t] = reinterpret_cast<const long long *>(A.el + sD.eoff + e)[kHotMapD.src[w]];
+        }
+        fem_sync();
+      }
+    }
+    // (a rigid body's sub-step does not reach the element fields)
+    fem_substep_body<HOT>(sD, A, work, vec_lds, cg_lds, coef_off, H, sP);
     fem_sync();
     if (threadIdx.x == 0) {
       g->newton_iters = sD.newton_iters;
       g->cg_iters = sD.cg_iters;
       g->status = sD.status;
       g->dnorm = sD.dnorm;
+    }
+  }
+  if (hot) {
+    const int nW = sD.nEL * kHotWords;
+    for (int t = threadIdx.x; t < nW; t += blockDim.x) {
+      const int e = t / kHotWords, w = t - e * kHotWords;
+      reinterpret_cast<long long *>(A.el + sD.eoff + e)[kHotMapD.src[w]] = hw[t];
     }
   }
 }
