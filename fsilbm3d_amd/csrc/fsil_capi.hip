@@ -279,6 +279,7 @@ struct Block {
   bool split = false;             // the coupling's launches go to the beam stream
   bool split_ok = true;           // FSIL_SPLIT=0 turns split steps off
   int split_margin = 3;           // cells the predicted band box adds to the last one (FSIL_SPLIT_MARGIN)
+  size_t bulk_lds = 0;            // dynamic LDS of an FP32 split step's bulk launch (its CU residency)
 };
 
 std::map<int, std::unique_ptr<Block>> g_blocks;
@@ -681,7 +682,8 @@ int collide_stream32(Block &b, int region = kRegionAll, const int *hole = nullpt
   const dim3 grid = region == kRegionHull ? dim3(hull_grid(b, hole, blk))
                                           : dim3((b.zd + blk.x - 1) / blk.x, (b.yd + blk.y - 1) / blk.y, b.xd);
   a.zrow = b.zrow;
-  const size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
+  size_t lds = zrow32_lds_bytes(a.zrow, blk.x * blk.y);
+  if (region == kRegionBulk) lds = std::max(lds, b.bulk_lds);  // (the kernel reads only the first bytes)
 #define L32M(M, F, B)                                                                                  \
   if (a.nt) hipLaunchKernelGGL((k_collide_push32<M, F, B, true>), grid, blk, lds, b.stream, a);          \
   else hipLaunchKernelGGL((k_collide_push32<M, F, B, false>), grid, blk, lds, b.stream, a);
@@ -974,6 +976,24 @@ int fsil_block_create(int id, const int *dims, const double *geom, const int *bc
     b.split_ok = !(e && !strcmp(e, "0"));
     const char *m = getenv("FSIL_SPLIT_MARGIN");
     if (m) b.split_margin = atoi(m);
+    // The bulk launch of an FP32 split step runs beside the coupling call,
+    // whose small launches wait for CU slots while the bulk's workgroups fill
+    // every CU.  The bulk asks for enough LDS that at most R of its
+    // workgroups share a CU (FSIL_BULK_RESIDENCY=R, default 2; 0: no limit).
+    // Measured on MI355X, flexible plate 384x192x192: 0.714 ms per step
+    // unlimited (6 per CU), 0.689 at 4, 0.685 at 3, 0.675 at 2; the bulk
+    // itself takes 489 us at 2 against 405 us, the coupling's band build
+    // 57 us against 200 us.
+    int R = 2;
+    if (const char *r = getenv("FSIL_BULK_RESIDENCY")) R = atoi(r);
+    int dev = 0, per_cu = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (R > 0 && hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) ==
+                     hipSuccess && per_cu > 0) {
+      const size_t want = (size_t)per_cu / (size_t)(R + 1) + 1024;
+      if (want <= 64 * 1024) b.bulk_lds = want;  // within the default dynamic-LDS limit
+    }
+    (void)hipGetLastError();
   }
   const size_t nf = (size_t)Q * b.N;
   if (precision == 0) {

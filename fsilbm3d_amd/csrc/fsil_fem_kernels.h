@@ -290,6 +290,153 @@ __device__ inline void fem_dot2(const double *a, const double *b, const double *
   fem_dot_chunks(a, b, c, d, n, shared, stage, cap, r1, r2, tree);
 }
 
+// Wave sum of a by lane shuffles (lane 0 holds it)
+__device__ inline double wave_sum_down(double a) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) a = a + __shfl_down(a, off, 64);
+  return a;
+}
+
+// Row g of the element-by-element product A v (Beam_MatrixMultipy,
+// SolidSolver.f90:2035-2049, then LocToGlobal): each element product in
+// ascending k, the products added in the DOF's element order, as fem_matvec
+// forms them.  vk(d) gives v at the body's DOF d.
+template <class VK>
+__device__ inline double fem_row(const FemVecs &V, const CgLds &C, int g, VK &&vk) {
+  double y = 0.0;
+  for (int q = C.ptr[g]; q < C.ptr[g + 1]; ++q) {
+    const int ent = C.ent[q], e = ent / NED, i = ent - e * NED;
+    const int n0 = 6 * V.en[2 * e], n1 = 6 * V.en[2 * e + 1];
+    const double *cc = C.coef + (size_t)e * kCoefStride + i;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NED; ++k) s = s + cc[k * NED] * vk(k < 6 ? n0 + k : n1 + (k - 6));
+    y = y + s;
+  }
+  return y;
+}
+
+// CG_Solve (SolidSolver.f90:1949-2020) for the forms whose dot products are
+// summed in a tree (FP32-deviation blocks), with the CG state in LDS: three
+// workgroup barriers per iteration (after A p and p.Ap, after the x, r, z
+// updates and r.r, r.z, after the direction update).  DOFs are owned sixty
+// (ten whole nodes) per wave, so that a node's block-Jacobi row reads r of its
+// own wave; each owner forms its row of A p, its updates and its terms of the
+// dot products, which are summed by wave shuffles and then over the waves in
+// wave order.  (Forming p = z + beta p at every reader instead, which saves
+// the third barrier, measured 29 us per sub-step for the A p phase against
+// 17 us: the repeated reads cost more than the barrier.)  x holds xF on
+// entry; returns 2 on a breakdown (zero p.Ap or r.z), else 0; adds the
+// iterations to cg_total.
+__device__ inline int fem_cg_fused(const FemArrays &A, const FemBodyDesc &D, const FemVecs &V, const CgLds &C,
+                                   int &cg_total) {
+  __shared__ double partA[16], partB[2][16];
+  const int nG = 6 * D.nND, g0 = D.goff;
+  const int nw = (int)blockDim.x >> 6, wv = (int)threadIdx.x >> 6, ln = (int)threadIdx.x & 63;
+  const int span = 60 * nw;
+  const double err = 1.0e-6;
+  auto zrow = [&](int g) {  // z_g = (M^-1 r)_g, block Jacobi
+    const int node = g / 6, i = g - node * 6;
+    const double *B = V.blk + (size_t)node * 36;
+    double sz = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sz = sz + B[i * 6 + k] * V.r[6 * node + k];
+    return V.fixed[g] ? 0.0 : sz;
+  };
+  // r = lodEffe - A x (fixed DOFs 0), z = M^-1 r, r.r and r.z
+  double a1 = 0.0, a2 = 0.0;
+  for (int base = 0; base < nG; base += span) {
+    const int g = base + wv * 60 + ln;
+    const bool on = ln < 60 && g < nG;
+    if (on) {
+      const double y = fem_row(V, C, g, [&](int d) { return V.x[d]; });
+      V.r[g] = V.fixed[g] ? 0.0 : A.lodEffe[g0 + g] - y;
+    }
+    wave_sync();
+    if (on) {
+      const double z = zrow(g), r = V.r[g];
+      V.z[g] = z;
+      a1 = a1 + r * r;
+      a2 = a2 + r * z;
+    }
+  }
+  a1 = wave_sum_down(a1);
+  a2 = wave_sum_down(a2);
+  if (ln == 0) { partB[0][wv] = a1; partB[1][wv] = a2; }
+  fem_sync();
+  double rr = 0.0, rsold = 0.0;
+  for (int w = 0; w < nw; ++w) { rr = rr + partB[0][w]; rsold = rsold + partB[1][w]; }
+  double rn = sqrt(rr);
+  if (rn <= err) return 0;
+  if (fabs(rsold) <= 1.0e-30) return 2;
+  bool first = true;
+  for (int it = 1; it <= 10000; ++it) {
+    ++cg_total;
+    // p (z on the first iteration), A p with the fixed DOFs zeroed, p.Ap
+    auto pk = [&](int d) { return first ? V.z[d] : V.p[d]; };
+    double a = 0.0;
+    for (int base = 0; base < nG; base += span) {
+      const int g = base + wv * 60 + ln;
+      if (ln < 60 && g < nG) {
+        double y = fem_row(V, C, g, pk);
+        if (V.fixed[g]) y = 0.0;
+        V.Ap[g] = y;
+        const double pg = pk(g);
+        if (first) V.p[g] = pg;  // p = z (read as z by every other reader)
+        a = a + pg * y;
+      }
+    }
+    a = wave_sum_down(a);
+    if (ln == 0) partA[wv] = a;
+    fem_sync();
+    double pAp = 0.0;
+    for (int w = 0; w < nw; ++w) pAp = pAp + partA[w];
+    if (fabs(pAp) <= 1.0e-30) return 2;
+    const double alpha = rsold / pAp;
+    // x, r update, z = M^-1 r, r.r and r.z
+    a1 = 0.0;
+    a2 = 0.0;
+    for (int base = 0; base < nG; base += span) {
+      const int g = base + wv * 60 + ln;
+      const bool on = ln < 60 && g < nG;
+      if (on) {
+        double xv = V.x[g] + alpha * V.p[g];
+        if (V.fixed[g]) xv = V.xF[g];
+        V.x[g] = xv;
+        double rv = V.r[g] - alpha * V.Ap[g];
+        if (V.fixed[g]) rv = 0.0;
+        V.r[g] = rv;
+      }
+      wave_sync();
+      if (on) {
+        const double z = zrow(g), r = V.r[g];
+        V.z[g] = z;
+        a1 = a1 + r * r;
+        a2 = a2 + r * z;
+      }
+    }
+    a1 = wave_sum_down(a1);
+    a2 = wave_sum_down(a2);
+    if (ln == 0) { partB[0][wv] = a1; partB[1][wv] = a2; }
+    fem_sync();
+    double rsnew = 0.0;
+    rr = 0.0;
+    for (int w = 0; w < nw; ++w) { rr = rr + partB[0][w]; rsnew = rsnew + partB[1][w]; }
+    rn = sqrt(rr);
+    if (rn <= err) break;
+    if (fabs(rsold) <= 1.0e-30) return 2;
+    const double beta = rsnew / rsold;
+    for (int base = 0; base < nG; base += span) {
+      const int g = base + wv * 60 + ln;
+      if (ln < 60 && g < nG) V.p[g] = V.fixed[g] ? 0.0 : V.z[g] + beta * V.p[g];
+    }
+    fem_sync();
+    first = false;
+    rsold = rsnew;
+  }
+  return 0;
+}
+
 // LDS of k_fem_substep, in doubles, behind one double of slack (fem_lds_base)
 // and each area an even number of doubles: a work area of `work` doubles (the
 // staging area of the dot products, 2 nG, and, when cg_lds, the CG's
@@ -509,9 +656,16 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
     }
     // block-Jacobi preconditioner (SolidSolver.f90:1966-1975, Invert6x6
     // :2234-2279): Gauss-Jordan on each node's 6x12 block with one lane per
-    // row holding its row in registers, ten nodes per wave (lanes 0..59);
-    // pivot columns, swapped rows and the pivot row move between the node's
-    // lanes by shuffles.  Every row operation is the sequential one.
+    // row holding its row in registers, ten nodes per wave (lanes 0..59).
+    // Row exchanges are kept as a permutation (pr: logical row -> the node's
+    // lane holding it, the same in the node's six lanes) instead of moving
+    // rows; pivot columns and the pivot row move between the node's lanes by
+    // shuffles.  Step i divides and eliminates only the columns later steps
+    // read (left-half columns after i, the whole right half): a column's
+    // values depend only on itself and the pivot column, and a left-half
+    // column is not read after its own step, so the inverse (the right half)
+    // and every pivot choice are the sequential routine's, operation for
+    // operation.
     {
       const int lane = (int)threadIdx.x & 63;
       const int wv = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
@@ -550,12 +704,15 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
 #pragma unroll
         for (int j = 0; j < 6; ++j) a[6 + j] = j == row ? 1.0 : 0.0;
         int bad = 0;
+        int pr[6];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) pr[k] = k;
 #pragma unroll
         for (int i = 0; i < 6; ++i) {
-          // column i of the node's rows, pivot as the sequential search picks it
+          // column i of the node's logical rows, pivot as the sequential search picks it
           double col[6];
 #pragma unroll
-          for (int k = 0; k < 6; ++k) col[k] = __shfl(a[i], lbase + k, 64);
+          for (int k = i; k < 6; ++k) col[k] = __shfl(a[i], lbase + pr[k], 64);
           int p = i;
           double pivot = fabs(col[i]);
 #pragma unroll
@@ -565,31 +722,43 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
               p = k;
             }
           if (!bad && pivot <= 1.0e-30) bad = i + 1;
-          // rows i and p exchanged
-          const int src = lbase + (row == i ? p : (row == p ? i : row));
+          // logical rows i and p exchanged
+          if (!bad) {
+            const int pi = pr[i];
+            int pp = pi;
 #pragma unroll
-          for (int j = 0; j < 12; ++j) {
-            const double v = __shfl(a[j], src, 64);
-            if (!bad) a[j] = v;
+            for (int k = i + 1; k < 6; ++k)
+              if (k == p) pp = pr[k];
+#pragma unroll
+            for (int k = i + 1; k < 6; ++k)
+              if (k == p) pr[k] = pi;
+            pr[i] = pp;
           }
-          if (!bad && row == i) {
+          const int P = pr[i];  // the node's lane holding the pivot row
+          if (!bad && row == P) {
             const double pv = a[i];
 #pragma unroll
-            for (int j = 0; j < 12; ++j) a[j] = a[j] / pv;
+            for (int j = i + 1; j < 12; ++j) a[j] = a[j] / pv;
           }
           double prow[12];
 #pragma unroll
-          for (int j = 0; j < 12; ++j) prow[j] = __shfl(a[j], lbase + i, 64);
-          if (!bad && row != i) {
+          for (int j = i + 1; j < 12; ++j) prow[j] = __shfl(a[j], lbase + P, 64);
+          if (!bad && row != P) {
             const double factor = a[i];
 #pragma unroll
-            for (int j = 0; j < 12; ++j) a[j] = a[j] - factor * prow[j];
+            for (int j = i + 1; j < 12; ++j) a[j] = a[j] - factor * prow[j];
           }
         }
+        // this lane's logical row
+        int L = 0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+          if (pr[k] == row) L = k;
         if (act) {
           double *out = V.blk + (size_t)n * 36;
+          // (a singular block leaves the identity; pr is then still the start)
 #pragma unroll
-          for (int j = 0; j < 6; ++j) out[row * 6 + j] = bad ? (row == j ? 1.0 : 0.0) : a[6 + j];
+          for (int j = 0; j < 6; ++j) out[L * 6 + j] = bad ? (L == j ? 1.0 : 0.0) : a[6 + j];
         }
       }
     }
@@ -610,6 +779,9 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       C.on = true;
       fem_sync();
     }
+    if (D.tree_sums && C.on && C.coef && V.en) {
+      if (const int st = fem_cg_fused(A, D, V, C, cg_total)) { status = st; break; }
+    } else {
     fem_matvec(A, D, V, C, V.x, V.Ap);
     for (int g = threadIdx.x; g < nG; g += blockDim.x)
       V.r[g] = V.fixed[g] ? 0.0 : A.lodEffe[g0 + g] - V.Ap[g];
@@ -680,6 +852,7 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
         rsold = rsnew;
       }
       if (status) break;
+    }
     }
     // UpdateDspANDTride (SolidSolver.f90:2304-2338); the ramp factor is 1
     for (int g = threadIdx.x; g < nG; g += blockDim.x) A.dsp[g0 + g] = A.dsp[g0 + g] + V.x[g];
