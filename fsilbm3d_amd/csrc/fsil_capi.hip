@@ -160,6 +160,7 @@ struct Ibm {
   DevBuf<double> Eprev, Enrm, Esnap;         // previous force, normals, force after `skip` rounds
   int last_warm = 0, last_fallback = 0, last_rounds = 0;
   int last_iters = 0;
+  int last_nband = 0;  // band cells of the last checked call (sizes the band-cell grids)
   double last_dmax = 0.0;
 };
 
@@ -1528,6 +1529,10 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   hipStream_t s = coupling_stream(*b);
   const int M = I.M, E = 64 * M;
+  // workgroups of the band-cell kernels (grid-stride): the last call's band
+  // and a quarter more, at most one per 256 entries
+  const int cell_blocks = I.last_nband > 0 ? std::min((E + 255) / 256, (I.last_nband + I.last_nband / 4 + 255) / 256 + 1)
+                                           : (E + 255) / 256;
   if (fem_order_after(*b)) return 1;  // k_ibm_finish rewrites the loads the beam sub-steps read
   if (!Exyz) {
     // markers computed on the device by fsil_fem_update_markers
@@ -1604,8 +1609,8 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     hipLaunchKernelGGL(k_mlp_infer_bf16, dim3((M + 63) / 64), dim3(256), 0, s, I.wsX.p, M, net->th.p, I.wsP.p);
     hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Eprev.p, I.Uref,
                        invh3p, I.Eforce.p, I.fT.p);
-    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, -1, MdfTest{}, I.cap, I.dptr.p,
-                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
+    hipLaunchKernelGGL(k_correct, dim3(cell_blocks), dim3(256), 0, s, I.ctl.p, -1, MdfTest{}, I.cap, I.dptr.p,
+                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p, cell_blocks);
     HIPCHK(hipGetLastError());
   }
   // MDF loop (Solidbody.f90:893-906): the rounds are queued (with the warm
@@ -1622,8 +1627,8 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     hipLaunchKernelGGL(k_interp, dim3((M + 3) / 4), dim3(256), 0, s, M, it, I.dbidx.p, I.dEw.p, I.band_u.p, I.cap,
                        I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p, snap, I.Esnap.p);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_correct, dim3((E + 255) / 256), dim3(256), 0, s, I.ctl.p, it, mt, I.cap, I.dptr.p,
-                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p);
+    hipLaunchKernelGGL(k_correct, dim3(cell_blocks + 1), dim3(256), 0, s, I.ctl.p, it, mt, I.cap, I.dptr.p,
+                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p, cell_blocks);
     HIPCHK(hipGetLastError());
     if (warm && it == 0) {
       // restart from the zero guess when the first round asked for it
@@ -1656,7 +1661,8 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   fa.dof_ptr = I.dof_ptr.p; fa.dof_list = I.dof_list.p; fa.dof_kind = I.dof_kind.p;
   fa.marker_elem = I.dmarker_elem.p; fa.elem_xc = I.elem_xc.p; fa.Exyz = I.Exyz.p;
   fa.lod = I.lod.p;
-  hipLaunchKernelGGL(k_ibm_finish, dim3(fa.dof_blocks + (E + 255) / 256), dim3(256), 0, s, fa);
+  fa.band_blocks = cell_blocks;
+  hipLaunchKernelGGL(k_ibm_finish, dim3(fa.dof_blocks + cell_blocks), dim3(256), 0, s, fa);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(I.rep->ev, s));
   I.rep->pending = true;
@@ -1768,6 +1774,7 @@ int ibm_check_report(Block &b) {
   I.last_fallback = R.ctl[kFallback] != 0;
   // forcing rounds run: interpolation rounds plus the guess's correction
   I.last_rounds = R.ctl[kRan] + 1 + R.ctl[kGuess];
+  I.last_nband = R.ctl[kNband];
   I.last_iters = R.ctl[kIters];
   I.last_dmax = R.dmax;
   return 0;

@@ -290,11 +290,28 @@ __device__ inline void fem_dot2(const double *a, const double *b, const double *
   fem_dot_chunks(a, b, c, d, n, shared, stage, cap, r1, r2, tree);
 }
 
-// Wave sum of a by lane shuffles (lane 0 holds it)
-__device__ inline double wave_sum_down(double a) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) a = a + __shfl_down(a, off, 64);
-  return a;
+// v from the lane the DPP control selects (0 where it selects none or the
+// masks leave the lane unwritten), as two 32-bit moves
+template <int CTRL, int ROW_MASK, int BANK_MASK>
+__device__ inline double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, BANK_MASK, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, BANK_MASK, true);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Wave sum by DPP (lane 63 holds it): row_shr 1, 2, 4, 8 leave each row's
+// sum of lanes 0..15 in its lane 15, row_bcast 15 and 31 then add the rows
+// into lane 63.  A fixed order; no LDS round trips (the shuffle tree takes
+// six ds_bpermute pairs).
+__device__ inline double wave_sum_dpp(double v) {
+  v = v + dpp_f64<0x111, 0xf, 0xf>(v);  // row_shr:1
+  v = v + dpp_f64<0x112, 0xf, 0xf>(v);  // row_shr:2
+  v = v + dpp_f64<0x114, 0xf, 0xf>(v);  // row_shr:4
+  v = v + dpp_f64<0x118, 0xf, 0xf>(v);  // row_shr:8
+  v = v + dpp_f64<0x142, 0xa, 0xf>(v);  // row_bcast:15 into rows 1, 3
+  v = v + dpp_f64<0x143, 0xc, 0xf>(v);  // row_bcast:31 into rows 2, 3
+  return v;
 }
 
 // Row g of the element-by-element product A v (Beam_MatrixMultipy,
@@ -322,8 +339,8 @@ __device__ inline double fem_row(const FemVecs &V, const CgLds &C, int g, VK &&v
 // updates and r.r, r.z, after the direction update).  DOFs are owned sixty
 // (ten whole nodes) per wave, so that a node's block-Jacobi row reads r of its
 // own wave; each owner forms its row of A p, its updates and its terms of the
-// dot products, which are summed by wave shuffles and then over the waves in
-// wave order.  (Forming p = z + beta p at every reader instead, which saves
+// dot products, which are summed within each wave by DPP (wave_sum_dpp) and
+// then over the waves in wave order.  (Forming p = z + beta p at every reader instead, which saves
 // the third barrier, measured 29 us per sub-step for the A p phase against
 // 17 us: the repeated reads cost more than the barrier.)  x holds xF on
 // entry; returns 2 on a breakdown (zero p.Ap or r.z), else 0; adds the
@@ -360,9 +377,9 @@ __device__ inline int fem_cg_fused(const FemArrays &A, const FemBodyDesc &D, con
       a2 = a2 + r * z;
     }
   }
-  a1 = wave_sum_down(a1);
-  a2 = wave_sum_down(a2);
-  if (ln == 0) { partB[0][wv] = a1; partB[1][wv] = a2; }
+  a1 = wave_sum_dpp(a1);
+  a2 = wave_sum_dpp(a2);
+  if (ln == 63) { partB[0][wv] = a1; partB[1][wv] = a2; }
   fem_sync();
   double rr = 0.0, rsold = 0.0;
   for (int w = 0; w < nw; ++w) { rr = rr + partB[0][w]; rsold = rsold + partB[1][w]; }
@@ -386,8 +403,8 @@ __device__ inline int fem_cg_fused(const FemArrays &A, const FemBodyDesc &D, con
         a = a + pg * y;
       }
     }
-    a = wave_sum_down(a);
-    if (ln == 0) partA[wv] = a;
+    a = wave_sum_dpp(a);
+    if (ln == 63) partA[wv] = a;
     fem_sync();
     double pAp = 0.0;
     for (int w = 0; w < nw; ++w) pAp = pAp + partA[w];
@@ -415,9 +432,9 @@ __device__ inline int fem_cg_fused(const FemArrays &A, const FemBodyDesc &D, con
         a2 = a2 + r * z;
       }
     }
-    a1 = wave_sum_down(a1);
-    a2 = wave_sum_down(a2);
-    if (ln == 0) { partB[0][wv] = a1; partB[1][wv] = a2; }
+    a1 = wave_sum_dpp(a1);
+    a2 = wave_sum_dpp(a2);
+    if (ln == 63) { partB[0][wv] = a1; partB[1][wv] = a2; }
     fem_sync();
     double rsnew = 0.0;
     rr = 0.0;

@@ -564,25 +564,29 @@ struct MdfTest {
 };
 
 // velocity correction of every band cell: u -= fT*rx*ry*rz over its entries
-// in (marker, stencil) order (Solidbody.f90:1034-1048).  round >= 0: the
-// correction of that forcing round (runs when its k_interp ran), and the
-// last workgroup (the grid covers 64 M cells, the band usually far fewer, so
-// it is mostly idle) makes the round's stop test (the markers' |dV|_1 summed
-// in a fixed order, then mdf_decide); round < 0 (the warm-start guess): runs
-// unless the loop is done, no stop test.
+// in (marker, stencil) order (Solidbody.f90:1034-1048), the cells by the
+// first cell_blocks workgroups in a grid-stride loop (the host sizes them to
+// the last call's band, so that few idle workgroups wait for CU slots).
+// round >= 0: the correction of that forcing round (runs when its k_interp
+// ran), and workgroup cell_blocks makes the round's stop test (the markers'
+// |dV|_1 summed in a fixed order, then mdf_decide); round < 0 (the
+// warm-start guess): runs unless the loop is done, no stop test.
 __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int round, MdfTest mt, int cap,
                                                  const int *__restrict__ ptr,
                                                  const int *__restrict__ entry,
                                                  const float *__restrict__ Ew,
                                                  const double *__restrict__ fT,
-                                                 double *__restrict__ band_u) {
+                                                 double *__restrict__ band_u, int cell_blocks) {
   if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
-  if (round >= 0 && blockIdx.x == gridDim.x - 1) {
-    const double tol = mdf_tl_sum(mt.tl, mt.M);
-    if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol, mt.skip);
+  if ((int)blockIdx.x >= cell_blocks) {
+    if (round >= 0) {
+      const double tol = mdf_tl_sum(mt.tl, mt.M);
+      if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol, mt.skip);
+    }
+    return;
   }
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= ctl[kNband]) return;
+  const int nb = ctl[kNband];
+  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += cell_blocks * blockDim.x) {
   double u1 = band_u[b], u2 = band_u[cap + b], u3 = band_u[2LL * cap + b];
   const int k1 = ptr[b + 1];
   for (int k = ptr[b]; k < k1;) {
@@ -600,6 +604,7 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
   band_u[b] = u1;
   band_u[cap + b] = u2;
   band_u[2LL * cap + b] = u3;
+  }
 }
 
 // Inclusive scan of the per-cell entry counts of the band's cells (ctl[kNew],
@@ -662,8 +667,9 @@ struct IbmReport {
 // Eulerian force of every band cell (ResetVolumeForce_ FluidDomain.f90:
 // 1195-1206, FluidVolumeForce_ Solidbody.f90:968-977, add_volume_force_
 // FluidDomain.f90:1182-1193): (0 - sum F/h^3*rx*ry*rz) + VF, by the other
-// workgroups, the first of which copies the control words into the report
-// (the last k_correct has made the stop test, so they are final).
+// band_blocks workgroups in a grid-stride loop, the first of which copies the
+// control words into the report (the last k_correct has made the stop test,
+// so they are final).
 constexpr int kLoadChunk = 256;
 
 struct IbmFinishArgs {
@@ -676,7 +682,7 @@ struct IbmFinishArgs {
   const double *Eforce;
   double invh3, VF1, VF2, VF3;
   double *band_F;
-  int ndof, dof_blocks;
+  int ndof, dof_blocks, band_blocks;
   const int *dof_ptr, *dof_list, *dof_kind, *marker_elem;
   const double *elem_xc, *Exyz;
   double *lod;
@@ -719,8 +725,8 @@ __global__ __launch_bounds__(256) void k_ibm_finish(IbmFinishArgs a) {
   const int blk = (int)blockIdx.x - a.dof_blocks;
   if (blk == 0 && threadIdx.x < kCtlWords) a.rep->ctl[threadIdx.x] = a.ctl[threadIdx.x];
   if (blk == 0 && threadIdx.x == 0) a.rep->dmax = a.dstate[0];
-  const int b = blk * blockDim.x + threadIdx.x;
-  if (b >= a.ctl[kNband]) return;
+  const int nb = a.ctl[kNband];
+  for (int b = blk * blockDim.x + threadIdx.x; b < nb; b += a.band_blocks * blockDim.x) {
   double g1 = 0.0, g2 = 0.0, g3 = 0.0;
   const int k1 = a.ptr[b + 1];
   for (int k = a.ptr[b]; k < k1;) {
@@ -739,6 +745,7 @@ __global__ __launch_bounds__(256) void k_ibm_finish(IbmFinishArgs a) {
   a.band_F[b] = g1 + a.VF1;
   a.band_F[a.cap + b] = g2 + a.VF2;
   a.band_F[2LL * a.cap + b] = g3 + a.VF3;
+  }
 }
 
 // ---------------------------------------------------------------------------
