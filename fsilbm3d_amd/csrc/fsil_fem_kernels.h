@@ -585,18 +585,35 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
   int cg_total = 0;
   int status = 0;
   double dnorm = 1.0;
-  for (iter = 1; iter <= D.ntolFEM; ++iter) {
-    // UpdateMatrixANDLoad (SolidSolver.f90:1932-1947).  Per element: internal
-    // force from the local stiffness, local geometric stiffness, rotation.
-    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
+  // the body's assembly lists for the CG in LDS (behind the dot-product
+  // staging area [0, 2 nG) and the element products), once per sub-step
+  CgLds C{};
+  C.coef = coefL;
+  if (cg_lds) {
+    C.lbuf = lds + 2 * nG;
+    int *ip = reinterpret_cast<int *>(lds + 2 * nG + NED * nE);
+    const int p0 = A.dof_ptr[g0], nent = A.dof_ptr[g0 + nG] - p0;
+    int *ep = ip + nG + 1;
+    for (int g = threadIdx.x; g <= nG; g += blockDim.x) ip[g] = A.dof_ptr[g0 + g] - p0;
+    for (int k = threadIdx.x; k < nent; k += blockDim.x) ep[k] = (A.dof_e[p0 + k] - D.eoff) * NED + A.dof_i[p0 + k];
+    C.ptr = ip;
+    C.ent = ep;
+    C.on = true;
+    fem_sync();
+  }
+  // UpdateMatrixANDLoad (SolidSolver.f90:1932-1947).  Per element: internal
+  // force from the local stiffness, local geometric stiffness.
+  auto stress_phase = [&](int t0, int step) {
+    for (int e = t0; e < nE; e += step) {
       auto &E = EL(e);
       el_body_stress(E);
       el_geom_scalars(E);
     }
-    fem_sync();
-    // Per 3x3 block of the element matrices: R^T K R, R^T G R and the Newmark
-    // tangent matrix (Segment_UpdateMatrix, SolidSolver.f90:131-157); each
-    // entry is summed as el_rkr sums it.
+  };
+  // Per 3x3 block of the element matrices: R^T K R, R^T G R and the Newmark
+  // tangent matrix (Segment_UpdateMatrix, SolidSolver.f90:131-157); each
+  // entry is summed as el_rkr sums it.
+  auto matrices_phase = [&]() {
     for (int t = threadIdx.x; t < nE * 16; t += blockDim.x) {
       const int e = t >> 4, blkij = t & 15;
       auto &E = EL(e);
@@ -647,11 +664,14 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
           }
         }
     }
-    fem_sync();
-    for (int e = threadIdx.x; e < nE; e += blockDim.x)
+  };
+  auto load_phase = [&](int t0, int step) {
+    for (int e = t0; e < nE; e += step)
       el_update_load(EL(e), c, D.dampM, D.dampK, A.dspO + g0, A.dsp + g0, A.vel + g0, A.acc + g0);
-    fem_sync();
-    for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+  };
+  // internal, external and effective loads per DOF (LocToGlobal order)
+  auto assembly_phase = [&](int t0, int step) {
+    for (int g = t0; g < nG; g += step) {
       double li = 0.0;
       for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
         li = li + EL(A.dof_e[k] - D.eoff).fint[A.dof_i[k]];
@@ -664,7 +684,30 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       }
       A.lodEffe[g0 + g] = le;
     }
-    fem_sync();
+  };
+  for (iter = 1; iter <= D.ntolFEM; ++iter) {
+    if constexpr (HOT) {
+      // Without stiffness damping the element loads do not read the rotated
+      // stiffness: the internal forces (first half of the workgroup) and the
+      // element loads (second half) run side by side, then the element
+      // matrices and the DOF loads (from the top thread down).
+      const int half = (int)blockDim.x / 2;
+      if ((int)threadIdx.x < half) stress_phase(threadIdx.x, half);
+      else load_phase((int)threadIdx.x - half, (int)blockDim.x - half);
+      fem_sync();
+      matrices_phase();
+      assembly_phase((int)blockDim.x - 1 - (int)threadIdx.x, blockDim.x);
+      fem_sync();
+    } else {
+      stress_phase(threadIdx.x, blockDim.x);
+      fem_sync();
+      matrices_phase();
+      fem_sync();
+      load_phase(threadIdx.x, blockDim.x);  // (stiffness damping reads what the matrices wrote)
+      fem_sync();
+      assembly_phase(threadIdx.x, blockDim.x);
+      fem_sync();
+    }
     // CG_Solve (SolidSolver.f90:1949-2020), block-Jacobi preconditioner
     for (int g = threadIdx.x; g < nG; g += blockDim.x) {
       const double xf = V.fixed[g] ? (iter == 1 ? A.vBC[g0 + g] : 0.0) : 0.0;
@@ -780,22 +823,6 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       }
     }
     fem_sync();
-    // the body's assembly lists and the mat-vec products into LDS (behind the
-    // dot-product staging area [0, 2 nG)) when they fit
-    CgLds C{};
-    C.coef = coefL;
-    if (cg_lds) {
-      C.lbuf = lds + 2 * nG;
-      int *ip = reinterpret_cast<int *>(lds + 2 * nG + NED * nE);
-      const int p0 = A.dof_ptr[g0], nent = A.dof_ptr[g0 + nG] - p0;
-      int *ep = ip + nG + 1;
-      for (int g = threadIdx.x; g <= nG; g += blockDim.x) ip[g] = A.dof_ptr[g0 + g] - p0;
-      for (int k = threadIdx.x; k < nent; k += blockDim.x) ep[k] = (A.dof_e[p0 + k] - D.eoff) * NED + A.dof_i[p0 + k];
-      C.ptr = ip;
-      C.ent = ep;
-      C.on = true;
-      fem_sync();
-    }
     if (D.tree_sums && C.on && C.coef && V.en) {
       if (const int st = fem_cg_fused(A, D, V, C, cg_total)) { status = st; break; }
     } else {
@@ -871,9 +898,22 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       if (status) break;
     }
     }
-    // UpdateDspANDTride (SolidSolver.f90:2304-2338); the ramp factor is 1
-    for (int g = threadIdx.x; g < nG; g += blockDim.x) A.dsp[g0 + g] = A.dsp[g0 + g] + V.x[g];
-    if (threadIdx.x == 0) sh_val = 0.0;
+    // UpdateDspANDTride (SolidSolver.f90:2304-2338); the ramp factor is 1.
+    // dnorm = |maxval(dsp_increment**2)| (SolidSolver.f90:2336): the maximum
+    // does not depend on the order; each wave's maximum goes to lds[wave]
+    {
+      double mx = -INFINITY;
+      for (int g = threadIdx.x; g < nG; g += blockDim.x) {
+        A.dsp[g0 + g] = A.dsp[g0 + g] + V.x[g];
+        const double v = V.x[g] * V.x[g];
+        if (v > mx) mx = v;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_down(mx, off, 64);
+        if (o > mx) mx = o;
+      }
+      if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
+    }
     fem_sync();
     for (int e = threadIdx.x; e < nE; e += blockDim.x) {
       auto &E = EL(e);
@@ -885,36 +925,20 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       el_cptdxyz1(E);
       el_update_triads(E, V.x);
     }
-    // dnorm = |maxval(dsp_increment**2)| (SolidSolver.f90:2336): the maximum
-    // does not depend on the order, so a wave reduction gives the same value
     {
-      double mx = -INFINITY;
-      for (int g = threadIdx.x; g < nG; g += blockDim.x) {
-        const double v = V.x[g] * V.x[g];
-        if (v > mx) mx = v;
-      }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_down(mx, off, 64);
-        if (o > mx) mx = o;
-      }
-      if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
-      fem_sync();
-      if (threadIdx.x == 0) {
-        double m = lds[0];
-        for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w)
-          if (lds[w] > m) m = lds[w];
-        sh_val = fabs(m);
-      }
+      double m = lds[0];
+      for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w)
+        if (lds[w] > m) m = lds[w];
+      dnorm = fabs(m);
     }
     fem_sync();
+    // (nothing before the next rewrite of lds[0..waves) reads pos)
     for (int n = threadIdx.x; n < nN; n += blockDim.x) {
       const int last = A.node_ptr[D.noff + n + 1] - 1;
       const auto &E = EL(A.node_e[last] - D.eoff);
       const double *src = A.node_h[last] ? E.x1 + 6 : E.x1;
       for (int k = 0; k < 6; ++k) A.pos[g0 + 6 * n + k] = src[k];
     }
-    dnorm = sh_val;
-    fem_sync();
     if (dnorm <= D.dtolFEM) break;
   }
   // UpdateVelAcc (SolidSolver.f90:2350-2359)
