@@ -137,10 +137,11 @@ struct Ibm {
   double dtol = 0.0, Uref = 1.0, denIn = 1.0;
   // device: markers and forces
   DevBuf<double> Exyz, Evel, Ea, Eforce, fT, tolpart, elem_xc, lod;
-  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_kind, first_marker;
+  DevBuf<int> dmarker_elem, dof_ptr, dof_list, dof_elem, dof_kind, first_marker;
   // device: stencils and band (built each call, see fsil_ibm_kernels.h)
   int cap = 0;                               // 64*M, stride of the band arrays
   DevBuf<float> dEw;                         // 12*M
+  DevBuf<float4> went;                       // per band entry (CSR order): rx, ry, rz
   DevBuf<unsigned> keys;                      // cell of each stencil entry
   DevBuf<int> tmp;                            // entries in arrival order per band cell
   DevBuf<int> blkbox;                         // per claim workgroup: its part of the band's box
@@ -1449,6 +1450,11 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   }
   if (upload(I.dof_ptr, ptr.data(), ptr.size(), b->stream)) return 1;
   if (upload(I.dof_list, lst.data(), lst.size(), b->stream)) return 1;
+  {
+    std::vector<int> lel(lst.size());  // the element of each list entry's marker
+    for (size_t j = 0; j < lst.size(); ++j) lel[j] = I.marker_elem[lst[j]];
+    if (upload(I.dof_elem, lel.data(), lel.size(), b->stream)) return 1;
+  }
   if (upload(I.dof_kind, kind.data(), kind.size(), b->stream)) return 1;
   if (upload(I.dmarker_elem, I.marker_elem.data(), I.marker_elem.size(), b->stream)) return 1;
   HIPCHK(I.Exyz.ensure(3 * (size_t)I.M));
@@ -1474,6 +1480,7 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   HIPCHK(I.flags.ensure(I.cap));
   HIPCHK(I.scan.ensure(I.cap));
   HIPCHK(I.dentry.ensure(I.cap));
+  HIPCHK(I.went.ensure(I.cap));
   HIPCHK(I.dbidx.ensure(I.cap));
   HIPCHK(I.dptr.ensure((size_t)I.cap + 1));
   HIPCHK(I.dcell.ensure(I.cap));
@@ -1529,10 +1536,13 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   }
   hipStream_t s = coupling_stream(*b);
   const int M = I.M, E = 64 * M;
-  // workgroups of the band-cell kernels (grid-stride): the last call's band
-  // and a quarter more, at most one per 256 entries
-  const int cell_blocks = I.last_nband > 0 ? std::min((E + 255) / 256, (I.last_nband + I.last_nband / 4 + 255) / 256 + 1)
-                                           : (E + 255) / 256;
+  // workgroups of the band-cell kernels (kCellsPerPass cells per pass): the
+  // last call's band and a quarter more, at most one per kCellsPerPass entries
+  const int cell_cap = (E + kCellsPerPass - 1) / kCellsPerPass;
+  const int cell_blocks =
+      I.last_nband > 0
+          ? std::min(cell_cap, (I.last_nband + I.last_nband / 4 + kCellsPerPass - 1) / kCellsPerPass + 1)
+          : cell_cap;
   if (fem_order_after(*b)) return 1;  // k_ibm_finish rewrites the loads the beam sub-steps read
   if (!Exyz) {
     // markers computed on the device by fsil_fem_update_markers
@@ -1589,11 +1599,11 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   hipLaunchKernelGGL(k_ibm_count, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys.p, I.cellmap.p, I.dbidx.p,
                      I.scan.p, I.flags.p, I.ctl.p, ma);
   HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL(k_band_scan, dim3(1), dim3(1024), 0, s, I.ctl.p, I.scan.p, I.dptr.p);
+  hipLaunchKernelGGL(k_band_scan, dim3(1), dim3(1024), 0, s, I.ctl.p, I.scan.p, I.dptr.p, I.blkbox.p, nclaim);
   hipLaunchKernelGGL(k_ibm_fill, dim3((E + 255) / 256), dim3(256), 0, s, E, I.dbidx.p, I.flags.p, I.dptr.p,
                      I.tmp.p);
   hipLaunchKernelGGL(k_ibm_rank, dim3((E + 255) / 256), dim3(256), 0, s, E, I.ctl.p, I.dbidx.p, I.dptr.p, I.tmp.p,
-                     I.dentry.p, I.blkbox.p, nclaim);
+                     I.dentry.p, I.dEw.p, I.went.p);
   HIPCHK(hipGetLastError());
   const double invh3p = 0.5 * dt * ((1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh)) / I.denIn;
   double dsum = 0.0;
@@ -1610,7 +1620,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Eprev.p, I.Uref,
                        invh3p, I.Eforce.p, I.fT.p);
     hipLaunchKernelGGL(k_correct, dim3(cell_blocks), dim3(256), 0, s, I.ctl.p, -1, MdfTest{}, I.cap, I.dptr.p,
-                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p, cell_blocks);
+                       I.dentry.p, I.went.p, I.fT.p, I.band_u.p, cell_blocks);
     HIPCHK(hipGetLastError());
   }
   // MDF loop (Solidbody.f90:893-906): the rounds are queued (with the warm
@@ -1628,7 +1638,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                        I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p, snap, I.Esnap.p);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_correct, dim3(cell_blocks + 1), dim3(256), 0, s, I.ctl.p, it, mt, I.cap, I.dptr.p,
-                       I.dentry.p, I.dEw.p, I.fT.p, I.band_u.p, cell_blocks);
+                       I.dentry.p, I.went.p, I.fT.p, I.band_u.p, cell_blocks);
     HIPCHK(hipGetLastError());
     if (warm && it == 0) {
       // restart from the zero guess when the first round asked for it
@@ -1653,12 +1663,12 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   // spreading (Solidbody.f90:920-979), beam loads and the report in one launch
   IbmFinishArgs fa{};
   fa.ctl = I.ctl.p; fa.dstate = I.dstate.p; fa.rep = I.rep->d;
-  fa.cap = I.cap; fa.ptr = I.dptr.p; fa.entry = I.dentry.p; fa.Ew = I.dEw.p; fa.Eforce = I.Eforce.p;
+  fa.cap = I.cap; fa.ptr = I.dptr.p; fa.entry = I.dentry.p; fa.went = I.went.p; fa.Eforce = I.Eforce.p;
   fa.invh3 = (1.0 / b->dh) * (1.0 / b->dh) * (1.0 / b->dh);
   fa.VF1 = b->VF[0]; fa.VF2 = b->VF[1]; fa.VF3 = b->VF[2];
   fa.band_F = I.band_F.p;
   fa.ndof = I.ndof; fa.dof_blocks = (I.ndof + 3) / 4;
-  fa.dof_ptr = I.dof_ptr.p; fa.dof_list = I.dof_list.p; fa.dof_kind = I.dof_kind.p;
+  fa.dof_ptr = I.dof_ptr.p; fa.dof_list = I.dof_list.p; fa.dof_elem = I.dof_elem.p; fa.dof_kind = I.dof_kind.p;
   fa.marker_elem = I.dmarker_elem.p; fa.elem_xc = I.elem_xc.p; fa.Exyz = I.Exyz.p;
   fa.lod = I.lod.p;
   fa.band_blocks = cell_blocks;

@@ -14,7 +14,8 @@
 // cell's winning entry a band number (from an atomic counter, so the band's
 // order varies from call to call while every per-cell result does not);
 // k_ibm_count gives every entry its band cell and a slot; an inclusive scan
-// of the per-cell counts makes the CSR pointers; k_ibm_fill places the
+// of the per-cell counts makes the CSR pointers (and the band's box);
+// k_ibm_fill places the
 // entries in arrival order and k_ibm_rank moves each to its rank by e among
 // its cell's entries.  Band arrays
 // have a fixed stride `cap` = 64*M (an upper bound of the band size) and the
@@ -171,7 +172,7 @@ __global__ __launch_bounds__(256) void k_ibm_bid(int E, const unsigned *__restri
 // cell map.  One 1024-thread workgroup makes one atomic add on the band
 // counter (atomics on one line from many workgroups serialize), and leaves
 // its part of the band's bounding box in blkbox[6 * workgroup ..] for
-// k_ibm_rank.
+// k_band_scan.
 constexpr int kClaimThreads = 1024;
 __global__ __launch_bounds__(kClaimThreads) void k_ibm_claim(int E, const unsigned *__restrict__ keys,
                                                              int *__restrict__ cellmap,
@@ -283,37 +284,31 @@ __global__ __launch_bounds__(256) void k_ibm_fill(int E, const int *__restrict__
 }
 
 // every entry at its rank in its cell's range (the number of the cell's
-// entries below it), so each cell lists its entries in ascending e; the band
-// size is published, and workgroup 0 reduces the claim workgroups' boxes
-// (nbox of them) into ctl[kBoxLo..kBoxHi+2]
+// entries below it, its range read eight entries at a time), so each cell
+// lists its entries in ascending e, its weights (rx, ry, rz) beside it in
+// went; the band size is published
 __global__ __launch_bounds__(256) void k_ibm_rank(int E, int *__restrict__ ctl, const int *__restrict__ bidx,
                                                   const int *__restrict__ band_ptr, const int *__restrict__ tmp,
-                                                  int *__restrict__ entry, const int *__restrict__ blkbox,
-                                                  int nbox) {
+                                                  int *__restrict__ entry, const float *__restrict__ Ew,
+                                                  float4 *__restrict__ went) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) ctl[kNband] = ctl[kNew];
-  if (blockIdx.x == 0) {
-    __shared__ int red[6][256];
-    int r[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
-    for (int q = threadIdx.x; q < nbox; q += blockDim.x)
-      for (int k = 0; k < 6; ++k) r[k] = k < 3 ? min(r[k], blkbox[6 * q + k]) : max(r[k], blkbox[6 * q + k]);
-    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = r[k];
-    __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-      if ((int)threadIdx.x < h)
-        for (int k = 0; k < 6; ++k)
-          red[k][threadIdx.x] = k < 3 ? min(red[k][threadIdx.x], red[k][threadIdx.x + h])
-                                      : max(red[k][threadIdx.x], red[k][threadIdx.x + h]);
-      __syncthreads();
-    }
-    if (threadIdx.x < 6) ctl[kBoxLo + threadIdx.x] = red[threadIdx.x][0];
-  }
   if (e >= E) return;
   const int b = bidx[e];
   const int p0 = band_ptr[b], n = band_ptr[b + 1] - p0;
   int r = 0;
-  for (int j = 0; j < n; ++j) r += tmp[p0 + j] < e ? 1 : 0;
+  for (int j0 = 0; j0 < n; j0 += 8) {
+    int v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = j0 + j < n ? tmp[p0 + j0 + j] : 0x7fffffff;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) r += v[j] < e ? 1 : 0;
+  }
   entry[p0 + r] = e;
+  // the entry's three stencil weights beside it, in the cells' order
+  const int m = e >> 6, sidx = e & 63;
+  const float *W = Ew + 12LL * m;
+  went[p0 + r] = make_float4(W[sidx >> 4], W[4 + ((sidx >> 2) & 3)], W[8 + (sidx & 3)], 0.0f);
 }
 
 // band macros: den/u of every band cell from the stored state, same
@@ -414,10 +409,32 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // sum of tl[0..M) in a fixed order by one 256-thread workgroup: strided
-// partial sums, a wave tree, then the four wave sums in order
-__device__ inline double mdf_tl_sum(const double *tl, int M) {
+// partial sums, a wave tree, then the four wave sums in order.  Each thread
+// reads its terms sixteen at a time before adding them (in the same order),
+// so the reads do not wait for one another; the first sixteen are read by
+// the caller (mdf_tl_first), ahead of its other reads.
+constexpr int kTlBatch = 16;
+__device__ inline void mdf_tl_first(const double *tl, int M, double *v) {
+  const int step = blockDim.x;
+#pragma unroll
+  for (int j = 0; j < kTlBatch; ++j) {
+    const int m = (int)threadIdx.x + j * step;
+    v[j] = m < M ? tl[m] : 0.0;
+  }
+}
+__device__ inline double mdf_tl_sum(const double *tl, int M, const double *first) {
+  constexpr int B = kTlBatch;
   double p = 0.0;
-  for (int m = threadIdx.x; m < M; m += blockDim.x) p = p + tl[m];
+  const int step = blockDim.x;
+  for (int base = threadIdx.x; base < M; base += B * step) {
+    double v[B];
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      v[j] = base == (int)threadIdx.x ? first[j] : (base + j * step < M ? tl[base + j * step] : 0.0);
+#pragma unroll
+    for (int j = 0; j < B; ++j)
+      if (base + j * step < M) p = p + v[j];
+  }
   __shared__ double part[4];
   p = wave_sum(p);
   if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = p;
@@ -436,9 +453,18 @@ __device__ inline double mdf_tl_sum(const double *tl, int M) {
 // from the zero guess (k_ws_fallback, k_band_macro on fallback), which then
 // runs the plain loop.  A negative dstate[2] (no previous call) always
 // falls back.
+// The words it reads, read up front (with the residual terms) so that the
+// decision waits for no further reads.
+struct MdfWords {
+  int iters, warm, done0;
+  double ref;
+};
+__device__ inline MdfWords mdf_words(const int *ctl, const double *dstate) {
+  return MdfWords{ctl[kIters], ctl[kWarm], ctl[kDone0], dstate[2]};
+}
 __device__ inline void mdf_decide(int *ctl, double *dstate, double tol, double dsum, double Uref, int ntol,
-                                  double dtol, int skip) {
-  const int it = ctl[kIters] + 1;
+                                  double dtol, int skip, const MdfWords &wd) {
+  const int it = wd.iters + 1;
   ctl[kIters] = it;
   if (!isfinite(tol)) {
     atomicMin(&ctl[kErr], 3);  // "Nan found in PenaltyForce" (Solidbody.f90:1028-1031)
@@ -448,13 +474,13 @@ __device__ inline void mdf_decide(int *ctl, double *dstate, double tol, double d
   const double dm = tol / (dsum * Uref);
   dstate[0] = dm;
   if (it == skip + 1) {
-    const double ref = dstate[2];
+    const double ref = wd.ref;
     dstate[2] = dm;
-    if (ctl[kWarm] == 1 && !(ref >= 0.0 && dm <= 2.0 * ref)) {
+    if (wd.warm == 1 && !(ref >= 0.0 && dm <= 2.0 * ref)) {
       ctl[kWarm] = 0;
       ctl[kFallback] = 1;
       ctl[kIters] = 0;
-      ctl[kDone] = ctl[kDone0];
+      ctl[kDone] = wd.done0;
       dstate[0] = 1e10;
       return;
     }
@@ -478,22 +504,30 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
                                                 double *__restrict__ fT,
                                                 double *__restrict__ tl, int snap = -1,
                                                 double *__restrict__ Esnap = nullptr) {
-  if (ctl[kDone]) return;  // uniform over the grid
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kRan] = round;
   __shared__ double t[4][3][64];
   const int w = threadIdx.x >> 6, s = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + w;
-  // the marker's own values, read while the stencil gathers
+  // the stencil's band cells and weights and the marker's own values are read
+  // with the control word (the reads are the kernel's latency)
+  int b = 0;
+  float wx = 0.0f, wy = 0.0f, wz = 0.0f;
+  if (m < M) {
+    b = bidx[64LL * m + s];
+    const float *W = Ew + 12LL * m;
+    wx = W[s >> 4];
+    wy = W[4 + ((s >> 2) & 3)];
+    wz = W[8 + (s & 3)];
+  }
   double ev1 = 0.0, ev2 = 0.0, ev3 = 0.0, am = 0.0, ef1 = 0.0, ef2 = 0.0, ef3 = 0.0;
   if (m < M && s == 0) {
     ev1 = Evel[3LL * m]; ev2 = Evel[3LL * m + 1]; ev3 = Evel[3LL * m + 2];
     am = Ea[m];
     ef1 = Eforce[3LL * m]; ef2 = Eforce[3LL * m + 1]; ef3 = Eforce[3LL * m + 2];
   }
+  if (ctl[kDone]) return;  // uniform over the grid
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl[kRan] = round;
   if (m < M) {
-    const int b = bidx[64LL * m + s];
-    const float *W = Ew + 12LL * m;
-    const double rx = (double)W[s >> 4], ry = (double)W[4 + ((s >> 2) & 3)], rz = (double)W[8 + (s & 3)];
+    const double rx = (double)wx, ry = (double)wy, rz = (double)wz;
     t[w][0][s] = ((band_u[b] * rx) * ry) * rz;
     t[w][1][s] = ((band_u[cap + b] * rx) * ry) * rz;
     t[w][2][s] = ((band_u[2LL * cap + b] * rx) * ry) * rz;
@@ -524,35 +558,93 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
   }
 }
 
-// Reads of a band cell's entries, eight at a time: the entry numbers, then
-// their weights and marker values, all issued before any is used.
-struct BandChunk {
-  double rx[8], ry[8], rz[8], v1[8], v2[8], v3[8];
-};
-__device__ __forceinline__ int band_chunk(int k, int k1, const int *__restrict__ entry,
-                                          const float *__restrict__ Ew, const double *__restrict__ val,
-                                          BandChunk &c) {
-  const int n = min(8, k1 - k);
-  int mm[8], ss[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int e = j < n ? entry[k + j] : 0;
-    mm[j] = e >> 6;
-    ss[j] = e & 63;
-  }
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    if (j < n) {
-      const float *W = Ew + 12LL * mm[j];
-      c.rx[j] = (double)W[ss[j] >> 4];
-      c.ry[j] = (double)W[4 + ((ss[j] >> 2) & 3)];
-      c.rz[j] = (double)W[8 + (ss[j] & 3)];
-      c.v1[j] = val[3LL * mm[j]];
-      c.v2[j] = val[3LL * mm[j] + 1];
-      c.v3[j] = val[3LL * mm[j] + 2];
+// The per-cell sums over a band cell's entries (the velocity correction and
+// the spreading), by workgroups of 256 threads taking kCellsPerPass
+// consecutive cells at a time: the cells' entries are one contiguous range of
+// the CSR arrays, so each thread forms one entry's term from coalesced reads
+// (entry, weights) and one gather (the marker's value) into LDS, and then one
+// thread per cell adds its terms in entry order, as the sequential loop does.
+// A pass whose range exceeds the LDS area (kPassEntries) falls back to one
+// thread per cell reading its entries itself.  SPREAD: term = -(((v h^-3 rx)
+// ry) rz), g = sum + VF (out = band_F); else u -= ((v rx) ry) rz (out =
+// band_u, read first).
+constexpr int kCellsPerPass = 32;
+constexpr int kPassEntries = 1024;  // 24 KiB of LDS: fits beside the bulk collide of a split step
+template <bool SPREAD>
+__device__ inline void band_cells_reduce(int blk, int nb, int nblk, int cap, const int *__restrict__ ptr,
+                                         const int *__restrict__ entry, const float4 *__restrict__ went,
+                                         const double *__restrict__ val, double invh3, double *__restrict__ out,
+                                         double VF1, double VF2, double VF3) {
+  __shared__ int sptr[kCellsPerPass + 1];
+  __shared__ double term[3][kPassEntries];
+  const int t = threadIdx.x;
+  for (int c0 = blk * kCellsPerPass; c0 < nb; c0 += nblk * kCellsPerPass) {
+    const int nc = min(kCellsPerPass, nb - c0);
+    double o1 = 0.0, o2 = 0.0, o3 = 0.0;
+    if (t <= nc) sptr[t] = ptr[c0 + t];
+    if (!SPREAD && t < nc) {
+      o1 = out[c0 + t];
+      o2 = out[cap + c0 + t];
+      o3 = out[2LL * cap + c0 + t];
     }
+    __syncthreads();
+    const int e0 = sptr[0], ne = sptr[nc] - e0;
+    const bool fits = ne <= kPassEntries;
+    if (fits) {
+      for (int k = t; k < ne; k += blockDim.x) {
+        const int m = entry[e0 + k] >> 6;
+        const float4 w = went[e0 + k];
+        const double rx = (double)w.x, ry = (double)w.y, rz = (double)w.z;
+        double v1 = val[3LL * m], v2 = val[3LL * m + 1], v3 = val[3LL * m + 2];
+        if (SPREAD) {
+          v1 = v1 * invh3; v2 = v2 * invh3; v3 = v3 * invh3;
+          term[0][k] = -(((v1 * rx) * ry) * rz);
+          term[1][k] = -(((v2 * rx) * ry) * rz);
+          term[2][k] = -(((v3 * rx) * ry) * rz);
+        } else {
+          term[0][k] = ((v1 * rx) * ry) * rz;
+          term[1][k] = ((v2 * rx) * ry) * rz;
+          term[2][k] = ((v3 * rx) * ry) * rz;
+        }
+      }
+    }
+    __syncthreads();
+    if (t < nc) {
+      const int k0 = sptr[t] - e0, k1 = sptr[t + 1] - e0;
+      for (int k = k0; k < k1; ++k) {
+        double x1, x2, x3;
+        if (fits) {
+          x1 = term[0][k]; x2 = term[1][k]; x3 = term[2][k];
+        } else {
+          const int m = entry[e0 + k] >> 6;
+          const float4 w = went[e0 + k];
+          const double rx = (double)w.x, ry = (double)w.y, rz = (double)w.z;
+          double v1 = val[3LL * m], v2 = val[3LL * m + 1], v3 = val[3LL * m + 2];
+          if (SPREAD) {
+            v1 = v1 * invh3; v2 = v2 * invh3; v3 = v3 * invh3;
+            x1 = -(((v1 * rx) * ry) * rz); x2 = -(((v2 * rx) * ry) * rz); x3 = -(((v3 * rx) * ry) * rz);
+          } else {
+            x1 = ((v1 * rx) * ry) * rz; x2 = ((v2 * rx) * ry) * rz; x3 = ((v3 * rx) * ry) * rz;
+          }
+        }
+        if (SPREAD) {
+          o1 = o1 + x1; o2 = o2 + x2; o3 = o3 + x3;
+        } else {
+          o1 = o1 - x1; o2 = o2 - x2; o3 = o3 - x3;
+        }
+      }
+      if (SPREAD) {
+        out[c0 + t] = o1 + VF1;
+        out[cap + c0 + t] = o2 + VF2;
+        out[2LL * cap + c0 + t] = o3 + VF3;
+      } else {
+        out[c0 + t] = o1;
+        out[cap + c0 + t] = o2;
+        out[2LL * cap + c0 + t] = o3;
+      }
+    }
+    __syncthreads();  // (the next pass rewrites sptr and term)
   }
-  return n;
 }
 
 struct MdfTest {
@@ -564,9 +656,9 @@ struct MdfTest {
 };
 
 // velocity correction of every band cell: u -= fT*rx*ry*rz over its entries
-// in (marker, stencil) order (Solidbody.f90:1034-1048), the cells by the
-// first cell_blocks workgroups in a grid-stride loop (the host sizes them to
-// the last call's band, so that few idle workgroups wait for CU slots).
+// in (marker, stencil) order (Solidbody.f90:1034-1048), by the first
+// cell_blocks workgroups (band_cells_reduce; the host sizes the grid to the
+// last call's band).
 // round >= 0: the correction of that forcing round (runs when its k_interp
 // ran), and workgroup cell_blocks makes the round's stop test (the markers'
 // |dV|_1 summed in a fixed order, then mdf_decide); round < 0 (the
@@ -574,77 +666,99 @@ struct MdfTest {
 __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int round, MdfTest mt, int cap,
                                                  const int *__restrict__ ptr,
                                                  const int *__restrict__ entry,
-                                                 const float *__restrict__ Ew,
+                                                 const float4 *__restrict__ went,
                                                  const double *__restrict__ fT,
                                                  double *__restrict__ band_u, int cell_blocks) {
-  if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
   if ((int)blockIdx.x >= cell_blocks) {
+    // the stop test: its reads issued together, then the sums
+    double first[kTlBatch];
+    mdf_tl_first(mt.tl, mt.M, first);
+    const MdfWords wd = mdf_words(ctl, mt.dstate);
+    if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
     if (round >= 0) {
-      const double tol = mdf_tl_sum(mt.tl, mt.M);
-      if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol, mt.skip);
+      const double tol = mdf_tl_sum(mt.tl, mt.M, first);
+      if (threadIdx.x == 0) mdf_decide(ctl, mt.dstate, tol, mt.dsum, mt.Uref, mt.ntol, mt.dtol, mt.skip, wd);
     }
     return;
   }
-  const int nb = ctl[kNband];
-  for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += cell_blocks * blockDim.x) {
-  double u1 = band_u[b], u2 = band_u[cap + b], u3 = band_u[2LL * cap + b];
-  const int k1 = ptr[b + 1];
-  for (int k = ptr[b]; k < k1;) {
-    BandChunk c;
-    const int n = band_chunk(k, k1, entry, Ew, fT, c);
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j < n) {
-        u1 = u1 - ((c.v1[j] * c.rx[j]) * c.ry[j]) * c.rz[j];
-        u2 = u2 - ((c.v2[j] * c.rx[j]) * c.ry[j]) * c.rz[j];
-        u3 = u3 - ((c.v3[j] * c.rx[j]) * c.ry[j]) * c.rz[j];
-      }
-    k += n;
-  }
-  band_u[b] = u1;
-  band_u[cap + b] = u2;
-  band_u[2LL * cap + b] = u3;
-  }
+  if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
+  band_cells_reduce<false>(blockIdx.x, ctl[kNband], cell_blocks, cap, ptr, entry, went, fT, 1.0, band_u, 0.0, 0.0, 0.0);
 }
 
 // Inclusive scan of the per-cell entry counts of the band's cells (ctl[kNew],
 // final after k_ibm_claim) into the CSR pointers ptr[1..] (ptr[0] = 0), one
-// 1024-thread workgroup: contiguous per-thread runs, their totals scanned
-// within each wave by lane shuffles and across the sixteen waves by one wave
-// (two barriers), then the runs written with their offsets.  Integer sums:
-// the result does not depend on the order.
-__global__ __launch_bounds__(1024) void k_band_scan(const int *__restrict__ ctl, const int *__restrict__ count,
-                                                     int *__restrict__ ptr) {
+// 1024-thread workgroup: each thread reads a run of kScanRun counts into
+// registers at once (the reads are the scan's latency), the run totals are
+// scanned within each wave by lane shuffles and across the sixteen waves by
+// one wave, then the runs are written with their offsets; bands above 16384
+// cells take further passes.  Integer sums: the result does not depend on the
+// order.  The workgroup also reduces the claim workgroups' parts of the
+// band's bounding box (nbox of them) into ctl[kBoxLo..kBoxHi+2].
+constexpr int kScanRun = 16;
+__global__ __launch_bounds__(1024) void k_band_scan(int *__restrict__ ctl, const int *__restrict__ count,
+                                                     int *__restrict__ ptr, const int *__restrict__ blkbox,
+                                                     int nbox) {
   __shared__ int wtot[16];
+  __shared__ int wbox[16][6];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  {
+    int r[6] = {0x7fffffff, 0x7fffffff, 0x7fffffff, -1, -1, -1};
+    for (int q = t; q < nbox; q += blockDim.x)
+#pragma unroll
+      for (int k = 0; k < 6; ++k) r[k] = k < 3 ? min(r[k], blkbox[6 * q + k]) : max(r[k], blkbox[6 * q + k]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      int v = r[k];
+      for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, 64);
+        v = k < 3 ? min(v, o) : max(v, o);
+      }
+      if (lane == 0) wbox[w][k] = v;
+    }
+  }
   // the cells claimed in this call (k_ibm_rank copies this count to kNband only later)
   const int n = ctl[kNew];
-  const int per = (n + 1023) / 1024;
-  const int t = threadIdx.x, lo = t * per, hi = min(n, lo + per);
-  const int lane = t & 63, w = t >> 6;
-  int s = 0;
-  for (int i = lo; i < hi; ++i) s += count[i];
-  int v = s;
+  int carry = 0;
+  for (int base = 0; base < n; base += 1024 * kScanRun) {
+    const int lo = base + t * kScanRun;
+    int c[kScanRun];
 #pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int o = __shfl_up(v, off, 64);
-    if (lane >= off) v += o;
-  }
-  if (lane == 63) wtot[w] = v;
-  __syncthreads();
-  if (w == 0) {
-    int x = lane < 16 ? wtot[lane] : 0;
+    for (int j = 0; j < kScanRun; ++j) c[j] = lo + j < n ? count[lo + j] : 0;
+    int s = 0;
 #pragma unroll
-    for (int off = 1; off < 16; off <<= 1) {
-      const int o = __shfl_up(x, off, 64);
-      if (lane >= off) x += o;
+    for (int j = 0; j < kScanRun; ++j) s += c[j];
+    int v = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int o = __shfl_up(v, off, 64);
+      if (lane >= off) v += o;
     }
-    if (lane < 16) wtot[lane] = x;
+    __syncthreads();  // (the previous pass is done with wtot)
+    if (lane == 63) wtot[w] = v;
+    __syncthreads();
+    if (w == 0) {
+      int x = lane < 16 ? wtot[lane] : 0;
+#pragma unroll
+      for (int off = 1; off < 16; off <<= 1) {
+        const int o = __shfl_up(x, off, 64);
+        if (lane >= off) x += o;
+      }
+      if (lane < 16) wtot[lane] = x;
+    }
+    __syncthreads();
+    int run = carry + (v - s) + (w ? wtot[w - 1] : 0);
+#pragma unroll
+    for (int j = 0; j < kScanRun; ++j) {
+      run += c[j];
+      if (lo + j < n) ptr[lo + j + 1] = run;
+    }
+    carry += wtot[15];
   }
   __syncthreads();
-  int run = (v - s) + (w ? wtot[w - 1] : 0);
-  for (int i = lo; i < hi; ++i) {
-    run += count[i];
-    ptr[i + 1] = run;
+  if (t < 6) {
+    int r = wbox[0][t];
+    for (int q = 1; q < 16; ++q) r = t < 3 ? min(r, wbox[q][t]) : max(r, wbox[q][t]);
+    ctl[kBoxLo + t] = r;
   }
 }
 
@@ -667,7 +781,7 @@ struct IbmReport {
 // Eulerian force of every band cell (ResetVolumeForce_ FluidDomain.f90:
 // 1195-1206, FluidVolumeForce_ Solidbody.f90:968-977, add_volume_force_
 // FluidDomain.f90:1182-1193): (0 - sum F/h^3*rx*ry*rz) + VF, by the other
-// band_blocks workgroups in a grid-stride loop, the first of which copies the
+// band_blocks workgroups (band_cells_reduce, workgroup blk of them), the first of which copies the
 // control words into the report (the last k_correct has made the stop test,
 // so they are final).
 constexpr int kLoadChunk = 256;
@@ -678,19 +792,18 @@ struct IbmFinishArgs {
   IbmReport *rep;
   int cap;
   const int *ptr, *entry;
-  const float *Ew;
+  const float4 *went;
   const double *Eforce;
   double invh3, VF1, VF2, VF3;
   double *band_F;
   int ndof, dof_blocks, band_blocks;
-  const int *dof_ptr, *dof_list, *dof_kind, *marker_elem;
+  const int *dof_ptr, *dof_list, *dof_elem, *dof_kind, *marker_elem;
   const double *elem_xc, *Exyz;
   double *lod;
 };
 
-__device__ __forceinline__ double load_term(const IbmFinishArgs &a, int kind, int m) {
+__device__ __forceinline__ double load_term(const IbmFinishArgs &a, int kind, int m, int el) {
   if (kind < 3) return 0.5 * a.Eforce[3LL * m + kind];
-  const int el = a.marker_elem[m];
   const double r1 = a.Exyz[3LL * m] - a.elem_xc[3LL * el];
   const double r2 = a.Exyz[3LL * m + 1] - a.elem_xc[3LL * el + 1];
   const double r3 = a.Exyz[3LL * m + 2] - a.elem_xc[3LL * el + 2];
@@ -713,7 +826,7 @@ __global__ __launch_bounds__(256) void k_ibm_finish(IbmFinishArgs a) {
     double acc = 0.0;
     for (int base = j0; base < j1; base += kLoadChunk) {
       const int n = min(kLoadChunk, j1 - base);
-      for (int q = lane; q < n; q += 64) term[w][q] = load_term(a, kind, a.dof_list[base + q]);
+      for (int q = lane; q < n; q += 64) term[w][q] = load_term(a, kind, a.dof_list[base + q], a.dof_elem[base + q]);
       wave_sync();
       if (lane == 0)
         for (int q = 0; q < n; ++q) acc = acc + term[w][q];
@@ -725,27 +838,8 @@ __global__ __launch_bounds__(256) void k_ibm_finish(IbmFinishArgs a) {
   const int blk = (int)blockIdx.x - a.dof_blocks;
   if (blk == 0 && threadIdx.x < kCtlWords) a.rep->ctl[threadIdx.x] = a.ctl[threadIdx.x];
   if (blk == 0 && threadIdx.x == 0) a.rep->dmax = a.dstate[0];
-  const int nb = a.ctl[kNband];
-  for (int b = blk * blockDim.x + threadIdx.x; b < nb; b += a.band_blocks * blockDim.x) {
-  double g1 = 0.0, g2 = 0.0, g3 = 0.0;
-  const int k1 = a.ptr[b + 1];
-  for (int k = a.ptr[b]; k < k1;) {
-    BandChunk c;
-    const int n = band_chunk(k, k1, a.entry, a.Ew, a.Eforce, c);
-#pragma unroll
-    for (int j = 0; j < 8; ++j)
-      if (j < n) {
-        const double h1 = c.v1[j] * a.invh3, h2 = c.v2[j] * a.invh3, h3 = c.v3[j] * a.invh3;
-        g1 = g1 + (-(((h1 * c.rx[j]) * c.ry[j]) * c.rz[j]));
-        g2 = g2 + (-(((h2 * c.rx[j]) * c.ry[j]) * c.rz[j]));
-        g3 = g3 + (-(((h3 * c.rx[j]) * c.ry[j]) * c.rz[j]));
-      }
-    k += n;
-  }
-  a.band_F[b] = g1 + a.VF1;
-  a.band_F[a.cap + b] = g2 + a.VF2;
-  a.band_F[2LL * a.cap + b] = g3 + a.VF3;
-  }
+  band_cells_reduce<true>(blk, a.ctl[kNband], a.band_blocks, a.cap, a.ptr, a.entry, a.went, a.Eforce, a.invh3,
+                          a.band_F, a.VF1, a.VF2, a.VF3);
 }
 
 // ---------------------------------------------------------------------------
