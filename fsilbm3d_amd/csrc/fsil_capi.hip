@@ -423,10 +423,9 @@ bool is_active_face(int code) {
 }
 
 int launch_faces(Block &b) {
-  for (int face = 0; face < 6; ++face) {
-    const int code = b.bc[face];
-    if (!is_active_face(code)) continue;
+  auto args = [&](int face) {
     FaceArgs a{};
+    const int code = b.bc[face];
     a.f = b.fA.p;
     a.g = b.gA.p;
     a.xd = b.xd; a.yd = b.yd; a.zd = b.zd; a.N = b.N;
@@ -444,28 +443,50 @@ int launch_faces(Block &b) {
     a.nb_den = b.nb_den[face].p;
     a.nb_u = b.nb_u[face].p;
     a.hw = b.hw_valid[face] ? b.hw[face].p : nullptr;
-    const int np = plane_size(b, face);
-    const dim3 grid((np + 255) / 256), block(256);
-    if (b.precision == 0) {
-      switch (face) {
+    return a;
+  };
+  const dim3 block(256);
+  const bool f32 = b.precision != 0;
+  for (int axis = 0; axis < 3; ++axis) {
+    const int lo = 2 * axis, hi = lo + 1;
+    const bool alo = is_active_face(b.bc[lo]), ahi = is_active_face(b.bc[hi]);
+    const int nda = axis == 0 ? b.xd : (axis == 1 ? b.yd : b.zd);
+    const int nblk = (plane_size(b, lo) + 255) / 256;
+    if (alo && ahi && nda >= 6) {
+      // both faces of the axis in one launch (k_face_pair: disjoint planes)
+      const FaceArgs A = args(lo), B = args(hi);
+      const dim3 grid(2 * nblk);
+      switch (axis + (f32 ? 3 : 0)) {
+        case 0: hipLaunchKernelGGL((k_face_pair<0, false>), grid, block, 0, b.stream, A, B, nblk); break;
+        case 1: hipLaunchKernelGGL((k_face_pair<1, false>), grid, block, 0, b.stream, A, B, nblk); break;
+        case 2: hipLaunchKernelGGL((k_face_pair<2, false>), grid, block, 0, b.stream, A, B, nblk); break;
+        case 3: hipLaunchKernelGGL((k_face_pair<0, true>), grid, block, 0, b.stream, A, B, nblk); break;
+        case 4: hipLaunchKernelGGL((k_face_pair<1, true>), grid, block, 0, b.stream, A, B, nblk); break;
+        case 5: hipLaunchKernelGGL((k_face_pair<2, true>), grid, block, 0, b.stream, A, B, nblk); break;
+      }
+      HIPCHK(hipGetLastError());
+      continue;
+    }
+    for (int face = lo; face <= hi; ++face) {
+      if (!is_active_face(b.bc[face])) continue;
+      const FaceArgs a = args(face);
+      const dim3 grid((plane_size(b, face) + 255) / 256);
+      switch (face + (f32 ? 6 : 0)) {
         case 0: hipLaunchKernelGGL((k_face<0, false>), grid, block, 0, b.stream, a); break;
         case 1: hipLaunchKernelGGL((k_face<1, false>), grid, block, 0, b.stream, a); break;
         case 2: hipLaunchKernelGGL((k_face<2, false>), grid, block, 0, b.stream, a); break;
         case 3: hipLaunchKernelGGL((k_face<3, false>), grid, block, 0, b.stream, a); break;
         case 4: hipLaunchKernelGGL((k_face<4, false>), grid, block, 0, b.stream, a); break;
         case 5: hipLaunchKernelGGL((k_face<5, false>), grid, block, 0, b.stream, a); break;
+        case 6: hipLaunchKernelGGL((k_face<0, true>), grid, block, 0, b.stream, a); break;
+        case 7: hipLaunchKernelGGL((k_face<1, true>), grid, block, 0, b.stream, a); break;
+        case 8: hipLaunchKernelGGL((k_face<2, true>), grid, block, 0, b.stream, a); break;
+        case 9: hipLaunchKernelGGL((k_face<3, true>), grid, block, 0, b.stream, a); break;
+        case 10: hipLaunchKernelGGL((k_face<4, true>), grid, block, 0, b.stream, a); break;
+        case 11: hipLaunchKernelGGL((k_face<5, true>), grid, block, 0, b.stream, a); break;
       }
-    } else {
-      switch (face) {
-        case 0: hipLaunchKernelGGL((k_face<0, true>), grid, block, 0, b.stream, a); break;
-        case 1: hipLaunchKernelGGL((k_face<1, true>), grid, block, 0, b.stream, a); break;
-        case 2: hipLaunchKernelGGL((k_face<2, true>), grid, block, 0, b.stream, a); break;
-        case 3: hipLaunchKernelGGL((k_face<3, true>), grid, block, 0, b.stream, a); break;
-        case 4: hipLaunchKernelGGL((k_face<4, true>), grid, block, 0, b.stream, a); break;
-        case 5: hipLaunchKernelGGL((k_face<5, true>), grid, block, 0, b.stream, a); break;
-      }
+      HIPCHK(hipGetLastError());
     }
-    HIPCHK(hipGetLastError());
   }
   return 0;
 }

@@ -581,13 +581,13 @@ struct FaceIO {
   }
 };
 
+// one boundary node t of face FACE
 template <int FACE, bool F32>
-__global__ __launch_bounds__(256) void k_face(FaceArgs a) {
+__device__ __forceinline__ void face_node(const FaceArgs &a, int t) {
   using IO = FaceIO<F32>;
   constexpr int axis = FACE >> 1, hi = FACE & 1;
   const int nd0 = axis == 0 ? a.yd : a.xd;             // outer in-plane extent
   const int nd1 = axis == 2 ? a.yd : a.zd;             // inner in-plane extent
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nd0 * nd1) return;
   const int A = t / nd1, B = t - A * nd1;
   const int nda = axis == 0 ? a.xd : (axis == 1 ? a.yd : a.zd);
@@ -701,6 +701,22 @@ __global__ __launch_bounds__(256) void k_face(FaceArgs a) {
     default:
       break;
   }
+}
+
+template <int FACE, bool F32>
+__global__ __launch_bounds__(256) void k_face(FaceArgs a) {
+  face_node<FACE, F32>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Both faces of one axis in one launch (workgroups [0, nblk) the low face,
+// the rest the high face).  A face writes only its boundary plane and reads
+// that plane and the next two inward, so the two faces of an axis at least
+// six nodes long touch disjoint planes and may run together; the axes stay
+// in order (edges belong to two axes).
+template <int AXIS, bool F32>
+__global__ __launch_bounds__(256) void k_face_pair(FaceArgs lo, FaceArgs hi, int nblk) {
+  if ((int)blockIdx.x < nblk) face_node<2 * AXIS, F32>(lo, blockIdx.x * blockDim.x + threadIdx.x);
+  else face_node<2 * AXIS + 1, F32>(hi, ((int)blockIdx.x - nblk) * blockDim.x + threadIdx.x);
 }
 
 }  // namespace fsil
