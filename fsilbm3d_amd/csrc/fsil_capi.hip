@@ -162,6 +162,7 @@ struct Ibm {
   int last_warm = 0, last_fallback = 0, last_rounds = 0;
   int last_iters = 0;
   int last_nband = 0;  // band cells of the last checked call (sizes the band-cell grids)
+  int pass_cap = 0;    // entries per LDS pass of the band-cell sums (at most kPassEntries)
   double last_dmax = 0.0;
 };
 
@@ -1440,6 +1441,9 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   Ibm &I = b->ibm;
   I = Ibm();
   I.nbodies = nbodies;
+  // entries per LDS pass of the band-cell sums (tests force the per-cell form with a small value)
+  I.pass_cap = kPassEntries;
+  if (const char *e = getenv("FSIL_BAND_PASS")) I.pass_cap = std::max(0, std::min(kPassEntries, atoi(e)));
   I.nmark.assign(nmarkers, nmarkers + nbodies);
   I.nnode.assign(nnodes, nnodes + nbodies);
   I.nelem.assign(nelems, nelems + nbodies);
@@ -1652,7 +1656,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
     hipLaunchKernelGGL(k_ws_apply, dim3((3 * M + 255) / 256), dim3(256), 0, s, M, I.wsP.p, I.Ea.p, I.Eprev.p, I.Uref,
                        invh3p, I.Eforce.p, I.fT.p);
     hipLaunchKernelGGL(k_correct, dim3(cell_blocks), dim3(256), 0, s, I.ctl.p, -1, MdfTest{}, I.cap, I.dptr.p,
-                       I.dentry.p, I.went.p, I.fT.p, I.band_u.p, cell_blocks);
+                       I.dentry.p, I.went.p, I.fT.p, I.band_u.p, cell_blocks, I.pass_cap);
     HIPCHK(hipGetLastError());
   }
   // MDF loop (Solidbody.f90:893-906): the rounds are queued (with the warm
@@ -1670,7 +1674,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
                        I.ctl.p, I.Evel.p, I.Ea.p, invh3p, I.Eforce.p, I.fT.p, I.tolpart.p, snap, I.Esnap.p);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(k_correct, dim3(cell_blocks + 1), dim3(256), 0, s, I.ctl.p, it, mt, I.cap, I.dptr.p,
-                       I.dentry.p, I.went.p, I.fT.p, I.band_u.p, cell_blocks);
+                       I.dentry.p, I.went.p, I.fT.p, I.band_u.p, cell_blocks, I.pass_cap);
     HIPCHK(hipGetLastError());
     if (warm && it == 0) {
       // restart from the zero guess when the first round asked for it
@@ -1704,6 +1708,7 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   fa.marker_elem = I.dmarker_elem.p; fa.elem_xc = I.elem_xc.p; fa.Exyz = I.Exyz.p;
   fa.lod = I.lod.p;
   fa.band_blocks = cell_blocks;
+  fa.pass_cap = I.pass_cap;
   hipLaunchKernelGGL(k_ibm_finish, dim3(fa.dof_blocks + cell_blocks), dim3(256), 0, s, fa);
   HIPCHK(hipGetLastError());
   HIPCHK(hipEventRecord(I.rep->ev, s));

@@ -564,8 +564,8 @@ __global__ __launch_bounds__(256) void k_interp(int M, int round, const int *__r
 // the CSR arrays, so each thread forms one entry's term from coalesced reads
 // (entry, weights) and one gather (the marker's value) into LDS, and then one
 // thread per cell adds its terms in entry order, as the sequential loop does.
-// A pass whose range exceeds the LDS area (kPassEntries) falls back to one
-// thread per cell reading its entries itself.  SPREAD: term = -(((v h^-3 rx)
+// A pass whose range exceeds the LDS area (pass_cap <= kPassEntries entries)
+// falls back to one thread per cell reading its entries itself.  SPREAD: term = -(((v h^-3 rx)
 // ry) rz), g = sum + VF (out = band_F); else u -= ((v rx) ry) rz (out =
 // band_u, read first).
 constexpr int kCellsPerPass = 32;
@@ -574,7 +574,7 @@ template <bool SPREAD>
 __device__ inline void band_cells_reduce(int blk, int nb, int nblk, int cap, const int *__restrict__ ptr,
                                          const int *__restrict__ entry, const float4 *__restrict__ went,
                                          const double *__restrict__ val, double invh3, double *__restrict__ out,
-                                         double VF1, double VF2, double VF3) {
+                                         double VF1, double VF2, double VF3, int pass_cap) {
   __shared__ int sptr[kCellsPerPass + 1];
   __shared__ double term[3][kPassEntries];
   const int t = threadIdx.x;
@@ -589,7 +589,7 @@ __device__ inline void band_cells_reduce(int blk, int nb, int nblk, int cap, con
     }
     __syncthreads();
     const int e0 = sptr[0], ne = sptr[nc] - e0;
-    const bool fits = ne <= kPassEntries;
+    const bool fits = ne <= pass_cap;
     if (fits) {
       for (int k = t; k < ne; k += blockDim.x) {
         const int m = entry[e0 + k] >> 6;
@@ -668,7 +668,7 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
                                                  const int *__restrict__ entry,
                                                  const float4 *__restrict__ went,
                                                  const double *__restrict__ fT,
-                                                 double *__restrict__ band_u, int cell_blocks) {
+                                                 double *__restrict__ band_u, int cell_blocks, int pass_cap) {
   if ((int)blockIdx.x >= cell_blocks) {
     // the stop test: its reads issued together, then the sums
     double first[kTlBatch];
@@ -682,7 +682,8 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
     return;
   }
   if (round >= 0 ? ctl[kRan] != round : ctl[kDone] != 0) return;  // uniform over the grid
-  band_cells_reduce<false>(blockIdx.x, ctl[kNband], cell_blocks, cap, ptr, entry, went, fT, 1.0, band_u, 0.0, 0.0, 0.0);
+  band_cells_reduce<false>(blockIdx.x, ctl[kNband], cell_blocks, cap, ptr, entry, went, fT, 1.0, band_u, 0.0, 0.0,
+                           0.0, pass_cap);
 }
 
 // Inclusive scan of the per-cell entry counts of the band's cells (ctl[kNew],
@@ -796,7 +797,7 @@ struct IbmFinishArgs {
   const double *Eforce;
   double invh3, VF1, VF2, VF3;
   double *band_F;
-  int ndof, dof_blocks, band_blocks;
+  int ndof, dof_blocks, band_blocks, pass_cap;
   const int *dof_ptr, *dof_list, *dof_elem, *dof_kind, *marker_elem;
   const double *elem_xc, *Exyz;
   double *lod;
@@ -839,7 +840,7 @@ __global__ __launch_bounds__(256) void k_ibm_finish(IbmFinishArgs a) {
   if (blk == 0 && threadIdx.x < kCtlWords) a.rep->ctl[threadIdx.x] = a.ctl[threadIdx.x];
   if (blk == 0 && threadIdx.x == 0) a.rep->dmax = a.dstate[0];
   band_cells_reduce<true>(blk, a.ctl[kNband], a.band_blocks, a.cap, a.ptr, a.entry, a.went, a.Eforce, a.invh3,
-                          a.band_F, a.VF1, a.VF2, a.VF3);
+                          a.band_F, a.VF1, a.VF2, a.VF3, a.pass_cap);
 }
 
 // ---------------------------------------------------------------------------
