@@ -1620,8 +1620,8 @@ int fsil_ibm_interaction_force(int h, double dt, const double *Exyz, const doubl
   // band: cells bid for by the stencils and claimed, per-cell counts (I.scan) and slots
   // (I.flags), CSR pointers by an inclusive scan into dptr[1..], entries
   // placed in arrival order (I.tmp), then at their ranks by entry number
+  sa.cellmap = I.cellmap.p;  // (the stencil kernel makes the bids)
   hipLaunchKernelGGL(k_ibm_stencil, dim3((M + 3) / 4), dim3(256), 0, s, sa);
-  hipLaunchKernelGGL(k_ibm_bid, dim3((E + 255) / 256), dim3(256), 0, s, E, I.keys.p, I.cellmap.p);
   const int nclaim = (E + kClaimThreads - 1) / kClaimThreads;
   hipLaunchKernelGGL(k_ibm_claim, dim3(nclaim), dim3(kClaimThreads), 0, s, E, I.keys.p, I.cellmap.p, I.dcell.p,
                      b->yd, b->zd, I.blkbox.p, I.ctl.p);

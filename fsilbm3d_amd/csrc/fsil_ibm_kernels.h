@@ -122,11 +122,15 @@ struct StencilArgs {
   float *__restrict__ Ew;                // 12*M
   unsigned *__restrict__ keys;           // 64*M: cell of entry e = 64*m + s
   int *__restrict__ ctl;
+  int *__restrict__ cellmap;             // the entries' bids
 };
 
 // UpdateElmtInterp_ (Solidbody.f90:760-821), one wave per marker: every lane
 // forms the marker's anchor (from its body's first marker) and trimmed 4x3
-// indices, lanes 0..11 write the FP32 weights, lane s the cell of entry s.
+// indices, lanes 0..11 write the FP32 weights, lane s the cell of entry s and
+// that entry's bid in the cell map: every entry bids -(e+2) for its cell, the
+// map keeps the lowest bid, so the cell's highest entry wins (cells outside
+// the band hold -1).
 __global__ __launch_bounds__(256) void k_ibm_stencil(StencilArgs a) {
   const int s = threadIdx.x & 63;
   const int m = blockIdx.x * 4 + ((int)threadIdx.x >> 6);
@@ -150,22 +154,18 @@ __global__ __launch_bounds__(256) void k_ibm_stencil(StencilArgs a) {
     if ((s >> 2) == k) a.Ew[12LL * m + s] = (float)peskin4((double)((s & 3) - 1) - off);
   }
   if (s == 0 && err) atomicMin(&a.ctl[kErr], (m + 1) * 4 + err);
-  a.keys[64LL * m + s] =
+  const unsigned key =
       ((unsigned)(cidx[0] - 1) * (unsigned)a.nd[1] + (unsigned)(cidx[1] - 1)) * (unsigned)a.nd[2] +
       (unsigned)(cidx[2] - 1);
+  a.keys[64LL * m + s] = key;
+  // the entry's bid for its cell
+  atomicMin(&a.cellmap[key], -(64 * m + s + 2));
 }
 
 // Cell-map accesses of the band build go through device-scope atomics (the
 // workgroups that bid for, claim and look up one cell run on different XCDs).
 __device__ __forceinline__ int cellmap_load(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// every entry bids -(e+2) for its cell: the cell map keeps the lowest bid, so
-// the cell's highest entry wins (cells outside the band hold -1)
-__global__ __launch_bounds__(256) void k_ibm_bid(int E, const unsigned *__restrict__ keys, int *__restrict__ cellmap) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < E) atomicMin(&cellmap[keys[e]], -(e + 2));
 }
 
 // the winning entry of each cell takes a band number and writes it into the
