@@ -1514,7 +1514,7 @@ int fsil_ibm_define_bodies(int h, int nbodies, const int *nmarkers, const int *n
   HIPCHK(I.tmp.ensure(I.cap));
   HIPCHK(I.blkbox.ensure(6 * (size_t)((I.cap + kClaimThreads - 1) / kClaimThreads)));
   HIPCHK(I.flags.ensure(I.cap));
-  HIPCHK(I.scan.ensure(I.cap));
+  HIPCHK(I.scan.ensure((size_t)I.cap + kScanRun));  // k_band_scan reads whole runs of kScanRun
   HIPCHK(I.dentry.ensure(I.cap));
   HIPCHK(I.went.ensure(I.cap));
   HIPCHK(I.dbidx.ensure(I.cap));

@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_ibm_fill(int E, const int *__restrict__
 }
 
 // every entry at its rank in its cell's range (the number of the cell's
-// entries below it, its range read eight entries at a time), so each cell
+// entries below it, its range read sixteen entries at a time), so each cell
 // lists its entries in ascending e, its weights (rx, ry, rz) beside it in
 // went; the band size is published
 __global__ __launch_bounds__(256) void k_ibm_rank(int E, int *__restrict__ ctl, const int *__restrict__ bidx,
@@ -294,21 +294,23 @@ __global__ __launch_bounds__(256) void k_ibm_rank(int E, int *__restrict__ ctl, 
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e == 0) ctl[kNband] = ctl[kNew];
   if (e >= E) return;
+  // the entry's weights are read with its cell (they need nothing else)
+  const int m = e >> 6, sidx = e & 63;
+  const float *W = Ew + 12LL * m;
+  const float wx = W[sidx >> 4], wy = W[4 + ((sidx >> 2) & 3)], wz = W[8 + (sidx & 3)];
   const int b = bidx[e];
   const int p0 = band_ptr[b], n = band_ptr[b + 1] - p0;
   int r = 0;
-  for (int j0 = 0; j0 < n; j0 += 8) {
-    int v[8];
+  for (int j0 = 0; j0 < n; j0 += 16) {
+    int v[16];
 #pragma unroll
-    for (int j = 0; j < 8; ++j) v[j] = j0 + j < n ? tmp[p0 + j0 + j] : 0x7fffffff;
+    for (int j = 0; j < 16; ++j) v[j] = j0 + j < n ? tmp[p0 + j0 + j] : 0x7fffffff;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) r += v[j] < e ? 1 : 0;
+    for (int j = 0; j < 16; ++j) r += v[j] < e ? 1 : 0;
   }
   entry[p0 + r] = e;
   // the entry's three stencil weights beside it, in the cells' order
-  const int m = e >> 6, sidx = e & 63;
-  const float *W = Ew + 12LL * m;
-  went[p0 + r] = make_float4(W[sidx >> 4], W[4 + ((sidx >> 2) & 3)], W[8 + (sidx & 3)], 0.0f);
+  went[p0 + r] = make_float4(wx, wy, wz, 0.0f);
 }
 
 // band macros: den/u of every band cell from the stored state, same
@@ -695,7 +697,7 @@ __global__ __launch_bounds__(256) void k_correct(int *__restrict__ ctl, int roun
 // cells take further passes.  Integer sums: the result does not depend on the
 // order.  The workgroup also reduces the claim workgroups' parts of the
 // band's bounding box (nbox of them) into ctl[kBoxLo..kBoxHi+2].
-constexpr int kScanRun = 16;
+constexpr int kScanRun = 16;  // (a multiple of 4: the runs are read as int4)
 __global__ __launch_bounds__(1024) void k_band_scan(int *__restrict__ ctl, const int *__restrict__ count,
                                                      int *__restrict__ ptr, const int *__restrict__ blkbox,
                                                      int nbox) {
@@ -723,8 +725,18 @@ __global__ __launch_bounds__(1024) void k_band_scan(int *__restrict__ ctl, const
   for (int base = 0; base < n; base += 1024 * kScanRun) {
     const int lo = base + t * kScanRun;
     int c[kScanRun];
+    if (lo < n) {
+      // the run in 16-byte reads (count holds kScanRun spare words past the band)
+      const int4 *q = reinterpret_cast<const int4 *>(count + lo);
 #pragma unroll
-    for (int j = 0; j < kScanRun; ++j) c[j] = lo + j < n ? count[lo + j] : 0;
+      for (int j = 0; j < kScanRun / 4; ++j) {
+        const int4 v = q[j];
+        c[4 * j] = v.x; c[4 * j + 1] = v.y; c[4 * j + 2] = v.z; c[4 * j + 3] = v.w;
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kScanRun; ++j)
+      if (!(lo + j < n)) c[j] = 0;
     int s = 0;
 #pragma unroll
     for (int j = 0; j < kScanRun; ++j) s += c[j];
