@@ -489,17 +489,75 @@ __device__ inline double *fem_lds_base() {
   return lds_raw + (((size_t)lds_raw & 15) ? 1 : 0);
 }
 
-// the stiffness and geometric-stiffness patterns, staged in LDS by the kernel
-struct FemPatterns {
-  signed char stiff[NED * NED], geo[NED * NED];
-};
+// Block BLK (block row BLK / 4, column BLK % 4) of an element's R^T K R,
+// R^T G R and the Newmark tangent matrix (Segment_UpdateMatrix,
+// SolidSolver.f90:131-157), each entry summed as el_rkr sums it.  The
+// patterns of K and G are constants here: a product with a zero entry (and a
+// product with an inner sum that has no terms) adds a signed zero to a sum
+// that starts at +0 and is left out.  cl: the element's LDS matrices
+// (transposed), or null for the element record's coef.
+template <int BLK, bool HOT, class ET>
+__device__ inline void fem_rkr_block(ET &E, double *cl, const double *c, const FemBodyDesc &D) {
+  constexpr int j1 = (BLK >> 2) * 3, j2 = (BLK & 3) * 3;
+  double r[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) r[q] = E.tee[(q % 3) * 3 + q / 3];  // R = tee^T (RotateMatrix)
+  double ks[9], kg[9];
+#pragma unroll
+  for (int a2 = 0; a2 < 3; ++a2)
+#pragma unroll
+    for (int b2 = 0; b2 < 3; ++b2) {
+      const int cs = kStiffPatternD.c[(j1 + a2) * NED + j2 + b2], cg = kGeoPatternD.c[(j1 + a2) * NED + j2 + b2];
+      ks[a2 * 3 + b2] = cs == 0 ? 0.0 : (cs > 0 ? E.ksc[cs - 1] : -E.ksc[-cs - 1]);
+      kg[a2 * 3 + b2] = cg == 0 ? 0.0 : (cg > 0 ? E.gsc[cg - 1] : -E.gsc[-cg - 1]);
+    }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int ii = 0; ii < 3; ++ii) {
+      double sR = 0.0, gR = 0.0;
+#pragma unroll
+      for (int jj = 0; jj < 3; ++jj) {
+        double kts = 0.0, ktg = 0.0;
+        bool any_s = false, any_g = false;
+#pragma unroll
+        for (int l = 0; l < 3; ++l) {
+          if (kStiffPatternD.c[(j1 + jj) * NED + j2 + l] != 0) {
+            kts = kts + ks[jj * 3 + l] * r[l * 3 + ii];
+            any_s = true;
+          }
+          if (kGeoPatternD.c[(j1 + jj) * NED + j2 + l] != 0) {
+            ktg = ktg + kg[jj * 3 + l] * r[l * 3 + ii];
+            any_g = true;
+          }
+        }
+        if (any_s) sR = sR + r[jj * 3 + k] * kts;
+        if (any_g) gR = gR + r[jj * 3 + k] * ktg;
+      }
+      const int idx = (j1 + k) * NED + j2 + ii;
+      const double tanv = sR + D.GeoGamma * gR;
+      // the rotated mass matrix is block diagonal: its off-diagonal 3x3
+      // blocks hold exact +0 (the rotation of a zero block)
+      const double m = j1 == j2 ? mas_entry(E, j1 + k, j2 + ii) : 0.0;
+      double cv = tanv + c[0] * m + c[1] * D.dampM * m;
+      if constexpr (!HOT) {  // the LDS form is taken without stiffness damping
+        if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
+        if (D.dampK > 0.0) E.stf[idx] = sR;  // read only by the stiffness-damping load
+      }
+      if (cl) {
+        cl[(j2 + ii) * NED + j1 + k] = cv;
+      } else {
+        if constexpr (!HOT) E.coef[idx] = cv;  // the LDS form always has cl
+      }
+    }
+}
 
 // One sub-step of one body by the whole workgroup.  HOT: the body's element
 // fields of the Newton iterations are in LDS (H, BeamHot per element), else
 // every element access goes to the global BeamElem array.
 template <bool HOT>
 __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int work, int vec_lds, int cg_lds,
-                                        int coef_off, BeamHot *H, const FemPatterns &P) {
+                                        int coef_off, BeamHot *H) {
   __shared__ double sh_val;
   __shared__ double sh_pair[2];
   __shared__ int sh_flag;
@@ -614,55 +672,23 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
   // tangent matrix (Segment_UpdateMatrix, SolidSolver.f90:131-157); each
   // entry is summed as el_rkr sums it.
   auto matrices_phase = [&]() {
-    for (int t = threadIdx.x; t < nE * 16; t += blockDim.x) {
-      const int e = t >> 4, blkij = t & 15;
+    // one 3x3 block per task, the block the same across a wave (64 element
+    // slots per block), so that the block's stiffness and geometric patterns
+    // are known where the products are formed (fem_rkr_block)
+    const int epad = (nE + 63) & ~63;
+    for (int t = threadIdx.x; t < 16 * epad; t += blockDim.x) {
+      const int blk = t / epad, e = t - blk * epad;
+      if (e >= nE) continue;
       auto &E = EL(e);
-      const int j1 = (blkij >> 2) * 3, j2 = (blkij & 3) * 3;
-      double r[9], ks[9], kg[9];
-      // the rotation R (RotateMatrix: the transposed triad tee)
-#pragma unroll
-      for (int q = 0; q < 9; ++q) r[q] = E.tee[(q % 3) * 3 + q / 3];
-#pragma unroll
-      for (int a2 = 0; a2 < 3; ++a2)
-#pragma unroll
-        for (int b2 = 0; b2 < 3; ++b2) {
-          const int at = (j1 + a2) * NED + j2 + b2;
-          const int cs = P.stiff[at], cg = P.geo[at];
-          ks[a2 * 3 + b2] = cs == 0 ? 0.0 : (cs > 0 ? E.ksc[cs - 1] : -E.ksc[-cs - 1]);
-          kg[a2 * 3 + b2] = cg == 0 ? 0.0 : (cg > 0 ? E.gsc[cg - 1] : -E.gsc[-cg - 1]);
-        }
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-#pragma unroll
-        for (int ii = 0; ii < 3; ++ii) {
-          double sR = 0.0, gR = 0.0;
-#pragma unroll
-          for (int jj = 0; jj < 3; ++jj) {
-            double kts = 0.0, ktg = 0.0;
-#pragma unroll
-            for (int l = 0; l < 3; ++l) {
-              kts = kts + ks[jj * 3 + l] * r[l * 3 + ii];
-              ktg = ktg + kg[jj * 3 + l] * r[l * 3 + ii];
-            }
-            sR = sR + r[jj * 3 + k] * kts;
-            gR = gR + r[jj * 3 + k] * ktg;
-          }
-          const int idx = (j1 + k) * NED + j2 + ii;
-          const double tanv = sR + D.GeoGamma * gR;
-          // the rotated mass matrix is block diagonal: its off-diagonal 3x3
-          // blocks hold exact +0 (the rotation of a zero block)
-          const double m = j1 == j2 ? mas_entry(E, j1 + k, j2 + ii) : 0.0;
-          double cv = tanv + c[0] * m + c[1] * D.dampM * m;
-          if constexpr (!HOT) {  // the LDS form is taken without stiffness damping
-            if (D.dampK > 0.0) cv = cv + c[1] * D.dampK * sR;
-            if (D.dampK > 0.0) E.stf[idx] = sR;  // read only by the stiffness-damping load
-          }
-          if (coefL) {
-            coefL[e * kCoefStride + (j2 + ii) * NED + j1 + k] = cv;
-          } else {
-            if constexpr (!HOT) E.coef[idx] = cv;  // the LDS form always has coefL
-          }
-        }
+      double *cl = coefL ? coefL + (size_t)e * kCoefStride : nullptr;
+      switch (blk) {
+#define FEM_BLK(B) \
+  case B: fem_rkr_block<B, HOT>(E, cl, c, D); break;
+        FEM_BLK(0) FEM_BLK(1) FEM_BLK(2) FEM_BLK(3) FEM_BLK(4) FEM_BLK(5) FEM_BLK(6) FEM_BLK(7)
+        FEM_BLK(8) FEM_BLK(9) FEM_BLK(10) FEM_BLK(11) FEM_BLK(12) FEM_BLK(13) FEM_BLK(14) FEM_BLK(15)
+#undef FEM_BLK
+        default: break;
+      }
     }
   };
   auto load_phase = [&](int t0, int step) {
@@ -672,16 +698,31 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
   // internal, external and effective loads per DOF (LocToGlobal order)
   auto assembly_phase = [&](int t0, int step) {
     for (int g = t0; g < nG; g += step) {
-      double li = 0.0;
-      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
-        li = li + EL(A.dof_e[k] - D.eoff).fint[A.dof_i[k]];
-      A.lodInte[g0 + g] = li;
-      double le = A.lodExte[g0 + g] - li;
-      for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k) {
-        le = le + EL(A.dof_e[k] - D.eoff).fload[A.dof_i[k]];
-        if constexpr (!HOT)
-          if (D.dampK > 0.0) le = le + A.el[A.dof_e[k]].fdamp[A.dof_i[k]];
+      const double ext = A.lodExte[g0 + g];
+      double li = 0.0, le;
+      if (HOT && C.on) {
+        // the same lists from LDS (the CG's copy, entry = element * 12 + index)
+        const int q0 = C.ptr[g], q1 = C.ptr[g + 1];
+        for (int q = q0; q < q1; ++q) {
+          const int ent = C.ent[q], e = ent / NED;
+          li = li + EL(e).fint[ent - e * NED];
+        }
+        le = ext - li;
+        for (int q = q0; q < q1; ++q) {
+          const int ent = C.ent[q], e = ent / NED;
+          le = le + EL(e).fload[ent - e * NED];
+        }
+      } else {
+        for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k)
+          li = li + EL(A.dof_e[k] - D.eoff).fint[A.dof_i[k]];
+        le = ext - li;
+        for (int k = A.dof_ptr[g0 + g]; k < A.dof_ptr[g0 + g + 1]; ++k) {
+          le = le + EL(A.dof_e[k] - D.eoff).fload[A.dof_i[k]];
+          if constexpr (!HOT)
+            if (D.dampK > 0.0) le = le + A.el[A.dof_e[k]].fdamp[A.dof_i[k]];
+        }
       }
+      A.lodInte[g0 + g] = li;
       A.lodEffe[g0 + g] = le;
     }
   };
@@ -975,12 +1016,7 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, int nbo
   // descs is pinned host memory: each sub-step's descriptor is read into LDS
   // once, and its results are written back
   __shared__ FemBodyDesc sD;
-  __shared__ FemPatterns sP;
   static_assert(sizeof(FemBodyDesc) % 8 == 0 && sizeof(FemBodyDesc) / 8 <= 512, "descriptor copy by 8-byte words");
-  for (int t = threadIdx.x; t < NED * NED; t += blockDim.x) {
-    sP.stiff[t] = kStiffPatternD.c[t];
-    sP.geo[t] = kGeoPatternD.c[t];
-  }
   BeamHot *H = HOT ? reinterpret_cast<BeamHot *>(lds + hot_off) : nullptr;
   long long *hw = reinterpret_cast<long long *>(H);
   bool hot = false;
@@ -1001,7 +1037,7 @@ __global__ __launch_bounds__(512) void k_fem_substep(FemBodyDesc *descs, int nbo
       }
     }
     // (a rigid body's sub-step does not reach the element fields)
-    fem_substep_body<HOT>(sD, A, work, vec_lds, cg_lds, coef_off, H, sP);
+    fem_substep_body<HOT>(sD, A, work, vec_lds, cg_lds, coef_off, H);
     fem_sync();
     if (threadIdx.x == 0) {
       g->newton_iters = sD.newton_iters;
