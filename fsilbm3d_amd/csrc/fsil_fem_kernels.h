@@ -729,12 +729,13 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
   for (iter = 1; iter <= D.ntolFEM; ++iter) {
     if constexpr (HOT) {
       // Without stiffness damping the element loads do not read the rotated
-      // stiffness: the internal forces (first half of the workgroup) and the
-      // element loads (second half) run side by side, then the element
-      // matrices and the DOF loads (from the top thread down).
-      const int half = (int)blockDim.x / 2;
-      if ((int)threadIdx.x < half) stress_phase(threadIdx.x, half);
-      else load_phase((int)threadIdx.x - half, (int)blockDim.x - half);
+      // stiffness: the internal forces (even waves) and the element loads
+      // (odd waves, so on other SIMDs than the first forces) run side by
+      // side, then the element matrices and the DOF loads (from the top
+      // thread down).
+      const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63, nw2 = (int)blockDim.x >> 7;
+      if ((w & 1) == 0) stress_phase((w >> 1) * 64 + lane, nw2 * 64);
+      else load_phase((w >> 1) * 64 + lane, nw2 * 64);
       fem_sync();
       matrices_phase();
       assembly_phase((int)blockDim.x - 1 - (int)threadIdx.x, blockDim.x);
