@@ -590,15 +590,20 @@ __host__ __device__ inline void el_update_load(ET &e, const double *c, double da
   }
 }
 
-// Segment_UpdateTriad_D + Segment_MakeTriad_ee (SolidSolver.f90:966-1064)
+// Segment_UpdateTriad_D (SolidSolver.f90:966-997) for one end of the
+// element: the end's triad rotated by its node's rotation increment
 template <class ET>
-__host__ __device__ inline void el_update_triads(ET &e, const double *dspnn) {
+__host__ __device__ inline void el_update_node_triad(ET &e, const double *dspnn, int end) {
   double rr[9];
-  finite_rot(dspnn[6 * e.node0 + 3], dspnn[6 * e.node0 + 4], dspnn[6 * e.node0 + 5], rr);
-  bmm3(rr, e.tn1, e.tn1);
-  finite_rot(dspnn[6 * e.node1 + 3], dspnn[6 * e.node1 + 4], dspnn[6 * e.node1 + 5], rr);
-  bmm3(rr, e.tn2, e.tn2);
-  // MakeTriad_ee
+  const int n = end ? e.node1 : e.node0;
+  finite_rot(dspnn[6 * n + 3], dspnn[6 * n + 4], dspnn[6 * n + 5], rr);
+  if (end) bmm3(rr, e.tn2, e.tn2);
+  else bmm3(rr, e.tn1, e.tn1);
+}
+// Segment_MakeTriad_ee (SolidSolver.f90:999-1064) from the updated end triads
+template <class ET>
+__host__ __device__ inline void el_make_triad_ee(ET &e) {
+  double rr[9];
   double *E = e.tee;
   for (int k = 0; k < 9; ++k) E[k] = 0.0;
   E[0] = e.xll1; E[3] = e.xmm1; E[6] = e.xnn1;
@@ -627,6 +632,14 @@ __host__ __device__ inline void el_update_triads(ET &e, const double *dspnn) {
   const double a1 = E[6] * E[1] - E[0] * E[7];
   const double a2 = E[0] * E[4] - E[3] * E[1];
   E[2] = a0; E[5] = a1; E[8] = a2;
+}
+
+// Segment_UpdateTriad_D + Segment_MakeTriad_ee (SolidSolver.f90:966-1064)
+template <class ET>
+__host__ __device__ inline void el_update_triads(ET &e, const double *dspnn) {
+  el_update_node_triad(e, dspnn, 0);
+  el_update_node_triad(e, dspnn, 1);
+  el_make_triad_ee(e);
 }
 
 // Invert6x6 (SolidSolver.f90:2234-2279), Gauss-Jordan with partial pivoting

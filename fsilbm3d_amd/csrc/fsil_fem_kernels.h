@@ -957,16 +957,28 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = mx;
     }
     fem_sync();
-    for (int e = threadIdx.x; e < nE; e += blockDim.x) {
-      auto &E = EL(e);
-      const double *x0 = A.el[D.eoff + e].x0;
-      for (int k = 0; k < 6; ++k) {
-        E.x1[k] = x0[k] + A.dsp[g0 + 6 * E.node0 + k];
-        E.x1[6 + k] = x0[6 + k] + A.dsp[g0 + 6 * E.node1 + k];
+    // the two end triads on waves of different SIMDs (even waves: the first
+    // end and the current geometry; odd waves: the second end), then the
+    // element triad
+    {
+      const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63, nw2 = (int)blockDim.x >> 7;
+      for (int e = (w >> 1) * 64 + lane; e < nE; e += nw2 * 64) {
+        auto &E = EL(e);
+        if ((w & 1) == 0) {
+          const double *x0 = A.el[D.eoff + e].x0;
+          for (int k = 0; k < 6; ++k) {
+            E.x1[k] = x0[k] + A.dsp[g0 + 6 * E.node0 + k];
+            E.x1[6 + k] = x0[6 + k] + A.dsp[g0 + 6 * E.node1 + k];
+          }
+          el_cptdxyz1(E);
+          el_update_node_triad(E, V.x, 0);
+        } else {
+          el_update_node_triad(E, V.x, 1);
+        }
       }
-      el_cptdxyz1(E);
-      el_update_triads(E, V.x);
     }
+    fem_sync();
+    for (int e = threadIdx.x; e < nE; e += blockDim.x) el_make_triad_ee(EL(e));
     {
       double m = lds[0];
       for (int w = 1; w < (int)(blockDim.x + 63) / 64; ++w)
