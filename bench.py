@@ -368,8 +368,9 @@ def main():
                          "training and BF16 inference kernels (mlp)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
-              "decomposition is a design note, not a measured figure.", file=sys.stderr)
+        print(f"bench.py: the benchmark runs on one GPU only (asked for {args.gpus}); the path's multi-GPU "
+              "decomposition (x-slabs with plane halos) is not implemented, so no multi-GPU figure is measured.",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     from tests.golden.cases import seeded_state
