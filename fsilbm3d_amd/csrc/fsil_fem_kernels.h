@@ -837,14 +837,35 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
             pr[i] = pp;
           }
           const int P = pr[i];  // the node's lane holding the pivot row
-          if (!bad && row == P) {
-            const double pv = a[i];
+          // the pivot row divided by the pivot: the 11 - i quotients are
+          // shared out over the node's six lanes (two divisions per lane
+          // instead of 11 - i by the pivot's lane, whose divisions the whole
+          // wave would issue), then gathered by every lane
+          const double pv = __shfl(a[i], lbase + P, 64);
+          double raw[12];
 #pragma unroll
-            for (int j = i + 1; j < 12; ++j) a[j] = a[j] / pv;
+          for (int j = i + 1; j < 12; ++j) raw[j] = __shfl(a[j], lbase + P, 64);
+          double n0 = 0.0, n1 = 0.0;
+#pragma unroll
+          for (int kk = 0; kk < 6; ++kk) {
+            if (row == kk && i + 1 + kk < 12) n0 = raw[i + 1 + kk];
+            if (row == kk && i + 7 + kk < 12) n1 = raw[i + 7 + kk];
+          }
+          double q0 = 0.0, q1 = 0.0;
+          if (!bad) {
+            q0 = n0 / pv;
+            q1 = n1 / pv;
           }
           double prow[12];
 #pragma unroll
-          for (int j = i + 1; j < 12; ++j) prow[j] = __shfl(a[j], lbase + P, 64);
+          for (int j = i + 1; j < 12; ++j) {
+            const int kk = j - i - 1;
+            prow[j] = kk < 6 ? __shfl(q0, lbase + kk, 64) : __shfl(q1, lbase + kk - 6, 64);
+          }
+          if (!bad && row == P) {
+#pragma unroll
+            for (int j = i + 1; j < 12; ++j) a[j] = prow[j];
+          }
           if (!bad && row != P) {
             const double factor = a[i];
 #pragma unroll
