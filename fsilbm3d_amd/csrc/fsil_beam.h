@@ -464,21 +464,28 @@ __host__ __device__ inline double rkr_entry(const double *ek, const double *r, i
 // Segment_BodyStress_D (SolidSolver.f90:750-839): local internal force -> e.fint,
 // with the local stiffness of Segment_FormStiffMatrix (e.stl, formed once)
 template <class ET>
-__host__ __device__ inline void el_body_stress(ET &e) {
+__host__ __device__ inline double el_stress_dl(const ET &e) {
   const double du = e.dx1 - e.dx0, dv = e.dy1 - e.dy0, dw = e.dz1 - e.dz0;
-  const double dl = ((e.dx0 + e.dx1) * du + (e.dy0 + e.dy1) * dv + (e.dz0 + e.dz1) * dw) / (e.len0 + e.len1);
-  double t[3], a[3], b[3];
+  return ((e.dx0 + e.dx1) * du + (e.dy0 + e.dy1) * dv + (e.dz0 + e.dz1) * dw) / (e.len0 + e.len1);
+}
+// the local rotation of one end (0: tn1, 1: tn2) relative to the element triad
+template <class ET>
+__host__ __device__ inline void el_stress_end(const ET &e, int end, double *out) {
+  double t[3];
   const double *T = e.tee;
-  angle_triad(e.tee, e.tn1, t);
-  a[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
-  a[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
-  a[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
-  angle_triad(e.tee, e.tn2, t);
-  b[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
-  b[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
-  b[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
+  angle_triad(e.tee, end ? e.tn2 : e.tn1, t);
+  out[0] = T[0] * t[0] + T[3] * t[1] + T[6] * t[2];
+  out[1] = T[1] * t[0] + T[4] * t[1] + T[7] * t[2];
+  out[2] = T[2] * t[0] + T[5] * t[1] + T[8] * t[2];
+}
+__host__ __device__ inline double el_geo_frm(double dl, const double *prop, double len0) {
+  return dl * prop[0] * prop[2] / len0;
+}
+// the internal force from the local deformation (dl, end rotations a, b)
+template <class ET>
+__host__ __device__ inline void el_stress_force(ET &e, double dl, const double *a, const double *b) {
+  const double *T = e.tee;
   const double ub[NED] = {0.0, 0.0, 0.0, a[0], a[1], a[2], dl, 0.0, 0.0, b[0], b[1], b[2]};
-  e.geoFRM = dl * e.prop[0] * e.prop[2] / e.len0;
   // fb = stl ub in ascending k.  The terms with a zero stiffness entry, and
   // those with k in {0, 1, 2, 7, 8} where ub is zero, add a signed zero to a
   // sum that starts at +0 and so change nothing; they are left out, and the
@@ -503,6 +510,15 @@ __host__ __device__ inline void el_body_stress(ET &e) {
       e.fint[6 + i] = e.fint[6 + i] + T[i * 3 + j] * fb[6 + j];
       e.fint[9 + i] = e.fint[9 + i] + T[i * 3 + j] * fb[9 + j];
     }
+}
+template <class ET>
+__host__ __device__ inline void el_body_stress(ET &e) {
+  const double dl = el_stress_dl(e);
+  double a[3], b[3];
+  el_stress_end(e, 0, a);
+  el_stress_end(e, 1, b);
+  e.geoFRM = el_geo_frm(dl, e.prop, e.len0);
+  el_stress_force(e, dl, a, b);
 }
 
 // Segment_StrainEnergy_D after FormStiffMatrix (SolidSolver.f90:841-907,

@@ -2104,7 +2104,9 @@ int fsil_fem_substeps(int h, double time, int numsubstep, double deltat) {
   };
   size_t cg_need = 0;
   for (const FemBodyDesc &d : F.desc) cg_need = std::max(cg_need, fem_cg_work(d.nND, d.nEL));
-  const int stage = std::max(12 * max_nd, 16);  // 2 nG of dot-product staging, >= one maximum per wave
+  // 2 nG of dot-product staging, >= one maximum per wave, >= the six end-rotation values per element of the
+  // element-stress phase
+  const int stage = std::max(std::max(12 * max_nd, 6 * max_ne), 16);
   // (the areas' offsets are even, from the kernel's 16-byte aligned base)
   int cg_lds = 1, work = ((int)std::max(cg_need, (size_t)stage) + 1) & ~1;
   if (fem_lds_bytes(work, 1, max_nd, max_ne) > lds_lim) {

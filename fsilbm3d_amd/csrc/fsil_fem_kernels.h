@@ -733,9 +733,31 @@ __device__ inline void fem_substep_body(FemBodyDesc &D, const FemArrays &A, int 
       // (odd waves, so on other SIMDs than the first forces) run side by
       // side, then the element matrices and the DOF loads (from the top
       // thread down).
-      const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63, nw2 = (int)blockDim.x >> 7;
-      if ((w & 1) == 0) stress_phase((w >> 1) * 64 + lane, nw2 * 64);
-      else load_phase((w >> 1) * 64 + lane, nw2 * 64);
+      // The internal forces in two parts on the four SIMDs: the first end's
+      // rotation (waves 0, 4), the element loads (1, 5), the second end's
+      // rotation (2, 6) and the axial force with the geometric-stiffness
+      // scalars (3, 7); then the forces themselves on waves 0, 4.  The end
+      // rotations wait in the dot-product staging area (free here).
+      const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63, nw4 = (int)blockDim.x >> 8;
+      double *rot_ab = lds;  // [e][6]: a, b
+      for (int e = (w >> 2) * 64 + lane; e < nE; e += nw4 * 64) {
+        auto &E = EL(e);
+        switch (w & 3) {
+          case 0: el_stress_end(E, 0, rot_ab + 6 * e); break;
+          case 1: el_update_load(E, c, D.dampM, D.dampK, A.dspO + g0, A.dsp + g0, A.vel + g0, A.acc + g0); break;
+          case 2: el_stress_end(E, 1, rot_ab + 6 * e + 3); break;
+          default:
+            E.geoFRM = el_geo_frm(el_stress_dl(E), E.prop, E.len0);
+            el_geom_scalars(E);
+            break;
+        }
+      }
+      fem_sync();
+      if ((w & 3) == 0)
+        for (int e = (w >> 2) * 64 + lane; e < nE; e += nw4 * 64) {
+          auto &E = EL(e);
+          el_stress_force(E, el_stress_dl(E), rot_ab + 6 * e, rot_ab + 6 * e + 3);
+        }
       fem_sync();
       matrices_phase();
       assembly_phase((int)blockDim.x - 1 - (int)threadIdx.x, blockDim.x);
